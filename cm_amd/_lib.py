@@ -1,0 +1,117 @@
+"""ctypes binding of liboc_asm.so (the C ABI declared in include/oc_asm.h).
+
+Status codes become :class:`CMISSError`, as the reference's Python package
+turns a nonzero ``cmfe`` status into an exception
+(bindings/python/opencmiss/_iron_utils.py:23-78).  The library must be the
+in-tree build; there is no fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboc_asm.so")
+
+OC_NO_ERROR = 0
+OC_ERR_INVALID_HANDLE = -1
+OC_ERR_INVALID_ARGUMENT = -2
+OC_ERR_HIP = -3
+OC_ERR_ALLOCATION = -4
+OC_ERR_NOT_CONVERGED = -5
+OC_ERR_PATTERN = -6
+OC_ERR_NOT_INITIALISED = -7
+OC_ERR_UNSUPPORTED = -8
+
+OC_EQUATIONS_SET_LAPLACE = 1
+OC_EQUATIONS_SET_MOONEY_RIVLIN = 2
+OC_FP64 = 0
+OC_FP32 = 1
+
+OC_BUFFER_MATRIX_VALUES = 0
+OC_BUFFER_MATRIX_VALUES_SP = 1
+OC_BUFFER_ROW_PTR = 2
+OC_BUFFER_COL_IDX = 3
+OC_BUFFER_DOFS = 4
+OC_BUFFER_RESIDUAL = 5
+OC_BUFFER_RHS = 6
+
+
+class CMISSError(RuntimeError):
+    """Raised when a liboc_asm entry point returns a nonzero status."""
+
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_H = ctypes.c_void_p
+_c_int, _c_double, _c_float = ctypes.c_int, ctypes.c_double, ctypes.c_float
+
+# name -> argtypes (all return int status)
+SIGNATURES = {
+    "oc_initialise": [_c_int],
+    "oc_finalise": [],
+    "oc_last_error": [ctypes.c_char_p, _c_int],
+    "oc_device_get": [ctypes.POINTER(_c_int)],
+    "oc_laplace_create": [_c_int, _dp, _c_int, _i32, _c_int, ctypes.POINTER(_H)],
+    "oc_mooney_rivlin_create": [_c_int, _dp, _c_int, _c_int, _i32, _i32, _c_int, _c_double, _c_double,
+                                ctypes.POINTER(_H)],
+    "oc_equations_set_destroy": [_H],
+    "oc_equations_set_info": [_H, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(ctypes.c_longlong),
+                              ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)],
+    "oc_materials_set": [_H, _c_double, _c_double],
+    "oc_matrix_structure_get": [_H, _c_int, _i32, _i32],
+    "oc_dofs_set": [_H, _dp],
+    "oc_dofs_get": [_H, _dp],
+    "oc_dofs_set_device": [_H, ctypes.c_void_p],
+    "oc_equations_set_assemble": [_H],
+    "oc_equations_set_residual_evaluate": [_H],
+    "oc_equations_set_jacobian_evaluate": [_H, _c_int],
+    "oc_matrix_values_get": [_H, _dp],
+    "oc_matrix_values_get_sp": [_H, _fp],
+    "oc_residual_get": [_H, _dp],
+    "oc_rhs_get": [_H, _dp],
+    "oc_device_pointer_get": [_H, _c_int, ctypes.POINTER(ctypes.c_void_p)],
+    "oc_event_record": [_c_int],
+    "oc_event_elapsed": [_c_int, _c_int, ctypes.POINTER(_c_float)],
+    "oc_synchronise": [],
+}
+
+_LIB = None
+
+
+def load():
+    """Load the in-tree library; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not found: build it with __graft_entry__.build() "
+                          "(make -C cm_amd/csrc); there is no fallback path")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = _c_int
+        f.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    load().oc_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(code):
+    if code != OC_NO_ERROR:
+        raise CMISSError(code, last_error())
+    return code
+
+
+def call(name, *args):
+    return check(getattr(load(), name)(*args))

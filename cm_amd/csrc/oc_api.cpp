@@ -1,0 +1,402 @@
+// oc_api.cpp -- the C ABI of liboc_asm (declared in include/oc_asm.h):
+// runtime, error reporting, equations-set lifetime, data movement and the
+// assembly entry points that dispatch to the HIP kernels.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "oc_internal.h"
+
+namespace oc {
+
+namespace {
+thread_local std::string g_last_error;
+Runtime g_runtime;
+std::mutex g_runtime_mutex;
+}  // namespace
+
+int set_error(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+Runtime &runtime() { return g_runtime; }
+
+int require_runtime()
+{
+    if (g_runtime.stream == nullptr)
+        return set_error(OC_ERR_NOT_INITIALISED, "oc_initialise has not been called");
+    return OC_NO_ERROR;
+}
+
+static int to_zero_based(const int *in, size_t n, int base, int limit, std::vector<int32_t> &out, const char *what)
+{
+    if (base != 0 && base != 1) return set_error(OC_ERR_INVALID_ARGUMENT, "index_base must be 0 or 1");
+    out.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        const int v = in[i] - base;
+        if (v < 0 || v >= limit)
+            return set_error(OC_ERR_INVALID_ARGUMENT, "%s[%zu] = %d out of range [%d, %d)", what, i, in[i], base,
+                             limit + base);
+        out[i] = v;
+    }
+    return OC_NO_ERROR;
+}
+
+static int finish_structure(EquationsSet &s, int ncomp, const int32_t *nn, const int32_t *nen,
+                            const int32_t *const *conn)
+{
+    OC_CHECK(build_pattern(ncomp, s.n_elements, nn, nen, conn, s.h_row_ptr, s.h_col_idx));
+    s.nnz = static_cast<int64_t>(s.h_col_idx.size());
+    std::vector<int32_t> colour_elems;
+    OC_CHECK(colour_elements(s.n_elements, ncomp, nn, nen, conn, s.colour_ptr, colour_elems));
+    OC_CHECK(s.colour_elems.upload(colour_elems.data(), colour_elems.size()));
+    OC_CHECK(s.row_ptr.upload(s.h_row_ptr.data(), s.h_row_ptr.size()));
+    OC_CHECK(s.col_idx.upload(s.h_col_idx.data(), s.h_col_idx.size()));
+    OC_CHECK(s.dofs.alloc(s.n_rows));
+    OC_CHECK(s.residual.alloc(s.n_rows));
+    OC_CHECK(s.rhs.alloc(s.n_rows));
+    OC_HIP_CHECK(hipMemset(s.dofs.ptr, 0, sizeof(double) * s.n_rows));
+    OC_HIP_CHECK(hipMemset(s.residual.ptr, 0, sizeof(double) * s.n_rows));
+    OC_HIP_CHECK(hipMemset(s.rhs.ptr, 0, sizeof(double) * s.n_rows));
+    return OC_NO_ERROR;
+}
+
+static int check_handle(oc_equations_set h)
+{
+    if (h == nullptr) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
+    return require_runtime();
+}
+
+}  // namespace oc
+
+using namespace oc;
+
+struct oc_equations_set_s {
+    EquationsSet s;
+};
+
+extern "C" {
+
+int oc_initialise(int device)
+{
+    std::lock_guard<std::mutex> lock(g_runtime_mutex);
+    Runtime &rt = g_runtime;
+    if (rt.stream != nullptr) {
+        if (rt.device == device) return OC_NO_ERROR;
+        return set_error(OC_ERR_INVALID_ARGUMENT, "already initialised on device %d", rt.device);
+    }
+    int count = 0;
+    OC_HIP_CHECK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "device %d not in [0, %d)", device, count);
+    OC_HIP_CHECK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    OC_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(OC_ERR_UNSUPPORTED, "device %d is %s; this library is built for gfx950 only", device,
+                         prop.gcnArchName);
+    rt.num_cus = prop.multiProcessorCount;
+    OC_HIP_CHECK(hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking));
+    for (auto &e : rt.events) OC_HIP_CHECK(hipEventCreate(&e));
+    rt.device = device;
+    return OC_NO_ERROR;
+}
+
+int oc_finalise(void)
+{
+    std::lock_guard<std::mutex> lock(g_runtime_mutex);
+    Runtime &rt = g_runtime;
+    if (rt.stream == nullptr) return OC_NO_ERROR;
+    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+    for (auto &e : rt.events) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    (void)hipStreamDestroy(rt.stream);
+    rt.stream = nullptr;
+    rt.device = -1;
+    return OC_NO_ERROR;
+}
+
+int oc_last_error(char *buffer, int buffer_size)
+{
+    if (buffer == nullptr || buffer_size <= 0) return OC_ERR_INVALID_ARGUMENT;
+    std::snprintf(buffer, static_cast<size_t>(buffer_size), "%s", g_last_error.c_str());
+    return OC_NO_ERROR;
+}
+
+int oc_device_get(int *device)
+{
+    if (!device) return set_error(OC_ERR_INVALID_ARGUMENT, "device is NULL");
+    *device = g_runtime.device;
+    return OC_NO_ERROR;
+}
+
+int oc_laplace_create(int n_nodes, const double *coords, int n_elements, const int *elem_nodes, int index_base,
+                      oc_equations_set *equations_set)
+{
+    OC_CHECK(require_runtime());
+    if (!coords || !elem_nodes || !equations_set || n_nodes <= 0 || n_elements <= 0)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "oc_laplace_create: invalid arguments");
+    auto *h = new (std::nothrow) oc_equations_set_s();
+    if (!h) return set_error(OC_ERR_ALLOCATION, "oc_laplace_create: out of host memory");
+    EquationsSet &s = h->s;
+    s.cls = OC_EQUATIONS_SET_LAPLACE;
+    s.n_nodes = n_nodes;
+    s.n_rows = n_nodes;
+    s.n_elements = n_elements;
+    s.nen = 8;
+    std::vector<int32_t> conn;
+    int rc = to_zero_based(elem_nodes, static_cast<size_t>(n_elements) * 8, index_base, n_nodes, conn, "elem_nodes");
+    const int32_t nn[1] = {n_nodes}, nen[1] = {8};
+    const int32_t *cp[1] = {conn.data()};
+    if (rc == OC_NO_ERROR) rc = finish_structure(s, 1, nn, nen, cp);
+    if (rc == OC_NO_ERROR) rc = s.conn.upload(conn.data(), conn.size());
+    if (rc == OC_NO_ERROR) rc = s.coords.upload(coords, static_cast<size_t>(3) * n_nodes);
+    if (rc == OC_NO_ERROR) rc = s.values.alloc(s.nnz);
+    if (rc != OC_NO_ERROR) {
+        delete h;
+        return rc;
+    }
+    *equations_set = h;
+    return OC_NO_ERROR;
+}
+
+int oc_mooney_rivlin_create(int n_nodes_q, const double *coords_q, int n_nodes_l, int n_elements,
+                            const int *elem_nodes_q, const int *elem_nodes_l, int index_base, double c10, double c01,
+                            oc_equations_set *equations_set)
+{
+    OC_CHECK(require_runtime());
+    if (!coords_q || !elem_nodes_q || !elem_nodes_l || !equations_set || n_nodes_q <= 0 || n_nodes_l <= 0 ||
+        n_elements <= 0)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "oc_mooney_rivlin_create: invalid arguments");
+    auto *h = new (std::nothrow) oc_equations_set_s();
+    if (!h) return set_error(OC_ERR_ALLOCATION, "oc_mooney_rivlin_create: out of host memory");
+    EquationsSet &s = h->s;
+    s.cls = OC_EQUATIONS_SET_MOONEY_RIVLIN;
+    s.n_nodes = n_nodes_q;
+    s.n_nodes_l = n_nodes_l;
+    s.n_rows = 3 * n_nodes_q + n_nodes_l;
+    s.n_elements = n_elements;
+    s.nen = 27;
+    s.c10 = c10;
+    s.c01 = c01;
+    std::vector<int32_t> cq, cl;
+    int rc = to_zero_based(elem_nodes_q, static_cast<size_t>(n_elements) * 27, index_base, n_nodes_q, cq,
+                           "elem_nodes_q");
+    if (rc == OC_NO_ERROR)
+        rc = to_zero_based(elem_nodes_l, static_cast<size_t>(n_elements) * 8, index_base, n_nodes_l, cl,
+                           "elem_nodes_l");
+    // four dependent components: z1, z2, z3 on the quadratic mesh component,
+    // p on the linear one (separated-component DOF order)
+    const int32_t nn[4] = {n_nodes_q, n_nodes_q, n_nodes_q, n_nodes_l}, nen[4] = {27, 27, 27, 8};
+    const int32_t *cp[4] = {cq.data(), cq.data(), cq.data(), cl.data()};
+    if (rc == OC_NO_ERROR) rc = finish_structure(s, 4, nn, nen, cp);
+    if (rc == OC_NO_ERROR) rc = s.conn.upload(cq.data(), cq.size());
+    if (rc == OC_NO_ERROR) rc = s.conn_l.upload(cl.data(), cl.size());
+    if (rc == OC_NO_ERROR) rc = s.coords.upload(coords_q, static_cast<size_t>(3) * n_nodes_q);
+    if (rc != OC_NO_ERROR) {
+        delete h;
+        return rc;
+    }
+    *equations_set = h;
+    return OC_NO_ERROR;
+}
+
+int oc_equations_set_destroy(oc_equations_set h)
+{
+    if (!h) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
+    if (g_runtime.stream) (void)hipStreamSynchronize(g_runtime.stream);
+    delete h;
+    return OC_NO_ERROR;
+}
+
+int oc_equations_set_info(oc_equations_set h, int *set_class, int *n_rows, long long *nnz, int *n_elements,
+                          int *n_colours)
+{
+    OC_CHECK(check_handle(h));
+    if (set_class) *set_class = h->s.cls;
+    if (n_rows) *n_rows = h->s.n_rows;
+    if (nnz) *nnz = h->s.nnz;
+    if (n_elements) *n_elements = h->s.n_elements;
+    if (n_colours) *n_colours = static_cast<int>(h->s.colour_ptr.size()) - 1;
+    return OC_NO_ERROR;
+}
+
+int oc_materials_set(oc_equations_set h, double c10, double c01)
+{
+    OC_CHECK(check_handle(h));
+    if (h->s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "materials are defined for Mooney-Rivlin sets only");
+    h->s.c10 = c10;
+    h->s.c01 = c01;
+    return OC_NO_ERROR;
+}
+
+int oc_matrix_structure_get(oc_equations_set h, int index_base, int *row_ptr, int *col_idx)
+{
+    OC_CHECK(check_handle(h));
+    if (index_base != 0 && index_base != 1) return set_error(OC_ERR_INVALID_ARGUMENT, "index_base must be 0 or 1");
+    const EquationsSet &s = h->s;
+    if (row_ptr)
+        for (size_t i = 0; i < s.h_row_ptr.size(); ++i) row_ptr[i] = s.h_row_ptr[i] + index_base;
+    if (col_idx)
+        for (size_t i = 0; i < s.h_col_idx.size(); ++i) col_idx[i] = s.h_col_idx[i] + index_base;
+    return OC_NO_ERROR;
+}
+
+int oc_dofs_set(oc_equations_set h, const double *dofs)
+{
+    OC_CHECK(check_handle(h));
+    if (!dofs) return set_error(OC_ERR_INVALID_ARGUMENT, "dofs is NULL");
+    OC_HIP_CHECK(hipMemcpyAsync(h->s.dofs.ptr, dofs, sizeof(double) * h->s.n_rows, hipMemcpyHostToDevice,
+                                g_runtime.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+int oc_dofs_set_device(oc_equations_set h, const double *dofs_device)
+{
+    OC_CHECK(check_handle(h));
+    if (!dofs_device) return set_error(OC_ERR_INVALID_ARGUMENT, "dofs_device is NULL");
+    OC_HIP_CHECK(hipMemcpyAsync(h->s.dofs.ptr, dofs_device, sizeof(double) * h->s.n_rows, hipMemcpyDeviceToDevice,
+                                g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+int oc_dofs_get(oc_equations_set h, double *dofs)
+{
+    OC_CHECK(check_handle(h));
+    if (!dofs) return set_error(OC_ERR_INVALID_ARGUMENT, "dofs is NULL");
+    OC_HIP_CHECK(hipMemcpyAsync(dofs, h->s.dofs.ptr, sizeof(double) * h->s.n_rows, hipMemcpyDeviceToHost,
+                                g_runtime.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+int oc_equations_set_assemble(oc_equations_set h)
+{
+    OC_CHECK(check_handle(h));
+    if (h->s.cls != OC_EQUATIONS_SET_LAPLACE)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "linear assembly is defined for Laplace sets only");
+    return launch_laplace_assemble(h->s);
+}
+
+int oc_equations_set_residual_evaluate(oc_equations_set h)
+{
+    OC_CHECK(check_handle(h));
+    if (h->s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "residual evaluation is defined for Mooney-Rivlin sets only");
+    return launch_mr_residual(h->s);
+}
+
+int oc_equations_set_jacobian_evaluate(oc_equations_set h, int precision)
+{
+    OC_CHECK(check_handle(h));
+    if (h->s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "Jacobian evaluation is defined for Mooney-Rivlin sets only");
+    if (precision != OC_FP64 && precision != OC_FP32)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "precision must be OC_FP64 or OC_FP32");
+    EquationsSet &s = h->s;
+    if (precision == OC_FP64 && s.values.ptr == nullptr) OC_CHECK(s.values.alloc(s.nnz));
+    if (precision == OC_FP32 && s.values_sp.ptr == nullptr) OC_CHECK(s.values_sp.alloc(s.nnz));
+    return launch_mr_jacobian(s, precision);
+}
+
+int oc_matrix_values_get(oc_equations_set h, double *values)
+{
+    OC_CHECK(check_handle(h));
+    if (!values) return set_error(OC_ERR_INVALID_ARGUMENT, "values is NULL");
+    if (!h->s.values.ptr) return set_error(OC_ERR_INVALID_ARGUMENT, "no FP64 matrix has been assembled");
+    OC_HIP_CHECK(hipMemcpyAsync(values, h->s.values.ptr, sizeof(double) * h->s.nnz, hipMemcpyDeviceToHost,
+                                g_runtime.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+int oc_matrix_values_get_sp(oc_equations_set h, float *values)
+{
+    OC_CHECK(check_handle(h));
+    if (!values) return set_error(OC_ERR_INVALID_ARGUMENT, "values is NULL");
+    if (!h->s.values_sp.ptr) return set_error(OC_ERR_INVALID_ARGUMENT, "no FP32 matrix has been assembled");
+    OC_HIP_CHECK(hipMemcpyAsync(values, h->s.values_sp.ptr, sizeof(float) * h->s.nnz, hipMemcpyDeviceToHost,
+                                g_runtime.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+static int vector_get(const DevBuf<double> &b, int n, double *out)
+{
+    if (!out) return set_error(OC_ERR_INVALID_ARGUMENT, "output is NULL");
+    OC_HIP_CHECK(hipMemcpyAsync(out, b.ptr, sizeof(double) * n, hipMemcpyDeviceToHost, g_runtime.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+int oc_residual_get(oc_equations_set h, double *residual)
+{
+    OC_CHECK(check_handle(h));
+    return vector_get(h->s.residual, h->s.n_rows, residual);
+}
+
+int oc_rhs_get(oc_equations_set h, double *rhs)
+{
+    OC_CHECK(check_handle(h));
+    return vector_get(h->s.rhs, h->s.n_rows, rhs);
+}
+
+int oc_device_pointer_get(oc_equations_set h, int buffer, void **device_pointer)
+{
+    OC_CHECK(check_handle(h));
+    if (!device_pointer) return set_error(OC_ERR_INVALID_ARGUMENT, "device_pointer is NULL");
+    EquationsSet &s = h->s;
+    void *p = nullptr;
+    switch (buffer) {
+    case OC_BUFFER_MATRIX_VALUES: p = s.values.ptr; break;
+    case OC_BUFFER_MATRIX_VALUES_SP: p = s.values_sp.ptr; break;
+    case OC_BUFFER_ROW_PTR: p = s.row_ptr.ptr; break;
+    case OC_BUFFER_COL_IDX: p = s.col_idx.ptr; break;
+    case OC_BUFFER_DOFS: p = s.dofs.ptr; break;
+    case OC_BUFFER_RESIDUAL: p = s.residual.ptr; break;
+    case OC_BUFFER_RHS: p = s.rhs.ptr; break;
+    default: return set_error(OC_ERR_INVALID_ARGUMENT, "unknown buffer %d", buffer);
+    }
+    if (!p) return set_error(OC_ERR_INVALID_ARGUMENT, "buffer %d has not been allocated yet", buffer);
+    *device_pointer = p;
+    return OC_NO_ERROR;
+}
+
+int oc_event_record(int slot)
+{
+    OC_CHECK(require_runtime());
+    if (slot < 0 || slot >= 64) return set_error(OC_ERR_INVALID_ARGUMENT, "event slot %d not in [0,64)", slot);
+    OC_HIP_CHECK(hipEventRecord(g_runtime.events[slot], g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+int oc_event_elapsed(int start, int end, float *milliseconds)
+{
+    OC_CHECK(require_runtime());
+    if (start < 0 || start >= 64 || end < 0 || end >= 64 || !milliseconds)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "invalid event slots");
+    OC_HIP_CHECK(hipEventSynchronize(g_runtime.events[end]));
+    OC_HIP_CHECK(hipEventElapsedTime(milliseconds, g_runtime.events[start], g_runtime.events[end]));
+    return OC_NO_ERROR;
+}
+
+int oc_synchronise(void)
+{
+    OC_CHECK(require_runtime());
+    OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+}  // extern "C"

@@ -1,0 +1,271 @@
+// oc_fe_device.h -- device-side finite element helpers for gfx950:
+// basis-table layout, 3x3 algebra, metrics, and the Mooney-Rivlin stress and
+// spatial elasticity tensor.  All FP64; every routine names the reference
+// routine whose arithmetic it restates.  3x3 matrices are column-major,
+// A[i + 3*j] == A(i+1, j+1) of the Fortran source.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace oc {
+
+// Basis tables: value and first derivatives of each basis function at each
+// Gauss point, GAUSS_BASIS_FNS(ns, {1,2,4,7}, ng) of BASIS_QUADRATURE_CREATE
+// (basis_routines.f90:2770-2890).  Layout [g][s][4] with nu 0 = value,
+// 1..3 = d/dxi1..3; Gauss points xi1-fastest; basis nodes xi1-fastest.
+struct BasisTables {
+    double w27[27];              // 3x3x3 Gauss weights
+    double w8[8];                // 2x2x2 Gauss weights
+    double quad27[27 * 27 * 4];  // triquadratic at 3x3x3
+    double lin27[27 * 8 * 4];    // trilinear at 3x3x3
+    double lin8[8 * 8 * 4];      // trilinear at 2x2x2
+};
+
+// Builds the tables on the host (restating GAUSS_LEGENDRE,
+// basis_routines.f90:5826-5875, and LAGRANGE_*_EVALUATE :7015-7137).
+void build_basis_tables(BasisTables &t);
+// Device copy owned by the runtime (built at oc_initialise).
+const BasisTables *device_basis_tables();
+
+// TENSOR_TO_VOIGT3, constants.f90:284 (0-based: 11,22,33,12,13,23).
+__device__ __forceinline__ int t2v(int i, int j) { return i == j ? i : 2 + i + j; }
+
+// INVERT_FULL_DP, 3x3 case, maths.f90:1455-1521 (cofactors over det;
+// zero matrix when |det| <= ZERO_TOLERANCE = 5*eps, constants.f90:68).
+__device__ __forceinline__ double invert3(const double *A, double *B)
+{
+#define a(i, j) A[(i - 1) + 3 * (j - 1)]
+#define b(i, j) B[(i - 1) + 3 * (j - 1)]
+    const double d = a(1, 1) * a(2, 2) * a(3, 3) + a(1, 2) * a(2, 3) * a(3, 1) + a(1, 3) * a(2, 1) * a(3, 2) -
+                     a(1, 1) * a(3, 2) * a(2, 3) - a(2, 1) * a(1, 2) * a(3, 3) - a(3, 1) * a(2, 2) * a(1, 3);
+    if (fabs(d) > 5.0 * 2.220446049250313e-16) {
+        b(1, 1) = (a(2, 2) * a(3, 3) - a(3, 2) * a(2, 3)) / d;
+        b(2, 1) = (a(2, 3) * a(3, 1) - a(3, 3) * a(2, 1)) / d;
+        b(3, 1) = (a(2, 1) * a(3, 2) - a(3, 1) * a(2, 2)) / d;
+        b(1, 2) = (a(3, 2) * a(1, 3) - a(1, 2) * a(3, 3)) / d;
+        b(2, 2) = (a(3, 3) * a(1, 1) - a(1, 3) * a(3, 1)) / d;
+        b(3, 2) = (a(3, 1) * a(1, 2) - a(1, 1) * a(3, 2)) / d;
+        b(1, 3) = (a(1, 2) * a(2, 3) - a(2, 2) * a(1, 3)) / d;
+        b(2, 3) = (a(1, 3) * a(2, 1) - a(2, 3) * a(1, 1)) / d;
+        b(3, 3) = (a(1, 1) * a(2, 2) - a(2, 1) * a(1, 2)) / d;
+    } else {
+        for (int i = 0; i < 9; ++i) B[i] = 0.0;
+    }
+    return d;
+#undef a
+#undef b
+}
+
+// COORDINATE_METRICS_CALCULATE, rectangular Cartesian, volume Jacobian
+// (coordinate_routines.f90:647-840): GL = DX_DXI^T DX_DXI, GU = inv(GL),
+// JACOBIAN = sqrt|det GL|.
+__device__ __forceinline__ double metric_jacobian(const double *dx, double *gu)
+{
+    double gl[9];
+    for (int mi = 0; mi < 3; ++mi)
+        for (int ni = 0; ni < 3; ++ni)
+            gl[mi + 3 * ni] = dx[0 + 3 * mi] * dx[0 + 3 * ni] + dx[1 + 3 * mi] * dx[1 + 3 * ni] +
+                              dx[2 + 3 * mi] * dx[2 + 3 * ni];
+    const double det = invert3(gl, gu);
+    return sqrt(fabs(det));
+}
+
+// Per-Gauss-point kinematics of a triquadratic element
+// (FiniteElasticity_FiniteElementResidualEvaluate :1491-1532 and
+// FiniteElasticity_FiniteElementJacobianEvaluate :954-989):
+//   dz/dxi, dX/dxi interpolated (FIELD_INTERPOLATE_GAUSS), J_z, J_X,
+//   DXI_DX = inv(dz/dxi), F = dz/dxi * inv(dX/dxi)
+//   (FiniteElasticity_GaussDeformationGradientTensor :3368-3425 with
+//   Coordinates_MaterialSystemCalculate, no fibres), DPHIDZ.
+struct GaussKinematics {
+    double F[9];
+    double jz, jx, Jznu;
+    double dxidz[9];
+};
+
+// ze, Xe: element parameters [c*27 + n]; fns: quad27 + g*27*4.
+__device__ __forceinline__ void gauss_kinematics(const double *fns, const double *ze, const double *Xe,
+                                                 GaussKinematics &k)
+{
+    double dz[9], dX[9], gu[9], dxidnu[9];
+    for (int x = 0; x < 3; ++x)
+        for (int ni = 0; ni < 3; ++ni) {
+            double sz = 0.0, sX = 0.0;
+            for (int n = 0; n < 27; ++n) {
+                const double f = fns[n * 4 + 1 + ni];
+                sz += f * ze[x * 27 + n];
+                sX += f * Xe[x * 27 + n];
+            }
+            dz[x + 3 * ni] = sz;
+            dX[x + 3 * ni] = sX;
+        }
+    k.jz = metric_jacobian(dz, gu);
+    k.jx = metric_jacobian(dX, gu);
+    invert3(dz, k.dxidz);
+    invert3(dX, dxidnu);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            k.F[i + 3 * j] = dz[i + 3 * 0] * dxidnu[0 + 3 * j] + dz[i + 3 * 1] * dxidnu[1 + 3 * j] +
+                             dz[i + 3 * 2] * dxidnu[2 + 3 * j];
+    k.Jznu = k.jz / k.jx;
+}
+
+// t(6x6) of FINITE_ELASTICITY_PUSH_STRESS_TENSOR / _ELASTICITY_TENSOR
+// (finite_elasticity_routines.f90:1196-1206, :1235-1245); t[i + 6j].
+__device__ __forceinline__ void voigt_push_matrix(const double *F, double *t)
+{
+    const int va[6] = {0, 1, 2, 0, 0, 1}, vb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            double v = F[va[i] + 3 * va[j]] * F[vb[i] + 3 * vb[j]];
+            if (j >= 3) v = v + F[va[i] + 3 * vb[j]] * F[vb[i] + 3 * va[j]];
+            t[i + 6 * j] = v;
+        }
+}
+
+// Modified right Cauchy-Green tensor and the MR isochoric fictitious 2nd
+// Piola tensor, Voigt (:4365-4384, :643-679).
+__device__ __forceinline__ void mr_fictitious_piola(const double *F, double Jznu, double c10, double c01,
+                                                    double *mod, double *S)
+{
+    const double s = pow(Jznu, -1.0 / 3.0);
+    for (int i = 0; i < 9; ++i) mod[i] = F[i] * s;
+    double azl[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            azl[i + 3 * j] = mod[0 + 3 * i] * mod[0 + 3 * j] + mod[1 + 3 * i] * mod[1 + 3 * j] +
+                             mod[2 + 3 * i] * mod[2 + 3 * j];
+    const double I1 = azl[0] + azl[4] + azl[8];
+    const double t1 = -2.0 * c01, t2 = 2.0 * (c10 + I1 * c01);
+    S[0] = t1 * azl[0] + t2;
+    S[1] = t1 * azl[4] + t2;
+    S[2] = t1 * azl[8] + t2;
+    S[3] = t1 * azl[1];
+    S[4] = t1 * azl[2];
+    S[5] = t1 * azl[5];
+}
+
+// FINITE_ELASTICITY_GAUSS_STRESS_TENSOR, MR branch
+// (finite_elasticity_routines.f90:4342-4410): Cauchy stress, Voigt.
+__device__ __forceinline__ void mr_cauchy_stress(const double *F, double Jznu, double p, double c10, double c01,
+                                                 double *sigma)
+{
+    double mod[9], S[6], t[36];
+    mr_fictitious_piola(F, Jznu, c10, c01, mod, S);
+    voigt_push_matrix(mod, t);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) v += t[i + 6 * j] * S[j];
+        sigma[i] = v / Jznu;
+    }
+    const double ot = (sigma[0] + sigma[1] + sigma[2]) / 3.0;
+    for (int i = 0; i < 3; ++i) sigma[i] = sigma[i] - ot + p;
+}
+
+// FINITE_ELASTICITY_GAUSS_ELASTICITY_TENSOR, MR branch
+// (finite_elasticity_routines.f90:609-740): spatial elasticity tensor c
+// (6x6 column-major) and Cauchy stress sigma (Voigt, pressure included).
+__device__ __forceinline__ void mr_elasticity_tensor(const double *F, double Jznu, double p, double c10, double c01,
+                                                     double *c, double *sigma)
+{
+    const double ONETHIRD = 1.0 / 3.0, TWOTHIRDS = 2.0 / 3.0;
+    double mod[9], S[6], t[36];
+    mr_fictitious_piola(F, Jznu, c10, c01, mod, S);
+    voigt_push_matrix(mod, t);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) v += t[i + 6 * j] * S[j];
+        sigma[i] = v / Jznu;
+    }
+    // fictitious material tangent: 4 c01 off-diagonal in the normal block,
+    // -2 c01 on the shear diagonal (:686-697); push-forward t E t^T / J.
+    const double e1 = 4.0 * c01, e2 = -2.0 * c01;
+    double tE[36];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        tE[i + 0] = t[i + 6 * 1] * e1 + t[i + 6 * 2] * e1;
+        tE[i + 6] = t[i + 6 * 0] * e1 + t[i + 6 * 2] * e1;
+        tE[i + 12] = t[i + 6 * 0] * e1 + t[i + 6 * 1] * e1;
+        tE[i + 18] = t[i + 18] * e2;
+        tE[i + 24] = t[i + 24] * e2;
+        tE[i + 30] = t[i + 30] * e2;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v += tE[i + 6 * k] * t[j + 6 * k];
+            c[i + 6 * j] = v / Jznu;
+        }
+    const double trace = sigma[0] + sigma[1] + sigma[2];
+    for (int i = 0; i < 3; ++i) sigma[i] = sigma[i] - ONETHIRD * trace;
+    const double ttr = TWOTHIRDS * trace;
+    for (int i = 0; i < 3; ++i) c[i + 6 * i] = c[i + 6 * i] + ttr;
+    for (int i = 3; i < 6; ++i) c[i + 6 * i] = c[i + 6 * i] + ttr * 0.5;
+    // deviatoric projection of the normal block and the coupling block
+    // (:705-717), DEV = I - 1/3 11^T on the normal block.
+    double a[9], b[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < 3; ++k) v += c[i + 6 * k] * (k == j ? TWOTHIRDS : -ONETHIRD);
+            a[i + 3 * j] = v;
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < 3; ++k) v += (i == k ? TWOTHIRDS : -ONETHIRD) * a[k + 3 * j];
+            b[i + 3 * j] = v;
+        }
+    double tmp[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < 3; ++k) v += (i == k ? TWOTHIRDS : -ONETHIRD) * c[k + 6 * (j + 3)];
+            tmp[i + 3 * j] = v;
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            c[i + 6 * j] = b[i + 3 * j];
+            c[i + 6 * (j + 3)] = tmp[i + 3 * j];
+            c[(j + 3) + 6 * i] = tmp[i + 3 * j];
+        }
+    // - (2/3 1 (x) sigma_dev + sigma_dev (x) 2/3 1)  (:719-724)
+    for (int j = 0; j < 6; ++j)
+        for (int i = 0; i < 6; ++i) {
+            const double ui = i < 3 ? TWOTHIRDS : 0.0, uj = j < 3 ? TWOTHIRDS : 0.0;
+            c[i + 6 * j] = c[i + 6 * j] - (ui * sigma[j] + uj * sigma[i]);
+        }
+    // volumetric part (:726-739)
+    for (int i = 0; i < 3; ++i) sigma[i] = sigma[i] + p;
+    for (int i = 0; i < 6; ++i) c[i + 6 * i] = c[i + 6 * i] - p;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (i != j) c[i + 6 * j] = c[i + 6 * j] + p;
+}
+
+// Position of column `col` in CSR row `row` (columns sorted ascending);
+// the assembly guarantees presence, so a miss returns -1 and is skipped.
+__device__ __forceinline__ int csr_slot(const int *__restrict__ row_ptr, const int *__restrict__ col_idx, int row,
+                                        int col)
+{
+    int lo = row_ptr[row], hi = row_ptr[row + 1] - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = col_idx[mid];
+        if (v == col) return mid;
+        if (v < col) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return -1;
+}
+
+}  // namespace oc

@@ -1,0 +1,111 @@
+// oc_internal.h -- internal declarations shared by the liboc_asm sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/oc_asm.h"
+
+namespace oc {
+
+// ---- error handling ---------------------------------------------------------
+int set_error(int code, const char *fmt, ...);
+#define OC_HIP_CHECK(call)                                                                              \
+    do {                                                                                                \
+        hipError_t _e = (call);                                                                         \
+        if (_e != hipSuccess)                                                                           \
+            return ::oc::set_error(OC_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                                   __FILE__, __LINE__);                                                 \
+    } while (0)
+#define OC_CHECK(call)              \
+    do {                            \
+        int _rc = (call);           \
+        if (_rc != OC_NO_ERROR)     \
+            return _rc;             \
+    } while (0)
+
+struct Runtime {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t events[64] = {};
+    int num_cus = 0;
+};
+Runtime &runtime();
+int require_runtime();
+
+// Device buffer owned by the library.
+template <typename T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t n = 0;
+    int alloc(size_t count)
+    {
+        release();
+        if (count == 0) return OC_NO_ERROR;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(T));
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return set_error(OC_ERR_ALLOCATION, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                             hipGetErrorString(e));
+        }
+        n = count;
+        return OC_NO_ERROR;
+    }
+    int upload(const T *host, size_t count)
+    {
+        OC_CHECK(alloc(count));
+        if (count) OC_HIP_CHECK(hipMemcpy(ptr, host, count * sizeof(T), hipMemcpyHostToDevice));
+        return OC_NO_ERROR;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+// Per-equations-set state.  Element arrays are element-major; node and DOF
+// indices are 0-based; CSR row_ptr/col_idx 0-based int32.
+struct EquationsSet {
+    int cls = 0;
+    int n_rows = 0;          // DOFs
+    int64_t nnz = 0;
+    int n_elements = 0;
+    int nen = 0;             // element nodes of the main component (8 or 27)
+    int n_nodes = 0;         // nodes of the main component
+    int n_nodes_l = 0;       // pressure component nodes (MR)
+    double c10 = 0.0, c01 = 0.0;
+
+    // host copies of the structure
+    std::vector<int32_t> h_row_ptr, h_col_idx;
+
+    // colouring: elements of colour c are colour_elems[colour_ptr[c] .. colour_ptr[c+1])
+    std::vector<int32_t> colour_ptr;
+    DevBuf<int32_t> colour_elems;
+
+    DevBuf<double> coords;   // 3 * n_nodes, separated
+    DevBuf<int32_t> conn;    // nen * E
+    DevBuf<int32_t> conn_l;  // 8 * E (MR pressure component)
+    DevBuf<int32_t> row_ptr, col_idx;
+    DevBuf<double> values;
+    DevBuf<float> values_sp;
+    DevBuf<double> dofs, residual, rhs;
+};
+
+// ---- host-side setup helpers (oc_setup.cpp) ---------------------------------
+int build_pattern(int ncomp, int n_elements, const int32_t *nn, const int32_t *nen, const int32_t *const *conn,
+                  std::vector<int32_t> &row_ptr, std::vector<int32_t> &col_idx);
+int colour_elements(int n_elements, int ncomp, const int32_t *nn, const int32_t *nen, const int32_t *const *conn,
+                    std::vector<int32_t> &colour_ptr, std::vector<int32_t> &colour_elems);
+
+// ---- kernel launchers --------------------------------------------------------
+int launch_laplace_assemble(EquationsSet &s);
+int launch_mr_residual(EquationsSet &s);
+int launch_mr_jacobian(EquationsSet &s, int precision);
+
+}  // namespace oc

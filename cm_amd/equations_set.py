@@ -1,0 +1,177 @@
+"""Host-side equations sets: the Python face of the assembly C ABI.
+
+Mirrors the reference's equations-set operator interface for this path:
+``assemble`` (EQUATIONS_SET_ASSEMBLE, equations_set_routines.f90:981),
+``residual_evaluate`` (EQUATIONS_SET_RESIDUAL_EVALUATE, :5184) and
+``jacobian_evaluate`` (EQUATIONS_SET_JACOBIAN_EVALUATE, :4694), with the
+matrix accessors of cmfe_Equations_LinearMatrixGet / JacobianMatrixGet
+(opencmiss_iron.f90:22861, 23233) and the CSR storage-location / data getters
+(:44843, :44920).  Every call goes to the HIP library; a failing status
+raises CMISSError.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+
+_initialised_device = None
+
+
+def initialise(device=0):
+    """Select the HIP device (gfx950) and create the library stream."""
+    global _initialised_device
+    call("oc_initialise", int(device))
+    _initialised_device = int(device)
+
+
+def finalise():
+    global _initialised_device
+    call("oc_finalise")
+    _initialised_device = None
+
+
+def synchronise():
+    call("oc_synchronise")
+
+
+def event_record(slot):
+    call("oc_event_record", int(slot))
+
+
+def event_elapsed(start, end):
+    ms = ctypes.c_float(0.0)
+    call("oc_event_elapsed", int(start), int(end), ctypes.byref(ms))
+    return float(ms.value)
+
+
+class EquationsSet:
+    """Common handle management for the two equations-set classes."""
+
+    def __init__(self, handle):
+        self._h = handle
+        cls, nrows, nnz, ne, ncol = (ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int(),
+                                     ctypes.c_int())
+        call("oc_equations_set_info", self._h, ctypes.byref(cls), ctypes.byref(nrows), ctypes.byref(nnz),
+             ctypes.byref(ne), ctypes.byref(ncol))
+        self.set_class = cls.value
+        self.n_rows = nrows.value
+        self.nnz = nnz.value
+        self.n_elements = ne.value
+        self.n_colours = ncol.value
+
+    def destroy(self):
+        if self._h:
+            call("oc_equations_set_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            if self._h and _lib._LIB is not None:
+                _lib._LIB.oc_equations_set_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- structure --------------------------------------------------------
+    def matrix_structure(self, index_base=0):
+        row_ptr = np.zeros(self.n_rows + 1, np.int32)
+        col_idx = np.zeros(self.nnz, np.int32)
+        call("oc_matrix_structure_get", self._h, int(index_base), row_ptr, col_idx)
+        return row_ptr, col_idx
+
+    # -- dependent field --------------------------------------------------
+    def set_dofs(self, dofs):
+        dofs = np.ascontiguousarray(dofs, dtype=np.float64)
+        if dofs.shape != (self.n_rows,):
+            raise ValueError(f"dofs must have shape ({self.n_rows},)")
+        call("oc_dofs_set", self._h, dofs)
+
+    def get_dofs(self):
+        out = np.zeros(self.n_rows)
+        call("oc_dofs_get", self._h, out)
+        return out
+
+    # -- results ----------------------------------------------------------
+    def matrix_values(self):
+        out = np.zeros(self.nnz)
+        call("oc_matrix_values_get", self._h, out)
+        return out
+
+    def matrix_values_sp(self):
+        out = np.zeros(self.nnz, np.float32)
+        call("oc_matrix_values_get_sp", self._h, out)
+        return out
+
+    def residual(self):
+        out = np.zeros(self.n_rows)
+        call("oc_residual_get", self._h, out)
+        return out
+
+    def rhs(self):
+        out = np.zeros(self.n_rows)
+        call("oc_rhs_get", self._h, out)
+        return out
+
+    def device_pointer(self, buffer):
+        p = ctypes.c_void_p()
+        call("oc_device_pointer_get", self._h, int(buffer), ctypes.byref(p))
+        return p.value
+
+
+class LaplaceEquationsSet(EquationsSet):
+    """Classical field / Laplace / standard subtype on trilinear hexahedra."""
+
+    def __init__(self, mesh):
+        if mesh.order != 1:
+            raise ValueError("the Laplace equations set uses a trilinear mesh")
+        h = ctypes.c_void_p()
+        coords = np.ascontiguousarray(mesh.coords, dtype=np.float64).ravel()
+        conn = np.ascontiguousarray(mesh.elem_nodes, dtype=np.int32).ravel()
+        call("oc_laplace_create", mesh.n_nodes, coords, mesh.n_elements, conn, 0, ctypes.byref(h))
+        super().__init__(h)
+        self.mesh = mesh
+
+    def assemble(self):
+        """Zero and assemble the linear matrix and RHS vector on the GPU."""
+        call("oc_equations_set_assemble", self._h)
+
+
+class MooneyRivlinEquationsSet(EquationsSet):
+    """Elasticity / finite elasticity / Mooney-Rivlin subtype:
+    triquadratic deformed coordinates plus trilinear hydrostatic pressure."""
+
+    def __init__(self, mesh, c10, c01):
+        if mesh.order != 2:
+            raise ValueError("the Mooney-Rivlin equations set uses a triquadratic mesh")
+        h = ctypes.c_void_p()
+        coords = np.ascontiguousarray(mesh.coords, dtype=np.float64).ravel()
+        cq = np.ascontiguousarray(mesh.elem_nodes, dtype=np.int32).ravel()
+        cl = np.ascontiguousarray(mesh.elem_nodes_l, dtype=np.int32).ravel()
+        call("oc_mooney_rivlin_create", mesh.n_nodes, coords, mesh.n_nodes_l, mesh.n_elements, cq, cl, 0,
+             float(c10), float(c01), ctypes.byref(h))
+        super().__init__(h)
+        self.mesh = mesh
+        self.c10, self.c01 = float(c10), float(c01)
+
+    def set_materials(self, c10, c01):
+        call("oc_materials_set", self._h, float(c10), float(c01))
+        self.c10, self.c01 = float(c10), float(c01)
+
+    def undeformed_dofs(self):
+        """z = X and p = 0 (the initial dependent field)."""
+        nq = self.mesh.n_nodes
+        d = np.zeros(self.n_rows)
+        d[:3 * nq] = self.mesh.coords.ravel()
+        return d
+
+    def residual_evaluate(self):
+        call("oc_equations_set_residual_evaluate", self._h)
+
+    def jacobian_evaluate(self, precision=_lib.OC_FP64):
+        call("oc_equations_set_jacobian_evaluate", self._h, int(precision))

@@ -1,0 +1,150 @@
+/*
+ * oc_asm.h -- C ABI of the MI355X (gfx950) finite element assembly library
+ * (liboc_asm.so).  It replaces, at the assembly level, the equations-set
+ * element loops of OpenCMISS-Iron for the Laplace (standard subtype) and
+ * Mooney-Rivlin finite elasticity equations sets, plus the scatter-add of
+ * element matrices into the CSR equations matrices.
+ *
+ * Conventions (mirroring the reference's generated C bindings,
+ * bindings/generate_bindings/c.py:20-40, 234-590):
+ *   - every entry point returns int: OC_NO_ERROR (0) or a negative code;
+ *     the message of the last failure on the calling thread is available
+ *     through oc_last_error (cmfe_ExtractErrorMessage analogue,
+ *     bindings/python/opencmiss/_iron_utils.py:23-78);
+ *   - objects are opaque handles; scalars IN by value, OUT by pointer;
+ *     arrays are (pointer, size) pairs; indices are 32-bit ints, values
+ *     double (float for FP32 Jacobians);
+ *   - host arrays are caller-owned and copied in; device buffers are owned by
+ *     the library and may be borrowed with oc_device_pointer_get
+ *     (cmfe_DistributedMatrix_DataGetDPObj analogue, opencmiss_iron.f90:
+ *     44920-44945);
+ *   - field vectors use separated-component DOF order
+ *     (field_routines.f90:10345-10347); coordinates are separated
+ *     (x[0..N) y[N..2N) z[2N..3N));
+ *   - index_base selects 0- or 1-based node/DOF indices on input and output
+ *     (the reference stores CSR 1-based, equations_matrices_routines.f90:
+ *     5133,5194-5198).
+ * A handle is not thread-safe; distinct handles may be used concurrently.
+ * Every call is ordered on the library's stream; copy-out calls synchronise.
+ */
+#ifndef OC_ASM_H
+#define OC_ASM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (CMFE_NO_ERROR = 0 analogue) ------------------------- */
+#define OC_NO_ERROR 0
+#define OC_ERR_INVALID_HANDLE (-1)
+#define OC_ERR_INVALID_ARGUMENT (-2)
+#define OC_ERR_HIP (-3)
+#define OC_ERR_ALLOCATION (-4)
+#define OC_ERR_NOT_CONVERGED (-5)
+#define OC_ERR_PATTERN (-6)
+#define OC_ERR_NOT_INITIALISED (-7)
+#define OC_ERR_UNSUPPORTED (-8)
+
+/* ---- equations-set classes ---------------------------------------------- */
+#define OC_EQUATIONS_SET_LAPLACE 1         /* classical field, Laplace, standard subtype */
+#define OC_EQUATIONS_SET_MOONEY_RIVLIN 2   /* elasticity, finite elasticity, MR subtype */
+
+/* ---- precision flags ----------------------------------------------------- */
+#define OC_FP64 0
+#define OC_FP32 1
+
+/* ---- device buffers that can be borrowed --------------------------------- */
+#define OC_BUFFER_MATRIX_VALUES 0     /* double[nnz]: linear matrix or FP64 Jacobian */
+#define OC_BUFFER_MATRIX_VALUES_SP 1  /* float[nnz]: FP32 Jacobian */
+#define OC_BUFFER_ROW_PTR 2           /* int[n_rows+1], 0-based */
+#define OC_BUFFER_COL_IDX 3           /* int[nnz], 0-based */
+#define OC_BUFFER_DOFS 4              /* double[n_rows]: dependent field values */
+#define OC_BUFFER_RESIDUAL 5          /* double[n_rows] */
+#define OC_BUFFER_RHS 6               /* double[n_rows] */
+
+typedef struct oc_equations_set_s *oc_equations_set;
+
+/* ---- library ------------------------------------------------------------- */
+/* Select the HIP device and create the library stream.  Replaces the
+ * computational-environment start-up (computational_environment.f90:539-633)
+ * for the single-GPU path. */
+int oc_initialise(int device);
+int oc_finalise(void);
+/* Copies the last error message of the calling thread (NUL-terminated). */
+int oc_last_error(char *buffer, int buffer_size);
+/* Number of the HIP device in use, or -1. */
+int oc_device_get(int *device);
+
+/* ---- equations sets ------------------------------------------------------ */
+/* Laplace, standard subtype, trilinear Lagrange hexahedra, 2x2x2 Gauss.
+ * Replaces the setup of Laplace_equations_routines.f90:1872-1900 plus the
+ * linear-matrix structure calculation (equations_matrices_routines.f90:
+ * 4572-4780).  coords: 3*n_nodes separated; elem_nodes: 8*n_elements,
+ * element-major, local node order xi1 fastest. */
+int oc_laplace_create(int n_nodes, const double *coords, int n_elements, const int *elem_nodes,
+                      int index_base, oc_equations_set *equations_set);
+
+/* Mooney-Rivlin finite elasticity (EQUATIONS_SET_MOONEY_RIVLIN_SUBTYPE):
+ * triquadratic displacement (deformed coordinates z, 3 components) and
+ * trilinear hydrostatic pressure (component 4), 3x3x3 Gauss; geometric field
+ * triquadratic.  Replaces finite_elasticity_routines.f90:4696-4711,
+ * 6962-7025 and the Jacobian structure calculation
+ * (equations_matrices_routines.f90:5048-5230).
+ * coords_q: 3*n_nodes_q; elem_nodes_q: 27*n_elements; elem_nodes_l:
+ * 8*n_elements indexing the pressure component's own nodes (0..n_nodes_l-1
+ * for base 0).  c10, c01: materials field components 1 and 2
+ * (finite_elasticity_routines.f90:6609-6613). */
+int oc_mooney_rivlin_create(int n_nodes_q, const double *coords_q, int n_nodes_l, int n_elements,
+                            const int *elem_nodes_q, const int *elem_nodes_l, int index_base, double c10,
+                            double c01, oc_equations_set *equations_set);
+int oc_equations_set_destroy(oc_equations_set equations_set);
+/* class (OC_EQUATIONS_SET_*), number of rows (= DOFs), number of nonzeros,
+ * number of elements, number of element colours. */
+int oc_equations_set_info(oc_equations_set equations_set, int *set_class, int *n_rows, long long *nnz,
+                          int *n_elements, int *n_colours);
+/* Materials (c10, c01) of a Mooney-Rivlin set. */
+int oc_materials_set(oc_equations_set equations_set, double c10, double c01);
+
+/* ---- CSR structure -------------------------------------------------------- */
+/* Copies the equations-matrix structure (row_ptr[n_rows+1], col_idx[nnz]),
+ * columns sorted ascending and unique per row (cmfe_DistributedMatrix_
+ * StorageLocationsGetObj analogue, opencmiss_iron.f90:44843). */
+int oc_matrix_structure_get(oc_equations_set equations_set, int index_base, int *row_ptr, int *col_idx);
+
+/* ---- dependent field ------------------------------------------------------- */
+int oc_dofs_set(oc_equations_set equations_set, const double *dofs);
+int oc_dofs_get(oc_equations_set equations_set, double *dofs);
+/* Device-to-device set from a borrowed/foreign device pointer. */
+int oc_dofs_set_device(oc_equations_set equations_set, const double *dofs_device);
+
+/* ---- assembly -------------------------------------------------------------- */
+/* Linear static assembly (EQUATIONS_SET_ASSEMBLE -> ..._STATIC_LINEAR_FEM,
+ * equations_set_routines.f90:981, 1295-1492): zero and assemble the linear
+ * matrix values and the RHS vector. */
+int oc_equations_set_assemble(oc_equations_set equations_set);
+/* Nonlinear residual (EQUATIONS_SET_RESIDUAL_EVALUATE ->
+ * ..._STATIC_FEM, equations_set_routines.f90:5184, 5483-5629). */
+int oc_equations_set_residual_evaluate(oc_equations_set equations_set);
+/* Analytic Jacobian (EQUATIONS_SET_JACOBIAN_EVALUATE -> ..._STATIC_FEM,
+ * equations_set_routines.f90:4694, 4878-5024).  precision OC_FP64 fills
+ * OC_BUFFER_MATRIX_VALUES, OC_FP32 fills OC_BUFFER_MATRIX_VALUES_SP. */
+int oc_equations_set_jacobian_evaluate(oc_equations_set equations_set, int precision);
+
+/* ---- copy-out / borrow ----------------------------------------------------- */
+int oc_matrix_values_get(oc_equations_set equations_set, double *values);
+int oc_matrix_values_get_sp(oc_equations_set equations_set, float *values);
+int oc_residual_get(oc_equations_set equations_set, double *residual);
+int oc_rhs_get(oc_equations_set equations_set, double *rhs);
+int oc_device_pointer_get(oc_equations_set equations_set, int buffer, void **device_pointer);
+
+/* ---- timing on the library stream ----------------------------------------- */
+/* Records event `slot` (0..63) on the library stream. */
+int oc_event_record(int slot);
+/* Milliseconds between two recorded events (synchronises on `end`). */
+int oc_event_elapsed(int start, int end, float *milliseconds);
+int oc_synchronise(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
