@@ -1,0 +1,214 @@
+#!/usr/bin/env python
+"""Benchmark of the MI355X finite element assembly path.
+
+Workload (one "step"): Mooney-Rivlin residual + FP64 analytic Jacobian
+assembly of the 16x16x16 triquadratic/trilinear unit-cube mesh (4,096
+elements, 112,724 DOFs, 22,626,496 Jacobian nonzeros), interior nodes
+jittered by U[-0.1h, 0.1h], c10 = 2, c01 = 6, at a deformed state (10%
+uniaxial stretch plus a seeded perturbation).  Inputs are resident in HBM
+before the timed region.  Prints one JSON line.
+
+Also reported (``secondary``): Laplace assembly throughput at 8^3 / 64^3 /
+128^3 trilinear and the FP32-Jacobian variant.  ``cpu_baseline`` times the
+C oracle (an FP64 restatement of the reference element loop with the
+reference's per-entry CSR bisection) on this host, single thread, on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
+# dense per-element G^T C G count, 27*(2*81*9*9 + 2*81*81*9) + 27*2*81*8
+MR_JACOBIAN_FLOP_PER_ELEMENT = 3577932
+
+
+def deformed_state(eq, mesh, stretch=1.10, seed=17):
+    """Homogeneous uniaxial stretch plus a small seeded perturbation."""
+    nq, nl = mesh.n_nodes, mesh.n_nodes_l
+    X = mesh.coords
+    lam = np.array([stretch, stretch ** -0.5, stretch ** -0.5])
+    rng = np.random.default_rng(seed)
+    z = X * lam[:, None] + 0.002 * rng.uniform(-1, 1, X.shape)
+    p = 0.5 * rng.standard_normal(nl)
+    return np.concatenate([z.ravel(), p])
+
+
+def laplace_bytes(n_elements, n_nodes, nnz):
+    # connectivity int32x8 + coordinates 3 doubles + CSR values + RHS
+    return 32 * n_elements + 24 * n_nodes + 8 * nnz + 8 * n_nodes
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy-kernel calibration); None if not measured."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def time_steps(fn, steps, warmup, cm):
+    for _ in range(warmup):
+        fn()
+    cm.synchronise()
+    cm.kernel_timing_enable(True)
+    cm.event_record(0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    cm.event_record(1)
+    cm.synchronise()
+    wall = time.perf_counter() - t0
+    ms = cm.event_elapsed(0, 1)
+    return ms, wall
+
+
+def laplace_line(cm, n, steps):
+    mesh = cm.RegularMesh.generate(1, n)
+    eq = cm.LaplaceEquationsSet(mesh)
+    ms, _ = time_steps(eq.assemble, steps, 2, cm)
+    kms, kl = cm.kernel_timing("laplace_assemble")
+    fms, _ = cm.kernel_timing("fill")
+    cm.kernel_timing_enable(False)
+    nbytes = laplace_bytes(mesh.n_elements, mesh.n_nodes, eq.nnz)
+    out = {
+        "n": n, "elements": mesh.n_elements, "nnz": eq.nnz,
+        "elements_per_s": mesh.n_elements * steps / (ms / 1e3),
+        "ms_per_assembly": ms / steps,
+        "kernel_ms_per_assembly": (kms + fms) / steps,
+        "algorithmic_bytes": nbytes,
+        "achieved_GBs": nbytes / ((kms + fms) / steps / 1e3) / 1e9,
+    }
+    out["hbm_frac"] = out["achieved_GBs"] / HBM_PEAK_GBS
+    eq.destroy()
+    return out
+
+
+def cpu_baseline(mesh, dofs, c10, c01, seconds):
+    """Oracle (C FP64 restatement of the reference loop) on this host, 1 thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    rp, ci = orc.pattern([(mesh.n_nodes, mesh.elem_nodes)] * 3 + [(mesh.n_nodes_l, mesh.elem_nodes_l)])
+    reps, t_total = 0, 0.0
+    while t_total < seconds or reps == 0:
+        t0 = time.perf_counter()
+        orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, dofs, c10, c01)
+        orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, dofs, c10, c01, rp, ci)
+        t_total += time.perf_counter() - t0
+        reps += 1
+    return {
+        "value": mesh.n_elements * reps / t_total,
+        "unit": "elements/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{mesh.n[0]}^3 MR residual + FP64 analytic Jacobian with per-entry CSR bisection add "
+                  f"({mesh.n_elements} elements) x {reps} repetitions, {t_total:.1f} s, setup excluded",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=16, help="elements per direction of the MR workload")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
+              "path is a design (element partitions with halo exchange) and is not run here.", file=sys.stderr)
+        sys.exit(2)
+
+    import cm_amd as cm
+    cm.initialise(0)
+    c10, c01 = 2.0, 6.0
+    mesh = cm.RegularMesh.generate(2, args.n, jitter=0.1, seed=2024)
+    eq = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
+    dofs = deformed_state(eq, mesh)
+    eq.set_dofs(dofs)
+
+    def step():
+        eq.residual_evaluate()
+        eq.jacobian_evaluate(cm.OC_FP64)
+
+    ms, wall = time_steps(step, args.steps, args.warmup, cm)
+    jac_ms, jac_launches = cm.kernel_timing("mr_jacobian_fp64")
+    res_ms, _ = cm.kernel_timing("mr_residual")
+    fill_ms, _ = cm.kernel_timing("fill")
+    cm.kernel_timing_enable(False)
+    E = mesh.n_elements
+    value = E * args.steps / (ms / 1e3)
+    per_launch_elems = E * args.steps / jac_launches
+    avg_launch_s = jac_ms / jac_launches / 1e3
+    achieved = MR_JACOBIAN_FLOP_PER_ELEMENT * per_launch_elems / avg_launch_s / 1e12
+    roofline = {
+        "kernel": "mr_jacobian_kernel<double>",
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP64_PEAK_TFLOPS,
+        "traffic": traffic_from_profile("mr_jacobian_fp64"),
+        "avg_launch_ms": avg_launch_s * 1e3,
+        "elements_per_launch": per_launch_elems,
+        "flop_per_element": MR_JACOBIAN_FLOP_PER_ELEMENT,
+    }
+
+    secondary = {
+        "kernel_ms_per_step": {"mr_jacobian_fp64": jac_ms / args.steps, "mr_residual": res_ms / args.steps,
+                               "fill": fill_ms / args.steps},
+        "wall_ms_per_step": wall * 1e3 / args.steps,
+    }
+    if not args.no_secondary:
+        eq.jacobian_evaluate(cm.OC_FP32)
+        ms32, _ = time_steps(lambda: eq.jacobian_evaluate(cm.OC_FP32), args.steps, 1, cm)
+        j32, l32 = cm.kernel_timing("mr_jacobian_fp32")
+        cm.kernel_timing_enable(False)
+        secondary["mr_jacobian_fp32"] = {
+            "elements_per_s": E * args.steps / (ms32 / 1e3),
+            "achieved_TFLOPs": MR_JACOBIAN_FLOP_PER_ELEMENT * E * args.steps / (j32 / 1e3) / 1e12,
+            "peak_TFLOPs": FP32_MFMA_PEAK_TFLOPS,
+        }
+        secondary["laplace"] = [laplace_line(cm, n, s) for n, s in ((8, 50), (64, 20), (128, 10))]
+
+    line = {
+        "metric": "elements/s assembled; time-to-1e-10-residual per load step, plain vs learned",
+        "value": value,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered unit-cube mesh, deformed state)",
+        "config": {"workload": f"Mooney-Rivlin {args.n}^3 triquadratic/trilinear, 3x3x3 Gauss, residual + FP64 "
+                               "analytic Jacobian assembly into CSR", "elements": E, "dofs": eq.n_rows,
+                   "nnz": eq.nnz, "c10": c10, "c01": c01, "jitter": 0.1},
+        "roofline": roofline,
+        "cpu_baseline": None,
+        "secondary": secondary,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(mesh, dofs, c10, c01, args.cpu_seconds)
+        line["cpu_baseline"]["host_cpus"] = os.cpu_count()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -8,7 +8,8 @@ from ._lib import (CMISSError, OC_FP32, OC_FP64, OC_BUFFER_COL_IDX, OC_BUFFER_DO
                    OC_BUFFER_MATRIX_VALUES, OC_BUFFER_MATRIX_VALUES_SP, OC_BUFFER_RESIDUAL, OC_BUFFER_RHS,
                    OC_BUFFER_ROW_PTR, LIB_PATH)
 from .equations_set import (EquationsSet, LaplaceEquationsSet, MooneyRivlinEquationsSet,  # noqa: F401
-                            event_elapsed, event_record, finalise, initialise, synchronise)
+                            event_elapsed, event_record, finalise, initialise, kernel_timing,
+                            kernel_timing_enable, synchronise)
 from .mesh import RegularMesh  # noqa: F401
 
 __all__ = ["CMISSError", "EquationsSet", "LaplaceEquationsSet", "MooneyRivlinEquationsSet", "RegularMesh",

@@ -79,6 +79,8 @@ SIGNATURES = {
     "oc_event_record": [_c_int],
     "oc_event_elapsed": [_c_int, _c_int, ctypes.POINTER(_c_float)],
     "oc_synchronise": [],
+    "oc_kernel_timing_enable": [_c_int],
+    "oc_kernel_timing_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
 }
 
 _LIB = None

@@ -29,6 +29,72 @@ int set_error(int code, const char *fmt, ...)
 
 Runtime &runtime() { return g_runtime; }
 
+// ---- kernel timing -------------------------------------------------------------
+namespace {
+struct PendingTiming {
+    std::string name;
+    hipEvent_t start, stop;
+};
+struct TimingState {
+    bool enabled = false;
+    std::vector<PendingTiming> pending;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<std::string, std::pair<double, long long>>> totals;
+} g_timing;
+
+hipEvent_t timing_event()
+{
+    if (!g_timing.pool.empty()) {
+        hipEvent_t e = g_timing.pool.back();
+        g_timing.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int timing_resolve()
+{
+    if (g_timing.pending.empty()) return OC_NO_ERROR;
+    OC_HIP_CHECK(hipEventSynchronize(g_timing.pending.back().stop));
+    for (auto &p : g_timing.pending) {
+        float ms = 0.0f;
+        OC_HIP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
+        bool found = false;
+        for (auto &t : g_timing.totals)
+            if (t.first == p.name) {
+                t.second.first += ms;
+                t.second.second += 1;
+                found = true;
+                break;
+            }
+        if (!found) g_timing.totals.push_back({p.name, {static_cast<double>(ms), 1}});
+        g_timing.pool.push_back(p.start);
+        g_timing.pool.push_back(p.stop);
+    }
+    g_timing.pending.clear();
+    return OC_NO_ERROR;
+}
+}  // namespace
+
+KernelTimer::KernelTimer(const char *name) : name_(name)
+{
+    if (!g_timing.enabled) return;
+    start_ = timing_event();
+    if (start_) (void)hipEventRecord(start_, g_runtime.stream);
+}
+
+KernelTimer::~KernelTimer()
+{
+    if (!start_) return;
+    hipEvent_t stop = timing_event();
+    if (!stop) return;
+    (void)hipEventRecord(stop, g_runtime.stream);
+    g_timing.pending.push_back({name_, start_, stop});
+    if (g_timing.pending.size() > 4096) (void)timing_resolve();
+}
+
 int require_runtime()
 {
     if (g_runtime.stream == nullptr)
@@ -396,6 +462,30 @@ int oc_synchronise(void)
 {
     OC_CHECK(require_runtime());
     OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+int oc_kernel_timing_enable(int enable)
+{
+    OC_CHECK(require_runtime());
+    OC_CHECK(timing_resolve());
+    g_timing.totals.clear();
+    g_timing.enabled = enable != 0;
+    return OC_NO_ERROR;
+}
+
+int oc_kernel_timing_get(const char *kernel_name, double *total_ms, long long *launches)
+{
+    OC_CHECK(require_runtime());
+    if (!kernel_name || !total_ms || !launches) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    OC_CHECK(timing_resolve());
+    *total_ms = 0.0;
+    *launches = 0;
+    for (auto &t : g_timing.totals)
+        if (t.first == kernel_name) {
+            *total_ms = t.second.first;
+            *launches = t.second.second;
+        }
     return OC_NO_ERROR;
 }
 

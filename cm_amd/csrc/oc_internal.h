@@ -36,6 +36,20 @@ struct Runtime {
 Runtime &runtime();
 int require_runtime();
 
+// Brackets one kernel launch with events on the library stream when kernel
+// timing is enabled (oc_kernel_timing_enable); no-op otherwise.
+class KernelTimer {
+public:
+    explicit KernelTimer(const char *name);
+    ~KernelTimer();
+    KernelTimer(const KernelTimer &) = delete;
+    KernelTimer &operator=(const KernelTimer &) = delete;
+
+private:
+    const char *name_;
+    hipEvent_t start_ = nullptr;
+};
+
 // Device buffer owned by the library.
 template <typename T>
 struct DevBuf {
@@ -95,7 +109,21 @@ struct EquationsSet {
     DevBuf<double> values;
     DevBuf<float> values_sp;
     DevBuf<double> dofs, residual, rhs;
+
+    // Mooney-Rivlin: per-Gauss-point records (field-major, see
+    // oc_elasticity.hip) and the CSR scatter map: for element e, element row
+    // i and element column j, slot = row_ptr[dof_i] + slot_offsets[(e*89+i)*89+j].
+    DevBuf<double> gp_records;
+    DevBuf<uint16_t> slot_offsets;
+    // Laplace: 8-bit offsets, slot = row_ptr[node_i] + slot_offsets8[(e*8+i)*8+j].
+    DevBuf<uint8_t> slot_offsets8;
 };
+
+// ---- shared device resources (oc_assembly.hip) --------------------------------
+struct BasisTables;
+int ensure_tables(const BasisTables **out);
+template <typename T>
+int fill(T *p, int64_t n, T v);
 
 // ---- host-side setup helpers (oc_setup.cpp) ---------------------------------
 int build_pattern(int ncomp, int n_elements, const int32_t *nn, const int32_t *nen, const int32_t *const *conn,
@@ -105,6 +133,7 @@ int colour_elements(int n_elements, int ncomp, const int32_t *nn, const int32_t 
 
 // ---- kernel launchers --------------------------------------------------------
 int launch_laplace_assemble(EquationsSet &s);
+int mr_prepare(EquationsSet &s);  // builds the scatter map once
 int launch_mr_residual(EquationsSet &s);
 int launch_mr_jacobian(EquationsSet &s, int precision);
 

@@ -46,6 +46,18 @@ def event_elapsed(start, end):
     return float(ms.value)
 
 
+def kernel_timing_enable(enable=True):
+    """Bracket every library kernel launch with events (clears totals)."""
+    call("oc_kernel_timing_enable", 1 if enable else 0)
+
+
+def kernel_timing(name):
+    """(total milliseconds, launches) of a library kernel since the last enable."""
+    ms, n = ctypes.c_double(0.0), ctypes.c_longlong(0)
+    call("oc_kernel_timing_get", name.encode(), ctypes.byref(ms), ctypes.byref(n))
+    return float(ms.value), int(n.value)
+
+
 class EquationsSet:
     """Common handle management for the two equations-set classes."""
 

@@ -144,6 +144,17 @@ int oc_event_record(int slot);
 int oc_event_elapsed(int start, int end, float *milliseconds);
 int oc_synchronise(void);
 
+/* ---- per-kernel timing (TAU phases analogue, equations_set_routines.f90:
+ * 1332-1336,1366,1414) ------------------------------------------------------ */
+/* When enabled, every kernel launch is bracketed by events on the library
+ * stream and its duration is accumulated under its kernel name
+ * ("laplace_assemble", "mr_residual", "mr_jacobian_fp64", "mr_jacobian_fp32",
+ * "fill", ...).  Enabling or resetting clears the accumulators. */
+int oc_kernel_timing_enable(int enable);
+/* Synchronises, then returns the total milliseconds and launch count of
+ * `kernel_name` since the last enable/reset. */
+int oc_kernel_timing_get(const char *kernel_name, double *total_ms, long long *launches);
+
 #ifdef __cplusplus
 }
 #endif
