@@ -126,11 +126,26 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--profile", choices=["jacobian", "copy"], default=None,
+                    help="run only one kernel sequence for rocprofv3 counter collection: the FP64 Jacobian "
+                         "assembly (--steps times), or a device copy of 512 MiB (the FETCH_SIZE/WRITE_SIZE "
+                         "calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
               "path is a design (element partitions with halo exchange) and is not run here.", file=sys.stderr)
         sys.exit(2)
+
+    if args.profile == "copy":
+        import torch
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": "copy", "bytes_read_per_launch": a.numel() * 8,
+                          "bytes_written_per_launch": b.numel() * 8, "launches": args.steps}))
+        return
 
     import cm_amd as cm
     cm.initialise(0)
@@ -144,10 +159,17 @@ def main():
         eq.residual_evaluate()
         eq.jacobian_evaluate(cm.OC_FP64)
 
+    if args.profile == "jacobian":
+        for _ in range(args.steps):
+            eq.jacobian_evaluate(cm.OC_FP64)
+        cm.synchronise()
+        print(json.dumps({"profile": "jacobian", "steps": args.steps, "elements": mesh.n_elements}))
+        return
+
     ms, wall = time_steps(step, args.steps, args.warmup, cm)
-    jac_ms, jac_launches = cm.kernel_timing("mr_jacobian_fp64")
-    res_ms, _ = cm.kernel_timing("mr_residual")
-    fill_ms, _ = cm.kernel_timing("fill")
+    kernels = {name: cm.kernel_timing(name) for name in
+               ("mr_gauss_jacobian", "mr_jacobian_fp64", "mr_residual", "dof_gather", "fill")}
+    jac_ms, jac_launches = kernels["mr_jacobian_fp64"]
     cm.kernel_timing_enable(False)
     E = mesh.n_elements
     value = E * args.steps / (ms / 1e3)
@@ -168,8 +190,8 @@ def main():
     }
 
     secondary = {
-        "kernel_ms_per_step": {"mr_jacobian_fp64": jac_ms / args.steps, "mr_residual": res_ms / args.steps,
-                               "fill": fill_ms / args.steps},
+        "kernel_ms_per_step": {name: v[0] / args.steps for name, v in kernels.items()},
+        "kernel_launches_per_step": {name: v[1] / args.steps for name, v in kernels.items()},
         "wall_ms_per_step": wall * 1e3 / args.steps,
     }
     if not args.no_secondary:

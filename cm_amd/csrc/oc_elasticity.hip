@@ -2,17 +2,16 @@
 // residual and analytic Jacobian (finite_elasticity_routines.f90:842-1175,
 // 1260-1640) scattered deterministically into the CSR equations matrix.
 //
-// Three stages:
-//  1. mr_gauss_kernel: one thread per (element, Gauss point) over all
-//     elements evaluates the FP64 kinematics and constitutive law and writes a
-//     Gauss-point record (field-major, see GpField);
-//  2. mr_jacobian_element_kernel: one workgroup per element of a colour
-//     contracts the records into the 89x89 element Jacobian and scatters it
-//     through the precomputed slot-offset map;
-//  3. mr_residual_element_kernel: one wavefront per element of a colour sums
-//     the residual and scatters it into the residual vector.
-// Elements of one colour share no DOF, so each launch writes every entry at
-// most once and the colour order fixes the summation order.
+// Jacobian: mr_gauss_kernel (one thread per element Gauss point, all
+// elements) evaluates the FP64 kinematics and tangent into Gauss-point
+// records (field-major, see GpField); mr_jacobian_element_kernel contracts
+// the records of each element of one colour into the 89x89 element Jacobian
+// and scatters it through the precomputed slot-offset map.  Elements of one
+// colour share no DOF, so each launch writes every CSR entry at most once and
+// the colour order fixes the summation order.
+// Residual: mr_residual_element_kernel evaluates element residuals of all
+// elements into a buffer and dof_gather_kernel sums them per DOF in
+// ascending element order.
 #include <algorithm>
 #include <cstdint>
 
@@ -27,21 +26,15 @@ enum GpField : int {
     GP_DXIDZ = 0,   // 9: DXI_DX of the deformed metrics, column-major
     GP_JGW = 9,     // J_z * w_g
     GP_C = 10,      // 36: spatial elasticity tensor (Voigt, column-major)
-    GP_SIGJ = 46,   // 6: Cauchy stress from the elasticity routine (Jacobian)
-    GP_SIGR = 52,   // 6: Cauchy stress from the stress routine (residual)
-    GP_RP = 58,     // J_X * w_g * (Jznu - 1)
-    GP_NFIELDS = 59
+    GP_SIGJ = 46,   // 6: Cauchy stress from the elasticity routine
+    GP_NFIELDS = 52
 };
-
-enum GaussMode : int { GAUSS_RESIDUAL = 1, GAUSS_JACOBIAN = 2, GAUSS_BOTH = 3 };
 
 constexpr int kGaussElemsPerBlock = 9;  // 9 elements x 27 Gauss points = 243 threads
 
-// Stage 1.  Restates, per Gauss point, the interpolation/metrics/F/J block of
-// the residual (:1491-1532) and Jacobian (:954-989) routines, the stress
-// routine FINITE_ELASTICITY_GAUSS_STRESS_TENSOR (:4342-4410) and the tangent
-// routine FINITE_ELASTICITY_GAUSS_ELASTICITY_TENSOR (:609-740).
-template <int kMode>
+// Restates, per Gauss point, the interpolation/metrics/F/J block of the
+// Jacobian routine (:954-989) and FINITE_ELASTICITY_GAUSS_ELASTICITY_TENSOR
+// (:609-740).
 __global__ void __launch_bounds__(256)
 mr_gauss_kernel(int n_elements, const int *__restrict__ conn_q, const int *__restrict__ conn_l,
                 const double *__restrict__ coords, int nq, const double *__restrict__ dofs, double c10, double c01,
@@ -81,21 +74,12 @@ mr_gauss_kernel(int n_elements, const int *__restrict__ conn_q, const int *__res
 #pragma unroll
     for (int i = 0; i < 9; ++i) rec[(GP_DXIDZ + i) * n_gp + q] = k.dxidz[i];
     rec[GP_JGW * n_gp + q] = k.jz * T->w27[g];
-    if (kMode & GAUSS_JACOBIAN) {
-        double c[36], sigma[6];
-        mr_elasticity_tensor(k.F, k.Jznu, P, c10, c01, c, sigma);
+    double c[36], sigma[6];
+    mr_elasticity_tensor(k.F, k.Jznu, P, c10, c01, c, sigma);
 #pragma unroll
-        for (int i = 0; i < 36; ++i) rec[(GP_C + i) * n_gp + q] = c[i];
+    for (int i = 0; i < 36; ++i) rec[(GP_C + i) * n_gp + q] = c[i];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) rec[(GP_SIGJ + i) * n_gp + q] = sigma[i];
-    }
-    if (kMode & GAUSS_RESIDUAL) {
-        double sigma[6];
-        mr_cauchy_stress(k.F, k.Jznu, P, c10, c01, sigma);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) rec[(GP_SIGR + i) * n_gp + q] = sigma[i];
-        rec[GP_RP * n_gp + q] = k.jx * T->w27[g] * (k.Jznu - 1.0);
-    }
+    for (int i = 0; i < 6; ++i) rec[(GP_SIGJ + i) * n_gp + q] = sigma[i];
 }
 
 // Element DOFs (component -> node order of the reference element matrix,
@@ -151,98 +135,133 @@ int mr_prepare(EquationsSet &s)
     OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
     if (h_missing) return set_error(OC_ERR_PATTERN, "%d element entries missing from the CSR structure", h_missing);
     OC_CHECK(s.gp_records.alloc(static_cast<size_t>(s.n_elements) * 27 * GP_NFIELDS));
+    OC_CHECK(s.relem.alloc(static_cast<size_t>(s.n_elements) * 89));
+    // DOF -> element entries, ascending element order
+    std::vector<int32_t> hconn(static_cast<size_t>(s.n_elements) * 27), hconn_l(static_cast<size_t>(s.n_elements) * 8);
+    OC_HIP_CHECK(hipMemcpy(hconn.data(), s.conn.ptr, hconn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    OC_HIP_CHECK(hipMemcpy(hconn_l.data(), s.conn_l.ptr, hconn_l.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int32_t> ptr(static_cast<size_t>(s.n_rows) + 1, 0), idx(static_cast<size_t>(s.n_elements) * 89);
+    auto dof_of = [&](int64_t e, int i) {
+        return i < 81 ? (i / 27) * s.n_nodes + hconn[e * 27 + i % 27] : 3 * s.n_nodes + hconn_l[e * 8 + i - 81];
+    };
+    for (int64_t e = 0; e < s.n_elements; ++e)
+        for (int i = 0; i < 89; ++i) ptr[dof_of(e, i) + 1]++;
+    for (int d = 0; d < s.n_rows; ++d) ptr[d + 1] += ptr[d];
+    std::vector<int32_t> fillp(ptr.begin(), ptr.end() - 1);
+    for (int64_t e = 0; e < s.n_elements; ++e)
+        for (int i = 0; i < 89; ++i) idx[fillp[dof_of(e, i)]++] = static_cast<int32_t>(e * 89 + i);
+    OC_CHECK(s.gather_ptr.upload(ptr.data(), ptr.size()));
+    OC_CHECK(s.gather_idx.upload(idx.data(), idx.size()));
     return OC_NO_ERROR;
 }
 
-static int launch_gauss(EquationsSet &s, int mode)
+static int launch_gauss(EquationsSet &s)
 {
     const BasisTables *T = nullptr;
     OC_CHECK(ensure_tables(&T));
     Runtime &rt = runtime();
     const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
-    KernelTimer timer(mode == GAUSS_RESIDUAL ? "mr_gauss_residual" : "mr_gauss_jacobian");
-    if (mode == GAUSS_RESIDUAL)
-        mr_gauss_kernel<GAUSS_RESIDUAL><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
-                                                                       s.coords.ptr, s.n_nodes, s.dofs.ptr, s.c10,
-                                                                       s.c01, T, s.gp_records.ptr);
-    else if (mode == GAUSS_JACOBIAN)
-        mr_gauss_kernel<GAUSS_JACOBIAN><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
-                                                                       s.coords.ptr, s.n_nodes, s.dofs.ptr, s.c10,
-                                                                       s.c01, T, s.gp_records.ptr);
-    else
-        mr_gauss_kernel<GAUSS_BOTH><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
-                                                                   s.coords.ptr, s.n_nodes, s.dofs.ptr, s.c10, s.c01,
-                                                                   T, s.gp_records.ptr);
+    KernelTimer timer("mr_gauss_jacobian");
+    mr_gauss_kernel<<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr, s.coords.ptr, s.n_nodes,
+                                                   s.dofs.ptr, s.c10, s.c01, T, s.gp_records.ptr);
     OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
 }
 
-
 // ---------------------------------------------------------------------------
-// Stage 3: residual (:1537-1574).  R_(a,m) += sum_i DPHIDZ(i,m) JGW sigma(i,a)
-// per Gauss point in order; R_(p,m) += J_X w (J-1) psi_m.  Persistent
-// 256-thread blocks hold the basis derivative table in LDS and process four
-// elements of the colour per iteration.
-constexpr int kResElems = 4;
-
+// Residual (:1491-1574), fused with the Gauss-point evaluation: one block of
+// 9 elements x 27 Gauss points evaluates kinematics and the stress routine,
+// keeps JGW*sigma and DXI_DX in LDS, then sums each element residual entry
+//   R_(a,m) += sum_i DPHIDZ(i,m) JGW sigma(i,a),  R_(p,m) += J_X w (J-1) psi_m
+// over the Gauss points in order into a per-element buffer.  A gather kernel
+// then adds the element contributions of every DOF in ascending element order
+// (the reference's element-loop order, equations_set_routines.f90:5535-5568).
 __global__ void __launch_bounds__(256)
-mr_residual_element_kernel(const int *__restrict__ colour_elems, int n_in_colour, int n_elements,
-                           const double *__restrict__ rec, const int *__restrict__ conn_q,
-                           const int *__restrict__ conn_l, int nq, const BasisTables *__restrict__ T,
-                           double *__restrict__ residual)
+mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const int *__restrict__ conn_l,
+                           const double *__restrict__ coords, int nq, const double *__restrict__ dofs, double c10,
+                           double c01, const BasisTables *__restrict__ T, double *__restrict__ relem)
 {
-    __shared__ double s_df[27 * 27 * 3];  // dphi_n/dxi_mi at g: [(g*27+n)*3 + mi]
-    __shared__ double s_psi[27 * 8];
-    __shared__ double s_dx[kResElems][9][27];
-    __shared__ double s_S[kResElems][9][27];  // JGW * sigma(i, a): [i + 3a][g]
-    __shared__ double s_rp[kResElems][27];
+    __shared__ double s_fq[27 * 27 * 4];
+    __shared__ double s_X[kGaussElemsPerBlock][81];
+    __shared__ double s_z[kGaussElemsPerBlock][81];
+    __shared__ double s_p[kGaussElemsPerBlock][8];
+    __shared__ double s_dx[kGaussElemsPerBlock][9][27];
+    __shared__ double s_S[kGaussElemsPerBlock][9][27];  // JGW * sigma(i, a): [i + 3a][g]
+    __shared__ double s_rp[kGaussElemsPerBlock][27];
     const int t = threadIdx.x;
-    for (int k = t; k < 27 * 27 * 3; k += 256) s_df[k] = T->quad27[(k / 3) * 4 + 1 + k % 3];
-    for (int k = t; k < 27 * 8; k += 256) s_psi[k] = T->lin27[k * 4];
-    const int64_t n_gp = static_cast<int64_t>(n_elements) * 27;
-    for (int base = blockIdx.x * kResElems; base < n_in_colour; base += gridDim.x * kResElems) {
-        __syncthreads();
-        for (int k = t; k < kResElems * 9 * 27; k += 256) {
-            const int le = k / 243, rem = k % 243, f = rem / 27, g = rem % 27;
-            if (base + le >= n_in_colour) continue;
-            const int64_t q = static_cast<int64_t>(colour_elems[base + le]) * 27 + g;
-            s_dx[le][f][g] = rec[(GP_DXIDZ + f) * n_gp + q];
-            const int i = f % 3, a = f / 3;
-            s_S[le][f][g] = rec[GP_JGW * n_gp + q] * rec[(GP_SIGR + t2v(i, a)) * n_gp + q];
-        }
-        for (int k = t; k < kResElems * 27; k += 256) {
-            const int le = k / 27, g = k % 27;
-            if (base + le >= n_in_colour) continue;
-            s_rp[le][g] = rec[GP_RP * n_gp + static_cast<int64_t>(colour_elems[base + le]) * 27 + g];
-        }
-        __syncthreads();
-        for (int w = t; w < kResElems * 89; w += 256) {
-            const int le = w / 89, r = w % 89;
-            if (base + le >= n_in_colour) continue;
-            const int e = colour_elems[base + le];
-            double acc = 0.0;
-            if (r < 81) {
-                const int a = r / 27, m = r % 27;
-#pragma unroll 1
-                for (int g = 0; g < 27; ++g) {
-                    const double *f = s_df + (g * 27 + m) * 3;
-                    double d[3];
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-                        d[i] = s_dx[le][0 + 3 * i][g] * f[0] + s_dx[le][1 + 3 * i][g] * f[1] +
-                               s_dx[le][2 + 3 * i][g] * f[2];
-                    const double dot = d[0] * s_S[le][0 + 3 * a][g] + d[1] * s_S[le][1 + 3 * a][g] +
-                                       d[2] * s_S[le][2 + 3 * a][g];
-                    acc = acc + dot;
-                }
-            } else {
-                const int m = r - 81;
-#pragma unroll 1
-                for (int g = 0; g < 27; ++g) acc = acc + s_rp[le][g] * s_psi[g * 8 + m];
-            }
-            residual[element_dof(conn_q, conn_l, nq, e, r)] += acc;
+    const int e0 = blockIdx.x * kGaussElemsPerBlock;
+    for (int k = t; k < 27 * 27 * 4; k += 256) s_fq[k] = T->quad27[k];
+    for (int k = t; k < kGaussElemsPerBlock * 81; k += 256) {
+        const int le = k / 81, r = k % 81, c = r / 27, n = r % 27;
+        const int e = e0 + le;
+        if (e < n_elements) {
+            const int node = conn_q[e * 27 + n];
+            s_X[le][r] = coords[static_cast<int64_t>(c) * nq + node];
+            s_z[le][r] = dofs[static_cast<int64_t>(c) * nq + node];
         }
     }
+    for (int k = t; k < kGaussElemsPerBlock * 8; k += 256) {
+        const int le = k / 8, n = k % 8;
+        const int e = e0 + le;
+        if (e < n_elements) s_p[le][n] = dofs[static_cast<int64_t>(3) * nq + conn_l[e * 8 + n]];
+    }
+    __syncthreads();
+    {
+        const int le = t / 27, g = t % 27;
+        const int e = e0 + le;
+        if (t < kGaussElemsPerBlock * 27 && e < n_elements) {
+            GaussKinematics k;
+            gauss_kinematics(s_fq + g * 27 * 4, s_z[le], s_X[le], k);
+            double P = 0.0;
+            for (int n = 0; n < 8; ++n) P += T->lin27[(g * 8 + n) * 4] * s_p[le][n];
+            double sigma[6];
+            mr_cauchy_stress(k.F, k.Jznu, P, c10, c01, sigma);
+            const double jgw = k.jz * T->w27[g];
+            for (int i = 0; i < 9; ++i) s_dx[le][i][g] = k.dxidz[i];
+            for (int a = 0; a < 3; ++a)
+                for (int i = 0; i < 3; ++i) s_S[le][i + 3 * a][g] = jgw * sigma[t2v(i, a)];
+            s_rp[le][g] = k.jx * T->w27[g] * (k.Jznu - 1.0);
+        }
+    }
+    __syncthreads();
+    for (int w = t; w < kGaussElemsPerBlock * 89; w += 256) {
+        const int le = w / 89, r = w % 89;
+        const int e = e0 + le;
+        if (e >= n_elements) continue;
+        double acc = 0.0;
+        if (r < 81) {
+            const int a = r / 27, m = r % 27;
+#pragma unroll 1
+            for (int g = 0; g < 27; ++g) {
+                const double *f = s_fq + (g * 27 + m) * 4;
+                double d[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    d[i] = s_dx[le][0 + 3 * i][g] * f[1] + s_dx[le][1 + 3 * i][g] * f[2] +
+                           s_dx[le][2 + 3 * i][g] * f[3];
+                const double dot = d[0] * s_S[le][0 + 3 * a][g] + d[1] * s_S[le][1 + 3 * a][g] +
+                                   d[2] * s_S[le][2 + 3 * a][g];
+                acc = acc + dot;
+            }
+        } else {
+            const int m = r - 81;
+#pragma unroll 1
+            for (int g = 0; g < 27; ++g) acc = acc + s_rp[le][g] * T->lin27[(g * 8 + m) * 4];
+        }
+        relem[static_cast<int64_t>(e) * 89 + r] = acc;
+    }
+}
+
+// residual[d] = sum over the element entries of DOF d, ascending elements.
+__global__ void __launch_bounds__(256)
+dof_gather_kernel(int n_rows, const int *__restrict__ ptr, const int *__restrict__ idx,
+                  const double *__restrict__ relem, double *__restrict__ out)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n_rows) return;
+    double acc = 0.0;
+    for (int k = ptr[d]; k < ptr[d + 1]; ++k) acc = acc + relem[idx[k]];
+    out[d] = acc;
 }
 
 int launch_mr_residual(EquationsSet &s)
@@ -251,17 +270,18 @@ int launch_mr_residual(EquationsSet &s)
     const BasisTables *T = nullptr;
     OC_CHECK(ensure_tables(&T));
     Runtime &rt = runtime();
-    OC_CHECK(launch_gauss(s, GAUSS_RESIDUAL));
-    OC_CHECK(fill<double>(s.residual.ptr, s.n_rows, 0.0));
-    const int ncol = static_cast<int>(s.colour_ptr.size()) - 1;
-    for (int c = 0; c < ncol; ++c) {
-        const int n = s.colour_ptr[c + 1] - s.colour_ptr[c];
-        if (n == 0) continue;
-        const int blocks = std::min((n + kResElems - 1) / kResElems, rt.num_cus * 4);
+    const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
+    {
         KernelTimer timer("mr_residual");
-        mr_residual_element_kernel<<<static_cast<unsigned>(blocks), 256, 0, rt.stream>>>(
-            s.colour_elems.ptr + s.colour_ptr[c], n, s.n_elements, s.gp_records.ptr, s.conn.ptr, s.conn_l.ptr,
-            s.n_nodes, T, s.residual.ptr);
+        mr_residual_element_kernel<<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
+                                                                  s.coords.ptr, s.n_nodes, s.dofs.ptr, s.c10, s.c01,
+                                                                  T, s.relem.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+    }
+    {
+        KernelTimer timer("dof_gather");
+        dof_gather_kernel<<<static_cast<unsigned>((s.n_rows + 255) / 256), 256, 0, rt.stream>>>(
+            s.n_rows, s.gather_ptr.ptr, s.gather_idx.ptr, s.relem.ptr, s.residual.ptr);
         OC_HIP_CHECK(hipGetLastError());
     }
     return OC_NO_ERROR;
@@ -431,7 +451,7 @@ int launch_mr_jacobian(EquationsSet &s, int precision)
     const BasisTables *T = nullptr;
     OC_CHECK(ensure_tables(&T));
     Runtime &rt = runtime();
-    OC_CHECK(launch_gauss(s, GAUSS_JACOBIAN));
+    OC_CHECK(launch_gauss(s));
     if (precision == OC_FP64) OC_CHECK(fill<double>(s.values.ptr, s.nnz, 0.0));
     else OC_CHECK(fill<float>(s.values_sp.ptr, s.nnz, 0.0f));
     const int ncol = static_cast<int>(s.colour_ptr.size()) - 1;

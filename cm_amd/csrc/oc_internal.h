@@ -115,6 +115,9 @@ struct EquationsSet {
     // i and element column j, slot = row_ptr[dof_i] + slot_offsets[(e*89+i)*89+j].
     DevBuf<double> gp_records;
     DevBuf<uint16_t> slot_offsets;
+    // element residuals (E*89) and the DOF -> element-entry gather lists
+    DevBuf<double> relem;
+    DevBuf<int32_t> gather_ptr, gather_idx;
     // Laplace: 8-bit offsets, slot = row_ptr[node_i] + slot_offsets8[(e*8+i)*8+j].
     DevBuf<uint8_t> slot_offsets8;
 };

@@ -1,0 +1,357 @@
+"""Pins the C oracle (FP64 restatement of the reference assembly path) to
+what the mathematics fixes.  The reference holds no golden vectors for this
+path (its regression examples are not vendored), so every check here is a
+closed form, an invariant, or brute force on tiny inputs, chosen so that a
+plausible mistake (dropped term, wrong sign or index, transposed operand)
+fails one of them.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from cm_amd.mesh import RegularMesh, regular_coordinates, regular_elements, uniform
+
+
+# --- Gauss-Legendre (basis_routines.f90:5826-5875) ---------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 4])
+def test_gauss_legendre_exactness(n):
+    x, w = orc.gauss_legendre(n)
+    assert abs(w.sum() - 1.0) < 1e-15
+    for p in range(2 * n):  # exact for degree <= 2n-1 on [0,1]
+        assert abs(np.dot(w, x ** p) - 1.0 / (p + 1)) < 1e-14
+    # not exact at degree 2n
+    assert abs(np.dot(w, x ** (2 * n)) - 1.0 / (2 * n + 1)) > 1e-6
+
+
+def test_gauss_legendre_closed_form():
+    x, w = orc.gauss_legendre(2)
+    assert np.allclose(x, [(1 - 1 / np.sqrt(3)) / 2, (1 + 1 / np.sqrt(3)) / 2], atol=0, rtol=1e-15)
+    x, w = orc.gauss_legendre(3)
+    assert np.allclose(x, [(1 - np.sqrt(0.6)) / 2, 0.5, (1 + np.sqrt(0.6)) / 2], atol=0, rtol=1e-15)
+    assert np.allclose(w, [5 / 18, 4 / 9, 5 / 18], atol=0, rtol=1e-15)
+
+
+# --- Lagrange tensor-product tables (basis_routines.f90:2770-2890) -----------
+@pytest.mark.parametrize("order,ng", [(1, 2), (1, 3), (2, 3)])
+def test_basis_partition_of_unity(order, ng):
+    w, pos, fns = orc.basis_tables(order, ng)
+    assert np.allclose(fns[:, :, 0].sum(axis=1), 1.0, atol=1e-14)
+    assert np.allclose(fns[:, :, 1:].sum(axis=1), 0.0, atol=1e-13)
+    # Gauss ordering xi1 fastest: ng = i + (j + k*n)*n
+    x1, _ = orc.gauss_legendre(ng)
+    for g, (k, j, i) in enumerate(itertools.product(range(ng), range(ng), range(ng))):
+        assert np.allclose(pos[g], [x1[i], x1[j], x1[k]], atol=0)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_basis_interpolates_polynomials(order):
+    """Nodal interpolation of a polynomial of the basis degree is exact, so the
+    tabulated values and derivatives must reproduce it at every Gauss point."""
+    w, pos, fns = orc.basis_tables(order, 3)
+    nx = order + 1
+    nodes = np.array([(a / order, b / order, c / order) for c in range(nx) for b in range(nx) for a in range(nx)])
+    f = lambda p: 1 + 2 * p[..., 0] ** order - 3 * p[..., 1] * p[..., 2] ** (order - 1) + p[..., 0] * p[..., 1]
+    df = lambda p: np.stack([2 * order * p[..., 0] ** (order - 1) + p[..., 1],
+                             -3 * p[..., 2] ** (order - 1) + p[..., 0],
+                             -3 * (order - 1) * p[..., 1] * p[..., 2] ** max(order - 2, 0)], axis=-1)
+    vals = f(nodes)
+    assert np.allclose(fns[:, :, 0] @ vals, f(pos), atol=1e-13)
+    for d in range(3):
+        assert np.allclose(fns[:, :, 1 + d] @ vals, df(pos)[:, d], atol=1e-12)
+
+
+def test_invert3_cofactors():
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((3, 3)) + 3 * np.eye(3)
+    B, det = np.zeros(9), np.zeros(1)
+    orc.lib().oco_invert3(np.ascontiguousarray(A.T).ravel(), B, det)
+    assert np.allclose(B.reshape(3, 3).T, np.linalg.inv(A), rtol=1e-13)
+    assert abs(det[0] - np.linalg.det(A)) < 1e-12
+
+
+# --- Laplace element (Laplace_equations_routines.f90:474-571) ---------------
+def test_laplace_element_closed_form_cube():
+    """Axis-aligned cube of side h, exact 2x2x2 Gauss: K_ii = h/3, edge
+    neighbour 0, face and body diagonals -h/12."""
+    for h in (1.0, 0.125):
+        conn, xyz = orc.regular_mesh(1, 1, 1, 1)
+        K = orc.laplace_element(h * xyz[:, conn[0]])
+        for a, b in itertools.product(range(8), range(8)):
+            d = bin(a ^ b).count("1")  # local nodes differ in d coordinates
+            expected = {0: h / 3, 1: 0.0, 2: -h / 12, 3: -h / 12}[d]
+            assert abs(K[a, b] - expected) < 1e-15
+
+
+def test_laplace_element_invariants_distorted():
+    rng = np.random.default_rng(3)
+    conn, xyz = orc.regular_mesh(1, 1, 1, 1)
+    xe = xyz[:, conn[0]] + 0.15 * rng.uniform(-1, 1, (3, 8))
+    K = orc.laplace_element(xe)
+    assert np.abs(K - K.T).max() < 1e-15
+    assert np.abs(K.sum(axis=1)).max() < 1e-14          # constants in the kernel
+    ev = np.linalg.eigvalsh(K)
+    assert ev[0] > -1e-14 and (ev[1:] > 1e-6).all()      # PSD, one zero mode
+    # linear fields: K u = 0 is false, but u^T K u = int |grad u|^2 dV exactly
+    # for u = x on an affine element; rotate/scale to get an affine element
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    xa = Q @ (xyz[:, conn[0]] * np.array([[1.0], [2.0], [0.5]]))
+    Ka = orc.laplace_element(xa)
+    u = xa[0]
+    vol = abs(np.linalg.det(Q)) * 1.0 * 2.0 * 0.5
+    assert abs(u @ Ka @ u - vol) < 1e-13
+
+
+def test_laplace_patch_test_u_equals_x():
+    """Config-1 BVP (u=0 on x=0, u=1 on x=1, natural BCs elsewhere): u = x
+    solves the assembled discrete system exactly."""
+    mesh = RegularMesh.generate(1, 4)
+    rp, ci = orc.pattern([(mesh.n_nodes, mesh.elem_nodes)])
+    vals, rhs = orc.laplace_assemble(mesh.elem_nodes, mesh.coords, rp, ci)
+    assert np.all(rhs == 0)
+    import scipy.sparse as sp
+    K = sp.csr_matrix((vals, ci, rp), shape=(mesh.n_nodes,) * 2)
+    u = mesh.coords[0]
+    r = K @ u
+    fixed = np.union1d(mesh.face_nodes(0, 0), mesh.face_nodes(0, 1))
+    free = np.setdiff1d(np.arange(mesh.n_nodes), fixed)
+    assert np.abs(r[free]).max() < 1e-13
+
+
+# --- CSR structure (equations_matrices_routines.f90:4572-5230) ---------------
+def _brute_pattern(components):
+    off = np.cumsum([0] + [c[0] for c in components])
+    rows = []
+    E = components[0][1].shape[0]
+    for c, (nn, conn) in enumerate(components):
+        for node in range(nn):
+            cols = set()
+            for e in range(E):
+                if node in conn[e]:
+                    for c2, (_, conn2) in enumerate(components):
+                        cols.update(int(off[c2] + x) for x in conn2[e])
+            rows.append(sorted(cols))
+    return rows
+
+
+@pytest.mark.parametrize("n", [(1, 1, 1), (2, 1, 1), (2, 2, 2), (3, 1, 2)])
+def test_pattern_brute_force(n):
+    m1 = RegularMesh.generate(1, n)
+    m2 = RegularMesh.generate(2, n)
+    for comps in ([(m1.n_nodes, m1.elem_nodes)],
+                  [(m2.n_nodes, m2.elem_nodes)] * 3 + [(m2.n_nodes_l, m2.elem_nodes_l)]):
+        rp, ci = orc.pattern(comps)
+        brute = _brute_pattern(comps)
+        assert rp[-1] == sum(len(r) for r in brute)
+        for r, cols in enumerate(brute):
+            assert list(ci[rp[r]:rp[r + 1]]) == cols
+
+
+@pytest.mark.parametrize("n", [1, 2, 5])
+def test_pattern_closed_form_nnz(n):
+    m1 = RegularMesh.generate(1, n)
+    rp, _ = orc.pattern([(m1.n_nodes, m1.elem_nodes)])
+    assert rp[-1] == (3 * n + 1) ** 3
+    m2 = RegularMesh.generate(2, n)
+    rp, _ = orc.pattern([(m2.n_nodes, m2.elem_nodes)] * 3 + [(m2.n_nodes_l, m2.elem_nodes_l)])
+    assert rp[-1] == 9 * (8 * n + 1) ** 3 + 6 * (5 * n + 1) ** 3 + (3 * n + 1) ** 3
+
+
+def test_csr_add_bisection_and_missing_entry():
+    rp = np.array([0, 30], np.int64)
+    ci = np.arange(0, 60, 2, dtype=np.int32)  # 30 sorted columns
+    vals = np.zeros(30)
+    cols = np.array([58, 0, 30, 2, 56], np.int32)
+    K = np.arange(1.0, 6.0)
+    assert orc.lib().oco_csr_add(rp, ci, vals, 1, np.array([0], np.int32), 5, cols, K) == 0
+    assert vals[29] == 1 and vals[0] == 2 and vals[15] == 3 and vals[1] == 4 and vals[28] == 5
+    assert orc.lib().oco_csr_add(rp, ci, vals, 1, np.array([0], np.int32), 1, np.array([3], np.int32),
+                                 np.ones(1)) == -2
+
+
+# --- mesh generation and numbering ------------------------------------------
+def test_mesh_generators_agree_with_oracle():
+    for order, n, jit in ((1, (3, 2, 4), 0.0), (2, (2, 3, 2), 0.1), (1, (5, 5, 5), 0.2)):
+        conn, xyz = orc.regular_mesh(order, *n, jitter=jit, seed=99)
+        m = RegularMesh.generate(order, n, jitter=jit, seed=99)
+        assert np.array_equal(conn, m.elem_nodes) and np.array_equal(xyz, m.coords)
+    assert np.array_equal(orc.pressure_elements(2, 3, 2), RegularMesh.generate(2, (2, 3, 2)).elem_nodes_l)
+
+
+def test_splitmix_stream_matches():
+    idx = np.array([0, 1, 2, 1000, 2 ** 40])
+    assert np.array_equal(uniform(12345, idx), [orc.lib().oco_uniform(12345, int(i)) for i in idx])
+
+
+def test_pressure_corner_numbering():
+    """Corner (i,j,k) of the quadratic grid is pressure node i + j(n1+1) + k(n1+1)(n2+1)
+    and sorting the corner global numbers reproduces that order."""
+    m = RegularMesh.generate(2, (2, 3, 2))
+    T1, T2 = 5, 7
+    corners = sorted((2 * i + 2 * j * T1 + 2 * k * T1 * T2, i + j * 3 + k * 12)
+                     for k in range(3) for j in range(4) for i in range(3))
+    assert [c[1] for c in corners] == list(range(len(corners)))
+    for e in range(m.n_elements):
+        q = m.elem_nodes[e][[0, 2, 6, 8, 18, 20, 24, 26]]
+        rank = {g: r for g, r in corners}
+        assert [rank[int(g)] for g in q] == list(m.elem_nodes_l[e])
+
+
+# --- Mooney-Rivlin constitutive law -------------------------------------------
+def test_mr_stress_uniaxial_closed_form():
+    """Incompressible uniaxial stretch F = diag(l, l^-1/2, l^-1/2):
+    Cauchy sigma_11 - sigma_22 = 2(c10 + c01/l)(l^2 - 1/l)."""
+    c10, c01 = 2.0, 6.0
+    for lam in (1.0, 1.1, 0.8, 1.5):
+        F = np.diag([lam, lam ** -0.5, lam ** -0.5])
+        s = orc.mr_stress(F, 0.0, c10, c01)
+        assert abs((s[0] - s[1]) - 2 * (c10 + c01 / lam) * (lam ** 2 - 1 / lam)) < 1e-12
+        assert abs(s[1] - s[2]) < 1e-14 and np.abs(s[3:]).max() < 1e-14
+        assert abs(s[:3].sum()) < 1e-12          # deviatoric at p = 0
+        s = orc.mr_stress(F, 0.7, c10, c01)
+        assert abs(s[:3].sum() - 3 * 0.7) < 1e-12  # + p on the diagonal
+
+
+def test_mr_stress_frame_indifference():
+    rng = np.random.default_rng(5)
+    F = np.eye(3) + 0.2 * rng.standard_normal((3, 3))
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    Q *= np.sign(np.linalg.det(Q))
+    voigt = lambda s: np.array([[s[0], s[3], s[4]], [s[3], s[1], s[5]], [s[4], s[5], s[2]]])
+    s1 = voigt(orc.mr_stress(F, 0.3, 2.0, 6.0))
+    s2 = voigt(orc.mr_stress(Q @ F, 0.3, 2.0, 6.0))
+    assert np.abs(Q @ s1 @ Q.T - s2).max() < 1e-12
+
+
+def test_mr_tangent_symmetry_and_small_strain_limit():
+    c10, c01 = 2.0, 6.0
+    c, s = orc.mr_elasticity(np.eye(3), 0.0, c10, c01)
+    assert np.abs(c - c.T).max() < 1e-13
+    mu = 2 * (c10 + c01)
+    # deviatoric isotropic tangent at F = I: 2 mu (I_sym - 1/3 1x1), Voigt shear entries mu
+    expected = np.zeros((6, 6))
+    expected[:3, :3] = 2 * mu * (np.eye(3) - 1 / 3)
+    expected[3:, 3:] = mu * np.eye(3)
+    assert np.abs(c - expected).max() < 1e-12
+
+
+# --- Mooney-Rivlin element residual / Jacobian --------------------------------
+def _element(jitter=0.1, seed=1):
+    m = RegularMesh.generate(2, 1, jitter=0.0)
+    Xe = m.coords[:, m.elem_nodes[0]]
+    if jitter:
+        rng = np.random.default_rng(seed)
+        Xe = Xe + jitter * 0.5 * rng.uniform(-1, 1, Xe.shape)
+    return Xe
+
+
+def test_mr_residual_vanishes_undeformed():
+    Xe = _element()
+    assert np.abs(orc.mr_element_residual(Xe, Xe.copy(), np.zeros(8), 2.0, 6.0)).max() < 1e-14
+
+
+def test_mr_homogeneous_uniaxial_state_is_exact():
+    """z = diag(l, l^-1/2, l^-1/2) X with the traction-free pressure is an
+    exact discrete solution on affine elements: the assembled residual
+    vanishes at interior DOFs and on the lateral traction-free faces."""
+    c10, c01, lam = 2.0, 6.0, 1.1
+    m = RegularMesh.generate(2, 2)
+    z = m.coords * np.array([[lam], [lam ** -0.5], [lam ** -0.5]])
+    # sigma_22 = 0 fixes p: sigma = dev + p
+    s0 = orc.mr_stress(np.diag([lam, lam ** -0.5, lam ** -0.5]), 0.0, c10, c01)
+    p = -s0[1]
+    dofs = np.concatenate([z.ravel(), np.full(m.n_nodes_l, p)])
+    r = orc.mr_residual(m.elem_nodes, m.elem_nodes_l, m.coords, dofs, c10, c01)
+    nq = m.n_nodes
+    ru = r[:3 * nq].reshape(3, nq)
+    interior_x = np.setdiff1d(np.arange(nq), np.union1d(m.face_nodes(0, 0), m.face_nodes(0, 1)))
+    assert np.abs(ru[:, interior_x]).max() < 1e-12   # only the loaded x faces carry reactions
+    assert np.abs(r[3 * nq:]).max() < 1e-13          # J = 1 everywhere
+
+
+def test_mr_residual_frame_indifference():
+    rng = np.random.default_rng(2)
+    Xe = _element()
+    ze = Xe + 0.05 * rng.uniform(-1, 1, Xe.shape)
+    pe = rng.standard_normal(8)
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    Q *= np.sign(np.linalg.det(Q))
+    R1 = orc.mr_element_residual(Xe, ze, pe, 2.0, 6.0)
+    R2 = orc.mr_element_residual(Xe, Q @ ze, pe, 2.0, 6.0)
+    assert np.abs(Q @ R1[:81].reshape(3, 27) - R2[:81].reshape(3, 27)).max() < 1e-12
+    assert np.abs(R1[81:] - R2[81:]).max() < 1e-13
+
+
+def test_mr_jacobian_matches_central_differences():
+    rng = np.random.default_rng(4)
+    Xe = _element()
+    ze = Xe + 0.05 * rng.uniform(-1, 1, Xe.shape)
+    pe = 0.5 * rng.standard_normal(8)
+    J = orc.mr_element_jacobian(Xe, ze, pe, 2.0, 6.0)
+    x0 = np.concatenate([ze.ravel(), pe])
+    Jc = np.zeros_like(J)
+    d = 1e-6
+    for k in range(89):
+        xp, xm = x0.copy(), x0.copy()
+        xp[k] += d
+        xm[k] -= d
+        Rp = orc.mr_element_residual(Xe, xp[:81].reshape(3, 27), xp[81:], 2.0, 6.0)
+        Rm = orc.mr_element_residual(Xe, xm[:81].reshape(3, 27), xm[81:], 2.0, 6.0)
+        Jc[:, k] = (Rp - Rm) / (2 * d)
+    assert np.abs(J - Jc).max() / np.abs(J).max() < 1e-8
+    assert np.abs(J - J.T).max() == 0.0          # mirrored lower triangle
+    assert np.all(J[81:, 81:] == 0.0)            # no p-p block
+
+
+def test_mr_fd_jacobian_reference_delta():
+    """Forward-difference Jacobian with delta = (1+||x||)1e-7 agrees with the
+    analytic one to O(delta)."""
+    rng = np.random.default_rng(6)
+    Xe = _element()
+    ze = Xe + 0.03 * rng.uniform(-1, 1, Xe.shape)
+    pe = 0.2 * rng.standard_normal(8)
+    delta = (1 + np.linalg.norm(np.concatenate([ze.ravel(), pe]))) * 1e-7
+    Jfd = orc.mr_element_jacobian_fd(Xe, ze, pe, 2.0, 6.0, delta)
+    J = orc.mr_element_jacobian(Xe, ze, pe, 2.0, 6.0)
+    assert np.abs(J - Jfd).max() / np.abs(J).max() < 1e-5
+
+
+def test_mr_small_strain_stiffness_is_incompressible_elasticity():
+    """At z = X, p = 0 the u-u block is the isotropic deviatoric stiffness with
+    shear modulus mu = 2(c10+c01): rigid translations are in its kernel and a
+    volume-preserving shear has energy mu * gamma^2 * V."""
+    Xe = _element(jitter=0.0)
+    J = orc.mr_element_jacobian(Xe, Xe.copy(), np.zeros(8), 2.0, 6.0)
+    Kuu = J[:81, :81]
+    for c in range(3):
+        t = np.zeros((3, 27))
+        t[c] = 1.0
+        assert np.abs(Kuu @ t.ravel()).max() < 1e-12
+    gamma = 1.0
+    u = np.zeros((3, 27))
+    u[0] = gamma * Xe[1]      # simple shear u_x = gamma y
+    mu = 2 * (2.0 + 6.0)
+    assert abs(u.ravel() @ Kuu @ u.ravel() - mu * gamma ** 2 * 1.0) < 1e-11
+
+
+# --- boundary-condition increments (equations_set_routines.f90:6560-6585) ---
+def test_bc_increment_schedule():
+    cur = np.array([1.0, 5.0])
+    full = np.array([1.1])
+    idx = np.array([0], np.int32)
+    seen = []
+    for it in range(1, 11):
+        cur = orc.bc_increment(idx, full, cur, it, 10)
+        seen.append(cur[0])
+    assert np.allclose(seen, 1.0 + 0.01 * np.arange(1, 11), atol=1e-15)
+    assert seen[-1] == 1.1 and cur[1] == 5.0
+
+
+def test_regular_coordinates_product_formula():
+    xyz = regular_coordinates(2, 2, 2, 2)
+    T = 5
+    p = np.arange(T ** 3)
+    assert np.array_equal(xyz[0], (p % T) * (1.0 / (T - 1)))
+    conn = regular_elements(1, 2, 2, 2)
+    assert list(conn[0]) == [0, 1, 3, 4, 9, 10, 12, 13]
