@@ -45,6 +45,45 @@ class CMISSError(RuntimeError):
         self.code = code
 
 
+OC_PRECONDITIONER_JACOBI = 0
+OC_PRECONDITIONER_BLOCK_DIAGONAL = 1
+OC_PRECONDITIONER_BLOCK_TRIANGULAR = 2
+
+
+class NewtonOptions(ctypes.Structure):
+    """oc_newton_options (include/oc_asm.h)."""
+    _fields_ = [
+        ("absolute_tolerance", ctypes.c_double),
+        ("relative_tolerance", ctypes.c_double),
+        ("reference_norm", ctypes.c_double),
+        ("maximum_iterations", ctypes.c_int),
+        ("jacobian_precision", ctypes.c_int),
+        ("krylov_restart", ctypes.c_int),
+        ("krylov_maximum_iterations", ctypes.c_int),
+        ("krylov_relative_tolerance", ctypes.c_double),
+        ("line_search", ctypes.c_int),
+        ("line_search_alpha", ctypes.c_double),
+        ("preconditioner", ctypes.c_int),
+    ]
+
+
+class NewtonStats(ctypes.Structure):
+    """oc_newton_stats (include/oc_asm.h)."""
+    _fields_ = [
+        ("converged", ctypes.c_int),
+        ("newton_iterations", ctypes.c_int),
+        ("krylov_iterations", ctypes.c_int),
+        ("residual_evaluations", ctypes.c_int),
+        ("jacobian_evaluations", ctypes.c_int),
+        ("initial_residual_norm", ctypes.c_double),
+        ("final_residual_norm", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -79,6 +118,12 @@ SIGNATURES = {
     "oc_event_record": [_c_int],
     "oc_event_elapsed": [_c_int, _c_int, ctypes.POINTER(_c_float)],
     "oc_synchronise": [],
+    "oc_boundary_conditions_set": [_H, _c_int, _i32, _dp, _c_int],
+    "oc_load_increment_apply": [_H, _c_int, _c_int],
+    "oc_newton_options_default": [ctypes.POINTER(NewtonOptions)],
+    "oc_newton_solve": [_H, ctypes.POINTER(NewtonOptions), ctypes.POINTER(NewtonStats)],
+    "oc_linear_solve": [_H, ctypes.POINTER(NewtonOptions), ctypes.POINTER(NewtonStats)],
+    "oc_residual_norm": [_H, ctypes.POINTER(ctypes.c_double)],
     "oc_kernel_timing_enable": [_c_int],
     "oc_kernel_timing_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
 }

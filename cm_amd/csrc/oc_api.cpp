@@ -145,10 +145,6 @@ static int check_handle(oc_equations_set h)
 
 using namespace oc;
 
-struct oc_equations_set_s {
-    EquationsSet s;
-};
-
 extern "C" {
 
 int oc_initialise(int device)

@@ -120,6 +120,18 @@ struct EquationsSet {
     DevBuf<int32_t> gather_ptr, gather_idx;
     // Laplace: 8-bit offsets, slot = row_ptr[node_i] + slot_offsets8[(e*8+i)*8+j].
     DevBuf<uint8_t> slot_offsets8;
+
+    // Dirichlet conditions (0-based DOFs, full values) and the free-DOF mask
+    int n_bc = 0;
+    DevBuf<int32_t> bc_idx;
+    DevBuf<double> bc_full;
+    DevBuf<uint8_t> free_mask;
+    // solver workspace (oc_solver.hip)
+    DevBuf<double> krylov;      // (restart+1) * n_rows
+    DevBuf<double> work;        // vectors
+    DevBuf<double> precond;     // preconditioner storage
+    DevBuf<double> partials;    // reduction partial sums
+    int krylov_capacity = 0;
 };
 
 // ---- shared device resources (oc_assembly.hip) --------------------------------
@@ -141,3 +153,8 @@ int launch_mr_residual(EquationsSet &s);
 int launch_mr_jacobian(EquationsSet &s, int precision);
 
 }  // namespace oc
+
+// The opaque handle of the C ABI.
+struct oc_equations_set_s {
+    oc::EquationsSet s;
+};

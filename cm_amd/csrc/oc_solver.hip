@@ -1,0 +1,757 @@
+// oc_solver.hip -- the GPU solver that judges the assembly path: Dirichlet
+// elimination by masking, CSR SpMV, block preconditioners, right-
+// preconditioned GMRES and Newton with backtracking line search, plus the
+// load-increment rule for fixed-incremented boundary conditions.
+//
+// The reference solves with PETSc SNES/KSP (solver_routines.f90:18306-19028,
+// 10957-11006) on a solver matrix from which Dirichlet rows and columns are
+// removed (solver_matrices_routines.f90:1258-1483).  Here the equations CSR is
+// kept and the operator is applied as  y_f = J_ff x_f, y_d = x_d  with x_d = 0
+// in every Krylov vector, which is the same eliminated system.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "oc_internal.h"
+
+namespace oc {
+
+namespace {
+
+constexpr int kReduceBlocks = 256;
+
+// ---- vector kernels -----------------------------------------------------------
+__global__ void __launch_bounds__(256) axpby_kernel(int n, double a, const double *__restrict__ x, double b,
+                                                   double *__restrict__ y)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        y[i] = a * x[i] + b * y[i];
+}
+
+__global__ void __launch_bounds__(256) copy_scale_kernel(int n, double a, const double *__restrict__ x,
+                                                        double *__restrict__ y)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = a * x[i];
+}
+
+__global__ void __launch_bounds__(256) mask_kernel(int n, const uint8_t *__restrict__ free_mask,
+                                                  double *__restrict__ y)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (!free_mask[i]) y[i] = 0.0;
+}
+
+// partial[j * gridDim.x + blockIdx.x] = sum over this block's slice of V_j . w
+// for j = 0..k-1 (blockIdx.y = j).  Deterministic: fixed grid, fixed tree.
+__global__ void __launch_bounds__(256) multi_dot_kernel(int n, const double *__restrict__ V, int64_t ldv,
+                                                       const double *__restrict__ w, double *__restrict__ partial)
+{
+    __shared__ double s[256];
+    const double *v = V + static_cast<int64_t>(blockIdx.y) * ldv;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += v[i] * w[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = s[0];
+}
+
+// w += sign * sum_j c_j V_j  (j < k <= kCoefChunk); coefficients by value
+constexpr int kCoefChunk = 64;
+struct Coefficients {
+    double c[kCoefChunk];
+};
+
+__global__ void __launch_bounds__(256) combine_kernel(int n, int k, const double *__restrict__ V, int64_t ldv,
+                                                     Coefficients coef, double sign, double *__restrict__ w)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int j = 0; j < k; ++j) acc += coef.c[j] * V[static_cast<int64_t>(j) * ldv + i];
+        w[i] += sign * acc;
+    }
+}
+
+// ---- CSR SpMV with the Dirichlet operator: y_r = sum_k v_k x_c on free rows,
+// y_r = x_r on fixed rows.  `lanes` lanes per row, shuffle reduction.
+template <typename VT, int kLanes>
+__global__ void __launch_bounds__(256) spmv_kernel(int n, const int *__restrict__ row_ptr,
+                                                  const int *__restrict__ col_idx, const VT *__restrict__ vals,
+                                                  const uint8_t *__restrict__ free_mask, const double *__restrict__ x,
+                                                  double *__restrict__ y)
+{
+    const int lane = threadIdx.x % kLanes;
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
+    if (row >= n) return;
+    double acc = 0.0;
+    const bool is_free = free_mask == nullptr || free_mask[row];
+    if (is_free)
+        for (int k = row_ptr[row] + lane; k < row_ptr[row + 1]; k += kLanes)
+            acc += static_cast<double>(vals[k]) * x[col_idx[k]];
+#pragma unroll
+    for (int o = kLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kLanes);
+    if (lane == 0) y[row] = is_free ? acc : x[row];
+}
+
+// ---- preconditioners ----------------------------------------------------------------
+__device__ __forceinline__ double csr_entry(const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
+                                            const double *__restrict__ vals, int r, int c)
+{
+    int lo = row_ptr[r], hi = row_ptr[r + 1] - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = col_idx[mid];
+        if (v == c) return vals[mid];
+        if (v < c) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return 0.0;
+}
+
+template <typename VT>
+__device__ __forceinline__ double csr_entry_t(const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
+                                              const VT *__restrict__ vals, int r, int c)
+{
+    int lo = row_ptr[r], hi = row_ptr[r + 1] - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = col_idx[mid];
+        if (v == c) return static_cast<double>(vals[mid]);
+        if (v < c) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return 0.0;
+}
+
+// point Jacobi: inv[d] = 1/a_dd (1 on zero diagonals and fixed rows, as PCJACOBI)
+template <typename VT>
+__global__ void __launch_bounds__(256) jacobi_setup_kernel(int n, const int *__restrict__ row_ptr,
+                                                          const int *__restrict__ col_idx,
+                                                          const VT *__restrict__ vals,
+                                                          const uint8_t *__restrict__ free_mask,
+                                                          double *__restrict__ inv)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    const double a = csr_entry_t(row_ptr, col_idx, vals, d, d);
+    inv[d] = (!free_mask[d] || a == 0.0) ? 1.0 : 1.0 / a;
+}
+
+// nodal 3x3 blocks of the displacement DOFs (d, d+nq, d+2nq) with fixed
+// rows/columns replaced by the identity, inverted; layout inv[node*9 + a*3+b].
+template <typename VT>
+__global__ void __launch_bounds__(256) block_setup_kernel(int nq, const int *__restrict__ row_ptr,
+                                                         const int *__restrict__ col_idx,
+                                                         const VT *__restrict__ vals,
+                                                         const uint8_t *__restrict__ free_mask,
+                                                         double *__restrict__ inv, double *__restrict__ diag_u)
+{
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nq) return;
+    double A[9];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            const int ra = a * nq + node, cb = b * nq + node;
+            double v;
+            if (free_mask[ra] && free_mask[cb]) v = csr_entry_t(row_ptr, col_idx, vals, ra, cb);
+            else v = (a == b) ? 1.0 : 0.0;
+            A[a * 3 + b] = v;
+        }
+    for (int a = 0; a < 3; ++a) diag_u[a * nq + node] = A[a * 3 + a];
+    const double c00 = A[4] * A[8] - A[5] * A[7], c01 = A[5] * A[6] - A[3] * A[8], c02 = A[3] * A[7] - A[4] * A[6];
+    const double det = A[0] * c00 + A[1] * c01 + A[2] * c02;
+    double *o = inv + static_cast<int64_t>(node) * 9;
+    if (fabs(det) < 1e-300) {
+        for (int i = 0; i < 9; ++i) o[i] = (i % 4 == 0) ? 1.0 : 0.0;
+        return;
+    }
+    const double id = 1.0 / det;
+    o[0] = c00 * id;
+    o[1] = (A[2] * A[7] - A[1] * A[8]) * id;
+    o[2] = (A[1] * A[5] - A[2] * A[4]) * id;
+    o[3] = c01 * id;
+    o[4] = (A[0] * A[8] - A[2] * A[6]) * id;
+    o[5] = (A[2] * A[3] - A[0] * A[5]) * id;
+    o[6] = c02 * id;
+    o[7] = (A[1] * A[6] - A[0] * A[7]) * id;
+    o[8] = (A[0] * A[4] - A[1] * A[3]) * id;
+}
+
+// pressure rows: S_p = sum_j B_pj^2 / K_jj over free displacement columns
+// (diagonal estimate of the Schur complement B diag(K)^-1 B^T); inv = -1/S.
+template <typename VT>
+__global__ void __launch_bounds__(256) schur_setup_kernel(int nq, int nl, const int *__restrict__ row_ptr,
+                                                         const int *__restrict__ col_idx,
+                                                         const VT *__restrict__ vals,
+                                                         const uint8_t *__restrict__ free_mask,
+                                                         const double *__restrict__ diag_u, double *__restrict__ inv)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nl) return;
+    const int r = 3 * nq + p;
+    double s = 0.0;
+    for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+        const int c = col_idx[k];
+        if (c >= 3 * nq || !free_mask[c]) continue;
+        const double b = static_cast<double>(vals[k]);
+        const double kd = diag_u[c];
+        if (kd != 0.0) s += b * b / kd;
+    }
+    inv[p] = s > 0.0 ? -1.0 / s : -1.0;
+}
+
+__global__ void __launch_bounds__(256) apply_jacobi_kernel(int n, const double *__restrict__ inv,
+                                                          const double *__restrict__ r, double *__restrict__ z)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) z[i] = inv[i] * r[i];
+}
+
+__global__ void __launch_bounds__(256) apply_block_kernel(int nq, int nl, const double *__restrict__ inv_u,
+                                                         const double *__restrict__ inv_p,
+                                                         const double *__restrict__ r, double *__restrict__ z)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) {
+        const double *o = inv_u + static_cast<int64_t>(i) * 9;
+        const double r0 = r[i], r1 = r[nq + i], r2 = r[2 * nq + i];
+        z[i] = o[0] * r0 + o[1] * r1 + o[2] * r2;
+        z[nq + i] = o[3] * r0 + o[4] * r1 + o[5] * r2;
+        z[2 * nq + i] = o[6] * r0 + o[7] * r1 + o[8] * r2;
+    }
+    if (i < nl) z[3 * nq + i] = inv_p[i] * r[3 * nq + i];
+}
+
+// ---- boundary conditions ----------------------------------------------------------
+__global__ void __launch_bounds__(256) bc_increment_kernel(int n, const int *__restrict__ idx,
+                                                          const double *__restrict__ full, double *__restrict__ dofs,
+                                                          int iteration, int max_iterations)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = idx[i];
+    if (iteration >= max_iterations) dofs[d] = full[i];
+    else dofs[d] = dofs[d] + (full[i] - dofs[d]) / static_cast<double>(max_iterations - iteration + 1);
+}
+
+unsigned grid_for(int64_t n, int per_block = 256)
+{
+    const int64_t b = (n + per_block - 1) / per_block;
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(b, runtime().num_cus * 8)));
+}
+
+// ---- host-side solver ------------------------------------------------------------------
+struct Krylov {
+    EquationsSet &s;
+    int n;
+    int m;                   // restart
+    double *V;               // (m+1) x n
+    double *r, *z, *w, *u;   // work vectors
+    double *inv_u, *inv_p, *diag_u, *inv_pt;  // preconditioner
+    int precond = OC_PRECONDITIONER_BLOCK_DIAGONAL;
+    bool fp32 = false;
+    std::vector<double> hpart;
+
+    int spmv(const double *x, double *y)
+    {
+        Runtime &rt = runtime();
+        KernelTimer timer("spmv");
+        const bool lap = s.cls == OC_EQUATIONS_SET_LAPLACE;
+        if (lap) {
+            spmv_kernel<double, 8><<<grid_rows(8), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                        s.values.ptr, s.free_mask.ptr, x, y);
+        } else if (fp32) {
+            spmv_kernel<float, 32><<<grid_rows(32), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                         s.values_sp.ptr, s.free_mask.ptr, x, y);
+        } else {
+            spmv_kernel<double, 32><<<grid_rows(32), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                          s.values.ptr, s.free_mask.ptr, x, y);
+        }
+        OC_HIP_CHECK(hipGetLastError());
+        return OC_NO_ERROR;
+    }
+    unsigned grid_rows(int lanes) const
+    {
+        return static_cast<unsigned>((static_cast<int64_t>(n) * lanes + 255) / 256);
+    }
+
+    int setup_preconditioner()
+    {
+        Runtime &rt = runtime();
+        KernelTimer timer("preconditioner_setup");
+        if (s.cls == OC_EQUATIONS_SET_LAPLACE || precond == OC_PRECONDITIONER_JACOBI) {
+            if (fp32)
+                jacobi_setup_kernel<float><<<grid_for(n), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                               s.values_sp.ptr, s.free_mask.ptr,
+                                                                               inv_u);
+            else
+                jacobi_setup_kernel<double><<<grid_for(n), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                                s.values.ptr, s.free_mask.ptr,
+                                                                                inv_u);
+            OC_HIP_CHECK(hipGetLastError());
+            return OC_NO_ERROR;
+        }
+        const int nq = s.n_nodes, nl = s.n_nodes_l;
+        const unsigned gq = static_cast<unsigned>((nq + 255) / 256), gl = static_cast<unsigned>((nl + 255) / 256);
+        if (fp32) {
+            block_setup_kernel<float><<<gq, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.col_idx.ptr, s.values_sp.ptr,
+                                                                 s.free_mask.ptr, inv_u, diag_u);
+            schur_setup_kernel<float><<<gl, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                 s.values_sp.ptr, s.free_mask.ptr, diag_u, inv_p);
+        } else {
+            block_setup_kernel<double><<<gq, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.col_idx.ptr, s.values.ptr,
+                                                                  s.free_mask.ptr, inv_u, diag_u);
+            schur_setup_kernel<double><<<gl, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                  s.values.ptr, s.free_mask.ptr, diag_u, inv_p);
+        }
+        OC_HIP_CHECK(hipGetLastError());
+        return OC_NO_ERROR;
+    }
+
+    // z = P^-1 r (block upper-triangular: z_p = S^-1 r_p, z_u = K^-1 (r_u - B^T z_p))
+    int apply_preconditioner(const double *rin, double *zout)
+    {
+        Runtime &rt = runtime();
+        if (s.cls == OC_EQUATIONS_SET_LAPLACE || precond == OC_PRECONDITIONER_JACOBI) {
+            apply_jacobi_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, inv_u, rin, zout);
+            OC_HIP_CHECK(hipGetLastError());
+            return OC_NO_ERROR;
+        }
+        const int nq = s.n_nodes, nl = s.n_nodes_l;
+        if (precond == OC_PRECONDITIONER_BLOCK_TRIANGULAR) {
+            // tmp = [0; S^-1 r_p]; u = J tmp (u-rows give B^T z_p); z_u = K^-1 (r_u - u_u)
+            OC_HIP_CHECK(hipMemsetAsync(inv_pt, 0, sizeof(double) * 3 * nq, rt.stream));
+            apply_jacobi_kernel<<<grid_for(nl), 256, 0, rt.stream>>>(nl, inv_p, rin + 3 * nq, inv_pt + 3 * nq);
+            OC_CHECK(spmv(inv_pt, u));
+            axpby_kernel<<<grid_for(3 * nq), 256, 0, rt.stream>>>(3 * nq, 1.0, rin, -1.0, u);
+            hipMemcpyAsync(u + 3 * nq, rin + 3 * nq, sizeof(double) * nl, hipMemcpyDeviceToDevice, rt.stream);
+            mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, u);
+            apply_block_kernel<<<static_cast<unsigned>((std::max(nq, nl) + 255) / 256), 256, 0, rt.stream>>>(
+                nq, nl, inv_u, inv_p, u, zout);
+        } else {
+            apply_block_kernel<<<static_cast<unsigned>((std::max(nq, nl) + 255) / 256), 256, 0, rt.stream>>>(
+                nq, nl, inv_u, inv_p, rin, zout);
+        }
+        OC_HIP_CHECK(hipGetLastError());
+        return OC_NO_ERROR;
+    }
+
+    // h[j] = V_j . w for j < k (one device->host copy)
+    int multi_dot(const double *Vb, int k, const double *wv, double *h)
+    {
+        Runtime &rt = runtime();
+        dim3 grid(kReduceBlocks, k);
+        multi_dot_kernel<<<grid, 256, 0, rt.stream>>>(n, Vb, n, wv, s.partials.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+        hpart.resize(static_cast<size_t>(k) * kReduceBlocks);
+        OC_HIP_CHECK(hipMemcpyAsync(hpart.data(), s.partials.ptr, sizeof(double) * hpart.size(),
+                                    hipMemcpyDeviceToHost, rt.stream));
+        OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+        for (int j = 0; j < k; ++j) {
+            double acc = 0.0;
+            for (int b = 0; b < kReduceBlocks; ++b) acc += hpart[static_cast<size_t>(j) * kReduceBlocks + b];
+            h[j] = acc;
+        }
+        return OC_NO_ERROR;
+    }
+
+    int norm(const double *x, double *out)
+    {
+        double h = 0.0;
+        OC_CHECK(multi_dot(x, 1, x, &h));
+        *out = std::sqrt(std::max(h, 0.0));
+        return OC_NO_ERROR;
+    }
+
+    int combine(const double *Vb, int k, const std::vector<double> &coef, double sign, double *out)
+    {
+        Runtime &rt = runtime();
+        for (int j0 = 0; j0 < k; j0 += kCoefChunk) {
+            const int kk = std::min(kCoefChunk, k - j0);
+            Coefficients c{};
+            for (int j = 0; j < kk; ++j) c.c[j] = coef[j0 + j];
+            combine_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, kk, Vb + static_cast<int64_t>(j0) * n, n, c, sign,
+                                                               out);
+            OC_HIP_CHECK(hipGetLastError());
+        }
+        return OC_NO_ERROR;
+    }
+
+    // Solves A x = b (x starts at 0) to ||b - A x|| <= tol_abs; b has zeros on
+    // fixed DOFs.  Right preconditioning: x = P^-1 V y.
+    int gmres(const double *b, double *x, double tol_abs, int max_iter, int *iters, double *res_out)
+    {
+        Runtime &rt = runtime();
+        OC_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, rt.stream));
+        int total = 0;
+        double beta = 0.0;
+        OC_CHECK(norm(b, &beta));
+        double res = beta;
+        std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1), h2(m + 1);
+        bool first = true;
+        while (total < max_iter && res > tol_abs) {
+            // r = b - A x
+            if (first) {
+                OC_HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
+                first = false;
+            } else {
+                OC_CHECK(spmv(x, w));
+                OC_HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
+                axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, -1.0, w, 1.0, r);
+                mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, r);
+                OC_CHECK(norm(r, &beta));
+                res = beta;
+                if (res <= tol_abs) break;
+            }
+            copy_scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0 / beta, r, V);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int j = 0;
+            for (; j < m && total < max_iter; ++j, ++total) {
+                double *vj = V + static_cast<int64_t>(j) * n, *vn = V + static_cast<int64_t>(j + 1) * n;
+                OC_CHECK(apply_preconditioner(vj, z));
+                mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, z);
+                OC_CHECK(spmv(z, vn));
+                mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, vn);
+                // classical Gram-Schmidt, twice (CGS2)
+                double wn2 = 0.0;
+                OC_CHECK(multi_dot(vn, 1, vn, &wn2));
+                OC_CHECK(multi_dot(V, j + 1, vn, h.data()));
+                std::vector<double> hc(h.begin(), h.begin() + j + 1);
+                OC_CHECK(combine(V, j + 1, hc, -1.0, vn));
+                double proj2 = 0.0;
+                for (int i = 0; i <= j; ++i) proj2 += h[i] * h[i];
+                if (wn2 - proj2 < 0.25 * wn2) {
+                    OC_CHECK(multi_dot(V, j + 1, vn, h2.data()));
+                    std::vector<double> hc2(h2.begin(), h2.begin() + j + 1);
+                    OC_CHECK(combine(V, j + 1, hc2, -1.0, vn));
+                    for (int i = 0; i <= j; ++i) h[i] += h2[i];
+                }
+                double hn = 0.0;
+                OC_CHECK(norm(vn, &hn));
+                h[j + 1] = hn;
+                if (hn > 0.0) copy_scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0 / hn, vn, vn);
+                // Givens
+                for (int i = 0; i < j; ++i) {
+                    const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+                    h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                    h[i] = t;
+                }
+                const double den = std::hypot(h[j], h[j + 1]);
+                cs[j] = den > 0.0 ? h[j] / den : 1.0;
+                sn[j] = den > 0.0 ? h[j + 1] / den : 0.0;
+                h[j] = den;
+                h[j + 1] = 0.0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                for (int i = 0; i <= j; ++i) H[static_cast<size_t>(i) * m + j] = h[i];
+                res = std::fabs(g[j + 1]);
+                if (res <= tol_abs || hn == 0.0) {
+                    ++j;
+                    ++total;
+                    break;
+                }
+            }
+            // y = H^-1 g ; x += P^-1 (V y)
+            for (int i = j - 1; i >= 0; --i) {
+                double t = g[i];
+                for (int k = i + 1; k < j; ++k) t -= H[static_cast<size_t>(i) * m + k] * y[k];
+                y[i] = t / H[static_cast<size_t>(i) * m + i];
+            }
+            OC_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(double) * n, rt.stream));
+            std::vector<double> yc(y.begin(), y.begin() + j);
+            OC_CHECK(combine(V, j, yc, 1.0, w));
+            OC_CHECK(apply_preconditioner(w, z));
+            mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, z);
+            axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0, z, 1.0, x);
+            if (j == 0) break;
+        }
+        *iters = total;
+        *res_out = res;
+        return OC_NO_ERROR;
+    }
+};
+
+}  // namespace
+
+// Work vectors: x_prev, dx, rhs, r, z, w, u, tmp -> 8 * n; preconditioner:
+// 9*nq + nl + n + n (inv_u, inv_p, diag_u, triangular scratch)
+static int solver_prepare(EquationsSet &s, int restart, Krylov &K)
+{
+    const int n = s.n_rows;
+    if (!s.free_mask.ptr) {
+        OC_CHECK(s.free_mask.alloc(n));
+        OC_HIP_CHECK(hipMemset(s.free_mask.ptr, 1, n));
+    }
+    if (s.krylov_capacity < restart) {
+        OC_CHECK(s.krylov.alloc(static_cast<size_t>(restart + 1) * n));
+        s.krylov_capacity = restart;
+    }
+    if (!s.work.ptr) OC_CHECK(s.work.alloc(static_cast<size_t>(8) * n));
+    if (!s.precond.ptr) OC_CHECK(s.precond.alloc(static_cast<size_t>(9) * s.n_nodes + 3 * static_cast<size_t>(n)));
+    if (!s.partials.ptr) OC_CHECK(s.partials.alloc(static_cast<size_t>(kReduceBlocks) * (1024 + 1)));
+    if (restart + 1 > 1024) return set_error(OC_ERR_INVALID_ARGUMENT, "krylov_restart must be <= 1023");
+    K.n = n;
+    K.m = restart;
+    K.V = s.krylov.ptr;
+    K.r = s.work.ptr + 3 * static_cast<int64_t>(n);
+    K.z = s.work.ptr + 4 * static_cast<int64_t>(n);
+    K.w = s.work.ptr + 5 * static_cast<int64_t>(n);
+    K.u = s.work.ptr + 6 * static_cast<int64_t>(n);
+    double *pc = s.precond.ptr;
+    if (s.cls == OC_EQUATIONS_SET_LAPLACE) {
+        K.inv_u = pc;
+        K.inv_p = nullptr;
+        K.diag_u = nullptr;
+        K.inv_pt = nullptr;
+    } else {
+        K.inv_u = pc;
+        K.inv_p = pc + static_cast<int64_t>(9) * s.n_nodes;
+        K.diag_u = K.inv_p + n;
+        K.inv_pt = K.diag_u + n;
+        if (K.precond == OC_PRECONDITIONER_JACOBI) K.inv_u = K.inv_pt;  // n-sized point Jacobi
+    }
+    return OC_NO_ERROR;
+}
+
+}  // namespace oc
+
+using namespace oc;
+
+extern "C" {
+
+int oc_boundary_conditions_set(oc_equations_set h, int n, const int *dof_indices, const double *full_values,
+                               int index_base)
+{
+    OC_CHECK(require_runtime());
+    if (!h) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
+    if (n < 0 || (n > 0 && (!dof_indices || !full_values)) || (index_base != 0 && index_base != 1))
+        return set_error(OC_ERR_INVALID_ARGUMENT, "oc_boundary_conditions_set: invalid arguments");
+    EquationsSet &s = h->s;
+    std::vector<int32_t> idx(n);
+    std::vector<uint8_t> mask(s.n_rows, 1);
+    for (int i = 0; i < n; ++i) {
+        const int d = dof_indices[i] - index_base;
+        if (d < 0 || d >= s.n_rows)
+            return set_error(OC_ERR_INVALID_ARGUMENT, "Dirichlet DOF %d out of range", dof_indices[i]);
+        if (!mask[d]) return set_error(OC_ERR_INVALID_ARGUMENT, "Dirichlet DOF %d given twice", dof_indices[i]);
+        mask[d] = 0;
+        idx[i] = d;
+    }
+    OC_CHECK(s.bc_idx.upload(idx.data(), idx.size()));
+    OC_CHECK(s.bc_full.upload(full_values, static_cast<size_t>(n)));
+    OC_CHECK(s.free_mask.upload(mask.data(), mask.size()));
+    s.n_bc = n;
+    return OC_NO_ERROR;
+}
+
+int oc_load_increment_apply(oc_equations_set h, int iteration, int max_iterations)
+{
+    OC_CHECK(require_runtime());
+    if (!h) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
+    if (max_iterations < 1 || iteration < 1 || iteration > max_iterations)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "load step %d of %d", iteration, max_iterations);
+    EquationsSet &s = h->s;
+    if (s.n_bc == 0) return OC_NO_ERROR;
+    bc_increment_kernel<<<static_cast<unsigned>((s.n_bc + 255) / 256), 256, 0, runtime().stream>>>(
+        s.n_bc, s.bc_idx.ptr, s.bc_full.ptr, s.dofs.ptr, iteration, max_iterations);
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
+int oc_newton_options_default(oc_newton_options *o)
+{
+    if (!o) return set_error(OC_ERR_INVALID_ARGUMENT, "options is NULL");
+    o->absolute_tolerance = 1e-10;
+    o->relative_tolerance = 1e-5;
+    o->reference_norm = 0.0;
+    o->maximum_iterations = 50;
+    o->jacobian_precision = OC_FP64;
+    o->krylov_restart = 30;
+    o->krylov_maximum_iterations = 100000;
+    o->krylov_relative_tolerance = 1e-5;
+    o->line_search = 1;
+    o->line_search_alpha = 1e-4;
+    o->preconditioner = OC_PRECONDITIONER_JACOBI;
+    return OC_NO_ERROR;
+}
+
+static int masked_residual_norm(EquationsSet &s, Krylov &K, double *out)
+{
+    OC_CHECK(launch_mr_residual(s));
+    mask_kernel<<<grid_for(s.n_rows), 256, 0, runtime().stream>>>(s.n_rows, s.free_mask.ptr, s.residual.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    return K.norm(s.residual.ptr, out);
+}
+
+int oc_residual_norm(oc_equations_set h, double *norm)
+{
+    OC_CHECK(require_runtime());
+    if (!h || !norm) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    EquationsSet &s = h->s;
+    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "residual norm is defined for Mooney-Rivlin sets");
+    OC_CHECK(mr_prepare(s));
+    Krylov K{s};
+    OC_CHECK(solver_prepare(s, 1, K));
+    return masked_residual_norm(s, K, norm);
+}
+
+int oc_newton_solve(oc_equations_set h, const oc_newton_options *opt, oc_newton_stats *stats)
+{
+    OC_CHECK(require_runtime());
+    if (!h || !opt || !stats) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    EquationsSet &s = h->s;
+    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "Newton solve is defined for Mooney-Rivlin sets");
+    if (opt->krylov_restart < 1 || opt->maximum_iterations < 0)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "invalid Newton options");
+    const auto t0 = std::chrono::steady_clock::now();
+    *stats = oc_newton_stats{};
+    OC_CHECK(mr_prepare(s));
+    Krylov K{s};
+    K.precond = opt->preconditioner;
+    K.fp32 = opt->jacobian_precision == OC_FP32;
+    OC_CHECK(solver_prepare(s, opt->krylov_restart, K));
+    if (K.fp32 && !s.values_sp.ptr) OC_CHECK(s.values_sp.alloc(s.nnz));
+    if (!K.fp32 && !s.values.ptr) OC_CHECK(s.values.alloc(s.nnz));
+    Runtime &rt = runtime();
+    const int n = s.n_rows;
+    double *x_prev = s.work.ptr, *dx = s.work.ptr + n, *rhs = s.work.ptr + 2 * static_cast<int64_t>(n);
+
+    double fnorm = 0.0;
+    OC_CHECK(masked_residual_norm(s, K, &fnorm));
+    stats->residual_evaluations++;
+    stats->initial_residual_norm = fnorm;
+    const double ref = opt->reference_norm > 0.0 ? opt->reference_norm : fnorm;
+    const double target = std::max(opt->absolute_tolerance, opt->relative_tolerance * ref);
+    double fnorm_prev = fnorm, eta_prev = 0.5;
+    int it = 0;
+    while (!(fnorm <= target) && it < opt->maximum_iterations) {
+        if (!std::isfinite(fnorm)) break;
+        // Jacobian and preconditioner at the current DOFs
+        OC_CHECK(launch_mr_jacobian(s, opt->jacobian_precision));
+        stats->jacobian_evaluations++;
+        OC_CHECK(K.setup_preconditioner());
+        // forcing term: fixed, or Eisenstat-Walker (choice 2, gamma 0.9, alpha (1+sqrt5)/2)
+        double eta = opt->krylov_relative_tolerance;
+        if (eta <= 0.0) {
+            if (it == 0) eta = 0.5;
+            else {
+                eta = 0.9 * std::pow(fnorm / fnorm_prev, 1.618);
+                const double safe = 0.9 * std::pow(eta_prev, 1.618);
+                if (safe > 0.1) eta = std::max(eta, safe);
+                eta = std::min(eta, 0.5);
+            }
+            eta = std::max(eta, 0.5 * target / fnorm);  // do not over-solve the last step
+            eta = std::max(eta, 1e-14);
+            eta_prev = eta;
+        }
+        // rhs = -F on free DOFs
+        copy_scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, -1.0, s.residual.ptr, rhs);
+        int kit = 0;
+        double kres = 0.0;
+        OC_CHECK(K.gmres(rhs, dx, eta * fnorm, opt->krylov_maximum_iterations, &kit, &kres));
+        stats->krylov_iterations += kit;
+        // line search on f = 1/2 ||F||^2 with slope -||F||^2 (Newton direction)
+        OC_HIP_CHECK(hipMemcpyAsync(x_prev, s.dofs.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
+        double lambda = 1.0, fnew = 0.0;
+        const double f0 = 0.5 * fnorm * fnorm, slope = -fnorm * fnorm;
+        double lambda_prev = 0.0, f_prev_trial = 0.0;
+        for (int ls = 0; ls < 20; ++ls) {
+            OC_HIP_CHECK(hipMemcpyAsync(s.dofs.ptr, x_prev, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                        rt.stream));
+            axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, lambda, dx, 1.0, s.dofs.ptr);
+            OC_CHECK(masked_residual_norm(s, K, &fnew));
+            stats->residual_evaluations++;
+            const double f1 = 0.5 * fnew * fnew;
+            if (!opt->line_search || (std::isfinite(f1) && f1 <= f0 + opt->line_search_alpha * lambda * slope))
+                break;
+            // quadratic (first backtrack) or cubic interpolation, safeguarded to [0.1, 0.5] lambda
+            double lnew;
+            if (ls == 0 || !std::isfinite(f1)) {
+                lnew = std::isfinite(f1) ? -slope / (2.0 * (f1 - f0 - slope)) : 0.1;
+            } else {
+                const double r1 = f1 - f0 - lambda * slope, r2 = f_prev_trial - f0 - lambda_prev * slope;
+                const double a = (r1 / (lambda * lambda) - r2 / (lambda_prev * lambda_prev)) / (lambda - lambda_prev);
+                const double b = (-lambda_prev * r1 / (lambda * lambda) + lambda * r2 / (lambda_prev * lambda_prev)) /
+                                 (lambda - lambda_prev);
+                if (a == 0.0) lnew = -slope / (2.0 * b);
+                else {
+                    const double disc = b * b - 3.0 * a * slope;
+                    lnew = disc < 0.0 ? 0.5 * lambda : (-b + std::sqrt(disc)) / (3.0 * a);
+                }
+            }
+            lambda_prev = lambda;
+            f_prev_trial = f1;
+            lambda = std::min(0.5 * lambda, std::max(0.1 * lambda, lnew));
+        }
+        fnorm_prev = fnorm;
+        fnorm = fnew;
+        ++it;
+    }
+    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+    stats->newton_iterations = it;
+    stats->final_residual_norm = fnorm;
+    stats->converged = fnorm <= target ? 1 : 0;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!stats->converged)
+        return set_error(OC_ERR_NOT_CONVERGED, "Newton did not converge: ||F|| = %.3e > %.3e after %d iterations",
+                         fnorm, target, it);
+    return OC_NO_ERROR;
+}
+
+int oc_linear_solve(oc_equations_set h, const oc_newton_options *opt, oc_newton_stats *stats)
+{
+    OC_CHECK(require_runtime());
+    if (!h || !opt || !stats) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    EquationsSet &s = h->s;
+    if (s.cls != OC_EQUATIONS_SET_LAPLACE)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "linear solve is defined for Laplace sets");
+    const auto t0 = std::chrono::steady_clock::now();
+    *stats = oc_newton_stats{};
+    Krylov K{s};
+    K.precond = OC_PRECONDITIONER_JACOBI;
+    OC_CHECK(solver_prepare(s, opt->krylov_restart, K));
+    Runtime &rt = runtime();
+    const int n = s.n_rows;
+    double *rhs = s.work.ptr + 2 * static_cast<int64_t>(n), *dx = s.work.ptr + n, *tmp = s.work.ptr + 7 * static_cast<int64_t>(n);
+    OC_CHECK(launch_laplace_assemble(s));
+    // u = Dirichlet values on fixed DOFs, 0 elsewhere; rhs = RHS - K u on free rows
+    OC_HIP_CHECK(hipMemsetAsync(s.dofs.ptr, 0, sizeof(double) * n, rt.stream));
+    if (s.n_bc) OC_CHECK(oc_load_increment_apply(h, 1, 1));
+    spmv_kernel<double, 8><<<static_cast<unsigned>((static_cast<int64_t>(n) * 8 + 255) / 256), 256, 0, rt.stream>>>(
+        n, s.row_ptr.ptr, s.col_idx.ptr, s.values.ptr, nullptr, s.dofs.ptr, tmp);
+    OC_HIP_CHECK(hipMemcpyAsync(rhs, s.rhs.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
+    axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, -1.0, tmp, 1.0, rhs);
+    mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, rhs);
+    OC_CHECK(K.setup_preconditioner());
+    double bnorm = 0.0;
+    OC_CHECK(K.norm(rhs, &bnorm));
+    stats->initial_residual_norm = bnorm;
+    const double target = std::max(opt->absolute_tolerance, opt->krylov_relative_tolerance * bnorm);
+    int kit = 0;
+    double kres = 0.0;
+    OC_CHECK(K.gmres(rhs, dx, target, opt->krylov_maximum_iterations, &kit, &kres));
+    axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0, dx, 1.0, s.dofs.ptr);
+    // true residual of the free rows
+    spmv_kernel<double, 8><<<static_cast<unsigned>((static_cast<int64_t>(n) * 8 + 255) / 256), 256, 0, rt.stream>>>(
+        n, s.row_ptr.ptr, s.col_idx.ptr, s.values.ptr, nullptr, s.dofs.ptr, tmp);
+    axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0, s.rhs.ptr, -1.0, tmp);
+    mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, tmp);
+    double fres = 0.0;
+    OC_CHECK(K.norm(tmp, &fres));
+    stats->krylov_iterations = kit;
+    stats->newton_iterations = 1;
+    stats->final_residual_norm = fres;
+    stats->converged = fres <= std::max(target * 1.0001, opt->absolute_tolerance) ? 1 : 0;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!stats->converged)
+        return set_error(OC_ERR_NOT_CONVERGED, "linear solve did not converge: ||r|| = %.3e > %.3e", fres, target);
+    return OC_NO_ERROR;
+}
+
+}  // extern "C"

@@ -135,6 +135,29 @@ class EquationsSet:
         call("oc_device_pointer_get", self._h, int(buffer), ctypes.byref(p))
         return p.value
 
+    # -- boundary conditions and solution ------------------------------------
+    def set_boundary_conditions(self, dof_indices, full_values):
+        """Fixed(-incremented) Dirichlet DOFs (0-based) and their full values."""
+        idx = np.ascontiguousarray(dof_indices, dtype=np.int32)
+        val = np.ascontiguousarray(full_values, dtype=np.float64)
+        if idx.shape != val.shape:
+            raise ValueError("dof_indices and full_values must have the same shape")
+        call("oc_boundary_conditions_set", self._h, int(idx.size), idx, val, 0)
+
+    def load_increment_apply(self, iteration, max_iterations):
+        call("oc_load_increment_apply", self._h, int(iteration), int(max_iterations))
+
+
+def newton_options(**overrides):
+    """oc_newton_options with the reference defaults, then ``overrides``."""
+    o = _lib.NewtonOptions()
+    call("oc_newton_options_default", ctypes.byref(o))
+    for k, v in overrides.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
 
 class LaplaceEquationsSet(EquationsSet):
     """Classical field / Laplace / standard subtype on trilinear hexahedra."""
@@ -152,6 +175,15 @@ class LaplaceEquationsSet(EquationsSet):
     def assemble(self):
         """Zero and assemble the linear matrix and RHS vector on the GPU."""
         call("oc_equations_set_assemble", self._h)
+
+    def solve(self, options=None, raise_on_failure=True):
+        """Static linear solve with the current Dirichlet conditions."""
+        opts = options if options is not None else newton_options()
+        st = _lib.NewtonStats()
+        rc = _lib.load().oc_linear_solve(self._h, ctypes.byref(opts), ctypes.byref(st))
+        if rc != 0 and (raise_on_failure or rc != _lib.OC_ERR_NOT_CONVERGED):
+            _lib.check(rc)
+        return st.as_dict()
 
 
 class MooneyRivlinEquationsSet(EquationsSet):
@@ -187,3 +219,29 @@ class MooneyRivlinEquationsSet(EquationsSet):
 
     def jacobian_evaluate(self, precision=_lib.OC_FP64):
         call("oc_equations_set_jacobian_evaluate", self._h, int(precision))
+
+    def residual_norm(self):
+        """||R|| over the free DOFs at the current dependent field."""
+        v = ctypes.c_double(0.0)
+        call("oc_residual_norm", self._h, ctypes.byref(v))
+        return v.value
+
+    def newton_solve(self, options=None, raise_on_failure=True):
+        """Newton solve at the current Dirichlet values; returns the stats dict."""
+        opts = options if options is not None else newton_options()
+        st = _lib.NewtonStats()
+        rc = _lib.load().oc_newton_solve(self._h, ctypes.byref(opts), ctypes.byref(st))
+        if rc != 0 and (raise_on_failure or rc != _lib.OC_ERR_NOT_CONVERGED):
+            _lib.check(rc)
+        return st.as_dict()
+
+    def uniaxial_boundary_conditions(self, stretch):
+        """Symmetry planes z_x = 0 on x = 0, z_y = 0 on y = 0, z_z = 0 on z = 0
+        and z_x -> stretch on x = 1 (fixed-incremented)."""
+        m = self.mesh
+        nq = m.n_nodes
+        idx = [m.face_nodes(0, 0), nq + m.face_nodes(1, 0), 2 * nq + m.face_nodes(2, 0), m.face_nodes(0, 1)]
+        vals = [np.zeros(len(idx[0])), np.zeros(len(idx[1])), np.zeros(len(idx[2])),
+                np.full(len(idx[3]), float(stretch))]
+        self.set_boundary_conditions(np.concatenate(idx), np.concatenate(vals))
+        return np.concatenate(idx)

@@ -137,6 +137,65 @@ int oc_residual_get(oc_equations_set equations_set, double *residual);
 int oc_rhs_get(oc_equations_set equations_set, double *rhs);
 int oc_device_pointer_get(oc_equations_set equations_set, int buffer, void **device_pointer);
 
+/* ---- boundary conditions and solution (the solver "judge") ----------------- */
+/* Fixed (or fixed-incremented) Dirichlet conditions on `n` DOFs with their
+ * full values (BOUNDARY_CONDITION_FIXED[_INCREMENTED],
+ * boundary_condition_routines.f90).  Replaces any previous set.  Does not
+ * change the DOFs; see oc_load_increment_apply. */
+int oc_boundary_conditions_set(oc_equations_set equations_set, int n, const int *dof_indices,
+                               const double *full_values, int index_base);
+/* EQUATIONS_SET_BOUNDARY_CONDITIONS_INCREMENT (equations_set_routines.f90:
+ * 6491-6610): for load step `iteration` of `max_iterations`, each Dirichlet
+ * DOF becomes cur + (full - cur)/(max - iteration + 1); the full value at the
+ * last step. */
+int oc_load_increment_apply(oc_equations_set equations_set, int iteration, int max_iterations);
+
+#define OC_PRECONDITIONER_JACOBI 0          /* point Jacobi, zero diagonal -> 1 (PETSc PCJACOBI) */
+#define OC_PRECONDITIONER_BLOCK_DIAGONAL 1  /* nodal 3x3 block Jacobi + diagonal Schur estimate for p */
+#define OC_PRECONDITIONER_BLOCK_TRIANGULAR 2 /* block upper-triangular variant of the above */
+
+typedef struct {
+    double absolute_tolerance;   /* Newton: converged when ||F|| <= max(abs, rel * ||F_ref||) */
+    double relative_tolerance;   /* relative to ||F_ref|| (the step's initial residual, or reference_norm) */
+    double reference_norm;       /* > 0: use instead of the initial residual norm */
+    int maximum_iterations;      /* Newton iterations */
+    int jacobian_precision;      /* OC_FP64 or OC_FP32 */
+    int krylov_restart;          /* GMRES restart length */
+    int krylov_maximum_iterations;
+    double krylov_relative_tolerance; /* > 0: fixed forcing term; <= 0: Eisenstat-Walker */
+    int line_search;             /* 0: full steps, 1: backtracking (cubic/quadratic, alpha below) */
+    double line_search_alpha;    /* sufficient-decrease constant (reference default 1e-4) */
+    int preconditioner;          /* OC_PRECONDITIONER_* */
+} oc_newton_options;
+
+typedef struct {
+    int converged;
+    int newton_iterations;
+    int krylov_iterations;       /* total over the Newton iterations */
+    int residual_evaluations;
+    int jacobian_evaluations;
+    double initial_residual_norm;
+    double final_residual_norm;
+    double seconds;
+} oc_newton_stats;
+
+/* Reference defaults (solver_routines.f90:17780-17851, 10957-11006): abs tol
+ * 1e-10, rel tol 1e-5, 50 iterations, line search alpha 1e-4, GMRES(30),
+ * rtol 1e-5, Jacobi. */
+int oc_newton_options_default(oc_newton_options *options);
+/* Newton solve of R(x) = 0 at the current Dirichlet values for a Mooney-
+ * Rivlin set (SOLVER_NEWTON_LINESEARCH_SOLVE, solver_routines.f90:18871-
+ * 19028, with the residual and Jacobian callbacks of problem_routines.f90:
+ * 4122-4483).  Dirichlet DOFs are eliminated (their rows and columns are
+ * dropped from the linear systems, solver_matrices_routines.f90:1258-1483). */
+int oc_newton_solve(oc_equations_set equations_set, const oc_newton_options *options, oc_newton_stats *stats);
+/* Static linear solve of a Laplace set (Problem_SolverEquationsStaticLinear
+ * Solve, problem_routines.f90:2740-2830): assemble, set the Dirichlet values
+ * and solve the free rows with the Krylov settings of `options`. */
+int oc_linear_solve(oc_equations_set equations_set, const oc_newton_options *options, oc_newton_stats *stats);
+/* Euclidean norm of the residual over the free DOFs at the current DOFs. */
+int oc_residual_norm(oc_equations_set equations_set, double *norm);
+
 /* ---- timing on the library stream ----------------------------------------- */
 /* Records event `slot` (0..63) on the library stream. */
 int oc_event_record(int slot);

@@ -1,0 +1,94 @@
+"""GPU solver (the judge of the assembly path): every accepted answer is
+checked against the FP64 oracle residual, plus closed-form solutions."""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def test_laplace_config1_patch_solution(gpu):
+    """8^3 trilinear, u = 0 on x = 0 and u = 1 on x = 1: the discrete solution
+    is u = x exactly."""
+    cm = gpu
+    mesh = cm.RegularMesh.generate(1, 8)
+    eq = cm.LaplaceEquationsSet(mesh)
+    left, right = mesh.face_nodes(0, 0), mesh.face_nodes(0, 1)
+    eq.set_boundary_conditions(np.concatenate([left, right]),
+                               np.concatenate([np.zeros(len(left)), np.ones(len(right))]))
+    st = eq.solve(cm.newton_options(krylov_relative_tolerance=1e-13, krylov_restart=60, absolute_tolerance=0.0))
+    assert st["converged"]
+    u = eq.get_dofs()
+    assert np.abs(u - mesh.coords[0]).max() < 1e-10
+
+
+def _oracle_free_residual(eq, mesh, fixed, c10, c01):
+    d = eq.get_dofs()
+    r = orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, c10, c01)
+    r[fixed] = 0.0
+    return np.linalg.norm(r)
+
+
+def test_mr_homogeneous_uniaxial_solution(gpu):
+    """Unjittered mesh, symmetry planes and z_x -> lambda on x = 1: the exact
+    discrete solution is the homogeneous incompressible stretch."""
+    cm = gpu
+    c10, c01, lam = 2.0, 6.0, 1.1
+    mesh = cm.RegularMesh.generate(2, 2)
+    eq = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
+    eq.set_dofs(eq.undeformed_dofs())
+    fixed = eq.uniaxial_boundary_conditions(lam)
+    opts = cm.newton_options(relative_tolerance=1e-12, absolute_tolerance=1e-13, krylov_restart=200,
+                             krylov_relative_tolerance=-1.0, preconditioner=cm.OC_PRECONDITIONER_BLOCK_DIAGONAL)
+    for step in (1, 2):
+        eq.load_increment_apply(step, 2)
+        st = eq.newton_solve(opts)
+        assert st["converged"] and st["newton_iterations"] <= 8
+    d = eq.get_dofs()
+    nq = mesh.n_nodes
+    z = d[:3 * nq].reshape(3, nq)
+    expected = mesh.coords * np.array([[lam], [lam ** -0.5], [lam ** -0.5]])
+    assert np.abs(z - expected).max() < 1e-9
+    s0 = orc.mr_stress(np.diag([lam, lam ** -0.5, lam ** -0.5]), 0.0, c10, c01)
+    assert np.abs(d[3 * nq:] + s0[1]).max() < 1e-8
+    assert _oracle_free_residual(eq, mesh, fixed, c10, c01) < 1e-11
+
+
+@pytest.mark.parametrize("precision,precond", [(0, 1), (1, 1), (0, 2), (0, 0)])
+def test_mr_newton_jittered_judged_by_oracle(gpu, precision, precond):
+    cm = gpu
+    c10, c01 = 2.0, 6.0
+    mesh = cm.RegularMesh.generate(2, 3, jitter=0.1, seed=4)
+    eq = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
+    eq.set_dofs(eq.undeformed_dofs())
+    fixed = eq.uniaxial_boundary_conditions(1.1)
+    opts = cm.newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=300,
+                             krylov_relative_tolerance=-1.0, jacobian_precision=precision, preconditioner=precond,
+                             maximum_iterations=30)
+    for step in (1, 2, 3):
+        eq.load_increment_apply(step, 3)
+        st = eq.newton_solve(opts)
+        assert st["converged"], st
+        ref = st["initial_residual_norm"]
+        assert _oracle_free_residual(eq, mesh, fixed, c10, c01) <= 1.01e-10 * ref + 1e-14
+
+
+def test_newton_failure_modes_forced_from_host(gpu):
+    cm = gpu
+    mesh = cm.RegularMesh.generate(2, 2, jitter=0.1, seed=1)
+    eq = cm.MooneyRivlinEquationsSet(mesh, 2.0, 6.0)
+    eq.set_dofs(eq.undeformed_dofs())
+    eq.uniaxial_boundary_conditions(1.2)
+    eq.load_increment_apply(1, 1)
+    # iteration limit (SNES_DIVERGED_MAX_IT analogue)
+    with pytest.raises(cm.CMISSError) as exc:
+        eq.newton_solve(cm.newton_options(maximum_iterations=1, relative_tolerance=1e-12))
+    assert exc.value.code == -5 and "did not converge" in str(exc.value)
+    st = eq.newton_solve(cm.newton_options(maximum_iterations=1, relative_tolerance=1e-12),
+                         raise_on_failure=False)
+    assert not st["converged"]
+    # NaN material constant (DIVERGED_FNORM_NAN analogue): stops, reports, never hangs
+    eq.set_materials(float("nan"), 6.0)
+    st = eq.newton_solve(cm.newton_options(maximum_iterations=5), raise_on_failure=False)
+    assert not st["converged"] and not np.isfinite(st["final_residual_norm"])
