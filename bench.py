@@ -177,7 +177,7 @@ def main():
     avg_launch_s = jac_ms / jac_launches / 1e3
     achieved = MR_JACOBIAN_FLOP_PER_ELEMENT * per_launch_elems / avg_launch_s / 1e12
     roofline = {
-        "kernel": "mr_jacobian_kernel<double>",
+        "kernel": "mr_jacobian_element_kernel<double>",
         "bound": "mfma",
         "achieved": achieved,
         "peak": FP64_PEAK_TFLOPS,

@@ -131,6 +131,7 @@ struct EquationsSet {
     DevBuf<double> work;        // vectors
     DevBuf<double> precond;     // preconditioner storage
     DevBuf<double> partials;    // reduction partial sums
+    DevBuf<int32_t> pstart;     // first pressure column of each displacement row
     int krylov_capacity = 0;
 };
 

@@ -210,6 +210,54 @@ __global__ void __launch_bounds__(256) apply_jacobi_kernel(int n, const double *
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) z[i] = inv[i] * r[i];
 }
 
+// Block upper-triangular application, displacement part, one thread per node:
+// t = r_u - B^T z_p over the pressure columns of the node's three rows
+// (they start at pstart[row], columns being sorted), then z_u = Kblock^-1 t.
+// Fixed rows contribute 0.
+template <typename VT>
+__global__ void __launch_bounds__(256) triangular_u_kernel(int nq, const int *__restrict__ row_ptr,
+                                                          const int *__restrict__ pstart,
+                                                          const int *__restrict__ col_idx,
+                                                          const VT *__restrict__ vals,
+                                                          const uint8_t *__restrict__ free_mask,
+                                                          const double *__restrict__ inv_u,
+                                                          const double *__restrict__ r, double *__restrict__ z)
+{
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nq) return;
+    double t[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int row = a * nq + node;
+        double acc = 0.0;
+        if (free_mask[row]) {
+            acc = r[row];
+            for (int k = pstart[row]; k < row_ptr[row + 1]; ++k) acc -= static_cast<double>(vals[k]) * z[col_idx[k]];
+        }
+        t[a] = acc;
+    }
+    const double *o = inv_u + static_cast<int64_t>(node) * 9;
+    z[node] = o[0] * t[0] + o[1] * t[1] + o[2] * t[2];
+    z[nq + node] = o[3] * t[0] + o[4] * t[1] + o[5] * t[2];
+    z[2 * nq + node] = o[6] * t[0] + o[7] * t[1] + o[8] * t[2];
+}
+
+__global__ void __launch_bounds__(256) pressure_start_kernel(int n_u_rows, int first_p_col,
+                                                            const int *__restrict__ row_ptr,
+                                                            const int *__restrict__ col_idx,
+                                                            int *__restrict__ pstart)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_u_rows) return;
+    int lo = row_ptr[r], hi = row_ptr[r + 1];
+    while (lo < hi) {  // first k with col_idx[k] >= first_p_col
+        const int mid = (lo + hi) >> 1;
+        if (col_idx[mid] < first_p_col) lo = mid + 1;
+        else hi = mid;
+    }
+    pstart[r] = lo;
+}
+
 __global__ void __launch_bounds__(256) apply_block_kernel(int nq, int nl, const double *__restrict__ inv_u,
                                                          const double *__restrict__ inv_p,
                                                          const double *__restrict__ r, double *__restrict__ z)
@@ -251,6 +299,7 @@ struct Krylov {
     double *V;               // (m+1) x n
     double *r, *z, *w, *u;   // work vectors
     double *inv_u, *inv_p, *diag_u, *inv_pt;  // preconditioner
+    const int *pstart = nullptr;
     int precond = OC_PRECONDITIONER_BLOCK_DIAGONAL;
     bool fp32 = false;
     std::vector<double> hpart;
@@ -322,15 +371,17 @@ struct Krylov {
         }
         const int nq = s.n_nodes, nl = s.n_nodes_l;
         if (precond == OC_PRECONDITIONER_BLOCK_TRIANGULAR) {
-            // tmp = [0; S^-1 r_p]; u = J tmp (u-rows give B^T z_p); z_u = K^-1 (r_u - u_u)
-            OC_HIP_CHECK(hipMemsetAsync(inv_pt, 0, sizeof(double) * 3 * nq, rt.stream));
-            apply_jacobi_kernel<<<grid_for(nl), 256, 0, rt.stream>>>(nl, inv_p, rin + 3 * nq, inv_pt + 3 * nq);
-            OC_CHECK(spmv(inv_pt, u));
-            axpby_kernel<<<grid_for(3 * nq), 256, 0, rt.stream>>>(3 * nq, 1.0, rin, -1.0, u);
-            hipMemcpyAsync(u + 3 * nq, rin + 3 * nq, sizeof(double) * nl, hipMemcpyDeviceToDevice, rt.stream);
-            mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, u);
-            apply_block_kernel<<<static_cast<unsigned>((std::max(nq, nl) + 255) / 256), 256, 0, rt.stream>>>(
-                nq, nl, inv_u, inv_p, u, zout);
+            // z_p = S^-1 r_p ; z_u = Kblock^-1 (r_u - B^T z_p)
+            apply_jacobi_kernel<<<grid_for(nl), 256, 0, rt.stream>>>(nl, inv_p, rin + 3 * nq, zout + 3 * nq);
+            const unsigned g = static_cast<unsigned>((nq + 255) / 256);
+            if (fp32)
+                triangular_u_kernel<float><<<g, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, pstart, s.col_idx.ptr,
+                                                                     s.values_sp.ptr, s.free_mask.ptr, inv_u, rin,
+                                                                     zout);
+            else
+                triangular_u_kernel<double><<<g, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, pstart, s.col_idx.ptr,
+                                                                      s.values.ptr, s.free_mask.ptr, inv_u, rin,
+                                                                      zout);
         } else {
             apply_block_kernel<<<static_cast<unsigned>((std::max(nq, nl) + 255) / 256), 256, 0, rt.stream>>>(
                 nq, nl, inv_u, inv_p, rin, zout);
@@ -390,7 +441,7 @@ struct Krylov {
         double beta = 0.0;
         OC_CHECK(norm(b, &beta));
         double res = beta;
-        std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1), h2(m + 1);
+        std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 2), h2(m + 2);
         bool first = true;
         while (total < max_iter && res > tol_abs) {
             // r = b - A x
@@ -416,22 +467,26 @@ struct Krylov {
                 mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, z);
                 OC_CHECK(spmv(z, vn));
                 mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, vn);
-                // classical Gram-Schmidt, twice (CGS2)
-                double wn2 = 0.0;
-                OC_CHECK(multi_dot(vn, 1, vn, &wn2));
-                OC_CHECK(multi_dot(V, j + 1, vn, h.data()));
+                // classical Gram-Schmidt: one fused reduction gives V_0..V_j . w
+                // and w . w (V_{j+1} is w itself); the norm after projection
+                // follows by Pythagoras unless cancellation asks for a second
+                // pass (CGS2) and an explicit norm.
+                OC_CHECK(multi_dot(V, j + 2, vn, h.data()));
+                const double wn2 = h[j + 1];
                 std::vector<double> hc(h.begin(), h.begin() + j + 1);
                 OC_CHECK(combine(V, j + 1, hc, -1.0, vn));
                 double proj2 = 0.0;
                 for (int i = 0; i <= j; ++i) proj2 += h[i] * h[i];
+                double hn = 0.0;
                 if (wn2 - proj2 < 0.25 * wn2) {
                     OC_CHECK(multi_dot(V, j + 1, vn, h2.data()));
                     std::vector<double> hc2(h2.begin(), h2.begin() + j + 1);
                     OC_CHECK(combine(V, j + 1, hc2, -1.0, vn));
                     for (int i = 0; i <= j; ++i) h[i] += h2[i];
+                    OC_CHECK(norm(vn, &hn));
+                } else {
+                    hn = std::sqrt(wn2 - proj2);
                 }
-                double hn = 0.0;
-                OC_CHECK(norm(vn, &hn));
                 h[j + 1] = hn;
                 if (hn > 0.0) copy_scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0 / hn, vn, vn);
                 // Givens
@@ -513,6 +568,14 @@ static int solver_prepare(EquationsSet &s, int restart, Krylov &K)
         K.diag_u = K.inv_p + n;
         K.inv_pt = K.diag_u + n;
         if (K.precond == OC_PRECONDITIONER_JACOBI) K.inv_u = K.inv_pt;  // n-sized point Jacobi
+        if (!s.pstart.ptr) {
+            const int nu = 3 * s.n_nodes;
+            OC_CHECK(s.pstart.alloc(nu));
+            pressure_start_kernel<<<static_cast<unsigned>((nu + 255) / 256), 256, 0, runtime().stream>>>(
+                nu, nu, s.row_ptr.ptr, s.col_idx.ptr, s.pstart.ptr);
+            OC_HIP_CHECK(hipGetLastError());
+        }
+        K.pstart = s.pstart.ptr;
     }
     return OC_NO_ERROR;
 }

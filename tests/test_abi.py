@@ -46,3 +46,16 @@ def test_error_paths_without_device():
         _lib.call("oc_equations_set_destroy", None)
     buf = ctypes.create_string_buffer(4)
     assert lib.oc_last_error(buf, 4) == 0 and len(buf.value) <= 3
+
+
+FORTRAN_DRIVER = os.path.join(ROOT, "tests", "fortran", "drive_laplace")
+
+
+@pytest.mark.skipif(not os.path.exists(FORTRAN_DRIVER), reason="Fortran driver not built (no amdflang)")
+def test_fortran_host_reaches_the_abi():
+    """The ISO_C_BINDING module (include/oc_asm.f90) binds the C ABI: a Fortran
+    program gets the status code and the error message of a bad call."""
+    import subprocess
+    out = subprocess.run([FORTRAN_DRIVER], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert "null handle: -1 equations set handle is NULL" in out.stdout
