@@ -4,11 +4,10 @@
 //
 // Jacobian: mr_gauss_kernel (one thread per element Gauss point, all
 // elements) evaluates the FP64 kinematics and tangent into Gauss-point
-// records (field-major, see GpField); mr_jacobian_element_kernel contracts
-// the records of each element of one colour into the 89x89 element Jacobian
-// and scatters it through the precomputed slot-offset map.  Elements of one
-// colour share no DOF, so each launch writes every CSR entry at most once and
-// the colour order fixes the summation order.
+// records (field-major, see GpField); mr_element_matrix_kernel contracts the
+// records of every element into its 89x89 element matrix on MFMA; and
+// csr_row_gather_kernel assembles each CSR row from the element rows that
+// touch it in ascending element order (deterministic, write-once).
 // Residual: mr_residual_element_kernel evaluates element residuals of all
 // elements into a buffer and dof_gather_kernel sums them per DOF in
 // ascending element order.
@@ -118,8 +117,9 @@ int mr_prepare(EquationsSet &s)
     if (s.slot_offsets.ptr) return OC_NO_ERROR;
     int max_row = 0;
     for (int r = 0; r < s.n_rows; ++r) max_row = std::max(max_row, s.h_row_ptr[r + 1] - s.h_row_ptr[r]);
-    if (max_row > 65535)
-        return set_error(OC_ERR_UNSUPPORTED, "CSR row of %d entries exceeds the 16-bit scatter map", max_row);
+    if (max_row > 8192)
+        return set_error(OC_ERR_UNSUPPORTED, "CSR row of %d entries exceeds the row-gather capacity", max_row);
+    s.max_row_len = max_row;
     OC_CHECK(s.slot_offsets.alloc(static_cast<size_t>(s.n_elements) * 89 * 89));
     DevBuf<int> missing;
     OC_CHECK(missing.alloc(1));
@@ -288,161 +288,205 @@ int launch_mr_residual(EquationsSet &s)
 }
 
 // ---------------------------------------------------------------------------
-// Stage 2: analytic Jacobian as
-//   K_(a,m),(b,n) = sum_g sum_i D_g[i][m] T_g^{ab}[i][n],
-//   T_g^{ab}[i][n] = sum_j W_g[(a,i),(b,j)] D_g[j][n],
+// Analytic Jacobian (FiniteElasticity_FiniteElementJacobianEvaluate,
+// finite_elasticity_routines.f90:842-1175) in two phases.
+//
+// Phase A (mr_element_matrix_kernel<Real>, all elements, no colours): for each
+// of the six lower displacement blocks (a, b), b <= a,
+//   K_(a,m),(b,n) = sum_(g,j) U_g^{ab}[m][j] D_g[j][n],
+//   U_g^{ab}[m][j] = sum_i D_g[i][m] W_g[(a,i),(b,j)],
 //   W_g[(a,i),(b,j)] = J_z w_g (c(V(i,a),V(j,b)) + delta_ab sigma_ij),
-// the reference's TEMPVEC / DOT_PRODUCT contraction (:1001-1037); pressure
-// rows K_(p,m),(b,n) = sum_g J_z w_g D_g[b][n] psi_m (:1039-1054); the
-// off-diagonal and pressure blocks are mirrored (:1149-1153).  The
-// contraction runs in Real (double, or float for FP32 Jacobians).
-// 162 threads own 3x9 tiles of the six lower displacement blocks, 81 own a
-// pressure column.  Persistent blocks keep the basis table in LDS.  Every
-// entry of one element has its own CSR slot, so the scatter issues all loads
-// of a batch before its stores.
+// i.e. the reference's TEMPVEC / DOT_PRODUCT contraction (:1001-1037)
+// re-associated as a 27x81 by 81x27 GEMM on MFMA (16x16x4, one wave per 16x16
+// output tile); pressure rows K_(p,m),(b,n) = sum_g (J_z w_g psi_m) D_g[b][n]
+// (:1039-1054) as an 8x27 by 27x81 GEMM.  Lower triangle kept, mirrored
+// (:1149-1153); the element matrix goes to an HBM buffer, row-major.
+// Phase B (csr_row_gather_kernel<Real>): one wavefront per CSR row adds the
+// element rows that touch it, in ascending element order, into an LDS copy of
+// the row and writes the row once (no zero fill, no read-modify-write).
+typedef float mfma_f32x4 __attribute__((ext_vector_type(4)));
+typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename Real>
+struct Mfma;
+
+template <>
+struct Mfma<float> {
+    using acc_t = mfma_f32x4;
+    __device__ static acc_t step(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    // output row of accumulator register r for this lane (column = lane % 16)
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <>
+struct Mfma<double> {
+    using acc_t = mfma_f64x4;
+    __device__ static acc_t step(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+constexpr int kKPad = 84;   // (g, j) = 81 padded to 21 MFMA k-steps of 4
+constexpr int kMPad = 32;   // 27 rows/columns padded to two 16-wide tiles
+constexpr int kPPad = 28;   // 27 Gauss points padded to 7 k-steps
+
 template <typename Real>
 __global__ void __launch_bounds__(256)
-mr_jacobian_element_kernel(const int *__restrict__ colour_elems, int n_in_colour, int n_elements,
-                           const double *__restrict__ rec, const int *__restrict__ conn_q,
-                           const int *__restrict__ conn_l, int nq, const int *__restrict__ row_ptr,
-                           const uint16_t *__restrict__ offs, const BasisTables *__restrict__ T,
-                           Real *__restrict__ values)
+mr_element_matrix_kernel(int n_elements, const double *__restrict__ rec, const BasisTables *__restrict__ T,
+                         Real *__restrict__ kbuf)
 {
-    __shared__ double s_df[27 * 27 * 3];
-    __shared__ Real s_psi[27 * 8];
-    __shared__ double s_dx[9][27];
-    __shared__ double s_c[36][27];
-    __shared__ double s_sig[6][27];
-    __shared__ double s_jgw[27];
-    __shared__ Real s_D[27][81];
+    // The Gauss-point records are only read while D and W are formed, so they
+    // share storage with the A operand (whose padding is re-zeroed after).
+    constexpr int kA = kKPad * kMPad;
+    constexpr int kUnion = (52 * 27 * 8 > kA * static_cast<int>(sizeof(Real))) ? 52 * 27 * 8
+                                                                                 : kA * static_cast<int>(sizeof(Real));
+    __shared__ __align__(16) unsigned char s_union[kUnion];
+    double(*s_rec)[27] = reinterpret_cast<double(*)[27]>(s_union);
+    Real *s_A = reinterpret_cast<Real *>(s_union);  // A[(g,j)][m] = U_g^{ab}[m][j] (k-major)
     __shared__ Real s_W[27][81];
-    __shared__ Real s_T[6][81];
-    __shared__ int s_base[89];
+    // B[(g,j)][n] = D_g[j][n], n padded to 32.  Read with stride 96 it is also
+    // the pressure B operand B[g][(b,n)] = D_g[b][n].
+    __shared__ Real s_B[kA];
+    __shared__ Real s_Pa[kPPad * 16];     // pressure A[g][m] = J_z w_g psi_m(g)
+    using M = Mfma<Real>;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int blk_a[6] = {0, 1, 1, 2, 2, 2}, blk_b[6] = {0, 0, 1, 0, 1, 2};
-    const int t = threadIdx.x;
-    for (int k = t; k < 27 * 27 * 3; k += 256) s_df[k] = T->quad27[(k / 3) * 4 + 1 + k % 3];
-    for (int k = t; k < 27 * 8; k += 256) s_psi[k] = static_cast<Real>(T->lin27[k * 4]);
     const int64_t n_gp = static_cast<int64_t>(n_elements) * 27;
+    // padding rows/columns of B and Pa stay zero for every element
+    for (int k = t; k < kA; k += 256) s_B[k] = Real(0);
+    for (int k = t; k < kPPad * 16; k += 256) s_Pa[k] = Real(0);
 
-    const bool is_uu = t < 162, is_p = t >= 162 && t < 243;
-    int blk = 0, m0 = 0, n0 = 0, pb = 0, pn = 0;
-    if (is_uu) {
-        blk = t / 27;
-        const int r = t % 27;
-        m0 = (r / 3) * 3;
-        n0 = (r % 3) * 9;
-    } else if (is_p) {
-        pb = (t - 162) / 27;
-        pn = (t - 162) % 27;
-    }
-
-    for (int idx = blockIdx.x; idx < n_in_colour; idx += gridDim.x) {
-        const int e = colour_elems[idx];
+    for (int e = blockIdx.x; e < n_elements; e += gridDim.x) {
         const int64_t q0 = static_cast<int64_t>(e) * 27;
+        Real *Ke = kbuf + static_cast<int64_t>(e) * 89 * 89;
         __syncthreads();
-        for (int k = t; k < 52 * 27; k += 256) {
-            const int f = k / 27, g = k % 27;
-            const double v = rec[f * n_gp + q0 + g];
-            if (f < 9) s_dx[f][g] = v;
-            else if (f == GP_JGW) s_jgw[g] = v;
-            else if (f < GP_SIGJ) s_c[f - GP_C][g] = v;
-            else s_sig[f - GP_SIGJ][g] = v;
-        }
-        if (t < 89) s_base[t] = row_ptr[element_dof(conn_q, conn_l, nq, e, t)];
+        for (int k = t; k < 52 * 27; k += 256) s_rec[k / 27][k % 27] = rec[(k / 27) * n_gp + q0 + (k % 27)];
         __syncthreads();
         for (int k = t; k < 27 * 81; k += 256) {
             const int g = k / 81, r = k % 81;
-            {  // D_g[i][n] = sum_mi DXI_DX(mi, i) dphi_n/dxi_mi (:974-986)
-                const int i = r / 27, n = r % 27;
-                const double *f = s_df + (g * 27 + n) * 3;
-                s_D[g][r] = static_cast<Real>(s_dx[0 + 3 * i][g] * f[0] + s_dx[1 + 3 * i][g] * f[1] +
-                                              s_dx[2 + 3 * i][g] * f[2]);
+            {  // D_g[j][n] = sum_mi DXI_DX(mi, j) dphi_n/dxi_mi (:974-986)
+                const int j = r / 27, n = r % 27;
+                const double *f = T->quad27 + (g * 27 + n) * 4;
+                const Real d = static_cast<Real>(s_rec[GP_DXIDZ + 0 + 3 * j][g] * f[1] +
+                                                 s_rec[GP_DXIDZ + 1 + 3 * j][g] * f[2] +
+                                                 s_rec[GP_DXIDZ + 2 + 3 * j][g] * f[3]);
+                s_B[(g * 3 + j) * kMPad + n] = d;
             }
             {  // W_g[(a,i),(b,j)]
                 const int row = r / 9, col = r % 9;
                 const int a = row / 3, i = row % 3, b = col / 3, j = col % 3;
-                double v = s_c[t2v(i, a) + 6 * t2v(j, b)][g];
-                if (a == b) v = v + s_sig[t2v(i, j)][g];
-                s_W[g][r] = static_cast<Real>(s_jgw[g] * v);
+                double v = s_rec[GP_C + t2v(i, a) + 6 * t2v(j, b)][g];
+                if (a == b) v = v + s_rec[GP_SIGJ + t2v(i, j)][g];
+                s_W[g][r] = static_cast<Real>(s_rec[GP_JGW][g] * v);
             }
+        }
+        for (int k = t; k < 27 * 8; k += 256) {
+            const int g = k / 8, m = k % 8;
+            s_Pa[g * 16 + m] = static_cast<Real>(s_rec[GP_JGW][g] * T->lin27[(g * 8 + m) * 4]);
         }
         __syncthreads();
-
-        Real acc[27];
-#pragma unroll
-        for (int i = 0; i < 27; ++i) acc[i] = Real(0);
-        for (int g = 0; g < 27; ++g) {
-            for (int q = t; q < 486; q += 256) {
-                const int bq = q / 81, rem = q % 81, i = rem / 27, n = rem % 27;
-                const int a = blk_a[bq], b = blk_b[bq];
-                const Real *w = &s_W[g][(a * 3 + i) * 9 + b * 3];
-                s_T[bq][i * 27 + n] = w[0] * s_D[g][n] + w[1] * s_D[g][27 + n] + w[2] * s_D[g][54 + n];
+        // records consumed: zero the padding of the A operand they overlapped
+        for (int k = t; k < kA; k += 256)
+            if (k >= 81 * kMPad || (k % kMPad) >= 27) s_A[k] = Real(0);
+        __syncthreads();
+        for (int bq = 0; bq < 6; ++bq) {
+            const int a = blk_a[bq], b = blk_b[bq];
+            // U_g^{ab}[m][j] = sum_i D_g[i][m] W_g[(a,i),(b,j)] into s_A[(g,j)][m]
+            for (int k = t; k < 27 * 81; k += 256) {
+                const int g = k / 81, r = k % 81, j = r / 27, m = r % 27;
+                const Real *Dm = s_B + (g * 3) * kMPad + m;
+                const Real *w = &s_W[g][(a * 3) * 9 + b * 3 + j];
+                s_A[(g * 3 + j) * kMPad + m] = Dm[0] * w[0] + Dm[kMPad] * w[9] + Dm[2 * kMPad] * w[18];
             }
             __syncthreads();
-            if (is_uu) {
+            {   // wave -> 16x16 tile (mt, nt) of the 32x32 padded block
+                const int mt = wave >> 1, nt = wave & 1;
+                typename M::acc_t acc = {Real(0), Real(0), Real(0), Real(0)};
+                const int mi = mt * 16 + (lane & 15), ni = nt * 16 + (lane & 15), kk = lane >> 4;
+#pragma unroll 7
+                for (int ks = 0; ks < kKPad / 4; ++ks) {
+                    const int k = ks * 4 + kk;
+                    acc = M::step(s_A[k * kMPad + mi], s_B[k * kMPad + ni], acc);
+                }
 #pragma unroll
-                for (int mm = 0; mm < 3; ++mm) {
-                    const Real d0 = s_D[g][m0 + mm], d1 = s_D[g][27 + m0 + mm], d2 = s_D[g][54 + m0 + mm];
-#pragma unroll
-                    for (int nn = 0; nn < 9; ++nn) {
-                        const Real dot = d0 * s_T[blk][n0 + nn] + d1 * s_T[blk][27 + n0 + nn] +
-                                         d2 * s_T[blk][54 + n0 + nn];
-                        acc[mm * 9 + nn] = acc[mm * 9 + nn] + dot;
+                for (int r = 0; r < 4; ++r) {
+                    const int m = mt * 16 + M::row(lane, r), n = ni;
+                    if (m >= 27 || n >= 27) continue;
+                    const int i = a * 27 + m, j = b * 27 + n;
+                    const Real v = acc[r];
+                    if (a != b) {
+                        Ke[i * 89 + j] = v;
+                        Ke[j * 89 + i] = v;
+                    } else if (m >= n) {  // lower triangle, mirrored (:1149-1153)
+                        Ke[i * 89 + j] = v;
+                        Ke[j * 89 + i] = v;
                     }
                 }
-            } else if (is_p) {
-                const Real jd = static_cast<Real>(s_jgw[g]) * s_D[g][pb * 27 + pn];
-#pragma unroll
-                for (int m = 0; m < 8; ++m) acc[m] = acc[m] + jd * s_psi[g * 8 + m];
             }
             __syncthreads();
         }
+        // pressure rows: 16 (8 used) x 96 padded, K = 28; waves 0..3 take tiles nt = wave, wave + 4 (< 6)
+        for (int nt = wave; nt < 6; nt += 4) {
+            typename M::acc_t acc = {Real(0), Real(0), Real(0), Real(0)};
+            const int mi = lane & 15, ni = nt * 16 + (lane & 15), kk = lane >> 4;
+#pragma unroll
+            for (int ks = 0; ks < kPPad / 4; ++ks) {
+                const int k = ks * 4 + kk;
+                acc = M::step(s_Pa[k * 16 + mi], s_B[k * 96 + ni], acc);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = M::row(lane, r), bn = ni, bb = bn / 32, n = bn % 32;
+                if (m >= 8 || n >= 27) continue;
+                const int i = 81 + m, j = bb * 27 + n;
+                Ke[i * 89 + j] = acc[r];
+                Ke[j * 89 + i] = acc[r];
+            }
+        }
+        if (t < 64) Ke[(81 + t / 8) * 89 + 81 + t % 8] = Real(0);  // no p-p block
+    }
+}
 
-        const uint16_t *eo = offs + static_cast<int64_t>(e) * 89 * 89;
-        if (is_uu) {
-            const int a = blk_a[blk], b = blk_b[blk];
-            int slot[27];
-            Real old[27];
+template <typename Real>
+__global__ void __launch_bounds__(64)
+csr_row_gather_kernel(int n_rows, const int *__restrict__ row_ptr, const int *__restrict__ gptr,
+                      const int *__restrict__ gidx, const Real *__restrict__ kbuf,
+                      const uint16_t *__restrict__ offs, Real *__restrict__ values)
+{
+    extern __shared__ unsigned char s_raw[];
+    Real *s_row = reinterpret_cast<Real *>(s_raw);
+    const int r = blockIdx.x, lane = threadIdx.x;
+    if (r >= n_rows) return;
+    const int r0 = row_ptr[r], len = row_ptr[r + 1] - r0;
+    for (int k = lane; k < len; k += 64) s_row[k] = Real(0);
+    __syncthreads();
+    // contributions in chunks of 8 elements: all global loads of a chunk are
+    // issued before its LDS accumulation (lane j holds entries j and j + 64)
+    constexpr int kChunk = 8;
+    for (int c0 = gptr[r]; c0 < gptr[r + 1]; c0 += kChunk) {
+        const int nc = min(kChunk, gptr[r + 1] - c0);
+        Real v0[kChunk], v1[kChunk];
+        int o0[kChunk], o1[kChunk];
 #pragma unroll
-            for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-                for (int nn = 0; nn < 9; ++nn) {
-                    const int i = a * 27 + m0 + mm, j = b * 27 + n0 + nn;
-                    slot[mm * 9 + nn] = s_base[i] + eo[i * 89 + j];
-                }
-#pragma unroll
-            for (int k = 0; k < 27; ++k) old[k] = values[slot[k]];
-#pragma unroll
-            for (int k = 0; k < 27; ++k) values[slot[k]] = old[k] + acc[k];
-            if (a != b) {
-#pragma unroll
-                for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-                    for (int nn = 0; nn < 9; ++nn) {
-                        const int i = a * 27 + m0 + mm, j = b * 27 + n0 + nn;
-                        slot[mm * 9 + nn] = s_base[j] + eo[j * 89 + i];
-                    }
-#pragma unroll
-                for (int k = 0; k < 27; ++k) old[k] = values[slot[k]];
-#pragma unroll
-                for (int k = 0; k < 27; ++k) values[slot[k]] = old[k] + acc[k];
+        for (int c = 0; c < kChunk; ++c) {
+            if (c < nc) {
+                const int64_t base = static_cast<int64_t>(gidx[c0 + c]) * 89;  // (e*89 + i) * 89
+                v0[c] = kbuf[base + lane];
+                o0[c] = offs[base + lane];
+                const bool hi = lane + 64 < 89;
+                v1[c] = hi ? kbuf[base + lane + 64] : Real(0);
+                o1[c] = hi ? offs[base + lane + 64] : -1;
             }
-        } else if (is_p) {
-            const int j = pb * 27 + pn;
-            int slot[16];
-            Real old[16];
+        }
 #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                const int i = 81 + m;
-                slot[m] = s_base[i] + eo[i * 89 + j];
-                slot[8 + m] = s_base[j] + eo[j * 89 + i];
+        for (int c = 0; c < kChunk; ++c) {
+            if (c < nc) {
+                s_row[o0[c]] += v0[c];
+                if (o1[c] >= 0) s_row[o1[c]] += v1[c];
             }
-#pragma unroll
-            for (int k = 0; k < 16; ++k) old[k] = values[slot[k]];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) values[slot[k]] = old[k] + acc[k & 7];
+            __syncthreads();
         }
     }
+    for (int k = lane; k < len; k += 64) values[r0 + k] = s_row[k];
 }
 
 int launch_mr_jacobian(EquationsSet &s, int precision)
@@ -452,23 +496,34 @@ int launch_mr_jacobian(EquationsSet &s, int precision)
     OC_CHECK(ensure_tables(&T));
     Runtime &rt = runtime();
     OC_CHECK(launch_gauss(s));
-    if (precision == OC_FP64) OC_CHECK(fill<double>(s.values.ptr, s.nnz, 0.0));
-    else OC_CHECK(fill<float>(s.values_sp.ptr, s.nnz, 0.0f));
-    const int ncol = static_cast<int>(s.colour_ptr.size()) - 1;
-    for (int c = 0; c < ncol; ++c) {
-        const int n = s.colour_ptr[c + 1] - s.colour_ptr[c];
-        if (n == 0) continue;
-        const int *ce = s.colour_elems.ptr + s.colour_ptr[c];
-        const unsigned blocks = static_cast<unsigned>(std::min(n, rt.num_cus * 2));
-        KernelTimer timer(precision == OC_FP64 ? "mr_jacobian_fp64" : "mr_jacobian_fp32");
-        if (precision == OC_FP64)
-            mr_jacobian_element_kernel<double><<<blocks, 256, 0, rt.stream>>>(
-                ce, n, s.n_elements, s.gp_records.ptr, s.conn.ptr, s.conn_l.ptr, s.n_nodes, s.row_ptr.ptr,
-                s.slot_offsets.ptr, T, s.values.ptr);
-        else
-            mr_jacobian_element_kernel<float><<<blocks, 256, 0, rt.stream>>>(
-                ce, n, s.n_elements, s.gp_records.ptr, s.conn.ptr, s.conn_l.ptr, s.n_nodes, s.row_ptr.ptr,
-                s.slot_offsets.ptr, T, s.values_sp.ptr);
+    const size_t kn = static_cast<size_t>(s.n_elements) * 89 * 89;
+    const unsigned blocks = static_cast<unsigned>(std::min(s.n_elements, rt.num_cus * 4));
+    const size_t row_lds = static_cast<size_t>(s.max_row_len) * (precision == OC_FP64 ? 8 : 4);
+    if (precision == OC_FP64) {
+        if (!s.kbuf64.ptr) OC_CHECK(s.kbuf64.alloc(kn));
+        {
+            KernelTimer timer("mr_element_matrix_fp64");
+            mr_element_matrix_kernel<double><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
+                                                                           s.kbuf64.ptr);
+            OC_HIP_CHECK(hipGetLastError());
+        }
+        KernelTimer timer("csr_row_gather_fp64");
+        csr_row_gather_kernel<double><<<static_cast<unsigned>(s.n_rows), 64, row_lds, rt.stream>>>(
+            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.gather_idx.ptr, s.kbuf64.ptr, s.slot_offsets.ptr,
+            s.values.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+    } else {
+        if (!s.kbuf32.ptr) OC_CHECK(s.kbuf32.alloc(kn));
+        {
+            KernelTimer timer("mr_element_matrix_fp32");
+            mr_element_matrix_kernel<float><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
+                                                                          s.kbuf32.ptr);
+            OC_HIP_CHECK(hipGetLastError());
+        }
+        KernelTimer timer("csr_row_gather_fp32");
+        csr_row_gather_kernel<float><<<static_cast<unsigned>(s.n_rows), 64, row_lds, rt.stream>>>(
+            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.gather_idx.ptr, s.kbuf32.ptr, s.slot_offsets.ptr,
+            s.values_sp.ptr);
         OC_HIP_CHECK(hipGetLastError());
     }
     return OC_NO_ERROR;

@@ -115,6 +115,9 @@ struct EquationsSet {
     // i and element column j, slot = row_ptr[dof_i] + slot_offsets[(e*89+i)*89+j].
     DevBuf<double> gp_records;
     DevBuf<uint16_t> slot_offsets;
+    DevBuf<double> kbuf64;      // element matrices, E*89*89, row-major
+    DevBuf<float> kbuf32;
+    int max_row_len = 0;
     // element residuals (E*89) and the DOF -> element-entry gather lists
     DevBuf<double> relem;
     DevBuf<int32_t> gather_ptr, gather_idx;
