@@ -124,6 +124,17 @@ SIGNATURES = {
     "oc_newton_solve": [_H, ctypes.POINTER(NewtonOptions), ctypes.POINTER(NewtonStats)],
     "oc_linear_solve": [_H, ctypes.POINTER(NewtonOptions), ctypes.POINTER(NewtonStats)],
     "oc_residual_norm": [_H, ctypes.POINTER(ctypes.c_double)],
+    "oc_mlp_create": [_c_int, _c_int, _c_int, _c_int, ctypes.c_ulonglong, ctypes.POINTER(_H)],
+    "oc_mlp_destroy": [_H],
+    "oc_mlp_parameter_count": [_H, ctypes.POINTER(_c_int)],
+    "oc_mlp_parameters_get": [_H, _fp],
+    "oc_mlp_parameters_set": [_H, _fp],
+    "oc_mlp_normalisation_set": [_H, _fp, _fp],
+    "oc_mlp_forward": [_H, _c_int, _fp, _fp],
+    "oc_mlp_gradient": [_H, _c_int, _fp, _fp, ctypes.POINTER(_c_float), _fp],
+    "oc_mlp_dataset_set": [_H, _c_int, _fp, _fp],
+    "oc_mlp_train": [_H, _c_int, _c_int, _c_float, ctypes.c_ulonglong, ctypes.POINTER(_c_float)],
+    "oc_mlp_warm_start": [_H, _H, _c_double],
     "oc_kernel_timing_enable": [_c_int],
     "oc_kernel_timing_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
 }

@@ -196,6 +196,36 @@ int oc_linear_solve(oc_equations_set equations_set, const oc_newton_options *opt
 /* Euclidean norm of the residual over the free DOFs at the current DOFs. */
 int oc_residual_norm(oc_equations_set equations_set, double *norm);
 
+/* ---- learned warm start (no counterpart in the reference) ------------------
+ * A per-node MLP predicts each load step's nodal displacement increment as
+ * Newton's starting guess; the plain Newton solve stays the judge.  Inputs
+ * per quadratic node (9): X (3), current displacement z - X (3), load
+ * increment, c10, c01; output (3): displacement increment per unit load
+ * increment.  FP32 parameters, trained with Adam by the library's kernels. */
+typedef struct oc_mlp_s *oc_mlp;
+int oc_mlp_create(int n_inputs, int n_hidden, int n_hidden_layers, int n_outputs, unsigned long long seed,
+                  oc_mlp *mlp);
+int oc_mlp_destroy(oc_mlp mlp);
+int oc_mlp_parameter_count(oc_mlp mlp, int *count);
+int oc_mlp_parameters_get(oc_mlp mlp, float *parameters);
+int oc_mlp_parameters_set(oc_mlp mlp, const float *parameters);
+/* Per-feature affine normalisation x' = (x - shift) * scale (n_inputs each). */
+int oc_mlp_normalisation_set(oc_mlp mlp, const float *shift, const float *scale);
+/* y = MLP(x) for n samples (host arrays, x: n*n_inputs, y: n*n_outputs). */
+int oc_mlp_forward(oc_mlp mlp, int n, const float *x, float *y);
+/* Loss 1/(n*n_out) sum (y - t)^2 and its gradient w.r.t. the parameters
+ * (host arrays; gradient in parameter order). */
+int oc_mlp_gradient(oc_mlp mlp, int n, const float *x, const float *t, float *loss, float *gradient);
+/* Uploads a training set (n samples) to the device. */
+int oc_mlp_dataset_set(oc_mlp mlp, int n, const float *x, const float *t);
+/* `steps` Adam steps on minibatches drawn (seeded) from the device training
+ * set; returns the mean loss of the last `steps` minibatches. */
+int oc_mlp_train(oc_mlp mlp, int steps, int batch, float learning_rate, unsigned long long seed, float *mean_loss);
+/* Warm start of a Mooney-Rivlin set: z += load_increment * MLP(features) on
+ * every free displacement DOF, features built on the device from the set's
+ * coordinates and current DOFs.  Call before oc_load_increment_apply. */
+int oc_mlp_warm_start(oc_mlp mlp, oc_equations_set equations_set, double load_increment);
+
 /* ---- timing on the library stream ----------------------------------------- */
 /* Records event `slot` (0..63) on the library stream. */
 int oc_event_record(int slot);
