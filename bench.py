@@ -95,6 +95,28 @@ def laplace_line(cm, n, steps):
     return out
 
 
+def newton_line(cm, n, load_steps):
+    """Time to 1e-10 relative residual per load step, plain FP64 Newton versus
+    learned warm start + FP32 Jacobian, 10% uniaxial stretch over `load_steps`."""
+    from cm_amd.train_warm_start import DEFAULT_WEIGHTS, solve_steps
+    from cm_amd.warm_start import WarmStartMLP
+    case = dict(n=n, c10=2.0, c01=6.0, stretch=0.10, seed=2024)
+    plain = solve_steps(case, load_steps)
+    net = WarmStartMLP.load(DEFAULT_WEIGHTS)
+    learned = solve_steps(case, load_steps, net=net, precision=cm.OC_FP32)
+    net.destroy()
+
+    def summary(rows):
+        return {"converged_steps": sum(r["converged"] for r in rows), "steps": len(rows),
+                "seconds_per_step": float(np.mean([r["seconds"] for r in rows])),
+                "newton_iterations": [r["newton"] for r in rows], "krylov_iterations": [r["krylov"] for r in rows],
+                "start_residual_over_plain_start": [r["start_norm"] / r["reference_norm"] for r in rows]}
+    p, l = summary(plain), summary(learned)
+    return {"workload": f"{n}^3 Mooney-Rivlin, c10=2, c01=6, 10% stretch in {load_steps} load steps, "
+                        "1e-10 x plain start residual, block-triangular GMRES",
+            "plain_fp64": p, "learned_fp32": l, "speedup": p["seconds_per_step"] / l["seconds_per_step"]}
+
+
 def cpu_baseline(mesh, dofs, c10, c01, seconds):
     """Oracle (C FP64 restatement of the reference loop) on this host, 1 thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -126,6 +148,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--load-steps", type=int, default=10)
     ap.add_argument("--profile", choices=["jacobian", "copy"], default=None,
                     help="run only one kernel sequence for rocprofv3 counter collection: the FP64 Jacobian "
                          "assembly (--steps times), or a device copy of 512 MiB (the FETCH_SIZE/WRITE_SIZE "
@@ -207,6 +231,8 @@ def main():
             "kernel_ms_per_step": {"mr_element_matrix_fp32": j32 / args.steps, "csr_row_gather_fp32": g32 / args.steps},
         }
         secondary["laplace"] = [laplace_line(cm, n, s) for n, s in ((8, 50), (64, 20), (128, 10))]
+    if not args.no_newton:
+        secondary["newton_per_load_step"] = newton_line(cm, args.n, args.load_steps)
 
     line = {
         "metric": "elements/s assembled; time-to-1e-10-residual per load step, plain vs learned",

@@ -105,6 +105,7 @@ struct EquationsSet {
     DevBuf<double> coords;   // 3 * n_nodes, separated
     DevBuf<int32_t> conn;    // nen * E
     DevBuf<int32_t> conn_l;  // 8 * E (MR pressure component)
+    DevBuf<int32_t> lin_to_quad;  // pressure node -> coincident quadratic node (warm start)
     DevBuf<int32_t> row_ptr, col_idx;
     DevBuf<double> values;
     DevBuf<float> values_sp;

@@ -197,29 +197,60 @@ __global__ void __launch_bounds__(256) warm_features_kernel(int nq, const double
                                                            double c10, double c01, const float *__restrict__ shift,
                                                            const float *__restrict__ scale, float *__restrict__ out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    float f[9];
-    for (int c = 0; c < 3; ++c) {
-        const double X = coords[static_cast<int64_t>(c) * nq + i];
-        f[c] = static_cast<float>(X);
-        f[3 + c] = static_cast<float>(dofs[static_cast<int64_t>(c) * nq + i] - X);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        float f[9];
+        for (int c = 0; c < 3; ++c) {
+            const double X = coords[static_cast<int64_t>(c) * nq + i];
+            f[c] = static_cast<float>(X);
+            f[3 + c] = static_cast<float>(dofs[static_cast<int64_t>(c) * nq + i] - X);
+        }
+        f[6] = static_cast<float>(dload);
+        f[7] = static_cast<float>(c10);
+        f[8] = static_cast<float>(c01);
+        for (int k = 0; k < 9; ++k) out[static_cast<int64_t>(i) * 9 + k] = (f[k] - shift[k]) * scale[k];
     }
-    f[6] = static_cast<float>(dload);
-    f[7] = static_cast<float>(c10);
-    f[8] = static_cast<float>(c01);
-    for (int k = 0; k < 9; ++k) out[static_cast<int64_t>(i) * 9 + k] = (f[k] - shift[k]) * scale[k];
 }
 
-__global__ void __launch_bounds__(256) warm_apply_kernel(int nq, const float *__restrict__ y, double dload,
+// z += dload * y[:, 0:3] on the free displacement DOFs of every quadratic node
+__global__ void __launch_bounds__(256) warm_apply_kernel(int nq, const float *__restrict__ y, int ldy, double dload,
                                                         const uint8_t *__restrict__ free_mask,
                                                         double *__restrict__ dofs)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    for (int c = 0; c < 3; ++c) {
-        const int64_t d = static_cast<int64_t>(c) * nq + i;
-        if (free_mask == nullptr || free_mask[d]) dofs[d] += dload * static_cast<double>(y[static_cast<int64_t>(i) * 3 + c]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        for (int c = 0; c < 3; ++c) {
+            const int64_t d = static_cast<int64_t>(c) * nq + i;
+            if (free_mask == nullptr || free_mask[d])
+                dofs[d] += dload * static_cast<double>(y[static_cast<int64_t>(i) * ldy + c]);
+        }
+    }
+}
+
+// quadratic node coinciding with each pressure (trilinear) node: element
+// corner k of the linear basis is local node 2*(k&1) + 6*((k>>1)&1) + 18*(k>>2)
+// of the triquadratic one (xi1 fastest in both).  Every writer of one entry
+// writes the same value.
+__global__ void __launch_bounds__(256) linear_to_quadratic_kernel(int n_elements, const int *__restrict__ conn_q,
+                                                                 const int *__restrict__ conn_l,
+                                                                 int *__restrict__ map)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n_elements * 8; t += gridDim.x * blockDim.x) {
+        const int e = t >> 3, k = t & 7;
+        const int a = 2 * (k & 1) + 6 * ((k >> 1) & 1) + 18 * (k >> 2);
+        map[conn_l[t]] = conn_q[static_cast<int64_t>(e) * 27 + a];
+    }
+}
+
+// p += dload * (c10 + c01) * y[quad(node), 3] on the free pressure DOFs (the
+// network's pressure output is in units of c10 + c01)
+__global__ void __launch_bounds__(256) warm_apply_pressure_kernel(int nq, int nl, const int *__restrict__ map,
+                                                                 const float *__restrict__ y, double dload,
+                                                                 const uint8_t *__restrict__ free_mask,
+                                                                 double *__restrict__ dofs)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+        const int64_t d = static_cast<int64_t>(3) * nq + i;
+        if (free_mask == nullptr || free_mask[d])
+            dofs[d] += dload * static_cast<double>(y[static_cast<int64_t>(map[i]) * 4 + 3]);
     }
 }
 
@@ -540,9 +571,9 @@ int oc_mlp_warm_start(oc_mlp mlp, oc_equations_set h, double load_increment)
     if (!mlp || !h) return set_error(OC_ERR_INVALID_HANDLE, "NULL handle");
     Mlp &m = mlp->m;
     EquationsSet &s = h->s;
-    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN || m.n_in != 9 || m.n_out != 3)
-        return set_error(OC_ERR_INVALID_ARGUMENT, "warm start needs a Mooney-Rivlin set and a 9-in/3-out MLP");
-    const int nq = s.n_nodes;
+    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN || m.n_in != 9 || (m.n_out != 3 && m.n_out != 4))
+        return set_error(OC_ERR_INVALID_ARGUMENT, "warm start needs a Mooney-Rivlin set and a 9-in, 3- or 4-out MLP");
+    const int nq = s.n_nodes, nl = s.n_nodes_l;
     OC_CHECK(m.ensure_capacity(nq));
     Runtime &rt = runtime();
     KernelTimer timer("mlp_warm_start");
@@ -550,8 +581,20 @@ int oc_mlp_warm_start(oc_mlp mlp, oc_equations_set h, double load_increment)
                                                                 s.c01, m.shift.ptr, m.scale.ptr, m.act[0].ptr);
     OC_HIP_CHECK(hipGetLastError());
     OC_CHECK(m.forward(nq));
-    warm_apply_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, m.out.ptr, load_increment, s.free_mask.ptr, s.dofs.ptr);
+    warm_apply_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, m.out.ptr, m.n_out, load_increment,
+                                                             s.free_mask.ptr, s.dofs.ptr);
     OC_HIP_CHECK(hipGetLastError());
+    if (m.n_out == 4) {
+        if (!s.lin_to_quad.ptr) {
+            OC_CHECK(s.lin_to_quad.alloc(nl));
+            linear_to_quadratic_kernel<<<blocks_for(static_cast<int64_t>(s.n_elements) * 8), 256, 0, rt.stream>>>(
+                s.n_elements, s.conn.ptr, s.conn_l.ptr, s.lin_to_quad.ptr);
+            OC_HIP_CHECK(hipGetLastError());
+        }
+        warm_apply_pressure_kernel<<<blocks_for(nl), 256, 0, rt.stream>>>(
+            nq, nl, s.lin_to_quad.ptr, m.out.ptr, load_increment * (s.c10 + s.c01), s.free_mask.ptr, s.dofs.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+    }
     return OC_NO_ERROR;
 }
 

@@ -699,7 +699,10 @@ int oc_newton_solve(oc_equations_set h, const oc_newton_options *opt, oc_newton_
         OC_CHECK(launch_mr_jacobian(s, opt->jacobian_precision));
         stats->jacobian_evaluations++;
         OC_CHECK(K.setup_preconditioner());
-        // forcing term: fixed, or Eisenstat-Walker (choice 2, gamma 0.9, alpha (1+sqrt5)/2)
+        // forcing term: fixed, or Eisenstat-Walker (choice 2, gamma 0.9, alpha
+        // (1+sqrt5)/2) capped by ||F|| / ||F_ref||, the Dembo-Eisenstat-Steihaug
+        // rule that keeps Newton quadratic once the iterate is close: a start
+        // already near the solution is solved accurately from its first step
         double eta = opt->krylov_relative_tolerance;
         if (eta <= 0.0) {
             if (it == 0) eta = 0.5;
@@ -709,6 +712,7 @@ int oc_newton_solve(oc_equations_set h, const oc_newton_options *opt, oc_newton_
                 if (safe > 0.1) eta = std::max(eta, safe);
                 eta = std::min(eta, 0.5);
             }
+            eta = std::min(eta, fnorm / ref);
             eta = std::max(eta, 0.5 * target / fnorm);  // do not over-solve the last step
             eta = std::max(eta, 1e-14);
             eta_prev = eta;

@@ -5,7 +5,8 @@ drawn from the batched-meshes distribution (n ~ U{n_min..n_max},
 c10 ~ U[1,5], c01 ~ U[0,10], stretch ~ U[5%,20%], 0.1h jitter), each load
 step solved by plain FP64 Newton to a 1e-10 relative residual.  Every load
 step gives one sample per quadratic node: features (X, z - X, load increment,
-c10, c01) at the start of the step and target (z_end - z_start) / increment.
+c10, c01) at the start of the step and targets (z_end - z_start, p_end -
+p_start interpolated to the node) / increment.
 
 Evaluation solves the same load steps twice, plain (previous solution plus
 the boundary increment) and learned (plus the MLP increment), both to
@@ -24,7 +25,7 @@ import numpy as np
 from . import _lib
 from .equations_set import MooneyRivlinEquationsSet, initialise, newton_options, synchronise
 from .mesh import RegularMesh
-from .warm_start import WarmStartMLP, step_features
+from .warm_start import WarmStartMLP, step_features, step_targets
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "data", "warm_start_mlp.npz")
@@ -46,7 +47,7 @@ def make_problem(case, jitter=0.1):
     mesh = RegularMesh.generate(2, case["n"], jitter=jitter, seed=case["seed"])
     eq = MooneyRivlinEquationsSet(mesh, case["c10"], case["c01"])
     eq.set_dofs(eq.undeformed_dofs())
-    eq.uniaxial_boundary_conditions(1.0 + case["stretch"])
+    eq.fixed_dofs = eq.uniaxial_boundary_conditions(1.0 + case["stretch"])
     return mesh, eq
 
 
@@ -60,9 +61,8 @@ def collect(case, steps):
         eq.load_increment_apply(s, steps)
         eq.newton_solve(solver_options())
         d1 = eq.get_dofs()
-        nq = mesh.n_nodes
         xs.append(step_features(mesh, d0, dload, case["c10"], case["c01"]))
-        ts.append(((d1[:3 * nq] - d0[:3 * nq]).reshape(3, nq).T / dload).astype(np.float32))
+        ts.append(step_targets(mesh, d0, d1, dload, case["c10"], case["c01"]))
     eq.destroy()
     return np.concatenate(xs), np.concatenate(ts)
 
@@ -143,20 +143,25 @@ def main():
     ap.add_argument("--big", type=int, default=0, help="also evaluate the n^3 case with 20%% stretch in 10 steps")
     ap.add_argument("--big-steps", type=int, default=10)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--eval-only", action="store_true", help="evaluate the weights at --out without training")
     args = ap.parse_args()
     initialise(0)
     log = lambda m: print(m, flush=True)
-    t0 = time.time()
-    X, T = generate(args.solves, args.steps, args.seed, args.n_min, args.n_max, log)
-    log(f"data: {len(X)} samples from {args.solves} solves in {time.time() - t0:.1f} s")
-    net = WarmStartMLP(hidden=128, layers=4, seed=args.seed)
-    t1 = time.time()
-    loss = train(net, X, T, args.train_steps, log=log)
-    log(f"trained in {time.time() - t1:.1f} s, loss {loss:.4e}")
-    os.makedirs(os.path.dirname(args.out), exist_ok=True)
-    net.save(args.out)
-    report = dict(training=dict(solves=args.solves, samples=int(len(X)), steps=args.train_steps, loss=loss,
-                                n_range=[args.n_min, args.n_max]), heldout=[])
+    if args.eval_only:
+        net = WarmStartMLP.load(args.out)
+        report = dict(training=None, heldout=[])
+    else:
+        t0 = time.time()
+        X, T = generate(args.solves, args.steps, args.seed, args.n_min, args.n_max, log)
+        log(f"data: {len(X)} samples from {args.solves} solves in {time.time() - t0:.1f} s")
+        net = WarmStartMLP(hidden=128, layers=4, seed=args.seed)
+        t1 = time.time()
+        loss = train(net, X, T, args.train_steps, log=log)
+        log(f"trained in {time.time() - t1:.1f} s, loss {loss:.4e}")
+        os.makedirs(os.path.dirname(args.out), exist_ok=True)
+        net.save(args.out)
+        report = dict(training=dict(solves=args.solves, samples=int(len(X)), steps=args.train_steps, loss=loss,
+                                    n_range=[args.n_min, args.n_max]), heldout=[])
     rng = np.random.default_rng(args.seed + 7919)
     for k in range(args.heldout):
         case = draw_case(rng, 4, 16)

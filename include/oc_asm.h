@@ -162,7 +162,7 @@ typedef struct {
     int jacobian_precision;      /* OC_FP64 or OC_FP32 */
     int krylov_restart;          /* GMRES restart length */
     int krylov_maximum_iterations;
-    double krylov_relative_tolerance; /* > 0: fixed forcing term; <= 0: Eisenstat-Walker */
+    double krylov_relative_tolerance; /* > 0: fixed forcing term; <= 0: Eisenstat-Walker capped by ||F||/||F_ref|| */
     int line_search;             /* 0: full steps, 1: backtracking (cubic/quadratic, alpha below) */
     double line_search_alpha;    /* sufficient-decrease constant (reference default 1e-4) */
     int preconditioner;          /* OC_PRECONDITIONER_* */
@@ -221,9 +221,12 @@ int oc_mlp_dataset_set(oc_mlp mlp, int n, const float *x, const float *t);
 /* `steps` Adam steps on minibatches drawn (seeded) from the device training
  * set; returns the mean loss of the last `steps` minibatches. */
 int oc_mlp_train(oc_mlp mlp, int steps, int batch, float learning_rate, unsigned long long seed, float *mean_loss);
-/* Warm start of a Mooney-Rivlin set: z += load_increment * MLP(features) on
- * every free displacement DOF, features built on the device from the set's
- * coordinates and current DOFs.  Call before oc_load_increment_apply. */
+/* Warm start of a Mooney-Rivlin set: z += load_increment * MLP(features)[0:3]
+ * on every free displacement DOF, features built on the device from the set's
+ * coordinates and current DOFs.  A 4-output MLP also predicts the pressure
+ * increment per unit load at the quadratic nodes; each free pressure DOF takes
+ * the value of its coincident (element corner) quadratic node.  Call before
+ * oc_load_increment_apply.  The pressure output is in units of c10 + c01. */
 int oc_mlp_warm_start(oc_mlp mlp, oc_equations_set equations_set, double load_increment);
 
 /* ---- timing on the library stream ----------------------------------------- */

@@ -52,12 +52,12 @@ def numpy_grad(params, dims, x, t, shift, scale):
 def test_mlp_forward_and_gradient_match_numpy(gpu):
     from cm_amd.warm_start import WarmStartMLP, FEATURE_SHIFT, FEATURE_SCALE
     net = WarmStartMLP(hidden=32, layers=3, seed=3)
-    dims = [(9, 32), (32, 32), (32, 32), (32, 3)]
+    dims = [(9, 32), (32, 32), (32, 32), (32, 4)]
     rng = np.random.default_rng(0)
     n = 300  # not a multiple of the GEMM tiles
     x = np.concatenate([rng.uniform(0, 1, (n, 3)), rng.uniform(-0.1, 0.1, (n, 3)), rng.uniform(0, 0.04, (n, 1)),
                         rng.uniform(1, 5, (n, 1)), rng.uniform(0, 10, (n, 1))], axis=1).astype(np.float32)
-    t = rng.standard_normal((n, 3)).astype(np.float32)
+    t = rng.standard_normal((n, 4)).astype(np.float32)
     p = net.parameters()
     y = net.forward(x)
     yref, _, _ = numpy_mlp(p, dims, x, FEATURE_SHIFT.astype(np.float64), FEATURE_SCALE.astype(np.float64))
@@ -75,7 +75,8 @@ def test_mlp_training_reduces_loss(gpu):
     n = 20000
     x = np.concatenate([rng.uniform(0, 1, (n, 3)), rng.uniform(-0.1, 0.1, (n, 3)), rng.uniform(0, 0.04, (n, 1)),
                         rng.uniform(1, 5, (n, 1)), rng.uniform(0, 10, (n, 1))], axis=1).astype(np.float32)
-    t = np.stack([np.sin(3 * x[:, 0]) * x[:, 1], x[:, 3] * 5 + x[:, 7] * 0.1, x[:, 2] ** 2], axis=1).astype(np.float32)
+    t = np.stack([np.sin(3 * x[:, 0]) * x[:, 1], x[:, 3] * 5 + x[:, 7] * 0.1, x[:, 2] ** 2, x[:, 8] * x[:, 4]],
+                 axis=1).astype(np.float32)
     net.set_dataset(x, t)
     first = net.gradient(x[:4096], t[:4096])[0]
     last = net.train(steps=1500, batch=1024, learning_rate=3e-3, seed=2)

@@ -1,0 +1,90 @@
+"""What the learned warm start gains over the plain Newton start.
+
+Both methods solve the same load steps to 1e-10 times the plain start's
+residual norm; the learned form starts from previous solution + MLP increment
+and assembles its Jacobians in FP32.  The accepted answers are checked with
+the oracle's FP64 residual."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle_lib import mr_residual
+
+pytestmark = pytest.mark.gpu
+
+
+def free_residual_norm(mesh, eq, dofs):
+    """Oracle residual of `dofs` on the free DOFs."""
+    r = mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, dofs, eq.c10, eq.c01)
+    r[eq.fixed_dofs] = 0.0
+    return float(np.linalg.norm(r))
+
+
+def run_steps(case, steps, net, precision):
+    from cm_amd import _lib
+    from cm_amd.train_warm_start import make_problem, solver_options
+    mesh, eq = make_problem(case)
+    dload = case["stretch"] / steps
+    rows = []
+    for s in range(1, steps + 1):
+        d0 = eq.get_dofs()
+        eq.load_increment_apply(s, steps)
+        ref = eq.residual_norm()
+        if net is not None:
+            eq.set_dofs(d0)
+            net.warm_start(eq, dload)
+            eq.load_increment_apply(s, steps)
+        st = eq.newton_solve(solver_options(precision, reference_norm=ref))
+        rows.append(dict(ref=ref, start=st["initial_residual_norm"], final=st["final_residual_norm"],
+                         newton=st["newton_iterations"], oracle=free_residual_norm(mesh, eq, eq.get_dofs())))
+    eq.destroy()
+    return rows
+
+
+def check_accepted(rows):
+    for r in rows:
+        assert r["final"] <= 1e-10 * r["ref"]
+        # the oracle's FP64 residual of the accepted displacements meets the same bar
+        assert r["oracle"] <= 1.05e-10 * r["ref"], r
+
+
+def test_network_trained_by_own_kernels_improves_the_start(gpu):
+    """Train on a handful of plain solves, evaluate on a held-out seed."""
+    from cm_amd import _lib
+    from cm_amd.train_warm_start import generate, train
+    from cm_amd.warm_start import WarmStartMLP
+    X, T = generate(8, 4, seed=11, n_min=3, n_max=4, log=lambda m: None)
+    net = WarmStartMLP(hidden=64, layers=3, seed=4)
+    train(net, X, T, 800, batch=2048, lr=2e-3, log=lambda m: None)
+    case = dict(n=4, c10=2.5, c01=4.0, stretch=0.12, seed=987654)
+    plain = run_steps(case, 4, None, _lib.OC_FP64)
+    learned = run_steps(case, 4, net, _lib.OC_FP32)
+    check_accepted(plain)
+    check_accepted(learned)
+    ratios = [l["start"] / p["start"] for l, p in zip(learned, plain)]
+    print("start residual learned/plain per step:", ratios)
+    assert max(ratios) < 0.5
+    assert sum(r["newton"] for r in learned) <= sum(r["newton"] for r in plain)
+    net.destroy()
+
+
+def test_committed_network_on_heldout_mesh(gpu):
+    """The shipped weights (trained on plain solves of the batched-meshes
+    distribution by oc_mlp_train) on a mesh size and seed outside the training set."""
+    from cm_amd import _lib
+    from cm_amd.train_warm_start import DEFAULT_WEIGHTS
+    from cm_amd.warm_start import WarmStartMLP
+    assert os.path.exists(DEFAULT_WEIGHTS)
+    net = WarmStartMLP.load(DEFAULT_WEIGHTS)
+    case = dict(n=5, c10=3.0, c01=2.0, stretch=0.15, seed=31337)
+    plain = run_steps(case, 5, None, _lib.OC_FP64)
+    learned = run_steps(case, 5, net, _lib.OC_FP32)
+    check_accepted(plain)
+    check_accepted(learned)
+    ratios = [l["start"] / p["start"] for l, p in zip(learned, plain)]
+    print("start residual learned/plain per step:", ratios)
+    print("Newton iterations plain", [r["newton"] for r in plain], "learned", [r["newton"] for r in learned])
+    assert np.mean(ratios) < 0.3
+    assert sum(r["newton"] for r in learned) < sum(r["newton"] for r in plain)
+    net.destroy()
