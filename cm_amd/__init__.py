@@ -9,11 +9,12 @@ from ._lib import (OC_PRECONDITIONER_BLOCK_DIAGONAL, OC_PRECONDITIONER_BLOCK_TRI
 from ._lib import (CMISSError, OC_FP32, OC_FP64, OC_BUFFER_COL_IDX, OC_BUFFER_DOFS,  # noqa: F401
                    OC_BUFFER_MATRIX_VALUES, OC_BUFFER_MATRIX_VALUES_SP, OC_BUFFER_RESIDUAL, OC_BUFFER_RHS,
                    OC_BUFFER_ROW_PTR, LIB_PATH)
-from .equations_set import (EquationsSet, LaplaceEquationsSet, MooneyRivlinEquationsSet,  # noqa: F401
+from .equations_set import (EquationsSet, LaplaceEquationsSet, MooneyRivlinBatch, MooneyRivlinEquationsSet,  # noqa: F401
                             event_elapsed, event_record, finalise, initialise, kernel_timing,
                             kernel_timing_enable, newton_options, synchronise)
 from .mesh import RegularMesh  # noqa: F401
 
-__all__ = ["CMISSError", "EquationsSet", "LaplaceEquationsSet", "MooneyRivlinEquationsSet", "RegularMesh",
+__all__ = ["CMISSError", "EquationsSet", "LaplaceEquationsSet", "MooneyRivlinBatch", "MooneyRivlinEquationsSet",
+           "RegularMesh",
            "initialise", "finalise", "synchronise", "event_record", "event_elapsed", "OC_FP32", "OC_FP64",
            "newton_options"]

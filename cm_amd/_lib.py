@@ -135,6 +135,16 @@ SIGNATURES = {
     "oc_mlp_dataset_set": [_H, _c_int, _fp, _fp],
     "oc_mlp_train": [_H, _c_int, _c_int, _c_float, ctypes.c_ulonglong, ctypes.POINTER(_c_float)],
     "oc_mlp_warm_start": [_H, _H, _c_double],
+    "oc_mooney_rivlin_batch_create": [_c_int, _i32, _i32, _i32, _dp, _i32, _i32, _dp, _dp, _c_int,
+                                      ctypes.POINTER(_H)],
+    "oc_mesh_count": [_H, ctypes.POINTER(_c_int)],
+    "oc_equations_set_mesh_get": [_H, _c_int] + [ctypes.POINTER(_c_int)] * 6,
+    "oc_mesh_materials_set": [_H, _dp, _dp],
+    "oc_newton_solve_batch": [_H, ctypes.POINTER(NewtonOptions), ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.POINTER(NewtonStats)],
+    "oc_mesh_residual_norms": [_H, _dp],
+    "oc_autotune": [_H, _c_int, ctypes.c_char_p, _c_int],
+    "oc_jacobian_evaluate_graph": [_H, _c_int],
     "oc_kernel_timing_enable": [_c_int],
     "oc_kernel_timing_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
 }

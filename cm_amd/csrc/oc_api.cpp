@@ -78,9 +78,15 @@ int timing_resolve()
 }
 }  // namespace
 
+namespace {
+bool g_capturing = false;
+}
+void set_capturing(bool on) { g_capturing = on; }
+bool capturing() { return g_capturing; }
+
 KernelTimer::KernelTimer(const char *name) : name_(name)
 {
-    if (!g_timing.enabled) return;
+    if (!g_timing.enabled || g_capturing) return;
     start_ = timing_event();
     if (start_) (void)hipEventRecord(start_, g_runtime.stream);
 }
@@ -121,9 +127,11 @@ static int finish_structure(EquationsSet &s, int ncomp, const int32_t *nn, const
 {
     OC_CHECK(build_pattern(ncomp, s.n_elements, nn, nen, conn, s.h_row_ptr, s.h_col_idx));
     s.nnz = static_cast<int64_t>(s.h_col_idx.size());
-    std::vector<int32_t> colour_elems;
-    OC_CHECK(colour_elements(s.n_elements, ncomp, nn, nen, conn, s.colour_ptr, colour_elems));
-    OC_CHECK(s.colour_elems.upload(colour_elems.data(), colour_elems.size()));
+    if (s.cls == OC_EQUATIONS_SET_LAPLACE) {  // the Laplace scatter runs by colours
+        std::vector<int32_t> colour_elems;
+        OC_CHECK(colour_elements(s.n_elements, ncomp, nn, nen, conn, s.colour_ptr, colour_elems));
+        OC_CHECK(s.colour_elems.upload(colour_elems.data(), colour_elems.size()));
+    }
     OC_CHECK(s.row_ptr.upload(s.h_row_ptr.data(), s.h_row_ptr.size()));
     OC_CHECK(s.col_idx.upload(s.h_col_idx.data(), s.h_col_idx.size()));
     OC_CHECK(s.dofs.alloc(s.n_rows));
@@ -236,40 +244,113 @@ int oc_mooney_rivlin_create(int n_nodes_q, const double *coords_q, int n_nodes_l
                             const int *elem_nodes_q, const int *elem_nodes_l, int index_base, double c10, double c01,
                             oc_equations_set *equations_set)
 {
+    return oc_mooney_rivlin_batch_create(1, &n_nodes_q, &n_nodes_l, &n_elements, coords_q, elem_nodes_q,
+                                         elem_nodes_l, &c10, &c01, index_base, equations_set);
+}
+
+int oc_mooney_rivlin_batch_create(int n_meshes, const int *n_nodes_q, const int *n_nodes_l, const int *n_elements,
+                                  const double *coords_q, const int *elem_nodes_q, const int *elem_nodes_l,
+                                  const double *c10, const double *c01, int index_base,
+                                  oc_equations_set *equations_set)
+{
     OC_CHECK(require_runtime());
-    if (!coords_q || !elem_nodes_q || !elem_nodes_l || !equations_set || n_nodes_q <= 0 || n_nodes_l <= 0 ||
-        n_elements <= 0)
-        return set_error(OC_ERR_INVALID_ARGUMENT, "oc_mooney_rivlin_create: invalid arguments");
+    if (n_meshes < 1 || !n_nodes_q || !n_nodes_l || !n_elements || !coords_q || !elem_nodes_q || !elem_nodes_l ||
+        !c10 || !c01 || !equations_set)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "oc_mooney_rivlin_batch_create: invalid arguments");
+    std::vector<int32_t> qo(n_meshes + 1, 0), lo(n_meshes + 1, 0), eo(n_meshes + 1, 0);
+    for (int m = 0; m < n_meshes; ++m) {
+        if (n_nodes_q[m] <= 0 || n_nodes_l[m] <= 0 || n_elements[m] <= 0)
+            return set_error(OC_ERR_INVALID_ARGUMENT, "mesh %d: node and element counts must be positive", m);
+        const int64_t q = static_cast<int64_t>(qo[m]) + n_nodes_q[m], l = static_cast<int64_t>(lo[m]) + n_nodes_l[m],
+                      e = static_cast<int64_t>(eo[m]) + n_elements[m];
+        if (3 * q + l >= INT32_MAX || e * 27 >= INT32_MAX)
+            return set_error(OC_ERR_UNSUPPORTED, "batch exceeds int32 DOF or connectivity indices at mesh %d", m);
+        qo[m + 1] = static_cast<int32_t>(q);
+        lo[m + 1] = static_cast<int32_t>(l);
+        eo[m + 1] = static_cast<int32_t>(e);
+    }
+    const int NQ = qo[n_meshes], NL = lo[n_meshes], NE = eo[n_meshes];
     auto *h = new (std::nothrow) oc_equations_set_s();
-    if (!h) return set_error(OC_ERR_ALLOCATION, "oc_mooney_rivlin_create: out of host memory");
+    if (!h) return set_error(OC_ERR_ALLOCATION, "oc_mooney_rivlin_batch_create: out of host memory");
     EquationsSet &s = h->s;
     s.cls = OC_EQUATIONS_SET_MOONEY_RIVLIN;
-    s.n_nodes = n_nodes_q;
-    s.n_nodes_l = n_nodes_l;
-    s.n_rows = 3 * n_nodes_q + n_nodes_l;
-    s.n_elements = n_elements;
+    s.n_nodes = NQ;
+    s.n_nodes_l = NL;
+    s.n_rows = 3 * NQ + NL;
+    s.n_elements = NE;
     s.nen = 27;
-    s.c10 = c10;
-    s.c01 = c01;
-    std::vector<int32_t> cq, cl;
-    int rc = to_zero_based(elem_nodes_q, static_cast<size_t>(n_elements) * 27, index_base, n_nodes_q, cq,
-                           "elem_nodes_q");
-    if (rc == OC_NO_ERROR)
-        rc = to_zero_based(elem_nodes_l, static_cast<size_t>(n_elements) * 8, index_base, n_nodes_l, cl,
-                           "elem_nodes_l");
+    s.n_meshes = n_meshes;
+    s.c10 = c10[0];
+    s.c01 = c01[0];
+    s.h_q_off = qo;
+    s.h_l_off = lo;
+    s.h_e_off = eo;
+    // union numbering: mesh-local nodes shifted by the mesh's node offset;
+    // coordinates re-laid separated over the union
+    std::vector<int32_t> cq(static_cast<size_t>(NE) * 27), cl(static_cast<size_t>(NE) * 8);
+    std::vector<double> X(static_cast<size_t>(3) * NQ), mat(static_cast<size_t>(2) * NE);
+    int rc = OC_NO_ERROR;
+    int64_t cpos = 0;
+    for (int m = 0; m < n_meshes && rc == OC_NO_ERROR; ++m) {
+        const int nq = n_nodes_q[m], nl = n_nodes_l[m], ne = n_elements[m];
+        std::vector<int32_t> tq, tl;
+        rc = to_zero_based(elem_nodes_q + static_cast<int64_t>(eo[m]) * 27, static_cast<size_t>(ne) * 27,
+                           index_base, nq, tq, "elem_nodes_q");
+        if (rc == OC_NO_ERROR)
+            rc = to_zero_based(elem_nodes_l + static_cast<int64_t>(eo[m]) * 8, static_cast<size_t>(ne) * 8,
+                               index_base, nl, tl, "elem_nodes_l");
+        if (rc != OC_NO_ERROR) break;
+        for (size_t k = 0; k < tq.size(); ++k) cq[static_cast<size_t>(eo[m]) * 27 + k] = tq[k] + qo[m];
+        for (size_t k = 0; k < tl.size(); ++k) cl[static_cast<size_t>(eo[m]) * 8 + k] = tl[k] + lo[m];
+        for (int c = 0; c < 3; ++c)
+            for (int i = 0; i < nq; ++i) X[static_cast<size_t>(c) * NQ + qo[m] + i] = coords_q[cpos + c * nq + i];
+        cpos += 3 * static_cast<int64_t>(nq);
+        for (int e = 0; e < ne; ++e) {
+            mat[2 * (static_cast<size_t>(eo[m]) + e)] = c10[m];
+            mat[2 * (static_cast<size_t>(eo[m]) + e) + 1] = c01[m];
+        }
+    }
     // four dependent components: z1, z2, z3 on the quadratic mesh component,
     // p on the linear one (separated-component DOF order)
-    const int32_t nn[4] = {n_nodes_q, n_nodes_q, n_nodes_q, n_nodes_l}, nen[4] = {27, 27, 27, 8};
+    const int32_t nn[4] = {NQ, NQ, NQ, NL}, nen[4] = {27, 27, 27, 8};
     const int32_t *cp[4] = {cq.data(), cq.data(), cq.data(), cl.data()};
     if (rc == OC_NO_ERROR) rc = finish_structure(s, 4, nn, nen, cp);
     if (rc == OC_NO_ERROR) rc = s.conn.upload(cq.data(), cq.size());
     if (rc == OC_NO_ERROR) rc = s.conn_l.upload(cl.data(), cl.size());
-    if (rc == OC_NO_ERROR) rc = s.coords.upload(coords_q, static_cast<size_t>(3) * n_nodes_q);
+    if (rc == OC_NO_ERROR) rc = s.coords.upload(X.data(), X.size());
+    if (rc == OC_NO_ERROR) rc = s.mat.upload(mat.data(), mat.size());
+    if (rc == OC_NO_ERROR) rc = s.q_off.upload(qo.data(), qo.size());
+    if (rc == OC_NO_ERROR) rc = s.l_off.upload(lo.data(), lo.size());
     if (rc != OC_NO_ERROR) {
         delete h;
         return rc;
     }
     *equations_set = h;
+    return OC_NO_ERROR;
+}
+
+int oc_equations_set_mesh_get(oc_equations_set h, int mesh, int *node_offset_q, int *n_nodes_q, int *node_offset_l,
+                              int *n_nodes_l, int *element_offset, int *n_elements)
+{
+    OC_CHECK(check_handle(h));
+    const EquationsSet &s = h->s;
+    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN || mesh < 0 || mesh >= s.n_meshes)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "mesh %d of a set with %d Mooney-Rivlin meshes", mesh,
+                         s.cls == OC_EQUATIONS_SET_MOONEY_RIVLIN ? s.n_meshes : 0);
+    if (node_offset_q) *node_offset_q = s.h_q_off[mesh];
+    if (n_nodes_q) *n_nodes_q = s.h_q_off[mesh + 1] - s.h_q_off[mesh];
+    if (node_offset_l) *node_offset_l = s.h_l_off[mesh];
+    if (n_nodes_l) *n_nodes_l = s.h_l_off[mesh + 1] - s.h_l_off[mesh];
+    if (element_offset) *element_offset = s.h_e_off[mesh];
+    if (n_elements) *n_elements = s.h_e_off[mesh + 1] - s.h_e_off[mesh];
+    return OC_NO_ERROR;
+}
+
+int oc_mesh_count(oc_equations_set h, int *n_meshes)
+{
+    OC_CHECK(check_handle(h));
+    if (!n_meshes) return set_error(OC_ERR_INVALID_ARGUMENT, "n_meshes is NULL");
+    *n_meshes = h->s.n_meshes;
     return OC_NO_ERROR;
 }
 
@@ -298,8 +379,27 @@ int oc_materials_set(oc_equations_set h, double c10, double c01)
     OC_CHECK(check_handle(h));
     if (h->s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
         return set_error(OC_ERR_INVALID_ARGUMENT, "materials are defined for Mooney-Rivlin sets only");
-    h->s.c10 = c10;
-    h->s.c01 = c01;
+    std::vector<double> c10s(h->s.n_meshes, c10), c01s(h->s.n_meshes, c01);
+    return oc_mesh_materials_set(h, c10s.data(), c01s.data());
+}
+
+int oc_mesh_materials_set(oc_equations_set h, const double *c10, const double *c01)
+{
+    OC_CHECK(check_handle(h));
+    EquationsSet &s = h->s;
+    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN || !c10 || !c01)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "per-mesh materials of a Mooney-Rivlin set");
+    std::vector<double> mat(static_cast<size_t>(2) * s.n_elements);
+    for (int m = 0; m < s.n_meshes; ++m)
+        for (int e = s.h_e_off[m]; e < s.h_e_off[m + 1]; ++e) {
+            mat[2 * static_cast<size_t>(e)] = c10[m];
+            mat[2 * static_cast<size_t>(e) + 1] = c01[m];
+        }
+    OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));  // no launch may still read the old values
+    OC_HIP_CHECK(hipMemcpy(s.mat.ptr, mat.data(), mat.size() * sizeof(double), hipMemcpyHostToDevice));
+    s.node_mat.release();
+    s.c10 = c10[0];
+    s.c01 = c01[0];
     return OC_NO_ERROR;
 }
 
@@ -308,8 +408,11 @@ int oc_matrix_structure_get(oc_equations_set h, int index_base, int *row_ptr, in
     OC_CHECK(check_handle(h));
     if (index_base != 0 && index_base != 1) return set_error(OC_ERR_INVALID_ARGUMENT, "index_base must be 0 or 1");
     const EquationsSet &s = h->s;
+    if (s.nnz + index_base > INT32_MAX)
+        return set_error(OC_ERR_UNSUPPORTED, "%lld nonzeros exceed the 4-byte integer structure arrays",
+                         static_cast<long long>(s.nnz));
     if (row_ptr)
-        for (size_t i = 0; i < s.h_row_ptr.size(); ++i) row_ptr[i] = s.h_row_ptr[i] + index_base;
+        for (size_t i = 0; i < s.h_row_ptr.size(); ++i) row_ptr[i] = static_cast<int>(s.h_row_ptr[i] + index_base);
     if (col_idx)
         for (size_t i = 0; i < s.h_col_idx.size(); ++i) col_idx[i] = s.h_col_idx[i] + index_base;
     return OC_NO_ERROR;

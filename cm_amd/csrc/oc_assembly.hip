@@ -138,16 +138,16 @@ template int fill<float>(float *, int64_t, float);
 constexpr int kLapElemsPerBlock = 32;
 
 __global__ void __launch_bounds__(256)
-laplace_slot_offsets_kernel(int n_elements, const int *__restrict__ conn, const int *__restrict__ row_ptr,
+laplace_slot_offsets_kernel(int n_elements, const int *__restrict__ conn, const int64_t *__restrict__ row_ptr,
                             const int *__restrict__ col_idx, uint8_t *__restrict__ offs, int *__restrict__ n_bad)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gid >= static_cast<int64_t>(n_elements) * 8) return;
     const int64_t e = gid / 8;
     const int row = conn[gid];
-    const int base = row_ptr[row];
+    const int64_t base = row_ptr[row];
     for (int j = 0; j < 8; ++j) {
-        const int slot = csr_slot(row_ptr, col_idx, row, conn[e * 8 + j]);
+        const int64_t slot = csr_slot(row_ptr, col_idx, row, conn[e * 8 + j]);
         if (slot < 0 || slot - base > 255) {
             atomicAdd(n_bad, 1);
             offs[gid * 8 + j] = 0;
@@ -159,7 +159,7 @@ laplace_slot_offsets_kernel(int n_elements, const int *__restrict__ conn, const 
 
 __global__ void __launch_bounds__(256)
 laplace_assemble_kernel(const int *__restrict__ colour_elems, int n_in_colour, const int *__restrict__ conn,
-                        const double *__restrict__ coords, int n_nodes, const int *__restrict__ row_ptr,
+                        const double *__restrict__ coords, int n_nodes, const int64_t *__restrict__ row_ptr,
                         const uint8_t *__restrict__ offs, double *__restrict__ values,
                         const BasisTables *__restrict__ T)
 {
@@ -210,12 +210,12 @@ laplace_assemble_kernel(const int *__restrict__ colour_elems, int n_in_colour, c
             k[ns] = k[ns] + sum * rwg;
         }
     }
-    const int base = row_ptr[node];
+    const int64_t base = row_ptr[node];
     const uint64_t packed = *reinterpret_cast<const uint64_t *>(offs + (static_cast<int64_t>(e) * 8 + ms) * 8);
-    int slot[8];
+    int64_t slot[8];
     double old[8];
 #pragma unroll
-    for (int ns = 0; ns < 8; ++ns) slot[ns] = base + static_cast<int>((packed >> (8 * ns)) & 0xff);
+    for (int ns = 0; ns < 8; ++ns) slot[ns] = base + static_cast<int64_t>((packed >> (8 * ns)) & 0xff);
 #pragma unroll
     for (int ns = 0; ns < 8; ++ns) old[ns] = values[slot[ns]];
 #pragma unroll

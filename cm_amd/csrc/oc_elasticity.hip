@@ -36,8 +36,8 @@ constexpr int kGaussElemsPerBlock = 9;  // 9 elements x 27 Gauss points = 243 th
 // (:609-740).
 __global__ void __launch_bounds__(256)
 mr_gauss_kernel(int n_elements, const int *__restrict__ conn_q, const int *__restrict__ conn_l,
-                const double *__restrict__ coords, int nq, const double *__restrict__ dofs, double c10, double c01,
-                const BasisTables *__restrict__ T, double *__restrict__ rec)
+                const double *__restrict__ coords, int nq, const double *__restrict__ dofs,
+                const double *__restrict__ mat, const BasisTables *__restrict__ T, double *__restrict__ rec)
 {
     __shared__ double s_fq[27 * 27 * 4];
     __shared__ double s_X[kGaussElemsPerBlock][81];
@@ -74,7 +74,7 @@ mr_gauss_kernel(int n_elements, const int *__restrict__ conn_q, const int *__res
     for (int i = 0; i < 9; ++i) rec[(GP_DXIDZ + i) * n_gp + q] = k.dxidz[i];
     rec[GP_JGW * n_gp + q] = k.jz * T->w27[g];
     double c[36], sigma[6];
-    mr_elasticity_tensor(k.F, k.Jznu, P, c10, c01, c, sigma);
+    mr_elasticity_tensor(k.F, k.Jznu, P, mat[2 * e], mat[2 * e + 1], c, sigma);
 #pragma unroll
     for (int i = 0; i < 36; ++i) rec[(GP_C + i) * n_gp + q] = c[i];
 #pragma unroll
@@ -93,16 +93,16 @@ __device__ __forceinline__ int element_dof(const int *__restrict__ conn_q, const
 // Scatter map: offset of element column j inside CSR row dof_i.
 __global__ void __launch_bounds__(128)
 build_slot_offsets_kernel(int n_elements, const int *__restrict__ conn_q, const int *__restrict__ conn_l, int nq,
-                          const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
+                          const int64_t *__restrict__ row_ptr, const int *__restrict__ col_idx,
                           uint16_t *__restrict__ offs, int *__restrict__ n_missing)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gid >= static_cast<int64_t>(n_elements) * 89) return;
     const int e = static_cast<int>(gid / 89), i = static_cast<int>(gid % 89);
     const int row = element_dof(conn_q, conn_l, nq, e, i);
-    const int base = row_ptr[row];
+    const int64_t base = row_ptr[row];
     for (int j = 0; j < 89; ++j) {
-        const int slot = csr_slot(row_ptr, col_idx, row, element_dof(conn_q, conn_l, nq, e, j));
+        const int64_t slot = csr_slot(row_ptr, col_idx, row, element_dof(conn_q, conn_l, nq, e, j));
         if (slot < 0) {
             atomicAdd(n_missing, 1);
             offs[gid * 89 + j] = 0;
@@ -115,11 +115,14 @@ build_slot_offsets_kernel(int n_elements, const int *__restrict__ conn_q, const 
 int mr_prepare(EquationsSet &s)
 {
     if (s.slot_offsets.ptr) return OC_NO_ERROR;
-    int max_row = 0;
+    int64_t max_row = 0;
     for (int r = 0; r < s.n_rows; ++r) max_row = std::max(max_row, s.h_row_ptr[r + 1] - s.h_row_ptr[r]);
     if (max_row > 8192)
-        return set_error(OC_ERR_UNSUPPORTED, "CSR row of %d entries exceeds the row-gather capacity", max_row);
-    s.max_row_len = max_row;
+        return set_error(OC_ERR_UNSUPPORTED, "CSR row of %lld entries exceeds the row-gather capacity",
+                         static_cast<long long>(max_row));
+    if (static_cast<int64_t>(s.n_elements) * 89 >= INT32_MAX)
+        return set_error(OC_ERR_UNSUPPORTED, "%d elements exceed the int32 element-entry gather lists", s.n_elements);
+    s.max_row_len = static_cast<int>(max_row);
     OC_CHECK(s.slot_offsets.alloc(static_cast<size_t>(s.n_elements) * 89 * 89));
     DevBuf<int> missing;
     OC_CHECK(missing.alloc(1));
@@ -155,7 +158,7 @@ int mr_prepare(EquationsSet &s)
     return OC_NO_ERROR;
 }
 
-static int launch_gauss(EquationsSet &s)
+int launch_mr_gauss(EquationsSet &s)
 {
     const BasisTables *T = nullptr;
     OC_CHECK(ensure_tables(&T));
@@ -163,7 +166,7 @@ static int launch_gauss(EquationsSet &s)
     const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
     KernelTimer timer("mr_gauss_jacobian");
     mr_gauss_kernel<<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr, s.coords.ptr, s.n_nodes,
-                                                   s.dofs.ptr, s.c10, s.c01, T, s.gp_records.ptr);
+                                                   s.dofs.ptr, s.mat.ptr, T, s.gp_records.ptr);
     OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
 }
@@ -178,8 +181,9 @@ static int launch_gauss(EquationsSet &s)
 // (the reference's element-loop order, equations_set_routines.f90:5535-5568).
 __global__ void __launch_bounds__(256)
 mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const int *__restrict__ conn_l,
-                           const double *__restrict__ coords, int nq, const double *__restrict__ dofs, double c10,
-                           double c01, const BasisTables *__restrict__ T, double *__restrict__ relem)
+                           const double *__restrict__ coords, int nq, const double *__restrict__ dofs,
+                           const double *__restrict__ mat, const BasisTables *__restrict__ T,
+                           double *__restrict__ relem)
 {
     __shared__ double s_fq[27 * 27 * 4];
     __shared__ double s_X[kGaussElemsPerBlock][81];
@@ -215,7 +219,7 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
             double P = 0.0;
             for (int n = 0; n < 8; ++n) P += T->lin27[(g * 8 + n) * 4] * s_p[le][n];
             double sigma[6];
-            mr_cauchy_stress(k.F, k.Jznu, P, c10, c01, sigma);
+            mr_cauchy_stress(k.F, k.Jznu, P, mat[2 * e], mat[2 * e + 1], sigma);
             const double jgw = k.jz * T->w27[g];
             for (int i = 0; i < 9; ++i) s_dx[le][i][g] = k.dxidz[i];
             for (int a = 0; a < 3; ++a)
@@ -274,8 +278,8 @@ int launch_mr_residual(EquationsSet &s)
     {
         KernelTimer timer("mr_residual");
         mr_residual_element_kernel<<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
-                                                                  s.coords.ptr, s.n_nodes, s.dofs.ptr, s.c10, s.c01,
-                                                                  T, s.relem.ptr);
+                                                                  s.coords.ptr, s.n_nodes, s.dofs.ptr, s.mat.ptr, T,
+                                                                  s.relem.ptr);
         OC_HIP_CHECK(hipGetLastError());
     }
     {
@@ -446,9 +450,9 @@ mr_element_matrix_kernel(int n_elements, const double *__restrict__ rec, const B
     }
 }
 
-template <typename Real>
+template <typename Real, int kChunk>
 __global__ void __launch_bounds__(64)
-csr_row_gather_kernel(int n_rows, const int *__restrict__ row_ptr, const int *__restrict__ gptr,
+csr_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const int *__restrict__ gptr,
                       const int *__restrict__ gidx, const Real *__restrict__ kbuf,
                       const uint16_t *__restrict__ offs, Real *__restrict__ values)
 {
@@ -456,12 +460,13 @@ csr_row_gather_kernel(int n_rows, const int *__restrict__ row_ptr, const int *__
     Real *s_row = reinterpret_cast<Real *>(s_raw);
     const int r = blockIdx.x, lane = threadIdx.x;
     if (r >= n_rows) return;
-    const int r0 = row_ptr[r], len = row_ptr[r + 1] - r0;
+    const int64_t r0 = row_ptr[r];
+    const int len = static_cast<int>(row_ptr[r + 1] - r0);
     for (int k = lane; k < len; k += 64) s_row[k] = Real(0);
     __syncthreads();
-    // contributions in chunks of 8 elements: all global loads of a chunk are
-    // issued before its LDS accumulation (lane j holds entries j and j + 64)
-    constexpr int kChunk = 8;
+    // contributions in chunks of kChunk element rows: all global loads of a
+    // chunk are issued before its LDS accumulation (lane j holds entries j and
+    // j + 64)
     for (int c0 = gptr[r]; c0 < gptr[r + 1]; c0 += kChunk) {
         const int nc = min(kChunk, gptr[r + 1] - c0);
         Real v0[kChunk], v1[kChunk];
@@ -489,43 +494,108 @@ csr_row_gather_kernel(int n_rows, const int *__restrict__ row_ptr, const int *__
     for (int k = lane; k < len; k += 64) values[r0 + k] = s_row[k];
 }
 
-int launch_mr_jacobian(EquationsSet &s, int precision)
+template <typename Real>
+static int launch_row_gather(EquationsSet &s, const Real *kbuf, Real *values, int chunk)
+{
+    Runtime &rt = runtime();
+    const size_t row_lds = static_cast<size_t>(s.max_row_len) * sizeof(Real);
+    const unsigned grid = static_cast<unsigned>(s.n_rows);
+    switch (chunk) {
+    case 4:
+        csr_row_gather_kernel<Real, 4><<<grid, 64, row_lds, rt.stream>>>(s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr,
+                                                                        s.gather_idx.ptr, kbuf, s.slot_offsets.ptr,
+                                                                        values);
+        break;
+    case 16:
+        csr_row_gather_kernel<Real, 16><<<grid, 64, row_lds, rt.stream>>>(s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr,
+                                                                         s.gather_idx.ptr, kbuf, s.slot_offsets.ptr,
+                                                                         values);
+        break;
+    default:
+        csr_row_gather_kernel<Real, 8><<<grid, 64, row_lds, rt.stream>>>(s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr,
+                                                                        s.gather_idx.ptr, kbuf, s.slot_offsets.ptr,
+                                                                        values);
+        break;
+    }
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
+int mr_jacobian_buffers(EquationsSet &s, int precision)
 {
     OC_CHECK(mr_prepare(s));
     const BasisTables *T = nullptr;
     OC_CHECK(ensure_tables(&T));
-    Runtime &rt = runtime();
-    OC_CHECK(launch_gauss(s));
     const size_t kn = static_cast<size_t>(s.n_elements) * 89 * 89;
-    const unsigned blocks = static_cast<unsigned>(std::min(s.n_elements, rt.num_cus * 4));
-    const size_t row_lds = static_cast<size_t>(s.max_row_len) * (precision == OC_FP64 ? 8 : 4);
     if (precision == OC_FP64) {
         if (!s.kbuf64.ptr) OC_CHECK(s.kbuf64.alloc(kn));
-        {
-            KernelTimer timer("mr_element_matrix_fp64");
-            mr_element_matrix_kernel<double><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
-                                                                           s.kbuf64.ptr);
-            OC_HIP_CHECK(hipGetLastError());
-        }
-        KernelTimer timer("csr_row_gather_fp64");
-        csr_row_gather_kernel<double><<<static_cast<unsigned>(s.n_rows), 64, row_lds, rt.stream>>>(
-            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.gather_idx.ptr, s.kbuf64.ptr, s.slot_offsets.ptr,
-            s.values.ptr);
-        OC_HIP_CHECK(hipGetLastError());
+        if (!s.values.ptr) OC_CHECK(s.values.alloc(s.nnz));
     } else {
         if (!s.kbuf32.ptr) OC_CHECK(s.kbuf32.alloc(kn));
-        {
-            KernelTimer timer("mr_element_matrix_fp32");
-            mr_element_matrix_kernel<float><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
-                                                                          s.kbuf32.ptr);
-            OC_HIP_CHECK(hipGetLastError());
-        }
-        KernelTimer timer("csr_row_gather_fp32");
-        csr_row_gather_kernel<float><<<static_cast<unsigned>(s.n_rows), 64, row_lds, rt.stream>>>(
-            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.gather_idx.ptr, s.kbuf32.ptr, s.slot_offsets.ptr,
-            s.values_sp.ptr);
-        OC_HIP_CHECK(hipGetLastError());
+        if (!s.values_sp.ptr) OC_CHECK(s.values_sp.alloc(s.nnz));
     }
+    return OC_NO_ERROR;
+}
+
+// Element-matrix phase alone (the autotuner times it per launch shape).
+int launch_mr_element_matrices(EquationsSet &s, int precision, int blocks_per_cu)
+{
+    const BasisTables *T = nullptr;
+    OC_CHECK(ensure_tables(&T));
+    Runtime &rt = runtime();
+    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : 4;
+    const unsigned blocks = static_cast<unsigned>(std::min(s.n_elements, rt.num_cus * bpc));
+    if (precision == OC_FP64) {
+        KernelTimer timer("mr_element_matrix_fp64");
+        mr_element_matrix_kernel<double><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
+                                                                       s.kbuf64.ptr);
+    } else {
+        KernelTimer timer("mr_element_matrix_fp32");
+        mr_element_matrix_kernel<float><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
+                                                                      s.kbuf32.ptr);
+    }
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
+int launch_mr_row_gather(EquationsSet &s, int precision, int chunk)
+{
+    if (precision == OC_FP64) {
+        KernelTimer timer("csr_row_gather_fp64");
+        return launch_row_gather<double>(s, s.kbuf64.ptr, s.values.ptr, chunk);
+    }
+    KernelTimer timer("csr_row_gather_fp32");
+    return launch_row_gather<float>(s, s.kbuf32.ptr, s.values_sp.ptr, chunk);
+}
+
+int launch_mr_jacobian(EquationsSet &s, int precision)
+{
+    OC_CHECK(mr_jacobian_buffers(s, precision));
+    OC_CHECK(launch_mr_gauss(s));
+    OC_CHECK(launch_mr_element_matrices(s, precision, s.tuned_element_blocks_per_cu));
+    return launch_mr_row_gather(s, precision, s.tuned_gather_chunk);
+}
+
+// (c10, c01) of every quadratic node from its elements (all elements around
+// a node belong to one mesh, so every writer of an entry writes the same pair)
+__global__ void __launch_bounds__(256) node_materials_kernel(int n_elements, const int *__restrict__ conn_q,
+                                                            const double *__restrict__ mat,
+                                                            double *__restrict__ node_mat)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n_elements * 27; t += gridDim.x * blockDim.x) {
+        const int e = t / 27, node = conn_q[t];
+        node_mat[2 * node] = mat[2 * e];
+        node_mat[2 * node + 1] = mat[2 * e + 1];
+    }
+}
+
+int mr_node_materials(EquationsSet &s)
+{
+    if (!s.node_mat.ptr) OC_CHECK(s.node_mat.alloc(static_cast<size_t>(2) * s.n_nodes));
+    const int64_t n = static_cast<int64_t>(s.n_elements) * 27;
+    const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 8192));
+    node_materials_kernel<<<blocks, 256, 0, runtime().stream>>>(s.n_elements, s.conn.ptr, s.mat.ptr, s.node_mat.ptr);
+    OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
 }
 

@@ -254,12 +254,12 @@ __device__ __forceinline__ void mr_elasticity_tensor(const double *F, double Jzn
 
 // Position of column `col` in CSR row `row` (columns sorted ascending);
 // the assembly guarantees presence, so a miss returns -1 and is skipped.
-__device__ __forceinline__ int csr_slot(const int *__restrict__ row_ptr, const int *__restrict__ col_idx, int row,
-                                        int col)
+__device__ __forceinline__ int64_t csr_slot(const int64_t *__restrict__ row_ptr, const int *__restrict__ col_idx,
+                                            int row, int col)
 {
-    int lo = row_ptr[row], hi = row_ptr[row + 1] - 1;
+    int64_t lo = row_ptr[row], hi = row_ptr[row + 1] - 1;
     while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
+        const int64_t mid = (lo + hi) >> 1;
         const int v = col_idx[mid];
         if (v == col) return mid;
         if (v < col) lo = mid + 1;

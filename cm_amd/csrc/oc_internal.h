@@ -84,7 +84,15 @@ struct DevBuf {
 };
 
 // Per-equations-set state.  Element arrays are element-major; node and DOF
-// indices are 0-based; CSR row_ptr/col_idx 0-based int32.
+// indices are 0-based; CSR col_idx 0-based int32, row offsets int64 on the
+// device (a batch of meshes can exceed 2^31 nonzeros).
+//
+// A Mooney-Rivlin set holds one mesh or a batch of independent meshes as the
+// disjoint union: mesh m owns quadratic nodes [q_off[m], q_off[m+1]), pressure
+// nodes [l_off[m], l_off[m+1]) and elements [e_off[m], e_off[m+1]); DOFs are
+// component-major over the union (z1 of every node, z2, z3, then p), so the
+// batch is one block-diagonal system and every kernel runs over all meshes
+// in one launch.
 struct EquationsSet {
     int cls = 0;
     int n_rows = 0;          // DOFs
@@ -93,10 +101,16 @@ struct EquationsSet {
     int nen = 0;             // element nodes of the main component (8 or 27)
     int n_nodes = 0;         // nodes of the main component
     int n_nodes_l = 0;       // pressure component nodes (MR)
-    double c10 = 0.0, c01 = 0.0;
+    double c10 = 0.0, c01 = 0.0;  // materials of a single-mesh set
+    int n_meshes = 1;
+    std::vector<int32_t> h_q_off, h_l_off, h_e_off;  // n_meshes + 1 each
+    DevBuf<int32_t> q_off, l_off;
+    DevBuf<double> mat;       // (c10, c01) per element
+    DevBuf<double> node_mat;  // (c10, c01) per quadratic node (warm start)
 
     // host copies of the structure
-    std::vector<int32_t> h_row_ptr, h_col_idx;
+    std::vector<int64_t> h_row_ptr;
+    std::vector<int32_t> h_col_idx;
 
     // colouring: elements of colour c are colour_elems[colour_ptr[c] .. colour_ptr[c+1])
     std::vector<int32_t> colour_ptr;
@@ -106,7 +120,8 @@ struct EquationsSet {
     DevBuf<int32_t> conn;    // nen * E
     DevBuf<int32_t> conn_l;  // 8 * E (MR pressure component)
     DevBuf<int32_t> lin_to_quad;  // pressure node -> coincident quadratic node (warm start)
-    DevBuf<int32_t> row_ptr, col_idx;
+    DevBuf<int64_t> row_ptr;
+    DevBuf<int32_t> col_idx;
     DevBuf<double> values;
     DevBuf<float> values_sp;
     DevBuf<double> dofs, residual, rhs;
@@ -130,13 +145,29 @@ struct EquationsSet {
     DevBuf<int32_t> bc_idx;
     DevBuf<double> bc_full;
     DevBuf<uint8_t> free_mask;
+    // solver weights: wgt = 1/||F_ref|| of the row's mesh on free rows, 0 on
+    // fixed rows; iwgt = 1/wgt on free rows, 0 on fixed rows
+    DevBuf<double> wgt, iwgt;
     // solver workspace (oc_solver.hip)
     DevBuf<double> krylov;      // (restart+1) * n_rows
     DevBuf<double> work;        // vectors
     DevBuf<double> precond;     // preconditioner storage
     DevBuf<double> partials;    // reduction partial sums
-    DevBuf<int32_t> pstart;     // first pressure column of each displacement row
+    DevBuf<int64_t> pstart;     // first pressure column of each displacement row
     int krylov_capacity = 0;
+
+    // launch shapes (oc_autotune); 0 = default
+    int tuned_element_blocks_per_cu = 0;
+    int tuned_spmv_lanes = 0;
+    int tuned_gather_chunk = 0;
+
+    // Newton-step graph: Jacobian assembly + preconditioner setup
+    hipGraphExec_t step_graph = nullptr;
+    int step_graph_key[4] = {-1, -1, -1, -1};
+    ~EquationsSet()
+    {
+        if (step_graph) (void)hipGraphExecDestroy(step_graph);
+    }
 };
 
 // ---- shared device resources (oc_assembly.hip) --------------------------------
@@ -147,7 +178,7 @@ int fill(T *p, int64_t n, T v);
 
 // ---- host-side setup helpers (oc_setup.cpp) ---------------------------------
 int build_pattern(int ncomp, int n_elements, const int32_t *nn, const int32_t *nen, const int32_t *const *conn,
-                  std::vector<int32_t> &row_ptr, std::vector<int32_t> &col_idx);
+                  std::vector<int64_t> &row_ptr, std::vector<int32_t> &col_idx);
 int colour_elements(int n_elements, int ncomp, const int32_t *nn, const int32_t *nen, const int32_t *const *conn,
                     std::vector<int32_t> &colour_ptr, std::vector<int32_t> &colour_elems);
 
@@ -156,6 +187,17 @@ int launch_laplace_assemble(EquationsSet &s);
 int mr_prepare(EquationsSet &s);  // builds the scatter map once
 int launch_mr_residual(EquationsSet &s);
 int launch_mr_jacobian(EquationsSet &s, int precision);
+int mr_node_materials(EquationsSet &s);  // fills node_mat from mat
+int mr_jacobian_buffers(EquationsSet &s, int precision);  // element-matrix and CSR value buffers
+int launch_mr_element_matrices(EquationsSet &s, int precision, int blocks_per_cu);
+int launch_mr_row_gather(EquationsSet &s, int precision, int chunk);
+int launch_mr_gauss(EquationsSet &s);
+// y = A x with `lanes` lanes per CSR row (A in FP64 or FP32 values)
+int launch_spmv_lanes(EquationsSet &s, bool fp32, int lanes, const double *x, double *y);
+
+// Kernel timers are suspended while a stream capture records a graph.
+void set_capturing(bool on);
+bool capturing();
 
 }  // namespace oc
 

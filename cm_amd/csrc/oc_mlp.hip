@@ -194,7 +194,8 @@ __global__ void __launch_bounds__(256) normalise_kernel(int n, int n_in, const f
 // warm-start features of every quadratic node: X, z - X, load increment, c10, c01
 __global__ void __launch_bounds__(256) warm_features_kernel(int nq, const double *__restrict__ coords,
                                                            const double *__restrict__ dofs, double dload,
-                                                           double c10, double c01, const float *__restrict__ shift,
+                                                           const double *__restrict__ node_mat,
+                                                           const float *__restrict__ shift,
                                                            const float *__restrict__ scale, float *__restrict__ out)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
@@ -205,8 +206,8 @@ __global__ void __launch_bounds__(256) warm_features_kernel(int nq, const double
             f[3 + c] = static_cast<float>(dofs[static_cast<int64_t>(c) * nq + i] - X);
         }
         f[6] = static_cast<float>(dload);
-        f[7] = static_cast<float>(c10);
-        f[8] = static_cast<float>(c01);
+        f[7] = static_cast<float>(node_mat[2 * i]);
+        f[8] = static_cast<float>(node_mat[2 * i + 1]);
         for (int k = 0; k < 9; ++k) out[static_cast<int64_t>(i) * 9 + k] = (f[k] - shift[k]) * scale[k];
     }
 }
@@ -244,13 +245,15 @@ __global__ void __launch_bounds__(256) linear_to_quadratic_kernel(int n_elements
 // network's pressure output is in units of c10 + c01)
 __global__ void __launch_bounds__(256) warm_apply_pressure_kernel(int nq, int nl, const int *__restrict__ map,
                                                                  const float *__restrict__ y, double dload,
+                                                                 const double *__restrict__ node_mat,
                                                                  const uint8_t *__restrict__ free_mask,
                                                                  double *__restrict__ dofs)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
         const int64_t d = static_cast<int64_t>(3) * nq + i;
+        const int q = map[i];
         if (free_mask == nullptr || free_mask[d])
-            dofs[d] += dload * static_cast<double>(y[static_cast<int64_t>(map[i]) * 4 + 3]);
+            dofs[d] += dload * (node_mat[2 * q] + node_mat[2 * q + 1]) * static_cast<double>(y[static_cast<int64_t>(q) * 4 + 3]);
     }
 }
 
@@ -577,8 +580,10 @@ int oc_mlp_warm_start(oc_mlp mlp, oc_equations_set h, double load_increment)
     OC_CHECK(m.ensure_capacity(nq));
     Runtime &rt = runtime();
     KernelTimer timer("mlp_warm_start");
-    warm_features_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, s.coords.ptr, s.dofs.ptr, load_increment, s.c10,
-                                                                s.c01, m.shift.ptr, m.scale.ptr, m.act[0].ptr);
+    if (!s.node_mat.ptr) OC_CHECK(mr_node_materials(s));
+    warm_features_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, s.coords.ptr, s.dofs.ptr, load_increment,
+                                                                s.node_mat.ptr, m.shift.ptr, m.scale.ptr,
+                                                                m.act[0].ptr);
     OC_HIP_CHECK(hipGetLastError());
     OC_CHECK(m.forward(nq));
     warm_apply_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, m.out.ptr, m.n_out, load_increment,
@@ -592,7 +597,7 @@ int oc_mlp_warm_start(oc_mlp mlp, oc_equations_set h, double load_increment)
             OC_HIP_CHECK(hipGetLastError());
         }
         warm_apply_pressure_kernel<<<blocks_for(nl), 256, 0, rt.stream>>>(
-            nq, nl, s.lin_to_quad.ptr, m.out.ptr, load_increment * (s.c10 + s.c01), s.free_mask.ptr, s.dofs.ptr);
+            nq, nl, s.lin_to_quad.ptr, m.out.ptr, load_increment, s.node_mat.ptr, s.free_mask.ptr, s.dofs.ptr);
         OC_HIP_CHECK(hipGetLastError());
     }
     return OC_NO_ERROR;

@@ -47,7 +47,7 @@ int worker_count()
 // around np, sorted ascending and unique (LIST_REMOVE_DUPLICATES,
 // lists.f90:2647-2690).  0-based output.
 int build_pattern(int ncomp, int n_elements, const int32_t *nn, const int32_t *nen, const int32_t *const *conn,
-                  std::vector<int32_t> &row_ptr, std::vector<int32_t> &col_idx)
+                  std::vector<int64_t> &row_ptr, std::vector<int32_t> &col_idx)
 {
     if (ncomp < 1 || ncomp > 4) return set_error(OC_ERR_INVALID_ARGUMENT, "pattern: %d components", ncomp);
     std::vector<int64_t> off(ncomp + 1, 0);
@@ -95,10 +95,7 @@ int build_pattern(int ncomp, int n_elements, const int32_t *nn, const int32_t *n
     };
     run(false);
     for (int64_t r = 0; r < ndof; ++r) counts[r + 1] += counts[r];
-    if (counts[ndof] >= INT32_MAX)
-        return set_error(OC_ERR_PATTERN, "pattern: %lld nonzeros exceed int32", (long long)counts[ndof]);
-    row_ptr.resize(ndof + 1);
-    for (int64_t r = 0; r <= ndof; ++r) row_ptr[r] = static_cast<int32_t>(counts[r]);
+    row_ptr.assign(counts.begin(), counts.end());
     col_idx.resize(counts[ndof]);
     run(true);
     return OC_NO_ERROR;

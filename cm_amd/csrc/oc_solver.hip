@@ -35,11 +35,68 @@ __global__ void __launch_bounds__(256) copy_scale_kernel(int n, double a, const 
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = a * x[i];
 }
 
-__global__ void __launch_bounds__(256) mask_kernel(int n, const uint8_t *__restrict__ free_mask,
-                                                  double *__restrict__ y)
+__global__ void __launch_bounds__(256) scale_kernel(int n, const double *__restrict__ w, double *__restrict__ y)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        if (!free_mask[i]) y[i] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = w[i] * y[i];
+}
+
+// y = a * w .* x
+__global__ void __launch_bounds__(256) weighted_copy_kernel(int n, double a, const double *__restrict__ w,
+                                                           const double *__restrict__ x, double *__restrict__ y)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = a * w[i] * x[i];
+}
+
+// Solver weights of a Mooney-Rivlin union: mesh m = blockIdx.y owns rows
+// c*NQ + [q_off[m], q_off[m+1]) (c < 3) and 3NQ + [l_off[m], l_off[m+1]).
+__device__ __forceinline__ int64_t mesh_row(int m, int64_t i, const int *__restrict__ q_off,
+                                            const int *__restrict__ l_off, int nq_total, bool *valid)
+{
+    const int64_t nq = q_off[m + 1] - q_off[m], nl = l_off[m + 1] - l_off[m];
+    *valid = i < 3 * nq + nl;
+    if (i < 3 * nq) return (i / nq) * nq_total + q_off[m] + i % nq;
+    return 3 * static_cast<int64_t>(nq_total) + l_off[m] + (i - 3 * nq);
+}
+
+__global__ void __launch_bounds__(256) mesh_weights_kernel(const int *__restrict__ q_off,
+                                                          const int *__restrict__ l_off, int nq_total,
+                                                          const double *__restrict__ w_mesh,
+                                                          const uint8_t *__restrict__ free_mask,
+                                                          double *__restrict__ wgt, double *__restrict__ iwgt)
+{
+    const int m = blockIdx.y;
+    const double w = w_mesh[m];
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
+        bool valid;
+        const int64_t r = mesh_row(m, i, q_off, l_off, nq_total, &valid);
+        if (!valid) break;
+        const bool f = free_mask[r] != 0;
+        wgt[r] = f ? w : 0.0;
+        iwgt[r] = f ? 1.0 / w : 0.0;
+    }
+}
+
+// partial[m * gridDim.x + blockIdx.x] = sum of x_r^2 over the free rows of mesh m
+__global__ void __launch_bounds__(256) mesh_norms_kernel(const int *__restrict__ q_off, const int *__restrict__ l_off,
+                                                        int nq_total, const uint8_t *__restrict__ free_mask,
+                                                        const double *__restrict__ x, double *__restrict__ partial)
+{
+    __shared__ double sh[256];
+    const int m = blockIdx.y;
+    double acc = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
+        bool valid;
+        const int64_t r = mesh_row(m, i, q_off, l_off, nq_total, &valid);
+        if (!valid) break;
+        if (free_mask[r]) acc += x[r] * x[r];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[m * gridDim.x + blockIdx.x] = sh[0];
 }
 
 // partial[j * gridDim.x + blockIdx.x] = sum over this block's slice of V_j . w
@@ -76,49 +133,36 @@ __global__ void __launch_bounds__(256) combine_kernel(int n, int k, const double
     }
 }
 
-// ---- CSR SpMV with the Dirichlet operator: y_r = sum_k v_k x_c on free rows,
-// y_r = x_r on fixed rows.  `lanes` lanes per row, shuffle reduction.
+// ---- CSR SpMV of the weighted Dirichlet operator: y_r = w_r sum_k v_k x_c
+// (w_r = 0 on fixed rows, whose sum is skipped), or y = A x when w is NULL.
+// `lanes` lanes per row, shuffle reduction.
 template <typename VT, int kLanes>
-__global__ void __launch_bounds__(256) spmv_kernel(int n, const int *__restrict__ row_ptr,
+__global__ void __launch_bounds__(256) spmv_kernel(int n, const int64_t *__restrict__ row_ptr,
                                                   const int *__restrict__ col_idx, const VT *__restrict__ vals,
-                                                  const uint8_t *__restrict__ free_mask, const double *__restrict__ x,
+                                                  const double *__restrict__ w, const double *__restrict__ x,
                                                   double *__restrict__ y)
 {
     const int lane = threadIdx.x % kLanes;
-    const int row = (blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
+    const int row = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes);
     if (row >= n) return;
     double acc = 0.0;
-    const bool is_free = free_mask == nullptr || free_mask[row];
-    if (is_free)
-        for (int k = row_ptr[row] + lane; k < row_ptr[row + 1]; k += kLanes)
+    const double wr = w == nullptr ? 1.0 : w[row];
+    if (wr != 0.0)
+        for (int64_t k = row_ptr[row] + lane; k < row_ptr[row + 1]; k += kLanes)
             acc += static_cast<double>(vals[k]) * x[col_idx[k]];
 #pragma unroll
     for (int o = kLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kLanes);
-    if (lane == 0) y[row] = is_free ? acc : x[row];
+    if (lane == 0) y[row] = wr * acc;
 }
 
 // ---- preconditioners ----------------------------------------------------------------
-__device__ __forceinline__ double csr_entry(const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
-                                            const double *__restrict__ vals, int r, int c)
-{
-    int lo = row_ptr[r], hi = row_ptr[r + 1] - 1;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const int v = col_idx[mid];
-        if (v == c) return vals[mid];
-        if (v < c) lo = mid + 1;
-        else hi = mid - 1;
-    }
-    return 0.0;
-}
-
 template <typename VT>
-__device__ __forceinline__ double csr_entry_t(const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
+__device__ __forceinline__ double csr_entry_t(const int64_t *__restrict__ row_ptr, const int *__restrict__ col_idx,
                                               const VT *__restrict__ vals, int r, int c)
 {
-    int lo = row_ptr[r], hi = row_ptr[r + 1] - 1;
+    int64_t lo = row_ptr[r], hi = row_ptr[r + 1] - 1;
     while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
+        const int64_t mid = (lo + hi) >> 1;
         const int v = col_idx[mid];
         if (v == c) return static_cast<double>(vals[mid]);
         if (v < c) lo = mid + 1;
@@ -129,7 +173,7 @@ __device__ __forceinline__ double csr_entry_t(const int *__restrict__ row_ptr, c
 
 // point Jacobi: inv[d] = 1/a_dd (1 on zero diagonals and fixed rows, as PCJACOBI)
 template <typename VT>
-__global__ void __launch_bounds__(256) jacobi_setup_kernel(int n, const int *__restrict__ row_ptr,
+__global__ void __launch_bounds__(256) jacobi_setup_kernel(int n, const int64_t *__restrict__ row_ptr,
                                                           const int *__restrict__ col_idx,
                                                           const VT *__restrict__ vals,
                                                           const uint8_t *__restrict__ free_mask,
@@ -144,7 +188,7 @@ __global__ void __launch_bounds__(256) jacobi_setup_kernel(int n, const int *__r
 // nodal 3x3 blocks of the displacement DOFs (d, d+nq, d+2nq) with fixed
 // rows/columns replaced by the identity, inverted; layout inv[node*9 + a*3+b].
 template <typename VT>
-__global__ void __launch_bounds__(256) block_setup_kernel(int nq, const int *__restrict__ row_ptr,
+__global__ void __launch_bounds__(256) block_setup_kernel(int nq, const int64_t *__restrict__ row_ptr,
                                                          const int *__restrict__ col_idx,
                                                          const VT *__restrict__ vals,
                                                          const uint8_t *__restrict__ free_mask,
@@ -184,7 +228,7 @@ __global__ void __launch_bounds__(256) block_setup_kernel(int nq, const int *__r
 // pressure rows: S_p = sum_j B_pj^2 / K_jj over free displacement columns
 // (diagonal estimate of the Schur complement B diag(K)^-1 B^T); inv = -1/S.
 template <typename VT>
-__global__ void __launch_bounds__(256) schur_setup_kernel(int nq, int nl, const int *__restrict__ row_ptr,
+__global__ void __launch_bounds__(256) schur_setup_kernel(int nq, int nl, const int64_t *__restrict__ row_ptr,
                                                          const int *__restrict__ col_idx,
                                                          const VT *__restrict__ vals,
                                                          const uint8_t *__restrict__ free_mask,
@@ -194,7 +238,7 @@ __global__ void __launch_bounds__(256) schur_setup_kernel(int nq, int nl, const 
     if (p >= nl) return;
     const int r = 3 * nq + p;
     double s = 0.0;
-    for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+    for (int64_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
         const int c = col_idx[k];
         if (c >= 3 * nq || !free_mask[c]) continue;
         const double b = static_cast<double>(vals[k]);
@@ -215,8 +259,8 @@ __global__ void __launch_bounds__(256) apply_jacobi_kernel(int n, const double *
 // (they start at pstart[row], columns being sorted), then z_u = Kblock^-1 t.
 // Fixed rows contribute 0.
 template <typename VT>
-__global__ void __launch_bounds__(256) triangular_u_kernel(int nq, const int *__restrict__ row_ptr,
-                                                          const int *__restrict__ pstart,
+__global__ void __launch_bounds__(256) triangular_u_kernel(int nq, const int64_t *__restrict__ row_ptr,
+                                                          const int64_t *__restrict__ pstart,
                                                           const int *__restrict__ col_idx,
                                                           const VT *__restrict__ vals,
                                                           const uint8_t *__restrict__ free_mask,
@@ -232,7 +276,8 @@ __global__ void __launch_bounds__(256) triangular_u_kernel(int nq, const int *__
         double acc = 0.0;
         if (free_mask[row]) {
             acc = r[row];
-            for (int k = pstart[row]; k < row_ptr[row + 1]; ++k) acc -= static_cast<double>(vals[k]) * z[col_idx[k]];
+            for (int64_t k = pstart[row]; k < row_ptr[row + 1]; ++k)
+                acc -= static_cast<double>(vals[k]) * z[col_idx[k]];
         }
         t[a] = acc;
     }
@@ -243,15 +288,15 @@ __global__ void __launch_bounds__(256) triangular_u_kernel(int nq, const int *__
 }
 
 __global__ void __launch_bounds__(256) pressure_start_kernel(int n_u_rows, int first_p_col,
-                                                            const int *__restrict__ row_ptr,
+                                                            const int64_t *__restrict__ row_ptr,
                                                             const int *__restrict__ col_idx,
-                                                            int *__restrict__ pstart)
+                                                            int64_t *__restrict__ pstart)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_u_rows) return;
-    int lo = row_ptr[r], hi = row_ptr[r + 1];
+    int64_t lo = row_ptr[r], hi = row_ptr[r + 1];
     while (lo < hi) {  // first k with col_idx[k] >= first_p_col
-        const int mid = (lo + hi) >> 1;
+        const int64_t mid = (lo + hi) >> 1;
         if (col_idx[mid] < first_p_col) lo = mid + 1;
         else hi = mid;
     }
@@ -285,6 +330,16 @@ __global__ void __launch_bounds__(256) bc_increment_kernel(int n, const int *__r
     else dofs[d] = dofs[d] + (full[i] - dofs[d]) / static_cast<double>(max_iterations - iteration + 1);
 }
 
+__global__ void __launch_bounds__(256) unit_weights_kernel(int n, const uint8_t *__restrict__ free_mask,
+                                                          double *__restrict__ wgt, double *__restrict__ iwgt)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v = free_mask[i] ? 1.0 : 0.0;
+        wgt[i] = v;
+        iwgt[i] = v;
+    }
+}
+
 unsigned grid_for(int64_t n, int per_block = 256)
 {
     const int64_t b = (n + per_block - 1) / per_block;
@@ -292,6 +347,12 @@ unsigned grid_for(int64_t n, int per_block = 256)
 }
 
 // ---- host-side solver ------------------------------------------------------------------
+// GMRES works on the weighted system W J dx = -W F, W = diag(wgt) constant on
+// each mesh of a batch (1/||F_ref|| of that mesh), with the right
+// preconditioner W^-1 P^-1: per mesh the operator is J_m P_m^-1, unscaled,
+// while the Krylov norms weigh every mesh's residual relative to its own
+// reference.  A single mesh (or the Laplace set, with unit weights) is the
+// one-block case.
 struct Krylov {
     EquationsSet &s;
     int n;
@@ -299,32 +360,34 @@ struct Krylov {
     double *V;               // (m+1) x n
     double *r, *z, *w, *u;   // work vectors
     double *inv_u, *inv_p, *diag_u, *inv_pt;  // preconditioner
-    const int *pstart = nullptr;
+    const int64_t *pstart = nullptr;
     int precond = OC_PRECONDITIONER_BLOCK_DIAGONAL;
     bool fp32 = false;
     std::vector<double> hpart;
 
+    // y = W (A x)
     int spmv(const double *x, double *y)
     {
-        Runtime &rt = runtime();
         KernelTimer timer("spmv");
         const bool lap = s.cls == OC_EQUATIONS_SET_LAPLACE;
-        if (lap) {
-            spmv_kernel<double, 8><<<grid_rows(8), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
-                                                                        s.values.ptr, s.free_mask.ptr, x, y);
-        } else if (fp32) {
-            spmv_kernel<float, 32><<<grid_rows(32), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
-                                                                         s.values_sp.ptr, s.free_mask.ptr, x, y);
-        } else {
-            spmv_kernel<double, 32><<<grid_rows(32), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
-                                                                          s.values.ptr, s.free_mask.ptr, x, y);
+        const int lanes = s.tuned_spmv_lanes > 0 ? s.tuned_spmv_lanes : (lap ? 8 : 32);
+        if (lap) return launch_spmv<double>(s.values.ptr, lanes, s.wgt.ptr, x, y);
+        if (fp32) return launch_spmv<float>(s.values_sp.ptr, lanes, s.wgt.ptr, x, y);
+        return launch_spmv<double>(s.values.ptr, lanes, s.wgt.ptr, x, y);
+    }
+    template <typename VT>
+    int launch_spmv(const VT *vals, int lanes, const double *wv, const double *x, double *y)
+    {
+        Runtime &rt = runtime();
+        const unsigned g = static_cast<unsigned>((static_cast<int64_t>(n) * lanes + 255) / 256);
+        switch (lanes) {
+        case 8: spmv_kernel<VT, 8><<<g, 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr, vals, wv, x, y); break;
+        case 16: spmv_kernel<VT, 16><<<g, 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr, vals, wv, x, y); break;
+        case 64: spmv_kernel<VT, 64><<<g, 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr, vals, wv, x, y); break;
+        default: spmv_kernel<VT, 32><<<g, 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr, vals, wv, x, y); break;
         }
         OC_HIP_CHECK(hipGetLastError());
         return OC_NO_ERROR;
-    }
-    unsigned grid_rows(int lanes) const
-    {
-        return static_cast<unsigned>((static_cast<int64_t>(n) * lanes + 255) / 256);
     }
 
     int setup_preconditioner()
@@ -333,59 +396,62 @@ struct Krylov {
         KernelTimer timer("preconditioner_setup");
         if (s.cls == OC_EQUATIONS_SET_LAPLACE || precond == OC_PRECONDITIONER_JACOBI) {
             if (fp32)
-                jacobi_setup_kernel<float><<<grid_for(n), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
-                                                                               s.values_sp.ptr, s.free_mask.ptr,
-                                                                               inv_u);
-            else
-                jacobi_setup_kernel<double><<<grid_for(n), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
-                                                                                s.values.ptr, s.free_mask.ptr,
+                jacobi_setup_kernel<float><<<grid_rows(n), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                                s.values_sp.ptr, s.free_mask.ptr,
                                                                                 inv_u);
+            else
+                jacobi_setup_kernel<double><<<grid_rows(n), 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                                 s.values.ptr, s.free_mask.ptr,
+                                                                                 inv_u);
             OC_HIP_CHECK(hipGetLastError());
             return OC_NO_ERROR;
         }
         const int nq = s.n_nodes, nl = s.n_nodes_l;
-        const unsigned gq = static_cast<unsigned>((nq + 255) / 256), gl = static_cast<unsigned>((nl + 255) / 256);
         if (fp32) {
-            block_setup_kernel<float><<<gq, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.col_idx.ptr, s.values_sp.ptr,
-                                                                 s.free_mask.ptr, inv_u, diag_u);
-            schur_setup_kernel<float><<<gl, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.col_idx.ptr,
-                                                                 s.values_sp.ptr, s.free_mask.ptr, diag_u, inv_p);
+            block_setup_kernel<float><<<grid_rows(nq), 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                            s.values_sp.ptr, s.free_mask.ptr, inv_u,
+                                                                            diag_u);
+            schur_setup_kernel<float><<<grid_rows(nl), 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                            s.values_sp.ptr, s.free_mask.ptr, diag_u,
+                                                                            inv_p);
         } else {
-            block_setup_kernel<double><<<gq, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.col_idx.ptr, s.values.ptr,
-                                                                  s.free_mask.ptr, inv_u, diag_u);
-            schur_setup_kernel<double><<<gl, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.col_idx.ptr,
-                                                                  s.values.ptr, s.free_mask.ptr, diag_u, inv_p);
+            block_setup_kernel<double><<<grid_rows(nq), 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                             s.values.ptr, s.free_mask.ptr, inv_u,
+                                                                             diag_u);
+            schur_setup_kernel<double><<<grid_rows(nl), 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.col_idx.ptr,
+                                                                             s.values.ptr, s.free_mask.ptr, diag_u,
+                                                                             inv_p);
         }
         OC_HIP_CHECK(hipGetLastError());
         return OC_NO_ERROR;
     }
+    static unsigned grid_rows(int64_t count) { return static_cast<unsigned>(std::max<int64_t>(1, (count + 255) / 256)); }
 
-    // z = P^-1 r (block upper-triangular: z_p = S^-1 r_p, z_u = K^-1 (r_u - B^T z_p))
+    // z = W^-1 P^-1 r, with P block upper-triangular (z_p = S^-1 r_p, z_u =
+    // K^-1 (r_u - B^T z_p)), block diagonal or point Jacobi
     int apply_preconditioner(const double *rin, double *zout)
     {
         Runtime &rt = runtime();
         if (s.cls == OC_EQUATIONS_SET_LAPLACE || precond == OC_PRECONDITIONER_JACOBI) {
             apply_jacobi_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, inv_u, rin, zout);
-            OC_HIP_CHECK(hipGetLastError());
-            return OC_NO_ERROR;
-        }
-        const int nq = s.n_nodes, nl = s.n_nodes_l;
-        if (precond == OC_PRECONDITIONER_BLOCK_TRIANGULAR) {
-            // z_p = S^-1 r_p ; z_u = Kblock^-1 (r_u - B^T z_p)
-            apply_jacobi_kernel<<<grid_for(nl), 256, 0, rt.stream>>>(nl, inv_p, rin + 3 * nq, zout + 3 * nq);
-            const unsigned g = static_cast<unsigned>((nq + 255) / 256);
-            if (fp32)
-                triangular_u_kernel<float><<<g, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, pstart, s.col_idx.ptr,
-                                                                     s.values_sp.ptr, s.free_mask.ptr, inv_u, rin,
-                                                                     zout);
-            else
-                triangular_u_kernel<double><<<g, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, pstart, s.col_idx.ptr,
-                                                                      s.values.ptr, s.free_mask.ptr, inv_u, rin,
-                                                                      zout);
         } else {
-            apply_block_kernel<<<static_cast<unsigned>((std::max(nq, nl) + 255) / 256), 256, 0, rt.stream>>>(
-                nq, nl, inv_u, inv_p, rin, zout);
+            const int nq = s.n_nodes, nl = s.n_nodes_l;
+            if (precond == OC_PRECONDITIONER_BLOCK_TRIANGULAR) {
+                apply_jacobi_kernel<<<grid_for(nl), 256, 0, rt.stream>>>(nl, inv_p, rin + 3 * static_cast<int64_t>(nq),
+                                                                         zout + 3 * static_cast<int64_t>(nq));
+                if (fp32)
+                    triangular_u_kernel<float><<<grid_rows(nq), 256, 0, rt.stream>>>(
+                        nq, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values_sp.ptr, s.free_mask.ptr, inv_u, rin, zout);
+                else
+                    triangular_u_kernel<double><<<grid_rows(nq), 256, 0, rt.stream>>>(
+                        nq, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values.ptr, s.free_mask.ptr, inv_u, rin, zout);
+            } else {
+                apply_block_kernel<<<grid_rows(std::max(nq, nl)), 256, 0, rt.stream>>>(nq, nl, inv_u, inv_p, rin,
+                                                                                       zout);
+            }
         }
+        OC_HIP_CHECK(hipGetLastError());
+        scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.iwgt.ptr, zout);
         OC_HIP_CHECK(hipGetLastError());
         return OC_NO_ERROR;
     }
@@ -431,8 +497,8 @@ struct Krylov {
         return OC_NO_ERROR;
     }
 
-    // Solves A x = b (x starts at 0) to ||b - A x|| <= tol_abs; b has zeros on
-    // fixed DOFs.  Right preconditioning: x = P^-1 V y.
+    // Solves W A x = b (x starts at 0) to ||b - W A x|| <= tol_abs; b is zero
+    // on fixed DOFs.  Right preconditioning: x = W^-1 P^-1 V y.
     int gmres(const double *b, double *x, double tol_abs, int max_iter, int *iters, double *res_out)
     {
         Runtime &rt = runtime();
@@ -444,7 +510,7 @@ struct Krylov {
         std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 2), h2(m + 2);
         bool first = true;
         while (total < max_iter && res > tol_abs) {
-            // r = b - A x
+            // r = b - W A x
             if (first) {
                 OC_HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
                 first = false;
@@ -452,7 +518,6 @@ struct Krylov {
                 OC_CHECK(spmv(x, w));
                 OC_HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
                 axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, -1.0, w, 1.0, r);
-                mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, r);
                 OC_CHECK(norm(r, &beta));
                 res = beta;
                 if (res <= tol_abs) break;
@@ -464,9 +529,7 @@ struct Krylov {
             for (; j < m && total < max_iter; ++j, ++total) {
                 double *vj = V + static_cast<int64_t>(j) * n, *vn = V + static_cast<int64_t>(j + 1) * n;
                 OC_CHECK(apply_preconditioner(vj, z));
-                mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, z);
                 OC_CHECK(spmv(z, vn));
-                mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, vn);
                 // classical Gram-Schmidt: one fused reduction gives V_0..V_j . w
                 // and w . w (V_{j+1} is w itself); the norm after projection
                 // follows by Pythagoras unless cancellation asks for a second
@@ -510,17 +573,16 @@ struct Krylov {
                     break;
                 }
             }
-            // y = H^-1 g ; x += P^-1 (V y)
+            // y = H^-1 g ; x += W^-1 P^-1 (V y)
             for (int i = j - 1; i >= 0; --i) {
                 double t = g[i];
                 for (int k = i + 1; k < j; ++k) t -= H[static_cast<size_t>(i) * m + k] * y[k];
                 y[i] = t / H[static_cast<size_t>(i) * m + i];
             }
-            OC_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(double) * n, rt.stream));
+            OC_CHECK(fill<double>(w, n, 0.0));
             std::vector<double> yc(y.begin(), y.begin() + j);
             OC_CHECK(combine(V, j, yc, 1.0, w));
             OC_CHECK(apply_preconditioner(w, z));
-            mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, z);
             axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0, z, 1.0, x);
             if (j == 0) break;
         }
@@ -537,18 +599,22 @@ struct Krylov {
 static int solver_prepare(EquationsSet &s, int restart, Krylov &K)
 {
     const int n = s.n_rows;
+    if (restart < 1 || restart + 1 > 1024) return set_error(OC_ERR_INVALID_ARGUMENT, "krylov_restart must be in [1, 1023]");
     if (!s.free_mask.ptr) {
         OC_CHECK(s.free_mask.alloc(n));
         OC_HIP_CHECK(hipMemset(s.free_mask.ptr, 1, n));
     }
+    if (!s.wgt.ptr) OC_CHECK(s.wgt.alloc(n));
+    if (!s.iwgt.ptr) OC_CHECK(s.iwgt.alloc(n));
     if (s.krylov_capacity < restart) {
         OC_CHECK(s.krylov.alloc(static_cast<size_t>(restart + 1) * n));
         s.krylov_capacity = restart;
     }
     if (!s.work.ptr) OC_CHECK(s.work.alloc(static_cast<size_t>(8) * n));
     if (!s.precond.ptr) OC_CHECK(s.precond.alloc(static_cast<size_t>(9) * s.n_nodes + 3 * static_cast<size_t>(n)));
-    if (!s.partials.ptr) OC_CHECK(s.partials.alloc(static_cast<size_t>(kReduceBlocks) * (1024 + 1)));
-    if (restart + 1 > 1024) return set_error(OC_ERR_INVALID_ARGUMENT, "krylov_restart must be <= 1023");
+    const size_t npart = std::max<size_t>(static_cast<size_t>(kReduceBlocks) * (1024 + 1),
+                                          static_cast<size_t>(s.n_meshes) * 256);
+    if (!s.partials.ptr || s.partials.n < npart) OC_CHECK(s.partials.alloc(npart));
     K.n = n;
     K.m = restart;
     K.V = s.krylov.ptr;
@@ -576,6 +642,235 @@ static int solver_prepare(EquationsSet &s, int restart, Krylov &K)
             OC_HIP_CHECK(hipGetLastError());
         }
         K.pstart = s.pstart.ptr;
+    }
+    return OC_NO_ERROR;
+}
+
+int launch_spmv_lanes(EquationsSet &s, bool fp32, int lanes, const double *x, double *y)
+{
+    Krylov K{s};
+    K.n = s.n_rows;
+    if (fp32) return K.launch_spmv<float>(s.values_sp.ptr, lanes, nullptr, x, y);
+    return K.launch_spmv<double>(s.values.ptr, lanes, nullptr, x, y);
+}
+
+// Per-mesh norms of x over free rows (Mooney-Rivlin sets), one device->host copy.
+static int mesh_norms(EquationsSet &s, const double *x, std::vector<double> &out)
+{
+    Runtime &rt = runtime();
+    const int M = s.n_meshes;
+    const int chunks = M == 1 ? 256 : 32;
+    dim3 grid(chunks, M);
+    mesh_norms_kernel<<<grid, 256, 0, rt.stream>>>(s.q_off.ptr, s.l_off.ptr, s.n_nodes, s.free_mask.ptr, x,
+                                                   s.partials.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    std::vector<double> h(static_cast<size_t>(M) * chunks);
+    OC_HIP_CHECK(hipMemcpyAsync(h.data(), s.partials.ptr, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                                rt.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+    out.assign(M, 0.0);
+    for (int m = 0; m < M; ++m) {
+        double acc = 0.0;
+        for (int c = 0; c < chunks; ++c) acc += h[static_cast<size_t>(m) * chunks + c];
+        out[m] = std::sqrt(acc);
+    }
+    return OC_NO_ERROR;
+}
+
+static int residual_mesh_norms(EquationsSet &s, std::vector<double> &out)
+{
+    OC_CHECK(launch_mr_residual(s));
+    return mesh_norms(s, s.residual.ptr, out);
+}
+
+// Jacobian assembly (and, with K, the preconditioner setup) of one Newton
+// step, recorded once as a graph of kernel nodes and replayed on later steps.
+static int step_assembly(EquationsSet &s, Krylov *K, int precision)
+{
+    Runtime &rt = runtime();
+    const int key[4] = {precision, K ? K->precond : -1, s.tuned_element_blocks_per_cu, s.tuned_gather_chunk};
+    if (!s.step_graph || !std::equal(key, key + 4, s.step_graph_key)) {
+        if (s.step_graph) {
+            (void)hipGraphExecDestroy(s.step_graph);
+            s.step_graph = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        OC_HIP_CHECK(hipStreamBeginCapture(rt.stream, hipStreamCaptureModeThreadLocal));
+        set_capturing(true);
+        int rc = launch_mr_jacobian(s, precision);
+        if (rc == OC_NO_ERROR && K) rc = K->setup_preconditioner();
+        set_capturing(false);
+        const hipError_t ec = hipStreamEndCapture(rt.stream, &graph);
+        OC_CHECK(rc);
+        if (ec != hipSuccess) return set_error(OC_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&s.step_graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            s.step_graph = nullptr;
+            return set_error(OC_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        std::copy(key, key + 4, s.step_graph_key);
+    }
+    KernelTimer timer("step_graph");
+    OC_HIP_CHECK(hipGraphLaunch(s.step_graph, rt.stream));
+    return OC_NO_ERROR;
+}
+
+static double weighted_norm(const std::vector<double> &nrm, const std::vector<double> &w)
+{
+    double acc = 0.0;
+    for (size_t m = 0; m < nrm.size(); ++m) acc += (w[m] * nrm[m]) * (w[m] * nrm[m]);
+    return std::sqrt(acc);
+}
+
+static double plain_norm(const std::vector<double> &nrm)
+{
+    double acc = 0.0;
+    for (double v : nrm) acc += v * v;
+    return std::sqrt(acc);
+}
+
+// Inexact Newton with backtracking line search on every mesh of the set at
+// once; mesh m has converged when ||F_m|| <= max(atol, rtol * ref_m).
+static int newton_core(EquationsSet &s, const oc_newton_options *opt, const double *refs_in, double *finals_out,
+                       oc_newton_stats *stats)
+{
+    if (opt->krylov_restart < 1 || opt->maximum_iterations < 0)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "invalid Newton options");
+    if (opt->jacobian_precision != OC_FP64 && opt->jacobian_precision != OC_FP32)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "jacobian_precision must be OC_FP64 or OC_FP32");
+    const auto t0 = std::chrono::steady_clock::now();
+    *stats = oc_newton_stats{};
+    OC_CHECK(mr_prepare(s));
+    Krylov K{s};
+    K.precond = opt->preconditioner;
+    K.fp32 = opt->jacobian_precision == OC_FP32;
+    OC_CHECK(solver_prepare(s, opt->krylov_restart, K));
+    OC_CHECK(mr_jacobian_buffers(s, opt->jacobian_precision));
+    const BasisTables *T = nullptr;
+    OC_CHECK(ensure_tables(&T));
+    Runtime &rt = runtime();
+    const int n = s.n_rows, M = s.n_meshes;
+    double *x_prev = s.work.ptr, *dx = s.work.ptr + n, *rhs = s.work.ptr + 2 * static_cast<int64_t>(n);
+
+    std::vector<double> nrm, ref(M), w(M), target(M);
+    OC_CHECK(residual_mesh_norms(s, nrm));
+    stats->residual_evaluations++;
+    double min_wt = HUGE_VAL;
+    for (int m = 0; m < M; ++m) {
+        ref[m] = refs_in ? refs_in[m] : nrm[m];
+        w[m] = ref[m] > 0.0 ? 1.0 / ref[m] : 1.0;
+        target[m] = std::max(opt->absolute_tolerance, opt->relative_tolerance * ref[m]);
+        min_wt = std::min(min_wt, w[m] * target[m]);
+    }
+    {
+        DevBuf<double> wd;
+        OC_CHECK(wd.upload(w.data(), w.size()));
+        dim3 grid(64, M);
+        mesh_weights_kernel<<<grid, 256, 0, rt.stream>>>(s.q_off.ptr, s.l_off.ptr, s.n_nodes, wd.ptr,
+                                                         s.free_mask.ptr, s.wgt.ptr, s.iwgt.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+        OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+    }
+    auto converged = [&](const std::vector<double> &v) {
+        for (int m = 0; m < M; ++m)
+            if (!(v[m] <= target[m])) return false;
+        return true;
+    };
+    double fnorm = weighted_norm(nrm, w);
+    const double ref_global = weighted_norm(ref, w);
+    stats->initial_residual_norm = plain_norm(nrm);
+    double fnorm_prev = fnorm, eta_prev = 0.5;
+    int it = 0;
+    while (!converged(nrm) && it < opt->maximum_iterations) {
+        if (!std::isfinite(fnorm)) break;
+        OC_CHECK(step_assembly(s, &K, opt->jacobian_precision));
+        stats->jacobian_evaluations++;
+        // forcing term: fixed, or Eisenstat-Walker (choice 2, gamma 0.9, alpha
+        // (1+sqrt5)/2) capped by ||F|| / ||F_ref||, the Dembo-Eisenstat-Steihaug
+        // rule that keeps Newton quadratic once the iterate is close: a start
+        // already near the solution is solved accurately from its first step
+        double eta = opt->krylov_relative_tolerance;
+        if (eta <= 0.0) {
+            if (it == 0) eta = 0.5;
+            else {
+                eta = 0.9 * std::pow(fnorm / fnorm_prev, 1.618);
+                const double safe = 0.9 * std::pow(eta_prev, 1.618);
+                if (safe > 0.1) eta = std::max(eta, safe);
+                eta = std::min(eta, 0.5);
+            }
+            if (ref_global > 0.0) eta = std::min(eta, fnorm / ref_global);
+            eta = std::max(eta, 0.5 * min_wt / fnorm);  // do not over-solve the last step
+            eta = std::max(eta, 1e-14);
+            eta_prev = eta;
+        }
+        // rhs = -W F (zero on fixed DOFs)
+        weighted_copy_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, -1.0, s.wgt.ptr, s.residual.ptr, rhs);
+        OC_HIP_CHECK(hipGetLastError());
+        int kit = 0;
+        double kres = 0.0;
+        OC_CHECK(K.gmres(rhs, dx, eta * fnorm, opt->krylov_maximum_iterations, &kit, &kres));
+        stats->krylov_iterations += kit;
+        // line search on f = 1/2 ||W F||^2 with slope -||W F||^2 (Newton direction)
+        OC_HIP_CHECK(hipMemcpyAsync(x_prev, s.dofs.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
+        double lambda = 1.0, fnew = 0.0;
+        const double f0 = 0.5 * fnorm * fnorm, slope = -fnorm * fnorm;
+        double lambda_prev = 0.0, f_prev_trial = 0.0;
+        std::vector<double> trial;
+        for (int ls = 0; ls < 20; ++ls) {
+            OC_HIP_CHECK(hipMemcpyAsync(s.dofs.ptr, x_prev, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                        rt.stream));
+            axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, lambda, dx, 1.0, s.dofs.ptr);
+            OC_CHECK(residual_mesh_norms(s, trial));
+            fnew = weighted_norm(trial, w);
+            stats->residual_evaluations++;
+            const double f1 = 0.5 * fnew * fnew;
+            if (!opt->line_search || (std::isfinite(f1) && f1 <= f0 + opt->line_search_alpha * lambda * slope))
+                break;
+            // quadratic (first backtrack) or cubic interpolation, safeguarded to [0.1, 0.5] lambda
+            double lnew;
+            if (ls == 0 || !std::isfinite(f1)) {
+                lnew = std::isfinite(f1) ? -slope / (2.0 * (f1 - f0 - slope)) : 0.1;
+            } else {
+                const double r1 = f1 - f0 - lambda * slope, r2 = f_prev_trial - f0 - lambda_prev * slope;
+                const double a = (r1 / (lambda * lambda) - r2 / (lambda_prev * lambda_prev)) / (lambda - lambda_prev);
+                const double b = (-lambda_prev * r1 / (lambda * lambda) + lambda * r2 / (lambda_prev * lambda_prev)) /
+                                 (lambda - lambda_prev);
+                if (a == 0.0) lnew = -slope / (2.0 * b);
+                else {
+                    const double disc = b * b - 3.0 * a * slope;
+                    lnew = disc < 0.0 ? 0.5 * lambda : (-b + std::sqrt(disc)) / (3.0 * a);
+                }
+            }
+            lambda_prev = lambda;
+            f_prev_trial = f1;
+            lambda = std::min(0.5 * lambda, std::max(0.1 * lambda, lnew));
+        }
+        nrm = trial;
+        fnorm_prev = fnorm;
+        fnorm = fnew;
+        ++it;
+    }
+    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+    stats->newton_iterations = it;
+    stats->final_residual_norm = plain_norm(nrm);
+    stats->converged = converged(nrm) ? 1 : 0;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (finals_out)
+        for (int m = 0; m < M; ++m) finals_out[m] = nrm[m];
+    if (!stats->converged) {
+        int worst = 0;
+        double worst_ratio = -1.0;
+        for (int m = 0; m < M; ++m) {
+            const double ratio = std::isfinite(nrm[m]) ? nrm[m] / std::max(target[m], 1e-300) : HUGE_VAL;
+            if (ratio > worst_ratio) {
+                worst_ratio = ratio;
+                worst = m;
+            }
+        }
+        return set_error(OC_ERR_NOT_CONVERGED,
+                         "Newton did not converge after %d iterations: mesh %d ||F|| = %.3e > %.3e", it, worst,
+                         nrm[worst], target[worst]);
     }
     return OC_NO_ERROR;
 }
@@ -642,133 +937,67 @@ int oc_newton_options_default(oc_newton_options *o)
     return OC_NO_ERROR;
 }
 
-static int masked_residual_norm(EquationsSet &s, Krylov &K, double *out)
+static int mr_solver_ready(oc_equations_set h)
 {
-    OC_CHECK(launch_mr_residual(s));
-    mask_kernel<<<grid_for(s.n_rows), 256, 0, runtime().stream>>>(s.n_rows, s.free_mask.ptr, s.residual.ptr);
-    OC_HIP_CHECK(hipGetLastError());
-    return K.norm(s.residual.ptr, out);
+    OC_CHECK(require_runtime());
+    if (!h) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
+    if (h->s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "defined for Mooney-Rivlin sets");
+    OC_CHECK(mr_prepare(h->s));
+    Krylov K{h->s};
+    return solver_prepare(h->s, 1, K);
 }
 
 int oc_residual_norm(oc_equations_set h, double *norm)
 {
-    OC_CHECK(require_runtime());
-    if (!h || !norm) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
-    EquationsSet &s = h->s;
-    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
-        return set_error(OC_ERR_INVALID_ARGUMENT, "residual norm is defined for Mooney-Rivlin sets");
-    OC_CHECK(mr_prepare(s));
-    Krylov K{s};
-    OC_CHECK(solver_prepare(s, 1, K));
-    return masked_residual_norm(s, K, norm);
+    if (!norm) return set_error(OC_ERR_INVALID_ARGUMENT, "norm is NULL");
+    OC_CHECK(mr_solver_ready(h));
+    std::vector<double> v;
+    OC_CHECK(residual_mesh_norms(h->s, v));
+    *norm = plain_norm(v);
+    return OC_NO_ERROR;
+}
+
+int oc_mesh_residual_norms(oc_equations_set h, double *norms)
+{
+    if (!norms) return set_error(OC_ERR_INVALID_ARGUMENT, "norms is NULL");
+    OC_CHECK(mr_solver_ready(h));
+    std::vector<double> v;
+    OC_CHECK(residual_mesh_norms(h->s, v));
+    std::copy(v.begin(), v.end(), norms);
+    return OC_NO_ERROR;
 }
 
 int oc_newton_solve(oc_equations_set h, const oc_newton_options *opt, oc_newton_stats *stats)
 {
-    OC_CHECK(require_runtime());
-    if (!h || !opt || !stats) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
-    EquationsSet &s = h->s;
-    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
-        return set_error(OC_ERR_INVALID_ARGUMENT, "Newton solve is defined for Mooney-Rivlin sets");
-    if (opt->krylov_restart < 1 || opt->maximum_iterations < 0)
-        return set_error(OC_ERR_INVALID_ARGUMENT, "invalid Newton options");
-    const auto t0 = std::chrono::steady_clock::now();
-    *stats = oc_newton_stats{};
-    OC_CHECK(mr_prepare(s));
-    Krylov K{s};
-    K.precond = opt->preconditioner;
-    K.fp32 = opt->jacobian_precision == OC_FP32;
-    OC_CHECK(solver_prepare(s, opt->krylov_restart, K));
-    if (K.fp32 && !s.values_sp.ptr) OC_CHECK(s.values_sp.alloc(s.nnz));
-    if (!K.fp32 && !s.values.ptr) OC_CHECK(s.values.alloc(s.nnz));
-    Runtime &rt = runtime();
-    const int n = s.n_rows;
-    double *x_prev = s.work.ptr, *dx = s.work.ptr + n, *rhs = s.work.ptr + 2 * static_cast<int64_t>(n);
+    if (!opt || !stats) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    OC_CHECK(mr_solver_ready(h));
+    if (h->s.n_meshes != 1 && opt->reference_norm > 0.0)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "a batch takes per-mesh reference norms (oc_newton_solve_batch)");
+    std::vector<double> refs(h->s.n_meshes, opt->reference_norm);
+    return newton_core(h->s, opt, opt->reference_norm > 0.0 ? refs.data() : nullptr, nullptr, stats);
+}
 
-    double fnorm = 0.0;
-    OC_CHECK(masked_residual_norm(s, K, &fnorm));
-    stats->residual_evaluations++;
-    stats->initial_residual_norm = fnorm;
-    const double ref = opt->reference_norm > 0.0 ? opt->reference_norm : fnorm;
-    const double target = std::max(opt->absolute_tolerance, opt->relative_tolerance * ref);
-    double fnorm_prev = fnorm, eta_prev = 0.5;
-    int it = 0;
-    while (!(fnorm <= target) && it < opt->maximum_iterations) {
-        if (!std::isfinite(fnorm)) break;
-        // Jacobian and preconditioner at the current DOFs
-        OC_CHECK(launch_mr_jacobian(s, opt->jacobian_precision));
-        stats->jacobian_evaluations++;
-        OC_CHECK(K.setup_preconditioner());
-        // forcing term: fixed, or Eisenstat-Walker (choice 2, gamma 0.9, alpha
-        // (1+sqrt5)/2) capped by ||F|| / ||F_ref||, the Dembo-Eisenstat-Steihaug
-        // rule that keeps Newton quadratic once the iterate is close: a start
-        // already near the solution is solved accurately from its first step
-        double eta = opt->krylov_relative_tolerance;
-        if (eta <= 0.0) {
-            if (it == 0) eta = 0.5;
-            else {
-                eta = 0.9 * std::pow(fnorm / fnorm_prev, 1.618);
-                const double safe = 0.9 * std::pow(eta_prev, 1.618);
-                if (safe > 0.1) eta = std::max(eta, safe);
-                eta = std::min(eta, 0.5);
-            }
-            eta = std::min(eta, fnorm / ref);
-            eta = std::max(eta, 0.5 * target / fnorm);  // do not over-solve the last step
-            eta = std::max(eta, 1e-14);
-            eta_prev = eta;
-        }
-        // rhs = -F on free DOFs
-        copy_scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, -1.0, s.residual.ptr, rhs);
-        int kit = 0;
-        double kres = 0.0;
-        OC_CHECK(K.gmres(rhs, dx, eta * fnorm, opt->krylov_maximum_iterations, &kit, &kres));
-        stats->krylov_iterations += kit;
-        // line search on f = 1/2 ||F||^2 with slope -||F||^2 (Newton direction)
-        OC_HIP_CHECK(hipMemcpyAsync(x_prev, s.dofs.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
-        double lambda = 1.0, fnew = 0.0;
-        const double f0 = 0.5 * fnorm * fnorm, slope = -fnorm * fnorm;
-        double lambda_prev = 0.0, f_prev_trial = 0.0;
-        for (int ls = 0; ls < 20; ++ls) {
-            OC_HIP_CHECK(hipMemcpyAsync(s.dofs.ptr, x_prev, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                                        rt.stream));
-            axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, lambda, dx, 1.0, s.dofs.ptr);
-            OC_CHECK(masked_residual_norm(s, K, &fnew));
-            stats->residual_evaluations++;
-            const double f1 = 0.5 * fnew * fnew;
-            if (!opt->line_search || (std::isfinite(f1) && f1 <= f0 + opt->line_search_alpha * lambda * slope))
-                break;
-            // quadratic (first backtrack) or cubic interpolation, safeguarded to [0.1, 0.5] lambda
-            double lnew;
-            if (ls == 0 || !std::isfinite(f1)) {
-                lnew = std::isfinite(f1) ? -slope / (2.0 * (f1 - f0 - slope)) : 0.1;
-            } else {
-                const double r1 = f1 - f0 - lambda * slope, r2 = f_prev_trial - f0 - lambda_prev * slope;
-                const double a = (r1 / (lambda * lambda) - r2 / (lambda_prev * lambda_prev)) / (lambda - lambda_prev);
-                const double b = (-lambda_prev * r1 / (lambda * lambda) + lambda * r2 / (lambda_prev * lambda_prev)) /
-                                 (lambda - lambda_prev);
-                if (a == 0.0) lnew = -slope / (2.0 * b);
-                else {
-                    const double disc = b * b - 3.0 * a * slope;
-                    lnew = disc < 0.0 ? 0.5 * lambda : (-b + std::sqrt(disc)) / (3.0 * a);
-                }
-            }
-            lambda_prev = lambda;
-            f_prev_trial = f1;
-            lambda = std::min(0.5 * lambda, std::max(0.1 * lambda, lnew));
-        }
-        fnorm_prev = fnorm;
-        fnorm = fnew;
-        ++it;
-    }
-    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
-    stats->newton_iterations = it;
-    stats->final_residual_norm = fnorm;
-    stats->converged = fnorm <= target ? 1 : 0;
-    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (!stats->converged)
-        return set_error(OC_ERR_NOT_CONVERGED, "Newton did not converge: ||F|| = %.3e > %.3e after %d iterations",
-                         fnorm, target, it);
-    return OC_NO_ERROR;
+int oc_newton_solve_batch(oc_equations_set h, const oc_newton_options *opt, const double *reference_norms,
+                          double *final_norms, oc_newton_stats *stats)
+{
+    if (!opt || !stats) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    OC_CHECK(mr_solver_ready(h));
+    return newton_core(h->s, opt, reference_norms, final_norms, stats);
+}
+
+int oc_jacobian_evaluate_graph(oc_equations_set h, int precision)
+{
+    OC_CHECK(require_runtime());
+    if (!h) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
+    if (h->s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "Jacobian evaluation is defined for Mooney-Rivlin sets only");
+    if (precision != OC_FP64 && precision != OC_FP32)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "precision must be OC_FP64 or OC_FP32");
+    OC_CHECK(mr_jacobian_buffers(h->s, precision));
+    const BasisTables *T = nullptr;
+    OC_CHECK(ensure_tables(&T));
+    return step_assembly(h->s, nullptr, precision);
 }
 
 int oc_linear_solve(oc_equations_set h, const oc_newton_options *opt, oc_newton_stats *stats)
@@ -786,15 +1015,16 @@ int oc_linear_solve(oc_equations_set h, const oc_newton_options *opt, oc_newton_
     Runtime &rt = runtime();
     const int n = s.n_rows;
     double *rhs = s.work.ptr + 2 * static_cast<int64_t>(n), *dx = s.work.ptr + n, *tmp = s.work.ptr + 7 * static_cast<int64_t>(n);
+    unit_weights_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, s.wgt.ptr, s.iwgt.ptr);
+    OC_HIP_CHECK(hipGetLastError());
     OC_CHECK(launch_laplace_assemble(s));
     // u = Dirichlet values on fixed DOFs, 0 elsewhere; rhs = RHS - K u on free rows
-    OC_HIP_CHECK(hipMemsetAsync(s.dofs.ptr, 0, sizeof(double) * n, rt.stream));
+    OC_CHECK(fill<double>(s.dofs.ptr, n, 0.0));
     if (s.n_bc) OC_CHECK(oc_load_increment_apply(h, 1, 1));
-    spmv_kernel<double, 8><<<static_cast<unsigned>((static_cast<int64_t>(n) * 8 + 255) / 256), 256, 0, rt.stream>>>(
-        n, s.row_ptr.ptr, s.col_idx.ptr, s.values.ptr, nullptr, s.dofs.ptr, tmp);
+    OC_CHECK(K.launch_spmv<double>(s.values.ptr, 8, nullptr, s.dofs.ptr, tmp));
     OC_HIP_CHECK(hipMemcpyAsync(rhs, s.rhs.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
     axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, -1.0, tmp, 1.0, rhs);
-    mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, rhs);
+    scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.wgt.ptr, rhs);
     OC_CHECK(K.setup_preconditioner());
     double bnorm = 0.0;
     OC_CHECK(K.norm(rhs, &bnorm));
@@ -805,10 +1035,9 @@ int oc_linear_solve(oc_equations_set h, const oc_newton_options *opt, oc_newton_
     OC_CHECK(K.gmres(rhs, dx, target, opt->krylov_maximum_iterations, &kit, &kres));
     axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0, dx, 1.0, s.dofs.ptr);
     // true residual of the free rows
-    spmv_kernel<double, 8><<<static_cast<unsigned>((static_cast<int64_t>(n) * 8 + 255) / 256), 256, 0, rt.stream>>>(
-        n, s.row_ptr.ptr, s.col_idx.ptr, s.values.ptr, nullptr, s.dofs.ptr, tmp);
+    OC_CHECK(K.launch_spmv<double>(s.values.ptr, 8, nullptr, s.dofs.ptr, tmp));
     axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0, s.rhs.ptr, -1.0, tmp);
-    mask_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.free_mask.ptr, tmp);
+    scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, s.wgt.ptr, tmp);
     double fres = 0.0;
     OC_CHECK(K.norm(tmp, &fres));
     stats->krylov_iterations = kit;

@@ -220,6 +220,18 @@ class MooneyRivlinEquationsSet(EquationsSet):
     def jacobian_evaluate(self, precision=_lib.OC_FP64):
         call("oc_equations_set_jacobian_evaluate", self._h, int(precision))
 
+    def jacobian_evaluate_graph(self, precision=_lib.OC_FP64):
+        """Jacobian assembly replayed from the set's recorded HIP graph."""
+        call("oc_jacobian_evaluate_graph", self._h, int(precision))
+
+    def autotune(self, precision=_lib.OC_FP64):
+        """Times the launch-shape candidates on this set, keeps the fastest;
+        returns the timings (dict)."""
+        import json
+        buf = ctypes.create_string_buffer(4096)
+        call("oc_autotune", self._h, int(precision), buf, 4096)
+        return json.loads(buf.value.decode())
+
     def residual_norm(self):
         """||R|| over the free DOFs at the current dependent field."""
         v = ctypes.c_double(0.0)
@@ -245,3 +257,96 @@ class MooneyRivlinEquationsSet(EquationsSet):
                 np.full(len(idx[3]), float(stretch))]
         self.set_boundary_conditions(np.concatenate(idx), np.concatenate(vals))
         return np.concatenate(idx)
+
+
+class MooneyRivlinBatch(MooneyRivlinEquationsSet):
+    """Independent Mooney-Rivlin meshes of uneven size as one equations set
+    (oc_mooney_rivlin_batch_create): one launch assembles every element of
+    every mesh; DOFs are component-major over the union of the meshes."""
+
+    def __init__(self, meshes, c10, c01):
+        for m in meshes:
+            if m.order != 2:
+                raise ValueError("the Mooney-Rivlin equations set uses triquadratic meshes")
+        k = len(meshes)
+        c10 = np.ascontiguousarray(np.broadcast_to(np.asarray(c10, np.float64), (k,)))
+        c01 = np.ascontiguousarray(np.broadcast_to(np.asarray(c01, np.float64), (k,)))
+        nq = np.array([m.n_nodes for m in meshes], np.int32)
+        nl = np.array([m.n_nodes_l for m in meshes], np.int32)
+        ne = np.array([m.n_elements for m in meshes], np.int32)
+        coords = np.concatenate([np.ascontiguousarray(m.coords, np.float64).ravel() for m in meshes])
+        cq = np.concatenate([np.ascontiguousarray(m.elem_nodes, np.int32).ravel() for m in meshes])
+        cl = np.concatenate([np.ascontiguousarray(m.elem_nodes_l, np.int32).ravel() for m in meshes])
+        h = ctypes.c_void_p()
+        call("oc_mooney_rivlin_batch_create", k, nq, nl, ne, coords, cq, cl, c10, c01, 0, ctypes.byref(h))
+        EquationsSet.__init__(self, h)
+        self.meshes = list(meshes)
+        self.mesh = None
+        self.c10s, self.c01s = c10.copy(), c01.copy()
+        self.c10, self.c01 = float(c10[0]), float(c01[0])
+        self.q_off = np.concatenate([[0], np.cumsum(nq)]).astype(np.int64)
+        self.l_off = np.concatenate([[0], np.cumsum(nl)]).astype(np.int64)
+        self.e_off = np.concatenate([[0], np.cumsum(ne)]).astype(np.int64)
+        self.n_nodes_total = int(self.q_off[-1])
+
+    @property
+    def n_meshes(self):
+        return len(self.meshes)
+
+    def mesh_dofs(self, m):
+        """Union DOF of each local DOF (z1 nodes, z2, z3, p) of mesh m."""
+        NQ = self.n_nodes_total
+        q0, q1, l0, l1 = self.q_off[m], self.q_off[m + 1], self.l_off[m], self.l_off[m + 1]
+        return np.concatenate([c * NQ + np.arange(q0, q1) for c in range(3)] + [3 * NQ + np.arange(l0, l1)])
+
+    def set_materials(self, c10, c01):
+        c10 = np.ascontiguousarray(np.broadcast_to(np.asarray(c10, np.float64), (self.n_meshes,)))
+        c01 = np.ascontiguousarray(np.broadcast_to(np.asarray(c01, np.float64), (self.n_meshes,)))
+        call("oc_mesh_materials_set", self._h, c10, c01)
+        self.c10s, self.c01s = c10.copy(), c01.copy()
+
+    def undeformed_dofs(self):
+        d = np.zeros(self.n_rows)
+        for m, mesh in enumerate(self.meshes):
+            idx = self.mesh_dofs(m)[:3 * mesh.n_nodes]
+            d[idx] = mesh.coords.ravel()
+        return d
+
+    def uniaxial_boundary_conditions(self, stretches):
+        """The uniaxial conditions of every mesh (stretch per mesh); returns
+        the union's fixed DOFs."""
+        stretches = np.broadcast_to(np.asarray(stretches, np.float64), (self.n_meshes,))
+        idx, vals = [], []
+        for m, mesh in enumerate(self.meshes):
+            nq = mesh.n_nodes
+            loc = [mesh.face_nodes(0, 0), nq + mesh.face_nodes(1, 0), 2 * nq + mesh.face_nodes(2, 0),
+                   mesh.face_nodes(0, 1)]
+            v = [np.zeros(len(loc[0])), np.zeros(len(loc[1])), np.zeros(len(loc[2])),
+                 np.full(len(loc[3]), float(stretches[m]))]
+            idx.append(self.mesh_dofs(m)[np.concatenate(loc)])
+            vals.append(np.concatenate(v))
+        idx, vals = np.concatenate(idx), np.concatenate(vals)
+        self.set_boundary_conditions(idx, vals)
+        return idx
+
+    def mesh_residual_norms(self):
+        out = np.zeros(self.n_meshes)
+        call("oc_mesh_residual_norms", self._h, out)
+        return out
+
+    def newton_solve(self, options=None, reference_norms=None, raise_on_failure=True):
+        """Newton solve of every mesh; returns (stats dict, per-mesh final ||F||)."""
+        opts = options if options is not None else newton_options()
+        st = _lib.NewtonStats()
+        finals = np.zeros(self.n_meshes)
+        refs = None
+        if reference_norms is not None:
+            refs = np.ascontiguousarray(reference_norms, np.float64)
+            if refs.shape != (self.n_meshes,):
+                raise ValueError(f"reference_norms must have shape ({self.n_meshes},)")
+        rc = _lib.load().oc_newton_solve_batch(self._h, ctypes.byref(opts),
+                                               None if refs is None else refs.ctypes.data, finals.ctypes.data,
+                                               ctypes.byref(st))
+        if rc != 0 and (raise_on_failure or rc != _lib.OC_ERR_NOT_CONVERGED):
+            _lib.check(rc)
+        return st.as_dict(), finals

@@ -56,7 +56,7 @@ extern "C" {
 /* ---- device buffers that can be borrowed --------------------------------- */
 #define OC_BUFFER_MATRIX_VALUES 0     /* double[nnz]: linear matrix or FP64 Jacobian */
 #define OC_BUFFER_MATRIX_VALUES_SP 1  /* float[nnz]: FP32 Jacobian */
-#define OC_BUFFER_ROW_PTR 2           /* int[n_rows+1], 0-based */
+#define OC_BUFFER_ROW_PTR 2           /* int64_t[n_rows+1], 0-based (device offsets are 64-bit) */
 #define OC_BUFFER_COL_IDX 3           /* int[nnz], 0-based */
 #define OC_BUFFER_DOFS 4              /* double[n_rows]: dependent field values */
 #define OC_BUFFER_RESIDUAL 5          /* double[n_rows] */
@@ -97,18 +97,44 @@ int oc_laplace_create(int n_nodes, const double *coords, int n_elements, const i
 int oc_mooney_rivlin_create(int n_nodes_q, const double *coords_q, int n_nodes_l, int n_elements,
                             const int *elem_nodes_q, const int *elem_nodes_l, int index_base, double c10,
                             double c01, oc_equations_set *equations_set);
+/* A batch of independent Mooney-Rivlin meshes of uneven size as one equations
+ * set (no single counterpart in the reference, where each mesh is its own
+ * equations set; the element routines are those of oc_mooney_rivlin_create).
+ * Per-mesh arrays: n_nodes_q, n_nodes_l, n_elements, c10, c01 [n_meshes];
+ * coords_q concatenates each mesh's 3*n_nodes_q (separated) array;
+ * elem_nodes_q / elem_nodes_l concatenate each mesh's connectivity with
+ * mesh-local node numbers.  The set is the disjoint union: mesh m's nodes are
+ * shifted by its node offset (oc_equations_set_mesh_get) and the DOFs are
+ * component-major over the union (z1 of all nodes, z2, z3, then p), so the
+ * equations matrix is block diagonal and each assembly is one launch over
+ * every element of every mesh. */
+int oc_mooney_rivlin_batch_create(int n_meshes, const int *n_nodes_q, const int *n_nodes_l, const int *n_elements,
+                                  const double *coords_q, const int *elem_nodes_q, const int *elem_nodes_l,
+                                  const double *c10, const double *c01, int index_base,
+                                  oc_equations_set *equations_set);
+/* Number of meshes of a set (1 unless created as a batch). */
+int oc_mesh_count(oc_equations_set equations_set, int *n_meshes);
+/* Mesh `mesh` of a batch: first union quadratic node, quadratic node count,
+ * first union pressure node, pressure node count, first element, elements.
+ * DOF of (component c < 3, mesh node i): c * n_nodes_total + node_offset_q + i;
+ * pressure node i: 3 * n_nodes_total + node_offset_l + i. */
+int oc_equations_set_mesh_get(oc_equations_set equations_set, int mesh, int *node_offset_q, int *n_nodes_q,
+                              int *node_offset_l, int *n_nodes_l, int *element_offset, int *n_elements);
 int oc_equations_set_destroy(oc_equations_set equations_set);
 /* class (OC_EQUATIONS_SET_*), number of rows (= DOFs), number of nonzeros,
  * number of elements, number of element colours. */
 int oc_equations_set_info(oc_equations_set equations_set, int *set_class, int *n_rows, long long *nnz,
                           int *n_elements, int *n_colours);
-/* Materials (c10, c01) of a Mooney-Rivlin set. */
+/* Materials (c10, c01) of a Mooney-Rivlin set (every mesh of a batch). */
 int oc_materials_set(oc_equations_set equations_set, double c10, double c01);
+/* Per-mesh materials of a batch: c10[n_meshes], c01[n_meshes]. */
+int oc_mesh_materials_set(oc_equations_set equations_set, const double *c10, const double *c01);
 
 /* ---- CSR structure -------------------------------------------------------- */
 /* Copies the equations-matrix structure (row_ptr[n_rows+1], col_idx[nnz]),
  * columns sorted ascending and unique per row (cmfe_DistributedMatrix_
- * StorageLocationsGetObj analogue, opencmiss_iron.f90:44843). */
+ * StorageLocationsGetObj analogue, opencmiss_iron.f90:44843).  Fails with
+ * OC_ERR_UNSUPPORTED when nnz does not fit the 4-byte integers. */
 int oc_matrix_structure_get(oc_equations_set equations_set, int index_base, int *row_ptr, int *col_idx);
 
 /* ---- dependent field ------------------------------------------------------- */
@@ -129,6 +155,10 @@ int oc_equations_set_residual_evaluate(oc_equations_set equations_set);
  * equations_set_routines.f90:4694, 4878-5024).  precision OC_FP64 fills
  * OC_BUFFER_MATRIX_VALUES, OC_FP32 fills OC_BUFFER_MATRIX_VALUES_SP. */
 int oc_equations_set_jacobian_evaluate(oc_equations_set equations_set, int precision);
+/* The same assembly replayed from a HIP graph recorded on the first call
+ * (kernel nodes only; re-recorded when the precision or launch shapes
+ * change).  Results are identical to oc_equations_set_jacobian_evaluate. */
+int oc_jacobian_evaluate_graph(oc_equations_set equations_set, int precision);
 
 /* ---- copy-out / borrow ----------------------------------------------------- */
 int oc_matrix_values_get(oc_equations_set equations_set, double *values);
@@ -189,12 +219,28 @@ int oc_newton_options_default(oc_newton_options *options);
  * 4122-4483).  Dirichlet DOFs are eliminated (their rows and columns are
  * dropped from the linear systems, solver_matrices_routines.f90:1258-1483). */
 int oc_newton_solve(oc_equations_set equations_set, const oc_newton_options *options, oc_newton_stats *stats);
+/* Newton solve of every mesh of a batch at once (no counterpart in the
+ * reference).  Mesh m has converged when ||F_m|| <= max(absolute_tolerance,
+ * relative_tolerance * ref_m), ref_m = reference_norms[m] or, when
+ * reference_norms is NULL, mesh m's own starting residual norm; GMRES and
+ * the line search weigh each mesh's residual by 1/ref_m.  final_norms
+ * (optional, n_meshes) receives ||F_m||.  The Jacobian assembly and
+ * preconditioner setup of each Newton step replay one recorded HIP graph. */
+int oc_newton_solve_batch(oc_equations_set equations_set, const oc_newton_options *options,
+                          const double *reference_norms, double *final_norms, oc_newton_stats *stats);
 /* Static linear solve of a Laplace set (Problem_SolverEquationsStaticLinear
  * Solve, problem_routines.f90:2740-2830): assemble, set the Dirichlet values
  * and solve the free rows with the Krylov settings of `options`. */
 int oc_linear_solve(oc_equations_set equations_set, const oc_newton_options *options, oc_newton_stats *stats);
+/* Launch-shape autotuner (no counterpart in the reference): times the
+ * element-matrix kernel's grid, the CSR row gather's prefetch depth and the
+ * SpMV's lanes per row on this set at its current DOFs, keeps the fastest of
+ * each for later launches, and writes the timings as JSON into report. */
+int oc_autotune(oc_equations_set equations_set, int precision, char *report, int report_size);
 /* Euclidean norm of the residual over the free DOFs at the current DOFs. */
 int oc_residual_norm(oc_equations_set equations_set, double *norm);
+/* The same per mesh of a batch: norms[n_meshes]. */
+int oc_mesh_residual_norms(oc_equations_set equations_set, double *norms);
 
 /* ---- learned warm start (no counterpart in the reference) ------------------
  * A per-node MLP predicts each load step's nodal displacement increment as
