@@ -9,7 +9,10 @@ uniaxial stretch plus a seeded perturbation).  Inputs are resident in HBM
 before the timed region.  Prints one JSON line.
 
 Also reported (``secondary``): Laplace assembly throughput at 8^3 / 64^3 /
-128^3 trilinear and the FP32-Jacobian variant.  ``cpu_baseline`` times the
+128^3 trilinear, the FP32-Jacobian variant, time to a 1e-10 relative
+residual per load step (plain FP64 Newton versus learned warm start + FP32
+Jacobian), and the batched workload (256 meshes of uneven size assembled in
+one launch per kernel).  ``cpu_baseline`` times the
 C oracle (an FP64 restatement of the reference element loop with the
 reference's per-entry CSR bisection) on this host, single thread, on a
 bounded sample of the same workload.
@@ -117,6 +120,37 @@ def newton_line(cm, n, load_steps):
             "plain_fp64": p, "learned_fp32": l, "speedup": p["seconds_per_step"] / l["seconds_per_step"]}
 
 
+def batched_line(cm, n_meshes, reps=3):
+    """Batched meshes (n ~ U{4..16}, seeded materials and stretches): residual
+    + Jacobian assembly of every mesh in one launch per kernel, FP32 and FP64,
+    after the launch-shape autotuner; and the same Jacobian replayed as a graph."""
+    from cm_amd.batch_workload import build, draw_cases
+    cases = draw_cases(n_meshes)
+    meshes, batch, setup = build(cases)
+    batch.load_increment_apply(1, 5)
+    E = batch.n_elements
+    out = {"workload": f"{n_meshes} Mooney-Rivlin meshes, n ~ U{{4..16}}, c10 ~ U[1,5], c01 ~ U[0,10], jitter 0.1h, "
+                       "first load step state", "meshes": n_meshes, "elements": E, "dofs": batch.n_rows,
+           "nnz": batch.nnz, "setup_s": setup}
+    for prec, name in ((cm.OC_FP32, "fp32"), (cm.OC_FP64, "fp64")):
+        batch.jacobian_evaluate(prec)
+        out[f"autotune_{name}"] = batch.autotune(prec)
+        ms, _ = time_steps(lambda: (batch.residual_evaluate(), batch.jacobian_evaluate(prec)), reps, 1, cm)
+        em, _ = cm.kernel_timing(f"mr_element_matrix_{name}")
+        ks = {k: cm.kernel_timing(k)[0] / reps for k in
+              ("mr_gauss_jacobian", f"mr_element_matrix_{name}", f"csr_row_gather_{name}", "mr_residual", "dof_gather")}
+        cm.kernel_timing_enable(False)
+        gms, _ = time_steps(lambda: batch.jacobian_evaluate_graph(prec), reps, 1, cm)
+        cm.kernel_timing_enable(False)
+        peak = FP32_MFMA_PEAK_TFLOPS if name == "fp32" else FP64_PEAK_TFLOPS
+        tflops = MR_JACOBIAN_FLOP_PER_ELEMENT * E / (em / reps / 1e3) / 1e12
+        out[name] = {"elements_per_s": E * reps / (ms / 1e3), "ms_per_residual_and_jacobian": ms / reps,
+                     "graph_jacobian_ms": gms / reps, "kernel_ms": ks,
+                     "element_matrix_TFLOPs": tflops, "element_matrix_frac": tflops / peak}
+    batch.destroy()
+    return out
+
+
 def cpu_baseline(mesh, dofs, c10, c01, seconds):
     """Oracle (C FP64 restatement of the reference loop) on this host, 1 thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -150,6 +184,7 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--load-steps", type=int, default=10)
+    ap.add_argument("--batched-meshes", type=int, default=256, help="0 skips the batched-meshes line")
     ap.add_argument("--profile", choices=["jacobian", "copy"], default=None,
                     help="run only one kernel sequence for rocprofv3 counter collection: the FP64 Jacobian "
                          "assembly (--steps times), or a device copy of 512 MiB (the FETCH_SIZE/WRITE_SIZE "
@@ -233,6 +268,8 @@ def main():
         secondary["laplace"] = [laplace_line(cm, n, s) for n, s in ((8, 50), (64, 20), (128, 10))]
     if not args.no_newton:
         secondary["newton_per_load_step"] = newton_line(cm, args.n, args.load_steps)
+    if args.batched_meshes > 0 and not args.no_secondary:
+        secondary["batched"] = batched_line(cm, args.batched_meshes)
 
     line = {
         "metric": "elements/s assembled; time-to-1e-10-residual per load step, plain vs learned",
