@@ -116,7 +116,7 @@ def newton_line(cm, n, load_steps):
                 "start_residual_over_plain_start": [r["start_norm"] / r["reference_norm"] for r in rows]}
     p, l = summary(plain), summary(learned)
     return {"workload": f"{n}^3 Mooney-Rivlin, c10=2, c01=6, 10% stretch in {load_steps} load steps, "
-                        "1e-10 x plain start residual, block-triangular GMRES",
+                        "1e-10 x plain start residual, GMRES with block-triangular multigrid preconditioner",
             "plain_fp64": p, "learned_fp32": l, "speedup": p["seconds_per_step"] / l["seconds_per_step"]}
 
 

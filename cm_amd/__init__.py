@@ -5,7 +5,7 @@ The compute path is liboc_asm.so (hand-written HIP kernels behind the C ABI
 in include/oc_asm.h); this package is its Python host side.
 """
 from ._lib import (OC_PRECONDITIONER_BLOCK_DIAGONAL, OC_PRECONDITIONER_BLOCK_TRIANGULAR,  # noqa: F401
-                   OC_PRECONDITIONER_JACOBI)
+                   OC_PRECONDITIONER_JACOBI, OC_PRECONDITIONER_MULTIGRID)
 from ._lib import (CMISSError, OC_FP32, OC_FP64, OC_BUFFER_COL_IDX, OC_BUFFER_DOFS,  # noqa: F401
                    OC_BUFFER_MATRIX_VALUES, OC_BUFFER_MATRIX_VALUES_SP, OC_BUFFER_RESIDUAL, OC_BUFFER_RHS,
                    OC_BUFFER_ROW_PTR, LIB_PATH)

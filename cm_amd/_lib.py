@@ -48,6 +48,7 @@ class CMISSError(RuntimeError):
 OC_PRECONDITIONER_JACOBI = 0
 OC_PRECONDITIONER_BLOCK_DIAGONAL = 1
 OC_PRECONDITIONER_BLOCK_TRIANGULAR = 2
+OC_PRECONDITIONER_MULTIGRID = 3
 
 
 class NewtonOptions(ctypes.Structure):
@@ -144,6 +145,8 @@ SIGNATURES = {
                               ctypes.POINTER(NewtonStats)],
     "oc_mesh_residual_norms": [_H, _dp],
     "oc_autotune": [_H, _c_int, ctypes.c_char_p, _c_int],
+    "oc_multigrid_structure_set": [_H, _i32],
+    "oc_multigrid_options_set": [_H, _c_int, _c_int, _c_double],
     "oc_jacobian_evaluate_graph": [_H, _c_int],
     "oc_kernel_timing_enable": [_c_int],
     "oc_kernel_timing_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],

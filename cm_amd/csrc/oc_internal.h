@@ -50,11 +50,31 @@ private:
     hipEvent_t start_ = nullptr;
 };
 
-// Device buffer owned by the library.
+// Device buffer owned by the library.  Move-only: a copy would free the
+// allocation twice (containers that grow move their elements).
 template <typename T>
 struct DevBuf {
     T *ptr = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), n(o.n)
+    {
+        o.ptr = nullptr;
+        o.n = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            n = o.n;
+            o.ptr = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
     int alloc(size_t count)
     {
         release();
@@ -81,6 +101,33 @@ struct DevBuf {
         n = 0;
     }
     ~DevBuf() { release(); }
+};
+
+// Geometric multigrid hierarchy of the displacement block (oc_multigrid.hip).
+// Level 0 is the triquadratic CSR operator itself; level 1 the trilinear
+// space on the same elements (its nodes are the element corners, numbered as
+// the pressure nodes); level l+1 coarsens level l by 2 per direction.  Levels
+// >= 1 store a 27-point 3x3-block stencil per node (node-major vectors).
+struct MgLevel {
+    int nn = 0;                           // nodes over all meshes
+    std::vector<int32_t> h_dim, h_off;    // nodes per direction / first node, per mesh (h_off: M+1)
+    DevBuf<int32_t> dim, off;
+    DevBuf<double> A;                     // nn*27*9: A[(node*27 + d)*9 + 3c + c']
+    DevBuf<double> Dinv;                  // nn*9 inverse diagonal blocks
+    DevBuf<uint8_t> freed;                // nn*3: DOF not fixed
+    DevBuf<double> x, b, r;               // nn*3 each
+};
+struct Multigrid {
+    std::vector<int32_t> h_n;             // elements per direction of each mesh
+    DevBuf<int32_t> n, e_off;
+    std::vector<MgLevel> levels;          // levels[0] holds only the per-mesh tables
+    DevBuf<double> ke1;                   // Galerkin trilinear element matrices, E*24*24
+    DevBuf<double> t0, r0, x0;            // level-0 scratch (3*n_nodes each)
+    DevBuf<double> coarse_inv;            // per mesh dense inverse of the coarsest level
+    int coarse_stride = 0;                // (max DOFs per mesh at the coarsest level)^2
+    int pre = 2, post = 2;
+    double omega = 0.6;
+    int coarse_sweeps = 0;                // 0: dense solve of the coarsest level
 };
 
 // Per-equations-set state.  Element arrays are element-major; node and DOF
@@ -164,9 +211,12 @@ struct EquationsSet {
     // Newton-step graph: Jacobian assembly + preconditioner setup
     hipGraphExec_t step_graph = nullptr;
     int step_graph_key[4] = {-1, -1, -1, -1};
+
+    Multigrid *mg = nullptr;  // set by oc_multigrid_structure_set
     ~EquationsSet()
     {
         if (step_graph) (void)hipGraphExecDestroy(step_graph);
+        delete mg;
     }
 };
 
@@ -192,6 +242,10 @@ int mr_jacobian_buffers(EquationsSet &s, int precision);  // element-matrix and 
 int launch_mr_element_matrices(EquationsSet &s, int precision, int blocks_per_cu);
 int launch_mr_row_gather(EquationsSet &s, int precision, int chunk);
 int launch_mr_gauss(EquationsSet &s);
+// multigrid (oc_multigrid.hip): setup after a Jacobian assembly in
+// `precision`; apply z_u = V-cycle(t) for the displacement block
+int mg_setup(EquationsSet &s, int precision);
+int mg_apply(EquationsSet &s, int precision, const double *t, double *z);
 // y = A x with `lanes` lanes per CSR row (A in FP64 or FP32 values)
 int launch_spmv_lanes(EquationsSet &s, bool fp32, int lanes, const double *x, double *y);
 

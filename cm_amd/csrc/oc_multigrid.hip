@@ -1,0 +1,743 @@
+// oc_multigrid.hip -- geometric multigrid V-cycle for the displacement block
+// K_uu of the Mooney-Rivlin Jacobian on regular generated meshes (no
+// counterpart in the reference, whose PETSc KSP uses the preconditioner the
+// user selects; solver_routines.f90:10957-11006).
+//
+// Hierarchy, per mesh of n^3 elements:
+//   level 0  triquadratic nodes (2n+1)^3: the assembled CSR itself (u-u part,
+//            columns left of the first pressure column of each row);
+//   level 1  trilinear nodes (n+1)^3 on the same elements (numbered like the
+//            pressure nodes); operator P0^T K P0 formed element by element
+//            from the element matrices of the last Jacobian assembly,
+//            K1_e = W^T M_e K_e M_e W (W: trilinear values at the 27 element
+//            nodes, M_e: free-DOF mask), assembled into a 27-point stencil;
+//   level l+1 (ceil(n_l/2)+1)^3 nodes, linear interpolation in each lattice
+//            direction (even fine index -> coarse i/2, odd -> mean of its two
+//            coarse neighbours), operator P^T A_l P on the stencils.
+// Dirichlet DOFs: interpolation rows of fixed fine DOFs are dropped, a coarse
+// DOF coinciding with a fixed fine DOF is fixed (its column dropped), and a
+// coarse DOF left with an empty row is fixed too; fixed rows and columns are
+// the identity.
+// Smoother: damped nodal 3x3 block Jacobi; coarsest level (<= 3 nodes per
+// direction): dense inverse per mesh.  Every reduction order is fixed, so the
+// V-cycle is a fixed linear operator (usable inside GMRES).
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+
+#include "oc_internal.h"
+
+namespace oc {
+namespace {
+
+__device__ __forceinline__ int find_mesh(const int *__restrict__ off, int M, int node)
+{
+    int lo = 0, hi = M - 1;  // off[lo] <= node < off[lo + 1]
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= node) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ double lin_weight(int delta) { return delta == 0 ? 1.0 : 0.5; }
+
+// ---- setup -----------------------------------------------------------------------------
+template <typename Real>
+__global__ void __launch_bounds__(256)
+q1_galerkin_element_kernel(int n_elements, const Real *__restrict__ kbuf, const int *__restrict__ conn_q, int nq,
+                           const uint8_t *__restrict__ free_mask, double *__restrict__ ke1)
+{
+    __shared__ double sK[81][82];
+    __shared__ double sT[81][25];
+    __shared__ double sW[27][8];
+    __shared__ uint8_t sf[81];
+    const int t = threadIdx.x;
+    for (int k = t; k < 27 * 8; k += 256) {
+        const int a = k / 8, c = k % 8;
+        double w = 1.0;
+        int ad[3] = {a % 3, (a / 3) % 3, a / 9};
+        for (int d = 0; d < 3; ++d) {
+            const double x = 0.5 * ad[d];
+            w *= ((c >> d) & 1) ? x : 1.0 - x;
+        }
+        sW[a][c] = w;
+    }
+    for (int e = blockIdx.x; e < n_elements; e += gridDim.x) {
+        __syncthreads();
+        if (t < 81) sf[t] = free_mask[static_cast<int64_t>(t / 27) * nq + conn_q[static_cast<int64_t>(e) * 27 + t % 27]];
+        __syncthreads();
+        const Real *Ke = kbuf + static_cast<int64_t>(e) * 89 * 89;
+        for (int k = t; k < 81 * 81; k += 256) {
+            const int i = k / 81, j = k % 81;
+            sK[i][j] = (sf[i] && sf[j]) ? static_cast<double>(Ke[i * 89 + j]) : 0.0;
+        }
+        __syncthreads();
+        for (int k = t; k < 81 * 24; k += 256) {
+            const int i = k / 24, col = k % 24, cp = col / 8, kk = col % 8;
+            double acc = 0.0;
+            for (int a = 0; a < 27; ++a) acc += sK[i][cp * 27 + a] * sW[a][kk];
+            sT[i][col] = acc;
+        }
+        __syncthreads();
+        for (int k = t; k < 24 * 24; k += 256) {
+            const int row = k / 24, col = k % 24, c = row / 8, kr = row % 8;
+            double acc = 0.0;
+            for (int a = 0; a < 27; ++a) acc += sW[a][kr] * sT[c * 27 + a][col];
+            ke1[static_cast<int64_t>(e) * 576 + k] = acc;
+        }
+    }
+}
+
+// level-1 stencil: block (I, I+d) = sum over the elements holding both
+// corners (ascending element order) of K1_e[(c, corner I), (c', corner J)]
+__global__ void __launch_bounds__(256)
+q1_stencil_kernel(int M, const int *__restrict__ n_mesh, const int *__restrict__ e_off, const int *__restrict__ off1,
+                  int nn, const double *__restrict__ ke1, double *__restrict__ A)
+{
+    const int64_t total = static_cast<int64_t>(nn) * 27;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int node = static_cast<int>(t / 27), d = static_cast<int>(t % 27);
+        const int m = find_mesh(off1, M, node);
+        const int n = n_mesh[m], N = n + 1, local = node - off1[m];
+        const int I[3] = {local % N, (local / N) % N, local / (N * N)};
+        const int J[3] = {I[0] + d % 3 - 1, I[1] + (d / 3) % 3 - 1, I[2] + d / 9 - 1};
+        double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (J[0] >= 0 && J[0] < N && J[1] >= 0 && J[1] < N && J[2] >= 0 && J[2] < N) {
+            int lo[3], hi[3];
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = max(max(I[k], J[k]) - 1, 0);
+                hi[k] = min(min(I[k], J[k]), n - 1);
+            }
+            for (int ez = lo[2]; ez <= hi[2]; ++ez)
+                for (int ey = lo[1]; ey <= hi[1]; ++ey)
+                    for (int ex = lo[0]; ex <= hi[0]; ++ex) {
+                        const int64_t e = e_off[m] + ex + static_cast<int64_t>(n) * (ey + static_cast<int64_t>(n) * ez);
+                        const int kI = (I[0] - ex) + 2 * (I[1] - ey) + 4 * (I[2] - ez);
+                        const int kJ = (J[0] - ex) + 2 * (J[1] - ey) + 4 * (J[2] - ez);
+                        const double *k1 = ke1 + e * 576;
+                        for (int c = 0; c < 3; ++c)
+                            for (int cp = 0; cp < 3; ++cp) blk[3 * c + cp] += k1[(c * 8 + kI) * 24 + cp * 8 + kJ];
+                    }
+        }
+        for (int k = 0; k < 9; ++k) A[t * 9 + k] = blk[k];
+    }
+}
+
+// Galerkin coarsening of a stencil level (masked by the fine level's free DOFs)
+__global__ void __launch_bounds__(256)
+galerkin_coarsen_kernel(int M, const int *__restrict__ dim_f, const int *__restrict__ off_f,
+                        const double *__restrict__ Af, const uint8_t *__restrict__ freed_f,
+                        const int *__restrict__ dim_c, const int *__restrict__ off_c, int nn_c,
+                        double *__restrict__ Ac)
+{
+    const int64_t total = static_cast<int64_t>(nn_c) * 27;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int node = static_cast<int>(t / 27), d = static_cast<int>(t % 27);
+        const int m = find_mesh(off_c, M, node);
+        const int Nc = dim_c[m], Nf = dim_f[m], local = node - off_c[m];
+        const int I[3] = {local % Nc, (local / Nc) % Nc, local / (Nc * Nc)};
+        const int J[3] = {I[0] + d % 3 - 1, I[1] + (d / 3) % 3 - 1, I[2] + d / 9 - 1};
+        double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (J[0] >= 0 && J[0] < Nc && J[1] >= 0 && J[1] < Nc && J[2] >= 0 && J[2] < Nc) {
+            for (int s = 0; s < 27; ++s) {
+                const int i[3] = {2 * I[0] + s % 3 - 1, 2 * I[1] + (s / 3) % 3 - 1, 2 * I[2] + s / 9 - 1};
+                if (i[0] < 0 || i[0] >= Nf || i[1] < 0 || i[1] >= Nf || i[2] < 0 || i[2] >= Nf) continue;
+                const double wi = lin_weight(s % 3 - 1) * lin_weight((s / 3) % 3 - 1) * lin_weight(s / 9 - 1);
+                const int64_t fi = off_f[m] + i[0] + static_cast<int64_t>(Nf) * (i[1] + static_cast<int64_t>(Nf) * i[2]);
+                for (int q = 0; q < 27; ++q) {
+                    const int j[3] = {2 * J[0] + q % 3 - 1, 2 * J[1] + (q / 3) % 3 - 1, 2 * J[2] + q / 9 - 1};
+                    if (j[0] < 0 || j[0] >= Nf || j[1] < 0 || j[1] >= Nf || j[2] < 0 || j[2] >= Nf) continue;
+                    const int e0 = j[0] - i[0], e1 = j[1] - i[1], e2 = j[2] - i[2];
+                    if (e0 < -1 || e0 > 1 || e1 < -1 || e1 > 1 || e2 < -1 || e2 > 1) continue;
+                    const double w = wi * lin_weight(q % 3 - 1) * lin_weight((q / 3) % 3 - 1) * lin_weight(q / 9 - 1);
+                    const int64_t fj = off_f[m] + j[0] + static_cast<int64_t>(Nf) * (j[1] + static_cast<int64_t>(Nf) * j[2]);
+                    const double *a = Af + (fi * 27 + (e0 + 1) + 3 * (e1 + 1) + 9 * (e2 + 1)) * 9;
+                    for (int c = 0; c < 3; ++c) {
+                        if (!freed_f[fi * 3 + c]) continue;
+                        for (int cp = 0; cp < 3; ++cp)
+                            if (freed_f[fj * 3 + cp]) blk[3 * c + cp] += w * a[3 * c + cp];
+                    }
+                }
+            }
+        }
+        for (int k = 0; k < 9; ++k) Ac[t * 9 + k] = blk[k];
+    }
+}
+
+// Coarse DOFs that coincide with a fixed fine DOF are fixed (a Dirichlet
+// face stays Dirichlet on every level).  Level 1 from the triquadratic mask:
+// coarse node I coincides with fine lattice node 2I.
+__global__ void __launch_bounds__(256) coincident_q2_kernel(int M, const int *__restrict__ n_mesh,
+                                                           const int *__restrict__ q_off, int nq_total,
+                                                           const int *__restrict__ off1, int nn1,
+                                                           const uint8_t *__restrict__ free_mask,
+                                                           uint8_t *__restrict__ freed1)
+{
+    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn1; node += gridDim.x * blockDim.x) {
+        const int m = find_mesh(off1, M, node);
+        const int n = n_mesh[m], N = n + 1, T = 2 * n + 1, local = node - off1[m];
+        const int64_t q = q_off[m] + 2 * (local % N) +
+                          static_cast<int64_t>(T) * (2 * ((local / N) % N) + static_cast<int64_t>(T) * 2 * (local / (N * N)));
+        for (int c = 0; c < 3; ++c) freed1[static_cast<int64_t>(node) * 3 + c] = free_mask[static_cast<int64_t>(c) * nq_total + q];
+    }
+}
+
+// level l -> l+1: coarse I coincides with fine 2I; the extra coarse node of
+// an odd fine grid (2I beyond the last fine index) takes the flag of the
+// boundary node it extends, so every Dirichlet face stays one on the coarse
+// grid and the masked interpolation keeps full column rank
+__global__ void __launch_bounds__(256) coincident_kernel(int M, const int *__restrict__ dim_f,
+                                                        const int *__restrict__ off_f,
+                                                        const uint8_t *__restrict__ freed_f,
+                                                        const int *__restrict__ dim_c, const int *__restrict__ off_c,
+                                                        int nn_c, uint8_t *__restrict__ freed_c)
+{
+    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn_c; node += gridDim.x * blockDim.x) {
+        const int m = find_mesh(off_c, M, node);
+        const int Nc = dim_c[m], Nf = dim_f[m], local = node - off_c[m];
+        const int i0 = min(2 * (local % Nc), Nf - 1), i1 = min(2 * ((local / Nc) % Nc), Nf - 1),
+                  i2 = min(2 * (local / (Nc * Nc)), Nf - 1);
+        const int64_t f = off_f[m] + i0 + static_cast<int64_t>(Nf) * (i1 + static_cast<int64_t>(Nf) * i2);
+        for (int c = 0; c < 3; ++c) freed_c[static_cast<int64_t>(node) * 3 + c] = freed_f[f * 3 + c];
+    }
+}
+
+// freed &= (diagonal != 0): a DOF whose Galerkin row is empty is fixed too
+__global__ void __launch_bounds__(256) level_freed_kernel(int nn, const double *__restrict__ A,
+                                                         uint8_t *__restrict__ freed)
+{
+    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn; node += gridDim.x * blockDim.x) {
+        const double *D = A + (static_cast<int64_t>(node) * 27 + 13) * 9;
+        for (int c = 0; c < 3; ++c) {
+            const int64_t k = static_cast<int64_t>(node) * 3 + c;
+            freed[k] = (freed[k] && D[4 * c] != 0.0) ? 1 : 0;
+        }
+    }
+}
+
+// Fixed rows and columns replaced by the identity; inverse diagonal blocks.
+__global__ void __launch_bounds__(256) level_mask_kernel(int M, const int *__restrict__ dim,
+                                                        const int *__restrict__ off, int nn, double *__restrict__ A,
+                                                        const uint8_t *__restrict__ freed, double *__restrict__ Dinv)
+{
+    const int64_t total = static_cast<int64_t>(nn) * 27;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int node = static_cast<int>(t / 27), d = static_cast<int>(t % 27);
+        const int m = find_mesh(off, M, node);
+        const int N = dim[m], local = node - off[m];
+        const int J[3] = {local % N + d % 3 - 1, (local / N) % N + (d / 3) % 3 - 1, local / (N * N) + d / 9 - 1};
+        double *a = A + t * 9;
+        if (J[0] < 0 || J[0] >= N || J[1] < 0 || J[1] >= N || J[2] < 0 || J[2] >= N) {
+            for (int k = 0; k < 9; ++k) a[k] = 0.0;
+            continue;
+        }
+        const int64_t j = off[m] + J[0] + static_cast<int64_t>(N) * (J[1] + static_cast<int64_t>(N) * J[2]);
+        for (int c = 0; c < 3; ++c)
+            for (int cp = 0; cp < 3; ++cp) {
+                const bool keep = freed[static_cast<int64_t>(node) * 3 + c] && freed[j * 3 + cp];
+                if (!keep) a[3 * c + cp] = (d == 13 && c == cp) ? 1.0 : 0.0;
+            }
+        if (d != 13) continue;
+        const double *D = a;
+        const double c00 = D[4] * D[8] - D[5] * D[7], c01 = D[5] * D[6] - D[3] * D[8], c02 = D[3] * D[7] - D[4] * D[6];
+        const double det = D[0] * c00 + D[1] * c01 + D[2] * c02;
+        double *o = Dinv + static_cast<int64_t>(node) * 9;
+        if (fabs(det) < 1e-300) {
+            for (int i = 0; i < 9; ++i) o[i] = (i % 4 == 0) ? 1.0 : 0.0;
+            continue;
+        }
+        const double id = 1.0 / det;
+        o[0] = c00 * id;
+        o[1] = (D[2] * D[7] - D[1] * D[8]) * id;
+        o[2] = (D[1] * D[5] - D[2] * D[4]) * id;
+        o[3] = c01 * id;
+        o[4] = (D[0] * D[8] - D[2] * D[6]) * id;
+        o[5] = (D[2] * D[3] - D[0] * D[5]) * id;
+        o[6] = c02 * id;
+        o[7] = (D[1] * D[6] - D[0] * D[7]) * id;
+        o[8] = (D[0] * D[4] - D[1] * D[3]) * id;
+    }
+}
+
+constexpr int kCoarseMax = 81;  // 27 nodes x 3 components
+
+// dense inverse of each mesh's coarsest level (in-place Gauss-Jordan, one
+// block per mesh; the masked operator is symmetric positive definite)
+__global__ void __launch_bounds__(256) coarse_inverse_kernel(int M, const int *__restrict__ dim,
+                                                            const int *__restrict__ off,
+                                                            const double *__restrict__ A, int stride,
+                                                            double *__restrict__ inv)
+{
+    __shared__ double a[kCoarseMax][kCoarseMax + 1];
+    __shared__ double fcol[kCoarseMax];
+    __shared__ double piv;
+    const int m = blockIdx.x, t = threadIdx.x;
+    if (m >= M) return;
+    const int N = dim[m], nodes = N * N * N, nd = 3 * nodes;
+    for (int k = t; k < nd * nd; k += 256) {
+        const int r = k / nd, col = k % nd;
+        const int nr = r / 3, c = r % 3, nc = col / 3, cp = col % 3;
+        const int e0 = nc % N - nr % N, e1 = (nc / N) % N - (nr / N) % N, e2 = nc / (N * N) - nr / (N * N);
+        double v = 0.0;
+        if (e0 >= -1 && e0 <= 1 && e1 >= -1 && e1 <= 1 && e2 >= -1 && e2 <= 1)
+            v = A[((static_cast<int64_t>(off[m]) + nr) * 27 + (e0 + 1) + 3 * (e1 + 1) + 9 * (e2 + 1)) * 9 + 3 * c + cp];
+        a[r][col] = v;
+    }
+    __syncthreads();
+    for (int k = 0; k < nd; ++k) {
+        if (t == 0) piv = (fabs(a[k][k]) > 1e-300) ? a[k][k] : 1.0;  // fixed DOFs are identity rows
+        __syncthreads();
+        const double p = piv;
+        for (int j = t; j < nd; j += 256) a[k][j] = (j == k ? 1.0 : a[k][j]) / p;
+        for (int i = t; i < nd; i += 256) fcol[i] = (i == k) ? 0.0 : a[i][k];
+        __syncthreads();
+        for (int q = t; q < nd * nd; q += 256) {
+            const int i = q / nd, j = q % nd;
+            if (i == k) continue;
+            a[i][j] = (j == k ? 0.0 : a[i][j]) - fcol[i] * a[k][j];
+        }
+        __syncthreads();
+    }
+    for (int k = t; k < nd * nd; k += 256) inv[static_cast<int64_t>(m) * stride + k] = a[k / nd][k % nd];
+}
+
+// ---- V-cycle kernels -------------------------------------------------------------------
+// level 0: r = b - K_uu x on free rows (columns left of the first pressure column)
+template <typename VT, int kLanes>
+__global__ void __launch_bounds__(256)
+uu_residual_kernel(int nu, const int64_t *__restrict__ row_ptr, const int64_t *__restrict__ pstart,
+                   const int *__restrict__ col_idx, const VT *__restrict__ vals, const uint8_t *__restrict__ free_mask,
+                   const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r)
+{
+    const int lane = threadIdx.x % kLanes;
+    const int row = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes);
+    if (row >= nu) return;
+    const bool f = free_mask[row] != 0;
+    double acc = 0.0;
+    if (f)
+        for (int64_t k = row_ptr[row] + lane; k < pstart[row]; k += kLanes)
+            acc += static_cast<double>(vals[k]) * x[col_idx[k]];
+#pragma unroll
+    for (int o = kLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kLanes);
+    if (lane == 0) r[row] = f ? b[row] - acc : 0.0;
+}
+
+// level 0: x = (first ? 0 : x) + omega Kblock^-1 r, component-major
+__global__ void __launch_bounds__(256) q2_jacobi_kernel(int nq, const double *__restrict__ inv_u,
+                                                       const double *__restrict__ r, double *__restrict__ x,
+                                                       double omega, int first)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        const double *o = inv_u + static_cast<int64_t>(i) * 9;
+        const double r0 = r[i], r1 = r[nq + i], r2 = r[2 * static_cast<int64_t>(nq) + i];
+        const double d[3] = {o[0] * r0 + o[1] * r1 + o[2] * r2, o[3] * r0 + o[4] * r1 + o[5] * r2,
+                             o[6] * r0 + o[7] * r1 + o[8] * r2};
+        for (int c = 0; c < 3; ++c) {
+            const int64_t k = static_cast<int64_t>(c) * nq + i;
+            x[k] = (first ? 0.0 : x[k]) + omega * d[c];
+        }
+    }
+}
+
+// level 0 -> 1 restriction b1 = P0^T r0 (r0 is zero on fixed DOFs)
+__global__ void __launch_bounds__(256) restrict_q2_kernel(int M, const int *__restrict__ n_mesh,
+                                                         const int *__restrict__ q_off, int nq_total,
+                                                         const int *__restrict__ off1, int nn1,
+                                                         const uint8_t *__restrict__ freed1,
+                                                         const double *__restrict__ r0, double *__restrict__ b1)
+{
+    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn1; node += gridDim.x * blockDim.x) {
+        const int m = find_mesh(off1, M, node);
+        const int n = n_mesh[m], N = n + 1, T = 2 * n + 1, local = node - off1[m];
+        const int I[3] = {local % N, (local / N) % N, local / (N * N)};
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int s = 0; s < 27; ++s) {
+            const int i[3] = {2 * I[0] + s % 3 - 1, 2 * I[1] + (s / 3) % 3 - 1, 2 * I[2] + s / 9 - 1};
+            if (i[0] < 0 || i[0] >= T || i[1] < 0 || i[1] >= T || i[2] < 0 || i[2] >= T) continue;
+            const double w = lin_weight(s % 3 - 1) * lin_weight((s / 3) % 3 - 1) * lin_weight(s / 9 - 1);
+            const int64_t q = q_off[m] + i[0] + static_cast<int64_t>(T) * (i[1] + static_cast<int64_t>(T) * i[2]);
+            for (int c = 0; c < 3; ++c) acc[c] += w * r0[static_cast<int64_t>(c) * nq_total + q];
+        }
+        for (int c = 0; c < 3; ++c) {
+            const int64_t k = static_cast<int64_t>(node) * 3 + c;
+            b1[k] = freed1[k] ? acc[c] : 0.0;
+        }
+    }
+}
+
+// level 1 -> 0 prolongation x0 += P0 x1 on free DOFs
+__global__ void __launch_bounds__(256) prolong_q2_kernel(int M, const int *__restrict__ n_mesh,
+                                                        const int *__restrict__ q_off, int nq_total,
+                                                        const int *__restrict__ off1,
+                                                        const uint8_t *__restrict__ free_mask,
+                                                        const double *__restrict__ x1, double *__restrict__ x0)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq_total; q += gridDim.x * blockDim.x) {
+        const int m = find_mesh(q_off, M, q);
+        const int n = n_mesh[m], N = n + 1, T = 2 * n + 1, local = q - q_off[m];
+        const int i[3] = {local % T, (local / T) % T, local / (T * T)};
+        int p0[3], p1[3];
+        double w0[3], w1[3];
+        for (int k = 0; k < 3; ++k) {
+            p0[k] = i[k] >> 1;
+            p1[k] = (i[k] + 1) >> 1;
+            w0[k] = (i[k] & 1) ? 0.5 : 1.0;
+            w1[k] = (i[k] & 1) ? 0.5 : 0.0;
+        }
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int s = 0; s < 8; ++s) {
+            const int b0 = s & 1, b1 = (s >> 1) & 1, b2 = s >> 2;
+            const double w = (b0 ? w1[0] : w0[0]) * (b1 ? w1[1] : w0[1]) * (b2 ? w1[2] : w0[2]);
+            if (w == 0.0) continue;
+            const int64_t P = off1[m] + (b0 ? p1[0] : p0[0]) +
+                              static_cast<int64_t>(N) * ((b1 ? p1[1] : p0[1]) + static_cast<int64_t>(N) * (b2 ? p1[2] : p0[2]));
+            for (int c = 0; c < 3; ++c) v[c] += w * x1[P * 3 + c];
+        }
+        for (int c = 0; c < 3; ++c) {
+            const int64_t k = static_cast<int64_t>(c) * nq_total + q;
+            if (free_mask[k]) x0[k] += v[c];
+        }
+    }
+}
+
+// levels >= 1: r = b - A x on free DOFs (27-point block stencil)
+__global__ void __launch_bounds__(256) stencil_residual_kernel(int M, const int *__restrict__ dim,
+                                                              const int *__restrict__ off, int nn,
+                                                              const double *__restrict__ A,
+                                                              const uint8_t *__restrict__ freed,
+                                                              const double *__restrict__ b,
+                                                              const double *__restrict__ x, double *__restrict__ r)
+{
+    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn; node += gridDim.x * blockDim.x) {
+        const int m = find_mesh(off, M, node);
+        const int N = dim[m], local = node - off[m];
+        const int I[3] = {local % N, (local / N) % N, local / (N * N)};
+        double acc[3] = {b[static_cast<int64_t>(node) * 3], b[static_cast<int64_t>(node) * 3 + 1],
+                         b[static_cast<int64_t>(node) * 3 + 2]};
+        for (int d = 0; d < 27; ++d) {
+            const int J[3] = {I[0] + d % 3 - 1, I[1] + (d / 3) % 3 - 1, I[2] + d / 9 - 1};
+            if (J[0] < 0 || J[0] >= N || J[1] < 0 || J[1] >= N || J[2] < 0 || J[2] >= N) continue;
+            const int64_t j = off[m] + J[0] + static_cast<int64_t>(N) * (J[1] + static_cast<int64_t>(N) * J[2]);
+            const double *a = A + (static_cast<int64_t>(node) * 27 + d) * 9;
+            const double x0 = x[j * 3], x1 = x[j * 3 + 1], x2 = x[j * 3 + 2];
+            for (int c = 0; c < 3; ++c) acc[c] -= a[3 * c] * x0 + a[3 * c + 1] * x1 + a[3 * c + 2] * x2;
+        }
+        for (int c = 0; c < 3; ++c) {
+            const int64_t k = static_cast<int64_t>(node) * 3 + c;
+            r[k] = freed[k] ? acc[c] : 0.0;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) stencil_jacobi_kernel(int nn, const double *__restrict__ Dinv,
+                                                            const double *__restrict__ r, double *__restrict__ x,
+                                                            double omega, int first)
+{
+    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn; node += gridDim.x * blockDim.x) {
+        const double *o = Dinv + static_cast<int64_t>(node) * 9;
+        const double *rr = r + static_cast<int64_t>(node) * 3;
+        for (int c = 0; c < 3; ++c) {
+            const int64_t k = static_cast<int64_t>(node) * 3 + c;
+            const double d = o[3 * c] * rr[0] + o[3 * c + 1] * rr[1] + o[3 * c + 2] * rr[2];
+            x[k] = (first ? 0.0 : x[k]) + omega * d;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) restrict_stencil_kernel(int M, const int *__restrict__ dim_f,
+                                                              const int *__restrict__ off_f,
+                                                              const int *__restrict__ dim_c,
+                                                              const int *__restrict__ off_c, int nn_c,
+                                                              const uint8_t *__restrict__ freed_c,
+                                                              const double *__restrict__ rf, double *__restrict__ bc)
+{
+    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn_c; node += gridDim.x * blockDim.x) {
+        const int m = find_mesh(off_c, M, node);
+        const int Nc = dim_c[m], Nf = dim_f[m], local = node - off_c[m];
+        const int I[3] = {local % Nc, (local / Nc) % Nc, local / (Nc * Nc)};
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int s = 0; s < 27; ++s) {
+            const int i[3] = {2 * I[0] + s % 3 - 1, 2 * I[1] + (s / 3) % 3 - 1, 2 * I[2] + s / 9 - 1};
+            if (i[0] < 0 || i[0] >= Nf || i[1] < 0 || i[1] >= Nf || i[2] < 0 || i[2] >= Nf) continue;
+            const double w = lin_weight(s % 3 - 1) * lin_weight((s / 3) % 3 - 1) * lin_weight(s / 9 - 1);
+            const int64_t f = off_f[m] + i[0] + static_cast<int64_t>(Nf) * (i[1] + static_cast<int64_t>(Nf) * i[2]);
+            for (int c = 0; c < 3; ++c) acc[c] += w * rf[f * 3 + c];
+        }
+        for (int c = 0; c < 3; ++c) {
+            const int64_t k = static_cast<int64_t>(node) * 3 + c;
+            bc[k] = freed_c[k] ? acc[c] : 0.0;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) prolong_stencil_kernel(int M, const int *__restrict__ dim_f,
+                                                             const int *__restrict__ off_f, int nn_f,
+                                                             const int *__restrict__ dim_c,
+                                                             const int *__restrict__ off_c,
+                                                             const uint8_t *__restrict__ freed_f,
+                                                             const double *__restrict__ xc, double *__restrict__ xf)
+{
+    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn_f; node += gridDim.x * blockDim.x) {
+        const int m = find_mesh(off_f, M, node);
+        const int Nf = dim_f[m], Nc = dim_c[m], local = node - off_f[m];
+        const int i[3] = {local % Nf, (local / Nf) % Nf, local / (Nf * Nf)};
+        int p0[3], p1[3];
+        double w0[3], w1[3];
+        for (int k = 0; k < 3; ++k) {
+            p0[k] = i[k] >> 1;
+            p1[k] = (i[k] + 1) >> 1;
+            w0[k] = (i[k] & 1) ? 0.5 : 1.0;
+            w1[k] = (i[k] & 1) ? 0.5 : 0.0;
+        }
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int s = 0; s < 8; ++s) {
+            const int b0 = s & 1, b1 = (s >> 1) & 1, b2 = s >> 2;
+            const double w = (b0 ? w1[0] : w0[0]) * (b1 ? w1[1] : w0[1]) * (b2 ? w1[2] : w0[2]);
+            if (w == 0.0) continue;
+            const int64_t P = off_c[m] + (b0 ? p1[0] : p0[0]) +
+                              static_cast<int64_t>(Nc) * ((b1 ? p1[1] : p0[1]) + static_cast<int64_t>(Nc) * (b2 ? p1[2] : p0[2]));
+            for (int c = 0; c < 3; ++c) v[c] += w * xc[P * 3 + c];
+        }
+        for (int c = 0; c < 3; ++c) {
+            const int64_t k = static_cast<int64_t>(node) * 3 + c;
+            if (freed_f[k]) xf[k] += v[c];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) coarse_apply_kernel(int M, const int *__restrict__ dim,
+                                                          const int *__restrict__ off, int nn,
+                                                          const double *__restrict__ inv, int stride,
+                                                          const double *__restrict__ b, double *__restrict__ x)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 3 * nn; k += gridDim.x * blockDim.x) {
+        const int node = k / 3;
+        const int m = find_mesh(off, M, node);
+        const int N = dim[m], nd = 3 * N * N * N;
+        const int r = k - 3 * off[m];
+        const double *row = inv + static_cast<int64_t>(m) * stride + static_cast<int64_t>(r) * nd;
+        const double *bm = b + static_cast<int64_t>(off[m]) * 3;
+        double acc = 0.0;
+        for (int j = 0; j < nd; ++j) acc += row[j] * bm[j];
+        x[k] = acc;
+    }
+}
+
+unsigned grid1(int64_t n) { return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384))); }
+
+}  // namespace
+
+int mg_setup(EquationsSet &s, int precision)
+{
+    Multigrid &g = *s.mg;
+    Runtime &rt = runtime();
+    const int M = s.n_meshes;
+    const unsigned eb = static_cast<unsigned>(std::min(s.n_elements, rt.num_cus * 2));
+    if (precision == OC_FP64)
+        q1_galerkin_element_kernel<double><<<eb, 256, 0, rt.stream>>>(s.n_elements, s.kbuf64.ptr, s.conn.ptr,
+                                                                     s.n_nodes, s.free_mask.ptr, g.ke1.ptr);
+    else
+        q1_galerkin_element_kernel<float><<<eb, 256, 0, rt.stream>>>(s.n_elements, s.kbuf32.ptr, s.conn.ptr,
+                                                                    s.n_nodes, s.free_mask.ptr, g.ke1.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    MgLevel &L1 = g.levels[1];
+    q1_stencil_kernel<<<grid1(static_cast<int64_t>(L1.nn) * 27), 256, 0, rt.stream>>>(
+        M, g.n.ptr, g.e_off.ptr, L1.off.ptr, L1.nn, g.ke1.ptr, L1.A.ptr);
+    coincident_q2_kernel<<<grid1(L1.nn), 256, 0, rt.stream>>>(M, g.n.ptr, s.q_off.ptr, s.n_nodes, L1.off.ptr, L1.nn,
+                                                              s.free_mask.ptr, L1.freed.ptr);
+    level_freed_kernel<<<grid1(L1.nn), 256, 0, rt.stream>>>(L1.nn, L1.A.ptr, L1.freed.ptr);
+    level_mask_kernel<<<grid1(static_cast<int64_t>(L1.nn) * 27), 256, 0, rt.stream>>>(
+        M, L1.dim.ptr, L1.off.ptr, L1.nn, L1.A.ptr, L1.freed.ptr, L1.Dinv.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    const int nl = static_cast<int>(g.levels.size());
+    for (int l = 1; l + 1 < nl; ++l) {
+        MgLevel &F = g.levels[l], &C = g.levels[l + 1];
+        galerkin_coarsen_kernel<<<grid1(static_cast<int64_t>(C.nn) * 27), 256, 0, rt.stream>>>(
+            M, F.dim.ptr, F.off.ptr, F.A.ptr, F.freed.ptr, C.dim.ptr, C.off.ptr, C.nn, C.A.ptr);
+        coincident_kernel<<<grid1(C.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.freed.ptr, C.dim.ptr,
+                                                             C.off.ptr, C.nn, C.freed.ptr);
+        level_freed_kernel<<<grid1(C.nn), 256, 0, rt.stream>>>(C.nn, C.A.ptr, C.freed.ptr);
+        level_mask_kernel<<<grid1(static_cast<int64_t>(C.nn) * 27), 256, 0, rt.stream>>>(
+            M, C.dim.ptr, C.off.ptr, C.nn, C.A.ptr, C.freed.ptr, C.Dinv.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+    }
+    MgLevel &Lc = g.levels[nl - 1];
+    coarse_inverse_kernel<<<static_cast<unsigned>(M), 256, 0, rt.stream>>>(M, Lc.dim.ptr, Lc.off.ptr, Lc.A.ptr,
+                                                                         g.coarse_stride, g.coarse_inv.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
+// z (displacement part, component-major) = V-cycle applied to t
+int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
+{
+    Multigrid &g = *s.mg;
+    Runtime &rt = runtime();
+    const int M = s.n_meshes, nq = s.n_nodes, nu = 3 * nq, nl = static_cast<int>(g.levels.size());
+    const double *inv_u = s.precond.ptr;  // nodal block inverses of the set's block preconditioner
+    const unsigned gu = static_cast<unsigned>((static_cast<int64_t>(nu) * 32 + 255) / 256);
+    auto residual0 = [&](const double *b, const double *x, double *r) -> int {
+        if (precision == OC_FP32)
+            uu_residual_kernel<float, 32><<<gu, 256, 0, rt.stream>>>(nu, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr,
+                                                                    s.values_sp.ptr, s.free_mask.ptr, b, x, r);
+        else
+            uu_residual_kernel<double, 32><<<gu, 256, 0, rt.stream>>>(nu, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr,
+                                                                      s.values.ptr, s.free_mask.ptr, b, x, r);
+        OC_HIP_CHECK(hipGetLastError());
+        return OC_NO_ERROR;
+    };
+    // level 0 pre-smoothing from x = 0
+    q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, t, z, g.omega, 1);
+    for (int k = 1; k < g.pre; ++k) {
+        OC_CHECK(residual0(t, z, g.r0.ptr));
+        q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, g.r0.ptr, z, g.omega, 0);
+    }
+    OC_CHECK(residual0(t, z, g.r0.ptr));
+    MgLevel &L1 = g.levels[1];
+    restrict_q2_kernel<<<grid1(L1.nn), 256, 0, rt.stream>>>(M, g.n.ptr, s.q_off.ptr, nq, L1.off.ptr, L1.nn,
+                                                            L1.freed.ptr, g.r0.ptr, L1.b.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    // down the stencil levels
+    for (int l = 1; l + 1 < nl; ++l) {
+        MgLevel &F = g.levels[l], &C = g.levels[l + 1];
+        stencil_jacobi_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(F.nn, F.Dinv.ptr, F.b.ptr, F.x.ptr, g.omega, 1);
+        for (int k = 1; k < g.pre; ++k) {
+            stencil_residual_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
+                                                                       F.freed.ptr, F.b.ptr, F.x.ptr, F.r.ptr);
+            stencil_jacobi_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(F.nn, F.Dinv.ptr, F.r.ptr, F.x.ptr, g.omega, 0);
+        }
+        stencil_residual_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
+                                                                   F.freed.ptr, F.b.ptr, F.x.ptr, F.r.ptr);
+        restrict_stencil_kernel<<<grid1(C.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, C.dim.ptr, C.off.ptr,
+                                                                   C.nn, C.freed.ptr, F.r.ptr, C.b.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+    }
+    MgLevel &Lc = g.levels[nl - 1];
+    coarse_apply_kernel<<<grid1(3 * static_cast<int64_t>(Lc.nn)), 256, 0, rt.stream>>>(
+        M, Lc.dim.ptr, Lc.off.ptr, Lc.nn, g.coarse_inv.ptr, g.coarse_stride, Lc.b.ptr, Lc.x.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    // up
+    for (int l = nl - 2; l >= 1; --l) {
+        MgLevel &F = g.levels[l], &C = g.levels[l + 1];
+        prolong_stencil_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, C.dim.ptr,
+                                                                  C.off.ptr, F.freed.ptr, C.x.ptr, F.x.ptr);
+        for (int k = 0; k < g.post; ++k) {
+            stencil_residual_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
+                                                                       F.freed.ptr, F.b.ptr, F.x.ptr, F.r.ptr);
+            stencil_jacobi_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(F.nn, F.Dinv.ptr, F.r.ptr, F.x.ptr, g.omega, 0);
+        }
+        OC_HIP_CHECK(hipGetLastError());
+    }
+    prolong_q2_kernel<<<grid1(nq), 256, 0, rt.stream>>>(M, g.n.ptr, s.q_off.ptr, nq, L1.off.ptr, s.free_mask.ptr,
+                                                        L1.x.ptr, z);
+    OC_HIP_CHECK(hipGetLastError());
+    for (int k = 0; k < g.post; ++k) {
+        OC_CHECK(residual0(t, z, g.r0.ptr));
+        q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, g.r0.ptr, z, g.omega, 0);
+    }
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
+}  // namespace oc
+
+using namespace oc;
+
+extern "C" int oc_multigrid_structure_set(oc_equations_set h, const int *n_per_mesh)
+{
+    OC_CHECK(require_runtime());
+    if (!h) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
+    EquationsSet &s = h->s;
+    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN || !n_per_mesh)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "multigrid structure: a Mooney-Rivlin set and n per mesh");
+    const int M = s.n_meshes;
+    // the connectivity must be that of regular generated n^3 meshes
+    std::vector<int32_t> cq(static_cast<size_t>(s.n_elements) * 27), cl(static_cast<size_t>(s.n_elements) * 8);
+    OC_HIP_CHECK(hipMemcpy(cq.data(), s.conn.ptr, cq.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    OC_HIP_CHECK(hipMemcpy(cl.data(), s.conn_l.ptr, cl.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int m = 0; m < M; ++m) {
+        const int64_t n = n_per_mesh[m];
+        const int64_t T = 2 * n + 1, N = n + 1;
+        if (n < 1 || n * n * n != s.h_e_off[m + 1] - s.h_e_off[m] || T * T * T != s.h_q_off[m + 1] - s.h_q_off[m] ||
+            N * N * N != s.h_l_off[m + 1] - s.h_l_off[m])
+            return set_error(OC_ERR_INVALID_ARGUMENT, "mesh %d is not a regular mesh of %lld^3 elements", m,
+                             static_cast<long long>(n));
+        for (int64_t e = 0; e < n * n * n; ++e) {
+            const int64_t ex = e % n, ey = (e / n) % n, ez = e / (n * n);
+            const int64_t ge = s.h_e_off[m] + e;
+            for (int a = 0; a < 27; ++a) {
+                const int64_t q = s.h_q_off[m] + (2 * ex + a % 3) + T * ((2 * ey + (a / 3) % 3) + T * (2 * ez + a / 9));
+                if (cq[ge * 27 + a] != q)
+                    return set_error(OC_ERR_INVALID_ARGUMENT, "mesh %d element %lld: not the regular node numbering",
+                                     m, static_cast<long long>(e));
+            }
+            for (int k = 0; k < 8; ++k) {
+                const int64_t p = s.h_l_off[m] + (ex + (k & 1)) + N * ((ey + ((k >> 1) & 1)) + N * (ez + (k >> 2)));
+                if (cl[ge * 8 + k] != p)
+                    return set_error(OC_ERR_INVALID_ARGUMENT, "mesh %d element %lld: not the regular pressure numbering",
+                                     m, static_cast<long long>(e));
+            }
+        }
+    }
+    auto g = new (std::nothrow) Multigrid();
+    if (!g) return set_error(OC_ERR_ALLOCATION, "out of host memory");
+    std::unique_ptr<Multigrid> guard(g);
+    g->h_n.assign(n_per_mesh, n_per_mesh + M);
+    OC_CHECK(g->n.upload(g->h_n.data(), M));
+    OC_CHECK(g->e_off.upload(s.h_e_off.data(), s.h_e_off.size()));
+    // levels: [0] placeholder, [1] trilinear (n+1)^3, then coarsening until every dim <= 3
+    g->levels.resize(2);
+    std::vector<int32_t> dims(M);
+    for (int m = 0; m < M; ++m) dims[m] = g->h_n[m] + 1;
+    while (true) {
+        MgLevel &L = g->levels.back();
+        L.h_dim = dims;
+        L.h_off.assign(M + 1, 0);
+        for (int m = 0; m < M; ++m) L.h_off[m + 1] = L.h_off[m] + dims[m] * dims[m] * dims[m];
+        L.nn = L.h_off[M];
+        OC_CHECK(L.dim.upload(L.h_dim.data(), M));
+        OC_CHECK(L.off.upload(L.h_off.data(), M + 1));
+        OC_CHECK(L.A.alloc(static_cast<size_t>(L.nn) * 27 * 9));
+        OC_CHECK(L.Dinv.alloc(static_cast<size_t>(L.nn) * 9));
+        OC_CHECK(L.freed.alloc(static_cast<size_t>(L.nn) * 3));
+        OC_CHECK(L.x.alloc(static_cast<size_t>(L.nn) * 3));
+        OC_CHECK(L.b.alloc(static_cast<size_t>(L.nn) * 3));
+        OC_CHECK(L.r.alloc(static_cast<size_t>(L.nn) * 3));
+        if (*std::max_element(dims.begin(), dims.end()) <= 3) break;
+        for (int m = 0; m < M; ++m) dims[m] = (dims[m] - 1 + 1) / 2 + 1;  // n -> ceil(n/2)
+        g->levels.emplace_back();
+    }
+    int maxnd = 0;
+    for (int m = 0; m < M; ++m) maxnd = std::max(maxnd, 3 * dims[m] * dims[m] * dims[m]);
+    if (maxnd > 81) return set_error(OC_ERR_UNSUPPORTED, "coarsest multigrid level too large");
+    g->coarse_stride = maxnd * maxnd;
+    OC_CHECK(g->coarse_inv.alloc(static_cast<size_t>(M) * g->coarse_stride));
+    OC_CHECK(g->ke1.alloc(static_cast<size_t>(s.n_elements) * 576));
+    OC_CHECK(g->r0.alloc(static_cast<size_t>(3) * s.n_nodes));
+    OC_CHECK(g->t0.alloc(static_cast<size_t>(3) * s.n_nodes));
+    delete s.mg;
+    s.mg = guard.release();
+    if (s.step_graph) {  // recorded without the multigrid setup
+        (void)hipGraphExecDestroy(s.step_graph);
+        s.step_graph = nullptr;
+    }
+    return OC_NO_ERROR;
+}
+
+extern "C" int oc_multigrid_options_set(oc_equations_set h, int pre_sweeps, int post_sweeps, double omega)
+{
+    OC_CHECK(require_runtime());
+    if (!h) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
+    if (!h->s.mg) return set_error(OC_ERR_INVALID_ARGUMENT, "no multigrid structure (oc_multigrid_structure_set)");
+    if (pre_sweeps < 1 || post_sweeps < 0 || !(omega > 0.0 && omega < 2.0))
+        return set_error(OC_ERR_INVALID_ARGUMENT, "multigrid options: pre >= 1, post >= 0, 0 < omega < 2");
+    h->s.mg->pre = pre_sweeps;
+    h->s.mg->post = post_sweeps;
+    h->s.mg->omega = omega;
+    return OC_NO_ERROR;
+}

@@ -287,6 +287,25 @@ __global__ void __launch_bounds__(256) triangular_u_kernel(int nq, const int64_t
     z[2 * nq + node] = o[6] * t[0] + o[7] * t[1] + o[8] * t[2];
 }
 
+// t = r_u - B^T z_p on the free displacement rows (pressure columns start at pstart[row])
+template <typename VT>
+__global__ void __launch_bounds__(256) schur_rhs_kernel(int nu, const int64_t *__restrict__ row_ptr,
+                                                       const int64_t *__restrict__ pstart,
+                                                       const int *__restrict__ col_idx, const VT *__restrict__ vals,
+                                                       const uint8_t *__restrict__ free_mask,
+                                                       const double *__restrict__ r, const double *__restrict__ z,
+                                                       double *__restrict__ t)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nu) return;
+    double acc = 0.0;
+    if (free_mask[row]) {
+        acc = r[row];
+        for (int64_t k = pstart[row]; k < row_ptr[row + 1]; ++k) acc -= static_cast<double>(vals[k]) * z[col_idx[k]];
+    }
+    t[row] = acc;
+}
+
 __global__ void __launch_bounds__(256) pressure_start_kernel(int n_u_rows, int first_p_col,
                                                             const int64_t *__restrict__ row_ptr,
                                                             const int *__restrict__ col_idx,
@@ -407,6 +426,7 @@ struct Krylov {
             return OC_NO_ERROR;
         }
         const int nq = s.n_nodes, nl = s.n_nodes_l;
+        if (precond == OC_PRECONDITIONER_MULTIGRID) OC_CHECK(mg_setup(s, fp32 ? OC_FP32 : OC_FP64));
         if (fp32) {
             block_setup_kernel<float><<<grid_rows(nq), 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.col_idx.ptr,
                                                                             s.values_sp.ptr, s.free_mask.ptr, inv_u,
@@ -436,7 +456,20 @@ struct Krylov {
             apply_jacobi_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, inv_u, rin, zout);
         } else {
             const int nq = s.n_nodes, nl = s.n_nodes_l;
-            if (precond == OC_PRECONDITIONER_BLOCK_TRIANGULAR) {
+            if (precond == OC_PRECONDITIONER_MULTIGRID) {
+                apply_jacobi_kernel<<<grid_for(nl), 256, 0, rt.stream>>>(nl, inv_p, rin + 3 * static_cast<int64_t>(nq),
+                                                                         zout + 3 * static_cast<int64_t>(nq));
+                const int nu = 3 * nq;
+                double *t = s.mg->t0.ptr;
+                if (fp32)
+                    schur_rhs_kernel<float><<<grid_rows(nu), 256, 0, rt.stream>>>(
+                        nu, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values_sp.ptr, s.free_mask.ptr, rin, zout, t);
+                else
+                    schur_rhs_kernel<double><<<grid_rows(nu), 256, 0, rt.stream>>>(
+                        nu, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values.ptr, s.free_mask.ptr, rin, zout, t);
+                OC_HIP_CHECK(hipGetLastError());
+                OC_CHECK(mg_apply(s, fp32 ? OC_FP32 : OC_FP64, t, zout));
+            } else if (precond == OC_PRECONDITIONER_BLOCK_TRIANGULAR) {
                 apply_jacobi_kernel<<<grid_for(nl), 256, 0, rt.stream>>>(nl, inv_p, rin + 3 * static_cast<int64_t>(nq),
                                                                          zout + 3 * static_cast<int64_t>(nq));
                 if (fp32)
@@ -600,6 +633,11 @@ static int solver_prepare(EquationsSet &s, int restart, Krylov &K)
 {
     const int n = s.n_rows;
     if (restart < 1 || restart + 1 > 1024) return set_error(OC_ERR_INVALID_ARGUMENT, "krylov_restart must be in [1, 1023]");
+    if (K.precond < OC_PRECONDITIONER_JACOBI || K.precond > OC_PRECONDITIONER_MULTIGRID)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "unknown preconditioner %d", K.precond);
+    if (K.precond == OC_PRECONDITIONER_MULTIGRID && (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN || !s.mg))
+        return set_error(OC_ERR_INVALID_ARGUMENT,
+                         "the multigrid preconditioner needs oc_multigrid_structure_set on a Mooney-Rivlin set");
     if (!s.free_mask.ptr) {
         OC_CHECK(s.free_mask.alloc(n));
         OC_HIP_CHECK(hipMemset(s.free_mask.ptr, 1, n));

@@ -220,6 +220,18 @@ class MooneyRivlinEquationsSet(EquationsSet):
     def jacobian_evaluate(self, precision=_lib.OC_FP64):
         call("oc_equations_set_jacobian_evaluate", self._h, int(precision))
 
+    def set_multigrid(self, pre_sweeps=2, post_sweeps=2, omega=0.6):
+        """Declare the mesh(es) regular n^3 generated meshes and build the
+        geometric multigrid hierarchy (OC_PRECONDITIONER_MULTIGRID)."""
+        meshes = getattr(self, "meshes", None) or [self.mesh]
+        ns = []
+        for m in meshes:
+            if len(set(m.n)) != 1:
+                raise ValueError("multigrid needs n x n x n meshes")
+            ns.append(m.n[0])
+        call("oc_multigrid_structure_set", self._h, np.array(ns, np.int32))
+        call("oc_multigrid_options_set", self._h, int(pre_sweeps), int(post_sweeps), float(omega))
+
     def jacobian_evaluate_graph(self, precision=_lib.OC_FP64):
         """Jacobian assembly replayed from the set's recorded HIP graph."""
         call("oc_jacobian_evaluate_graph", self._h, int(precision))

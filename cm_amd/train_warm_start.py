@@ -39,7 +39,7 @@ def draw_case(rng, n_min=4, n_max=16):
 def solver_options(precision=_lib.OC_FP64, restart=100, reference_norm=0.0):
     return newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, reference_norm=reference_norm,
                           krylov_restart=restart, krylov_relative_tolerance=-1.0, jacobian_precision=precision,
-                          preconditioner=_lib.OC_PRECONDITIONER_BLOCK_TRIANGULAR, maximum_iterations=30,
+                          preconditioner=_lib.OC_PRECONDITIONER_MULTIGRID, maximum_iterations=30,
                           krylov_maximum_iterations=50000)
 
 
@@ -48,6 +48,7 @@ def make_problem(case, jitter=0.1):
     eq = MooneyRivlinEquationsSet(mesh, case["c10"], case["c01"])
     eq.set_dofs(eq.undeformed_dofs())
     eq.fixed_dofs = eq.uniaxial_boundary_conditions(1.0 + case["stretch"])
+    eq.set_multigrid()
     return mesh, eq
 
 

@@ -183,6 +183,8 @@ int oc_load_increment_apply(oc_equations_set equations_set, int iteration, int m
 #define OC_PRECONDITIONER_JACOBI 0          /* point Jacobi, zero diagonal -> 1 (PETSc PCJACOBI) */
 #define OC_PRECONDITIONER_BLOCK_DIAGONAL 1  /* nodal 3x3 block Jacobi + diagonal Schur estimate for p */
 #define OC_PRECONDITIONER_BLOCK_TRIANGULAR 2 /* block upper-triangular variant of the above */
+#define OC_PRECONDITIONER_MULTIGRID 3       /* block upper-triangular, displacement block by one geometric
+                                               multigrid V-cycle (needs oc_multigrid_structure_set) */
 
 typedef struct {
     double absolute_tolerance;   /* Newton: converged when ||F|| <= max(abs, rel * ||F_ref||) */
@@ -232,6 +234,18 @@ int oc_newton_solve_batch(oc_equations_set equations_set, const oc_newton_option
  * Solve, problem_routines.f90:2740-2830): assemble, set the Dirichlet values
  * and solve the free rows with the Krylov settings of `options`. */
 int oc_linear_solve(oc_equations_set equations_set, const oc_newton_options *options, oc_newton_stats *stats);
+/* Geometric multigrid for OC_PRECONDITIONER_MULTIGRID (no counterpart in the
+ * reference, whose KSP preconditioner is chosen by the user): declares every
+ * mesh of the set a regular generated mesh of n_per_mesh[m]^3 elements
+ * (node numbering as GENERATED_MESH_REGULAR_CREATE_FINISH; checked, else
+ * OC_ERR_INVALID_ARGUMENT) and builds the level hierarchy: triquadratic ->
+ * trilinear on the same elements -> coarsening by 2 per direction down to
+ * <= 3 nodes per direction.  Coarse operators are Galerkin products of the
+ * current Jacobian (formed with the preconditioner setup of each Newton step). */
+int oc_multigrid_structure_set(oc_equations_set equations_set, const int *n_per_mesh);
+/* V-cycle smoothing: damped nodal block-Jacobi sweeps before and after the
+ * coarse correction (defaults 2, 2, 0.6). */
+int oc_multigrid_options_set(oc_equations_set equations_set, int pre_sweeps, int post_sweeps, double omega);
 /* Launch-shape autotuner (no counterpart in the reference): times the
  * element-matrix kernel's grid, the CSR row gather's prefetch depth and the
  * SpMV's lanes per row on this set at its current DOFs, keeps the fastest of

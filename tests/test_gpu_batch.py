@@ -152,3 +152,29 @@ def test_warm_start_on_a_batch_equals_per_mesh(gpu):
         np.testing.assert_array_equal(out[batch.mesh_dofs(m)], expect[m])
     batch.destroy()
     net.destroy()
+
+
+def test_batched_newton_with_multigrid(gpu):
+    cm = gpu
+    cases = [dict(n=2, seed=31, c10=2.0, c01=6.0, s=1.10), dict(n=5, seed=32, c10=1.0, c01=0.2, s=1.05),
+             dict(n=3, seed=33, c10=5.0, c01=10.0, s=1.20), dict(n=1, seed=34, c10=3.0, c01=1.0, s=1.15)]
+    meshes = make_meshes(cm, cases)
+    batch = cm.MooneyRivlinBatch(meshes, [c["c10"] for c in cases], [c["c01"] for c in cases])
+    batch.set_dofs(batch.undeformed_dofs())
+    fixed = batch.uniaxial_boundary_conditions([c["s"] for c in cases])
+    batch.set_multigrid()
+    opts = cm.newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=100,
+                             krylov_relative_tolerance=-1.0, preconditioner=cm.OC_PRECONDITIONER_MULTIGRID,
+                             maximum_iterations=30)
+    for step in (1, 2):
+        batch.load_increment_apply(step, 3)
+        refs = batch.mesh_residual_norms()
+        st, finals = batch.newton_solve(opts)
+        assert st["converged"], st
+        d = batch.get_dofs()
+        for m, (mesh, c) in enumerate(zip(meshes, cases)):
+            idx = batch.mesh_dofs(m)
+            r = orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d[idx], c["c10"], c["c01"])
+            r[np.isin(idx, fixed)] = 0.0
+            assert np.linalg.norm(r) <= 1.05e-10 * refs[m], (step, m)
+    batch.destroy()
