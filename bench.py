@@ -152,7 +152,9 @@ def batched_line(cm, n_meshes, reps=3):
 
 
 def cpu_baseline(mesh, dofs, c10, c01, seconds):
-    """Oracle (C FP64 restatement of the reference loop) on this host, 1 thread."""
+    """Oracle (C FP64 restatement of the reference loop) on this host: one
+    thread in the reference's element order, and all cores (up to 16 threads,
+    element colours over OpenMP threads) on the same bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
     rp, ci = orc.pattern([(mesh.n_nodes, mesh.elem_nodes)] * 3 + [(mesh.n_nodes_l, mesh.elem_nodes_l)])
@@ -163,6 +165,13 @@ def cpu_baseline(mesh, dofs, c10, c01, seconds):
         orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, dofs, c10, c01, rp, ci)
         t_total += time.perf_counter() - t0
         reps += 1
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    preps, p_total = 0, 0.0
+    while p_total < seconds / 4 or preps == 0:
+        t0 = time.perf_counter()
+        orc.mr_assemble_parallel(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, dofs, c10, c01, rp, ci, threads)
+        p_total += time.perf_counter() - t0
+        preps += 1
     return {
         "value": mesh.n_elements * reps / t_total,
         "unit": "elements/s",
@@ -170,6 +179,9 @@ def cpu_baseline(mesh, dofs, c10, c01, seconds):
         "kind": "port",
         "sample": f"{mesh.n[0]}^3 MR residual + FP64 analytic Jacobian with per-entry CSR bisection add "
                   f"({mesh.n_elements} elements) x {reps} repetitions, {t_total:.1f} s, setup excluded",
+        "all_cores": {"value": mesh.n_elements * preps / p_total, "threads": threads,
+                      "sample": f"same workload, element colours over OpenMP threads, x {preps} repetitions, "
+                                f"{p_total:.1f} s"},
     }
 
 
@@ -185,10 +197,10 @@ def main():
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--load-steps", type=int, default=10)
     ap.add_argument("--batched-meshes", type=int, default=256, help="0 skips the batched-meshes line")
-    ap.add_argument("--profile", choices=["jacobian", "copy"], default=None,
-                    help="run only one kernel sequence for rocprofv3 counter collection: the FP64 Jacobian "
-                         "assembly (--steps times), or a device copy of 512 MiB (the FETCH_SIZE/WRITE_SIZE "
-                         "calibration)")
+    ap.add_argument("--profile", choices=["jacobian", "jacobian32", "laplace", "copy"], default=None,
+                    help="run only one kernel sequence for rocprofv3 counter collection: the FP64 (or FP32) "
+                         "Jacobian assembly of the MR workload, the 128^3 Laplace assembly (--steps times), or a "
+                         "device copy of 512 MiB (the FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
@@ -208,6 +220,15 @@ def main():
 
     import cm_amd as cm
     cm.initialise(0)
+    if args.profile == "laplace":
+        mesh = cm.RegularMesh.generate(1, 128)
+        eq = cm.LaplaceEquationsSet(mesh)
+        for _ in range(args.steps):
+            eq.assemble()
+        cm.synchronise()
+        print(json.dumps({"profile": "laplace", "steps": args.steps, "elements": mesh.n_elements, "nnz": eq.nnz,
+                          "algorithmic_bytes": laplace_bytes(mesh.n_elements, mesh.n_nodes, eq.nnz)}))
+        return
     c10, c01 = 2.0, 6.0
     mesh = cm.RegularMesh.generate(2, args.n, jitter=0.1, seed=2024)
     eq = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
@@ -218,11 +239,12 @@ def main():
         eq.residual_evaluate()
         eq.jacobian_evaluate(cm.OC_FP64)
 
-    if args.profile == "jacobian":
+    if args.profile in ("jacobian", "jacobian32"):
+        prec = cm.OC_FP64 if args.profile == "jacobian" else cm.OC_FP32
         for _ in range(args.steps):
-            eq.jacobian_evaluate(cm.OC_FP64)
+            eq.jacobian_evaluate(prec)
         cm.synchronise()
-        print(json.dumps({"profile": "jacobian", "steps": args.steps, "elements": mesh.n_elements}))
+        print(json.dumps({"profile": args.profile, "steps": args.steps, "elements": mesh.n_elements}))
         return
 
     ms, wall = time_steps(step, args.steps, args.warmup, cm)

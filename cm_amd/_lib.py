@@ -136,6 +136,7 @@ SIGNATURES = {
     "oc_mlp_dataset_set": [_H, _c_int, _fp, _fp],
     "oc_mlp_train": [_H, _c_int, _c_int, _c_float, ctypes.c_ulonglong, ctypes.POINTER(_c_float)],
     "oc_mlp_warm_start": [_H, _H, _c_double],
+    "oc_mlp_warm_start_batch": [_H, _H, _dp],
     "oc_mooney_rivlin_batch_create": [_c_int, _i32, _i32, _i32, _dp, _i32, _i32, _dp, _dp, _c_int,
                                       ctypes.POINTER(_H)],
     "oc_mesh_count": [_H, ctypes.POINTER(_c_int)],

@@ -127,11 +127,6 @@ static int finish_structure(EquationsSet &s, int ncomp, const int32_t *nn, const
 {
     OC_CHECK(build_pattern(ncomp, s.n_elements, nn, nen, conn, s.h_row_ptr, s.h_col_idx));
     s.nnz = static_cast<int64_t>(s.h_col_idx.size());
-    if (s.cls == OC_EQUATIONS_SET_LAPLACE) {  // the Laplace scatter runs by colours
-        std::vector<int32_t> colour_elems;
-        OC_CHECK(colour_elements(s.n_elements, ncomp, nn, nen, conn, s.colour_ptr, colour_elems));
-        OC_CHECK(s.colour_elems.upload(colour_elems.data(), colour_elems.size()));
-    }
     OC_CHECK(s.row_ptr.upload(s.h_row_ptr.data(), s.h_row_ptr.size()));
     OC_CHECK(s.col_idx.upload(s.h_col_idx.data(), s.h_col_idx.size()));
     OC_CHECK(s.dofs.alloc(s.n_rows));
@@ -370,7 +365,7 @@ int oc_equations_set_info(oc_equations_set h, int *set_class, int *n_rows, long 
     if (n_rows) *n_rows = h->s.n_rows;
     if (nnz) *nnz = h->s.nnz;
     if (n_elements) *n_elements = h->s.n_elements;
-    if (n_colours) *n_colours = static_cast<int>(h->s.colour_ptr.size()) - 1;
+    if (n_colours) *n_colours = 0;  // assembly needs no element colouring
     return OC_NO_ERROR;
 }
 

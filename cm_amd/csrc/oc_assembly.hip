@@ -1,12 +1,10 @@
 // oc_assembly.hip -- basis tables, fill kernel and the gfx950 Laplace
-// linear-matrix assembly with deterministic CSR scatter.
-//
-// Determinism: elements are greedily coloured on the host so that no two
-// elements of one colour share a DOF.  Each colour is one launch; inside a
-// launch every CSR entry / residual entry receives at most one contribution,
-// and the colour order fixes the summation order of every entry.
+// linear-matrix assembly: element matrices, then a deterministic write-once
+// CSR row gather in ascending element order.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "oc_fe_device.h"
 #include "oc_internal.h"
@@ -131,10 +129,14 @@ template int fill<float>(float *, int64_t, float);
 // ---------------------------------------------------------------------------
 // Laplace, standard subtype (Laplace_equations_routines.f90:474-571):
 // K_e(ms,ns) += sum_{mi,ni} dphi_ms/dxi_mi GU(mi,ni) dphi_ns/dxi_ni * J w.
-// Eight lanes per element: lane g first evaluates the metric of Gauss point
-// g, then lane ms owns element row ms and scatters it into CSR row node(ms)
-// through the 8-bit slot-offset map (all loads of the row before its stores:
-// the eight slots are distinct).
+// Two phases, as the Jacobian: laplace_element_kernel writes every element
+// matrix to a buffer (eight lanes per element: lane g evaluates the metric of
+// Gauss point g, then lane ms owns element row ms and writes it as 64
+// contiguous bytes); laplace_row_gather_kernel gives each CSR row one thread,
+// which adds the element rows touching it in ascending element order (the
+// reference's element-loop order) into an LDS copy of its block's rows, and
+// the block writes its contiguous CSR segment once: no zero fill, no
+// read-modify-write, no colours.
 constexpr int kLapElemsPerBlock = 32;
 
 __global__ void __launch_bounds__(256)
@@ -158,20 +160,16 @@ laplace_slot_offsets_kernel(int n_elements, const int *__restrict__ conn, const 
 }
 
 __global__ void __launch_bounds__(256)
-laplace_assemble_kernel(const int *__restrict__ colour_elems, int n_in_colour, const int *__restrict__ conn,
-                        const double *__restrict__ coords, int n_nodes, const int64_t *__restrict__ row_ptr,
-                        const uint8_t *__restrict__ offs, double *__restrict__ values,
-                        const BasisTables *__restrict__ T)
+laplace_element_kernel(int n_elements, const int *__restrict__ conn, const double *__restrict__ coords, int n_nodes,
+                       const BasisTables *__restrict__ T, double *__restrict__ kbuf)
 {
     __shared__ double s_x[kLapElemsPerBlock][3][8];
     __shared__ double s_gu[kLapElemsPerBlock][8][10];
     const int local = threadIdx.x >> 3, lane = threadIdx.x & 7;
-    const int idx = blockIdx.x * kLapElemsPerBlock + local;
-    const bool active = idx < n_in_colour;
-    int e = 0, node = 0;
+    const int e = blockIdx.x * kLapElemsPerBlock + local;
+    const bool active = e < n_elements;
     if (active) {
-        e = colour_elems[idx];
-        node = conn[e * 8 + lane];
+        const int node = conn[static_cast<int64_t>(e) * 8 + lane];
         for (int c = 0; c < 3; ++c) s_x[local][c][lane] = coords[static_cast<int64_t>(c) * n_nodes + node];
     }
     __syncthreads();
@@ -210,16 +208,41 @@ laplace_assemble_kernel(const int *__restrict__ colour_elems, int n_in_colour, c
             k[ns] = k[ns] + sum * rwg;
         }
     }
-    const int64_t base = row_ptr[node];
-    const uint64_t packed = *reinterpret_cast<const uint64_t *>(offs + (static_cast<int64_t>(e) * 8 + ms) * 8);
-    int64_t slot[8];
-    double old[8];
+    double2 *out = reinterpret_cast<double2 *>(kbuf + (static_cast<int64_t>(e) * 8 + ms) * 8);
 #pragma unroll
-    for (int ns = 0; ns < 8; ++ns) slot[ns] = base + static_cast<int64_t>((packed >> (8 * ns)) & 0xff);
+    for (int q = 0; q < 4; ++q) out[q] = make_double2(k[2 * q], k[2 * q + 1]);
+}
+
+template <int kRows>
+__global__ void __launch_bounds__(kRows)
+laplace_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const int *__restrict__ gptr,
+                          const int *__restrict__ gidx, const double *__restrict__ kbuf,
+                          const uint8_t *__restrict__ offs, double *__restrict__ values)
+{
+    extern __shared__ unsigned char s_raw[];
+    double *s_seg = reinterpret_cast<double *>(s_raw);
+    const int r0 = blockIdx.x * kRows, t = threadIdx.x, r = r0 + t;
+    const int r_end = min(r0 + kRows, n_rows);
+    const int64_t base = row_ptr[r0];
+    const int seg = static_cast<int>(row_ptr[r_end] - base);
+    for (int k = t; k < seg; k += kRows) s_seg[k] = 0.0;
+    __syncthreads();
+    if (r < n_rows) {
+        double *row = s_seg + (row_ptr[r] - base);
+        for (int c = gptr[r]; c < gptr[r + 1]; ++c) {
+            const int64_t ei = gidx[c];  // element * 8 + element row
+            const uint64_t packed = *reinterpret_cast<const uint64_t *>(offs + ei * 8);
+            const double2 *kr = reinterpret_cast<const double2 *>(kbuf + ei * 8);
 #pragma unroll
-    for (int ns = 0; ns < 8; ++ns) old[ns] = values[slot[ns]];
-#pragma unroll
-    for (int ns = 0; ns < 8; ++ns) values[slot[ns]] = old[ns] + k[ns];
+            for (int q = 0; q < 4; ++q) {
+                const double2 v = kr[q];
+                row[(packed >> (16 * q)) & 0xff] += v.x;
+                row[(packed >> (16 * q + 8)) & 0xff] += v.y;
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = t; k < seg; k += kRows) values[base + k] = s_seg[k];
 }
 
 static int laplace_prepare(EquationsSet &s)
@@ -237,9 +260,39 @@ static int laplace_prepare(EquationsSet &s)
     int h_bad = 0;
     OC_HIP_CHECK(hipMemcpyAsync(&h_bad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, rt.stream));
     OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
-    if (h_bad)
+    if (h_bad) {
+        s.slot_offsets8.release();
         return set_error(OC_ERR_PATTERN, "%d Laplace element entries missing from the CSR structure or beyond "
                                          "the 8-bit scatter map", h_bad);
+    }
+    if (static_cast<int64_t>(s.n_elements) * 8 >= INT32_MAX)
+        return set_error(OC_ERR_UNSUPPORTED, "%d elements exceed the int32 element-row gather lists", s.n_elements);
+    // rows per gather block: the block's CSR segment must fit the LDS
+    int64_t max_seg = 0;
+    for (int r0 = 0; r0 < s.n_rows; r0 += 256)
+        max_seg = std::max(max_seg, s.h_row_ptr[std::min(r0 + 256, s.n_rows)] - s.h_row_ptr[r0]);
+    s.lap_rows_per_block = 256;
+    if (max_seg > 8192) {
+        max_seg = 0;
+        for (int r0 = 0; r0 < s.n_rows; r0 += 64)
+            max_seg = std::max(max_seg, s.h_row_ptr[std::min(r0 + 64, s.n_rows)] - s.h_row_ptr[r0]);
+        s.lap_rows_per_block = 64;
+        if (max_seg > 8192)
+            return set_error(OC_ERR_UNSUPPORTED, "Laplace CSR rows too long for the row gather (%lld entries in 64 rows)",
+                             static_cast<long long>(max_seg));
+    }
+    s.max_row_len = static_cast<int>(max_seg);
+    // DOF -> element rows, ascending element order
+    std::vector<int32_t> hconn(static_cast<size_t>(s.n_elements) * 8);
+    OC_HIP_CHECK(hipMemcpy(hconn.data(), s.conn.ptr, hconn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int32_t> ptr(static_cast<size_t>(s.n_rows) + 1, 0), idx(hconn.size());
+    for (size_t k = 0; k < hconn.size(); ++k) ptr[hconn[k] + 1]++;
+    for (int d = 0; d < s.n_rows; ++d) ptr[d + 1] += ptr[d];
+    std::vector<int32_t> fillp(ptr.begin(), ptr.end() - 1);
+    for (size_t k = 0; k < hconn.size(); ++k) idx[fillp[hconn[k]]++] = static_cast<int32_t>(k);
+    OC_CHECK(s.gather_ptr.upload(ptr.data(), ptr.size()));
+    OC_CHECK(s.gather_idx.upload(idx.data(), idx.size()));
+    OC_CHECK(s.kbuf64.alloc(static_cast<size_t>(s.n_elements) * 64));
     return OC_NO_ERROR;
 }
 
@@ -249,19 +302,25 @@ int launch_laplace_assemble(EquationsSet &s)
     OC_CHECK(ensure_tables(&T));
     OC_CHECK(laplace_prepare(s));
     Runtime &rt = runtime();
-    OC_CHECK(fill<double>(s.values.ptr, s.nnz, 0.0));
     OC_CHECK(fill<double>(s.rhs.ptr, s.n_rows, 0.0));  // RHS element vector is 0 (:542)
-    const int ncol = static_cast<int>(s.colour_ptr.size()) - 1;
-    for (int c = 0; c < ncol; ++c) {
-        const int n = s.colour_ptr[c + 1] - s.colour_ptr[c];
-        if (n == 0) continue;
-        const unsigned blocks = static_cast<unsigned>((n + kLapElemsPerBlock - 1) / kLapElemsPerBlock);
-        KernelTimer timer("laplace_assemble");
-        laplace_assemble_kernel<<<blocks, 256, 0, rt.stream>>>(s.colour_elems.ptr + s.colour_ptr[c], n, s.conn.ptr,
-                                                              s.coords.ptr, s.n_nodes, s.row_ptr.ptr,
-                                                              s.slot_offsets8.ptr, s.values.ptr, T);
+    {
+        KernelTimer timer("laplace_element");
+        const unsigned blocks = static_cast<unsigned>((s.n_elements + kLapElemsPerBlock - 1) / kLapElemsPerBlock);
+        laplace_element_kernel<<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.coords.ptr, s.n_nodes, T,
+                                                             s.kbuf64.ptr);
         OC_HIP_CHECK(hipGetLastError());
     }
+    KernelTimer timer("laplace_row_gather");
+    const size_t lds = static_cast<size_t>(s.max_row_len) * sizeof(double);
+    if (s.lap_rows_per_block == 256)
+        laplace_row_gather_kernel<256><<<static_cast<unsigned>((s.n_rows + 255) / 256), 256, lds, rt.stream>>>(
+            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.gather_idx.ptr, s.kbuf64.ptr, s.slot_offsets8.ptr,
+            s.values.ptr);
+    else
+        laplace_row_gather_kernel<64><<<static_cast<unsigned>((s.n_rows + 63) / 64), 64, lds, rt.stream>>>(
+            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.gather_idx.ptr, s.kbuf64.ptr, s.slot_offsets8.ptr,
+            s.values.ptr);
+    OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
 }
 

@@ -159,10 +159,6 @@ struct EquationsSet {
     std::vector<int64_t> h_row_ptr;
     std::vector<int32_t> h_col_idx;
 
-    // colouring: elements of colour c are colour_elems[colour_ptr[c] .. colour_ptr[c+1])
-    std::vector<int32_t> colour_ptr;
-    DevBuf<int32_t> colour_elems;
-
     DevBuf<double> coords;   // 3 * n_nodes, separated
     DevBuf<int32_t> conn;    // nen * E
     DevBuf<int32_t> conn_l;  // 8 * E (MR pressure component)
@@ -184,8 +180,10 @@ struct EquationsSet {
     // element residuals (E*89) and the DOF -> element-entry gather lists
     DevBuf<double> relem;
     DevBuf<int32_t> gather_ptr, gather_idx;
-    // Laplace: 8-bit offsets, slot = row_ptr[node_i] + slot_offsets8[(e*8+i)*8+j].
+    // Laplace: 8-bit offsets, slot = row_ptr[node_i] + slot_offsets8[(e*8+i)*8+j];
+    // element matrices in kbuf64 (E*64) and rows per row-gather block
     DevBuf<uint8_t> slot_offsets8;
+    int lap_rows_per_block = 256;
 
     // Dirichlet conditions (0-based DOFs, full values) and the free-DOF mask
     int n_bc = 0;
@@ -229,8 +227,6 @@ int fill(T *p, int64_t n, T v);
 // ---- host-side setup helpers (oc_setup.cpp) ---------------------------------
 int build_pattern(int ncomp, int n_elements, const int32_t *nn, const int32_t *nen, const int32_t *const *conn,
                   std::vector<int64_t> &row_ptr, std::vector<int32_t> &col_idx);
-int colour_elements(int n_elements, int ncomp, const int32_t *nn, const int32_t *nen, const int32_t *const *conn,
-                    std::vector<int32_t> &colour_ptr, std::vector<int32_t> &colour_elems);
 
 // ---- kernel launchers --------------------------------------------------------
 int launch_laplace_assemble(EquationsSet &s);

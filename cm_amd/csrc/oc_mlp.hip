@@ -192,13 +192,28 @@ __global__ void __launch_bounds__(256) normalise_kernel(int n, int n_in, const f
 }
 
 // warm-start features of every quadratic node: X, z - X, load increment, c10, c01
+// mesh of union node `node` given the per-mesh node offsets (M + 1 entries)
+__device__ __forceinline__ int mesh_of(const int *__restrict__ off, int M, int node)
+{
+    int lo = 0, hi = M - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= node) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __global__ void __launch_bounds__(256) warm_features_kernel(int nq, const double *__restrict__ coords,
-                                                           const double *__restrict__ dofs, double dload,
+                                                           const double *__restrict__ dofs, int M,
+                                                           const int *__restrict__ q_off,
+                                                           const double *__restrict__ dload_mesh,
                                                            const double *__restrict__ node_mat,
                                                            const float *__restrict__ shift,
                                                            const float *__restrict__ scale, float *__restrict__ out)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        const double dload = dload_mesh[mesh_of(q_off, M, i)];
         float f[9];
         for (int c = 0; c < 3; ++c) {
             const double X = coords[static_cast<int64_t>(c) * nq + i];
@@ -213,11 +228,14 @@ __global__ void __launch_bounds__(256) warm_features_kernel(int nq, const double
 }
 
 // z += dload * y[:, 0:3] on the free displacement DOFs of every quadratic node
-__global__ void __launch_bounds__(256) warm_apply_kernel(int nq, const float *__restrict__ y, int ldy, double dload,
+__global__ void __launch_bounds__(256) warm_apply_kernel(int nq, const float *__restrict__ y, int ldy, int M,
+                                                        const int *__restrict__ q_off,
+                                                        const double *__restrict__ dload_mesh,
                                                         const uint8_t *__restrict__ free_mask,
                                                         double *__restrict__ dofs)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        const double dload = dload_mesh[mesh_of(q_off, M, i)];
         for (int c = 0; c < 3; ++c) {
             const int64_t d = static_cast<int64_t>(c) * nq + i;
             if (free_mask == nullptr || free_mask[d])
@@ -244,7 +262,9 @@ __global__ void __launch_bounds__(256) linear_to_quadratic_kernel(int n_elements
 // p += dload * (c10 + c01) * y[quad(node), 3] on the free pressure DOFs (the
 // network's pressure output is in units of c10 + c01)
 __global__ void __launch_bounds__(256) warm_apply_pressure_kernel(int nq, int nl, const int *__restrict__ map,
-                                                                 const float *__restrict__ y, double dload,
+                                                                 const float *__restrict__ y, int M,
+                                                                 const int *__restrict__ q_off,
+                                                                 const double *__restrict__ dload_mesh,
                                                                  const double *__restrict__ node_mat,
                                                                  const uint8_t *__restrict__ free_mask,
                                                                  double *__restrict__ dofs)
@@ -252,6 +272,7 @@ __global__ void __launch_bounds__(256) warm_apply_pressure_kernel(int nq, int nl
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
         const int64_t d = static_cast<int64_t>(3) * nq + i;
         const int q = map[i];
+        const double dload = dload_mesh[mesh_of(q_off, M, q)];
         if (free_mask == nullptr || free_mask[d])
             dofs[d] += dload * (node_mat[2 * q] + node_mat[2 * q + 1]) * static_cast<double>(y[static_cast<int64_t>(q) * 4 + 3]);
     }
@@ -272,6 +293,7 @@ struct Mlp {
     std::vector<DevBuf<float>> pre, act;  // pre[l]: n x out_l ; act[l]: n x in_l (act[0] = input)
     DevBuf<float> dY, dA, dB, partial, out;
     DevBuf<float> data_x, data_t;
+    DevBuf<double> dload;  // per-mesh load increments of the last warm start
     int n_data = 0;
 
     int layers() const { return static_cast<int>(in_dim.size()); }
@@ -571,22 +593,33 @@ int oc_mlp_train(oc_mlp mlp, int steps, int batch, float learning_rate, unsigned
 int oc_mlp_warm_start(oc_mlp mlp, oc_equations_set h, double load_increment)
 {
     OC_CHECK(require_runtime());
+    if (!h) return set_error(OC_ERR_INVALID_HANDLE, "NULL handle");
+    std::vector<double> d(h->s.n_meshes, load_increment);
+    return oc_mlp_warm_start_batch(mlp, h, d.data());
+}
+
+int oc_mlp_warm_start_batch(oc_mlp mlp, oc_equations_set h, const double *load_increments)
+{
+    OC_CHECK(require_runtime());
     if (!mlp || !h) return set_error(OC_ERR_INVALID_HANDLE, "NULL handle");
+    if (!load_increments) return set_error(OC_ERR_INVALID_ARGUMENT, "load_increments is NULL");
     Mlp &m = mlp->m;
     EquationsSet &s = h->s;
     if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN || m.n_in != 9 || (m.n_out != 3 && m.n_out != 4))
         return set_error(OC_ERR_INVALID_ARGUMENT, "warm start needs a Mooney-Rivlin set and a 9-in, 3- or 4-out MLP");
-    const int nq = s.n_nodes, nl = s.n_nodes_l;
+    const int nq = s.n_nodes, nl = s.n_nodes_l, M = s.n_meshes;
     OC_CHECK(m.ensure_capacity(nq));
     Runtime &rt = runtime();
+    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));  // the increments buffer may still be read by a previous call
+    OC_CHECK(m.dload.upload(load_increments, static_cast<size_t>(M)));
     KernelTimer timer("mlp_warm_start");
     if (!s.node_mat.ptr) OC_CHECK(mr_node_materials(s));
-    warm_features_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, s.coords.ptr, s.dofs.ptr, load_increment,
-                                                                s.node_mat.ptr, m.shift.ptr, m.scale.ptr,
+    warm_features_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, s.coords.ptr, s.dofs.ptr, M, s.q_off.ptr,
+                                                                m.dload.ptr, s.node_mat.ptr, m.shift.ptr, m.scale.ptr,
                                                                 m.act[0].ptr);
     OC_HIP_CHECK(hipGetLastError());
     OC_CHECK(m.forward(nq));
-    warm_apply_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, m.out.ptr, m.n_out, load_increment,
+    warm_apply_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, m.out.ptr, m.n_out, M, s.q_off.ptr, m.dload.ptr,
                                                              s.free_mask.ptr, s.dofs.ptr);
     OC_HIP_CHECK(hipGetLastError());
     if (m.n_out == 4) {
@@ -596,8 +629,9 @@ int oc_mlp_warm_start(oc_mlp mlp, oc_equations_set h, double load_increment)
                 s.n_elements, s.conn.ptr, s.conn_l.ptr, s.lin_to_quad.ptr);
             OC_HIP_CHECK(hipGetLastError());
         }
-        warm_apply_pressure_kernel<<<blocks_for(nl), 256, 0, rt.stream>>>(
-            nq, nl, s.lin_to_quad.ptr, m.out.ptr, load_increment, s.node_mat.ptr, s.free_mask.ptr, s.dofs.ptr);
+        warm_apply_pressure_kernel<<<blocks_for(nl), 256, 0, rt.stream>>>(nq, nl, s.lin_to_quad.ptr, m.out.ptr, M,
+                                                                          s.q_off.ptr, m.dload.ptr, s.node_mat.ptr,
+                                                                          s.free_mask.ptr, s.dofs.ptr);
         OC_HIP_CHECK(hipGetLastError());
     }
     return OC_NO_ERROR;

@@ -1,6 +1,6 @@
 // oc_setup.cpp -- host-side setup for liboc_asm: the CSR structure of the
-// equations matrices and the element colouring that makes the device scatter
-// deterministic.  Runs once per equations set (not on the per-assembly path).
+// equations matrices.  Runs once per equations set (not on the per-assembly
+// path).
 #include <algorithm>
 #include <cstring>
 #include <thread>
@@ -98,37 +98,6 @@ int build_pattern(int ncomp, int n_elements, const int32_t *nn, const int32_t *n
     row_ptr.assign(counts.begin(), counts.end());
     col_idx.resize(counts[ndof]);
     run(true);
-    return OC_NO_ERROR;
-}
-
-// Greedy element colouring: two elements sharing any node of any component
-// get different colours, so one colour's scatter has no write conflicts and
-// the colour order fixes the summation order of every CSR entry.  Elements
-// keep ascending order inside a colour (spatial locality of the scatter).
-int colour_elements(int n_elements, int ncomp, const int32_t *nn, const int32_t *nen, const int32_t *const *conn,
-                    std::vector<int32_t> &colour_ptr, std::vector<int32_t> &colour_elems)
-{
-    std::vector<std::vector<uint64_t>> used(ncomp);
-    for (int c = 0; c < ncomp; ++c) used[c].assign(nn[c], 0);
-    std::vector<int32_t> colour(n_elements);
-    int ncol = 0;
-    for (int64_t e = 0; e < n_elements; ++e) {
-        uint64_t forbidden = 0;
-        for (int c = 0; c < ncomp; ++c)
-            for (int k = 0; k < nen[c]; ++k) forbidden |= used[c][conn[c][e * nen[c] + k]];
-        if (~forbidden == 0) return set_error(OC_ERR_UNSUPPORTED, "element colouring needs more than 64 colours");
-        const int col = __builtin_ctzll(~forbidden);
-        colour[e] = col;
-        ncol = std::max(ncol, col + 1);
-        for (int c = 0; c < ncomp; ++c)
-            for (int k = 0; k < nen[c]; ++k) used[c][conn[c][e * nen[c] + k]] |= (1ull << col);
-    }
-    colour_ptr.assign(ncol + 1, 0);
-    for (int64_t e = 0; e < n_elements; ++e) colour_ptr[colour[e] + 1]++;
-    for (int c = 0; c < ncol; ++c) colour_ptr[c + 1] += colour_ptr[c];
-    colour_elems.resize(n_elements);
-    std::vector<int32_t> fill(colour_ptr.begin(), colour_ptr.end() - 1);
-    for (int64_t e = 0; e < n_elements; ++e) colour_elems[fill[colour[e]]++] = static_cast<int32_t>(e);
     return OC_NO_ERROR;
 }
 

@@ -1,0 +1,47 @@
+"""One load step of the 32^3 Mooney-Rivlin case (20% stretch in 10 steps),
+plain FP64 Newton or learned warm start + FP32 Jacobian, for kernel traces.
+
+Usage: python -m cm_amd.newton_profile [--learned] [--n 32] [--steps 1]
+"""
+import argparse
+import json
+import time
+
+from . import _lib
+from .equations_set import initialise, synchronise
+from .train_warm_start import DEFAULT_WEIGHTS, make_problem, solver_options
+from .warm_start import WarmStartMLP
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--learned", action="store_true")
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=1)
+    args = ap.parse_args()
+    initialise(0)
+    case = dict(n=args.n, c10=2.0, c01=6.0, stretch=0.20, seed=2024)
+    mesh, eq = make_problem(case)
+    net = WarmStartMLP.load(DEFAULT_WEIGHTS) if args.learned else None
+    dload = case["stretch"] / 10
+    for s in range(1, args.steps + 1):
+        d0 = eq.get_dofs()
+        eq.load_increment_apply(s, 10)
+        ref = eq.residual_norm()
+        t0 = time.perf_counter()
+        prec = _lib.OC_FP64
+        if net is not None:
+            eq.set_dofs(d0)
+            net.warm_start(eq, dload)
+            eq.load_increment_apply(s, 10)
+            prec = _lib.OC_FP32
+        st = eq.newton_solve(solver_options(prec, reference_norm=ref))
+        synchronise()
+        print(json.dumps({"step": s, "learned": net is not None, "seconds": time.perf_counter() - t0,
+                          "newton": st["newton_iterations"], "krylov": st["krylov_iterations"],
+                          "final_over_ref": st["final_residual_norm"] / ref}), flush=True)
+    eq.destroy()
+
+
+if __name__ == "__main__":
+    main()

@@ -39,8 +39,8 @@ def draw_case(rng, n_min=4, n_max=16):
 def solver_options(precision=_lib.OC_FP64, restart=100, reference_norm=0.0):
     return newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, reference_norm=reference_norm,
                           krylov_restart=restart, krylov_relative_tolerance=-1.0, jacobian_precision=precision,
-                          preconditioner=_lib.OC_PRECONDITIONER_MULTIGRID, maximum_iterations=30,
-                          krylov_maximum_iterations=50000)
+                          preconditioner=_lib.OC_PRECONDITIONER_MULTIGRID, maximum_iterations=100,
+                          krylov_maximum_iterations=100000)
 
 
 def make_problem(case, jitter=0.1):
@@ -82,6 +82,57 @@ def generate(n_solves, steps, seed, n_min, n_max, log=print):
     return np.concatenate(X), np.concatenate(T)
 
 
+def generate_batched(n_solves, steps, seed, n_min, n_max, net=None, batch_size=256, samples_per_step=2000,
+                     log=print):
+    """Training data from batches of meshes solved together (one equations
+    set per batch): each load step starts from the network's guess when a
+    network is given (the answers do not depend on the start: every accepted
+    step meets 1e-10 x the plain start's residual, FP64) and falls back to the
+    plain start with FP64 Jacobians when that solve does not converge.  Up to
+    `samples_per_step` random nodes per mesh and step are kept."""
+    from .batch_workload import build
+    rng = np.random.default_rng(seed)
+    pick = np.random.default_rng(seed + 1)
+    cases = [draw_case(rng, n_min, n_max) for _ in range(n_solves)]
+    X, T = [], []
+    t0 = time.time()
+    fallbacks = 0
+    for b0 in range(0, n_solves, batch_size):
+        cs = cases[b0:b0 + batch_size]
+        meshes, batch, _ = build(cs)
+        batch.set_multigrid()
+        dload = np.array([c["stretch"] / steps for c in cs])
+        for s in range(1, steps + 1):
+            d0 = batch.get_dofs()
+            batch.load_increment_apply(s, steps)
+            refs = batch.mesh_residual_norms()
+            ok = False
+            if net is not None:
+                batch.set_dofs(d0)
+                net.warm_start(batch, dload)
+                batch.load_increment_apply(s, steps)
+                st, _ = batch.newton_solve(solver_options(_lib.OC_FP32), reference_norms=refs, raise_on_failure=False)
+                ok = bool(st["converged"])
+            if not ok:
+                if net is not None:
+                    fallbacks += 1
+                    batch.set_dofs(d0)
+                    batch.load_increment_apply(s, steps)
+                batch.newton_solve(solver_options(_lib.OC_FP64), reference_norms=refs)
+            d1 = batch.get_dofs()
+            for m, (mesh, c) in enumerate(zip(meshes, cs)):
+                idx = batch.mesh_dofs(m)
+                x = step_features(mesh, d0[idx], dload[m], c["c10"], c["c01"])
+                t = step_targets(mesh, d0[idx], d1[idx], dload[m], c["c10"], c["c01"])
+                k = pick.choice(len(x), size=min(len(x), samples_per_step), replace=False)
+                X.append(x[k])
+                T.append(t[k])
+        batch.destroy()
+        log(f"  data: {min(b0 + batch_size, n_solves)}/{n_solves} solves, {sum(len(a) for a in X)} samples, "
+            f"{fallbacks} plain fallbacks, {time.time() - t0:.1f} s")
+    return np.concatenate(X), np.concatenate(T)
+
+
 def train(net, X, T, steps, batch=4096, lr=1e-3, seed=1, log=print):
     net.set_dataset(X, T)
     loss = None
@@ -95,7 +146,7 @@ def train(net, X, T, steps, batch=4096, lr=1e-3, seed=1, log=print):
     return loss
 
 
-def solve_steps(case, steps, net=None, precision=_lib.OC_FP64, max_steps=None, restart=100):
+def solve_steps(case, steps, net=None, precision=_lib.OC_FP64, max_steps=None, restart=100, log=None):
     """Solves the load steps; with `net`, each step starts from the MLP guess.
     Returns per-step dicts (iterations, seconds, residual norms)."""
     mesh, eq = make_problem(case)
@@ -117,6 +168,8 @@ def solve_steps(case, steps, net=None, precision=_lib.OC_FP64, max_steps=None, r
         out.append(dict(step=s, converged=bool(st["converged"]), newton=st["newton_iterations"],
                         krylov=st["krylov_iterations"], seconds=wall, start_norm=st["initial_residual_norm"],
                         reference_norm=ref, final_norm=st["final_residual_norm"]))
+        if log is not None:
+            log(f"    {'learned' if net is not None else 'plain'} step {s}: {out[-1]}")
         if not st["converged"]:
             break
     eq.destroy()
@@ -145,6 +198,10 @@ def main():
     ap.add_argument("--big-steps", type=int, default=10)
     ap.add_argument("--report", default=None)
     ap.add_argument("--eval-only", action="store_true", help="evaluate the weights at --out without training")
+    ap.add_argument("--batched", action="store_true",
+                    help="generate the data with batched solves (256 meshes per equations set)")
+    ap.add_argument("--bootstrap", default=None,
+                    help="weights used as the warm start while generating batched data")
     args = ap.parse_args()
     initialise(0)
     log = lambda m: print(m, flush=True)
@@ -153,7 +210,11 @@ def main():
         report = dict(training=None, heldout=[])
     else:
         t0 = time.time()
-        X, T = generate(args.solves, args.steps, args.seed, args.n_min, args.n_max, log)
+        if args.batched:
+            boot = WarmStartMLP.load(args.bootstrap) if args.bootstrap else None
+            X, T = generate_batched(args.solves, args.steps, args.seed, args.n_min, args.n_max, net=boot, log=log)
+        else:
+            X, T = generate(args.solves, args.steps, args.seed, args.n_min, args.n_max, log)
         log(f"data: {len(X)} samples from {args.solves} solves in {time.time() - t0:.1f} s")
         net = WarmStartMLP(hidden=128, layers=4, seed=args.seed)
         t1 = time.time()
@@ -170,10 +231,13 @@ def main():
         learned = summarise(solve_steps(case, args.steps, net=net, precision=_lib.OC_FP32))
         report["heldout"].append(dict(case=case, plain_fp64=plain, learned_fp32=learned))
         log(f"heldout {case}: plain {plain} | learned {learned}")
+        if args.report:
+            with open(args.report, "w") as f:
+                json.dump(report, f, indent=1)
     if args.big:
         case = dict(n=args.big, c10=2.0, c01=6.0, stretch=0.20, seed=2024)
-        plain = solve_steps(case, args.big_steps)
-        learned = solve_steps(case, args.big_steps, net=net, precision=_lib.OC_FP32)
+        learned = solve_steps(case, args.big_steps, net=net, precision=_lib.OC_FP32, log=log)
+        plain = solve_steps(case, args.big_steps, log=log)
         report["big"] = dict(case=case, plain_fp64=summarise(plain), learned_fp32=summarise(learned),
                              plain_rows=plain, learned_rows=learned)
         log(f"big: plain {summarise(plain)} | learned {summarise(learned)}")

@@ -102,8 +102,13 @@ class WarmStartMLP:
         return float(loss.value)
 
     def warm_start(self, equations_set, load_increment):
-        """z += load_increment * MLP(features) on the free displacement DOFs."""
-        call("oc_mlp_warm_start", self._h, equations_set.handle, float(load_increment))
+        """z += load_increment * MLP(features) on the free DOFs; a batch takes
+        one load increment per mesh (scalar or array)."""
+        inc = np.asarray(load_increment, np.float64)
+        if inc.ndim == 0:
+            call("oc_mlp_warm_start", self._h, equations_set.handle, float(inc))
+        else:
+            call("oc_mlp_warm_start_batch", self._h, equations_set.handle, np.ascontiguousarray(inc))
 
     def save(self, path):
         np.savez(path, parameters=self.parameters(), hidden=self.hidden, layers=self.layers, outputs=self.outputs)

@@ -122,7 +122,7 @@ int oc_equations_set_mesh_get(oc_equations_set equations_set, int mesh, int *nod
                               int *node_offset_l, int *n_nodes_l, int *element_offset, int *n_elements);
 int oc_equations_set_destroy(oc_equations_set equations_set);
 /* class (OC_EQUATIONS_SET_*), number of rows (= DOFs), number of nonzeros,
- * number of elements, number of element colours. */
+ * number of elements, number of element colours (0: the assembly needs none). */
 int oc_equations_set_info(oc_equations_set equations_set, int *set_class, int *n_rows, long long *nnz,
                           int *n_elements, int *n_colours);
 /* Materials (c10, c01) of a Mooney-Rivlin set (every mesh of a batch). */
@@ -288,6 +288,8 @@ int oc_mlp_train(oc_mlp mlp, int steps, int batch, float learning_rate, unsigned
  * the value of its coincident (element corner) quadratic node.  Call before
  * oc_load_increment_apply.  The pressure output is in units of c10 + c01. */
 int oc_mlp_warm_start(oc_mlp mlp, oc_equations_set equations_set, double load_increment);
+/* The same for a batch of meshes: load_increments[n_meshes], one per mesh. */
+int oc_mlp_warm_start_batch(oc_mlp mlp, oc_equations_set equations_set, const double *load_increments);
 
 /* ---- timing on the library stream ----------------------------------------- */
 /* Records event `slot` (0..63) on the library stream. */

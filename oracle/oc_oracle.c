@@ -792,6 +792,61 @@ int oco_mr_jacobian(int nelem, const int32_t *elem_q, const int32_t *elem_l, int
     return rc;
 }
 
+/* Multi-threaded CPU baseline (not a restatement of one reference routine):
+ * residual + analytic Jacobian with the elements split into greedy colours
+ * (no two elements of a colour share a node), each colour's elements in
+ * parallel over OpenMP threads, every element added with oco_csr_add as in
+ * the sequential loop.  Summation order differs from the reference's element
+ * order, so values agree with oco_mr_jacobian to rounding only. */
+int oco_mr_assemble_parallel(int nelem, const int32_t *elem_q, const int32_t *elem_l, int nq, int nl,
+                             const double *Xq, const double *dofs, double c10, double c01,
+                             const int64_t *row_ptr, const int32_t *col_idx, double *values, double *res,
+                             int threads)
+{
+    const int64_t ndof = (int64_t)3 * nq + nl;
+    uint64_t *used = (uint64_t *)calloc((size_t)nq, sizeof(uint64_t));
+    int32_t *colour = (int32_t *)malloc(sizeof(int32_t) * (size_t)nelem);
+    if (!used || !colour) {
+        free(used);
+        free(colour);
+        return -2;
+    }
+    int ncol = 0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        uint64_t forbidden = 0;
+        for (int k = 0; k < 27; ++k) forbidden |= used[elem_q[e * 27 + k]];
+        if (~forbidden == 0) {
+            free(used);
+            free(colour);
+            return -3;
+        }
+        const int c = __builtin_ctzll(~forbidden);
+        colour[e] = c;
+        if (c + 1 > ncol) ncol = c + 1;
+        for (int k = 0; k < 27; ++k) used[elem_q[e * 27 + k]] |= (1ull << c);
+    }
+    free(used);
+    memset(values, 0, sizeof(double) * (size_t)row_ptr[ndof]);
+    memset(res, 0, sizeof(double) * (size_t)ndof);
+    int rc = 0;
+    for (int c = 0; c < ncol; ++c) {
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 8) reduction(| : rc)
+        for (int64_t e = 0; e < nelem; ++e) {
+            if (colour[e] != c) continue;
+            double Xe[81], ze[81], pe[8], Re[89];
+            double Je[89 * 89];
+            int32_t ed[89];
+            mr_gather(e, elem_q, elem_l, nq, Xq, dofs, Xe, ze, pe, ed);
+            oco_mr_element_residual(Xe, ze, pe, c10, c01, Re);
+            for (int i = 0; i < 89; ++i) res[ed[i]] += Re[i];
+            oco_mr_element_jacobian(Xe, ze, pe, c10, c01, Je);
+            rc |= oco_csr_add(row_ptr, col_idx, values, 89, ed, 89, ed, Je) != 0;
+        }
+    }
+    free(colour);
+    return rc ? -1 : 0;
+}
+
 /* EQUATIONS_SET_BOUNDARY_CONDITIONS_INCREMENT, equations_set_routines.f90:
  * 6560-6585: full value at the last step, else cur + (full-cur)/(N-i+1). */
 int oco_bc_increment(int ndof, const int32_t *dof_idx, const double *full, double *current,

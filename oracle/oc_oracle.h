@@ -89,6 +89,13 @@ int oco_mr_jacobian(int nelem, const int32_t *elem_q, const int32_t *elem_l, int
                     const double *Xq, const double *dofs, double c10, double c01, int mode,
                     const int64_t *row_ptr, const int32_t *col_idx, double *values);
 
+/* Multi-threaded baseline: residual + analytic Jacobian, greedy element
+ * colours in sequence, each colour's elements over `threads` OpenMP threads. */
+int oco_mr_assemble_parallel(int nelem, const int32_t *elem_q, const int32_t *elem_l, int nq, int nl,
+                             const double *Xq, const double *dofs, double c10, double c01,
+                             const int64_t *row_ptr, const int32_t *col_idx, double *values, double *res,
+                             int threads);
+
 /* --- boundary-condition increment (equations_set_routines.f90:6491-6610) -- */
 int oco_bc_increment(int ndof, const int32_t *dof_idx, const double *full, double *current,
                      int iteration, int max_iterations);

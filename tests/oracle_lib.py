@@ -40,6 +40,8 @@ def load():
         "oco_mr_residual": (c_int, [c_int, _i32, _i32, c_int, c_int, _dp, _dp, c_double, c_double, _dp]),
         "oco_mr_jacobian": (c_int, [c_int, _i32, _i32, c_int, c_int, _dp, _dp, c_double, c_double, c_int,
                                     _i64, _i32, _dp]),
+        "oco_mr_assemble_parallel": (c_int, [c_int, _i32, _i32, c_int, c_int, _dp, _dp, c_double, c_double,
+                                             _i64, _i32, _dp, _dp, c_int]),
         "oco_bc_increment": (c_int, [c_int, _i32, _dp, _dp, c_int, c_int]),
         "oco_uniform": (c_double, [c_u64, c_u64]),
     }
@@ -195,6 +197,20 @@ def mr_jacobian(elem_q, elem_l, Xq, dofs, c10, c01, row_ptr, col_idx, mode=0):
                                  np.ascontiguousarray(Xq).ravel(), np.ascontiguousarray(dofs), c10, c01, mode,
                                  row_ptr, col_idx, vals), "mr_jacobian")
     return vals
+
+
+def mr_assemble_parallel(elem_q, elem_l, Xq, dofs, c10, c01, row_ptr, col_idx, threads):
+    """Residual + analytic Jacobian over element colours on `threads` threads."""
+    E = elem_q.shape[0]
+    nq = Xq.shape[1]
+    nl = dofs.shape[0] - 3 * nq
+    vals = np.zeros(col_idx.shape[0])
+    res = np.zeros(dofs.shape[0])
+    _check(lib().oco_mr_assemble_parallel(E, np.ascontiguousarray(elem_q, np.int32).ravel(),
+                                          np.ascontiguousarray(elem_l, np.int32).ravel(), nq, nl,
+                                          np.ascontiguousarray(Xq).ravel(), np.ascontiguousarray(dofs), c10, c01,
+                                          row_ptr, col_idx, vals, res, int(threads)), "mr_assemble_parallel")
+    return res, vals
 
 
 def bc_increment(dof_idx, full, current, iteration, max_iterations):

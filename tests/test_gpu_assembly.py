@@ -108,7 +108,7 @@ def test_mooney_rivlin_16_config(gpu):
     oJ = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, rp.astype(np.int64), ci)
     J = eq.matrix_values()
     assert rel(J, oJ) < FP64_TOL
-    # repeated assembly is deterministic (coloured scatter)
+    # repeated assembly is deterministic (ordered row gather)
     eq.jacobian_evaluate(cm.OC_FP64)
     assert np.array_equal(eq.matrix_values(), J)
 

@@ -132,7 +132,7 @@ def test_batched_newton_meets_every_mesh_tolerance(gpu):
 def test_warm_start_on_a_batch_equals_per_mesh(gpu):
     cm = gpu
     from cm_amd.warm_start import WarmStartMLP
-    net = WarmStartMLP(hidden=32, layers=2, seed=9)
+    net = WarmStartMLP(hidden=32, layers=2, seed=9)  # 4 outputs: displacement and pressure
     meshes = make_meshes(cm, CASES)
     batch = cm.MooneyRivlinBatch(meshes, [c["c10"] for c in CASES], [c["c01"] for c in CASES])
     d = np.zeros(batch.n_rows)
@@ -142,11 +142,11 @@ def test_warm_start_on_a_batch_equals_per_mesh(gpu):
         d[batch.mesh_dofs(m)] = dm
         eq = cm.MooneyRivlinEquationsSet(mesh, c["c10"], c["c01"])
         eq.set_dofs(dm)
-        net.warm_start(eq, 0.01)
+        net.warm_start(eq, 0.01 * (m + 1))
         expect.append(eq.get_dofs())
         eq.destroy()
     batch.set_dofs(d)
-    net.warm_start(batch, 0.01)
+    net.warm_start(batch, 0.01 * np.arange(1, len(CASES) + 1))
     out = batch.get_dofs()
     for m in range(len(CASES)):
         np.testing.assert_array_equal(out[batch.mesh_dofs(m)], expect[m])

@@ -355,3 +355,18 @@ def test_regular_coordinates_product_formula():
     assert np.array_equal(xyz[0], (p % T) * (1.0 / (T - 1)))
     conn = regular_elements(1, 2, 2, 2)
     assert list(conn[0]) == [0, 1, 3, 4, 9, 10, 12, 13]
+
+
+def test_parallel_baseline_matches_sequential_oracle():
+    """The multi-threaded CPU baseline (element colours over threads) agrees
+    with the sequential reference-order loop to rounding."""
+    mesh = RegularMesh.generate(2, 3, jitter=0.1, seed=8)
+    rng = np.random.default_rng(3)
+    d = np.concatenate([(mesh.coords * 1.03 + 0.01 * rng.uniform(-1, 1, mesh.coords.shape)).ravel(),
+                        rng.standard_normal(mesh.n_nodes_l)])
+    rp, ci = orc.pattern([(mesh.n_nodes, mesh.elem_nodes)] * 3 + [(mesh.n_nodes_l, mesh.elem_nodes_l)])
+    r_seq = orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0)
+    v_seq = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, rp, ci)
+    r_par, v_par = orc.mr_assemble_parallel(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, rp, ci, 4)
+    assert np.linalg.norm(r_par - r_seq) <= 1e-13 * np.linalg.norm(r_seq)
+    assert np.linalg.norm(v_par - v_seq) <= 1e-13 * np.linalg.norm(v_seq)
