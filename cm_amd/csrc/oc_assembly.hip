@@ -165,9 +165,11 @@ laplace_element_kernel(int n_elements, const int *__restrict__ conn, const doubl
 {
     __shared__ double s_x[kLapElemsPerBlock][3][8];
     __shared__ double s_gu[kLapElemsPerBlock][8][10];
+    __shared__ double s_f[8 * 8 * 4];  // trilinear basis at the 2x2x2 Gauss points
     const int local = threadIdx.x >> 3, lane = threadIdx.x & 7;
     const int e = blockIdx.x * kLapElemsPerBlock + local;
     const bool active = e < n_elements;
+    s_f[threadIdx.x] = T->lin8[threadIdx.x];  // 256 entries, one per thread
     if (active) {
         const int node = conn[static_cast<int64_t>(e) * 8 + lane];
         for (int c = 0; c < 3; ++c) s_x[local][c][lane] = coords[static_cast<int64_t>(c) * n_nodes + node];
@@ -175,7 +177,7 @@ laplace_element_kernel(int n_elements, const int *__restrict__ conn, const doubl
     __syncthreads();
     if (active) {
         const int g = lane;
-        const double *f = T->lin8 + g * 8 * 4;
+        const double *f = s_f + g * 8 * 4;
         double dx[9], gu[9];
         for (int x = 0; x < 3; ++x)
             for (int ni = 0; ni < 3; ++ni) {
@@ -193,9 +195,9 @@ laplace_element_kernel(int n_elements, const int *__restrict__ conn, const doubl
     double k[8];
 #pragma unroll
     for (int ns = 0; ns < 8; ++ns) k[ns] = 0.0;
-#pragma unroll 1
+#pragma unroll
     for (int g = 0; g < 8; ++g) {
-        const double *f = T->lin8 + g * 8 * 4;
+        const double *f = s_f + g * 8 * 4;
         const double *gu = s_gu[local][g];
         double v[3];
 #pragma unroll

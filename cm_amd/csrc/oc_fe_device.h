@@ -268,4 +268,25 @@ __device__ __forceinline__ int64_t csr_slot(const int64_t *__restrict__ row_ptr,
     return -1;
 }
 
+// Partial dot product of CSR entries [k0, k1) with x for lane `lane` of a row
+// handled by kLanes lanes: four entries in flight per lane, fixed order.
+template <typename VT, int kLanes>
+__device__ __forceinline__ double csr_row_dot(int64_t k0, int64_t k1, int lane, const int *__restrict__ col,
+                                              const VT *__restrict__ val, const double *__restrict__ x)
+{
+    double a0 = 0.0, a1 = 0.0;
+    int64_t k = k0 + lane;
+    for (; k + 3 * kLanes < k1; k += 4 * kLanes) {
+        const int c0 = col[k], c1 = col[k + kLanes], c2 = col[k + 2 * kLanes], c3 = col[k + 3 * kLanes];
+        const double v0 = static_cast<double>(val[k]), v1 = static_cast<double>(val[k + kLanes]);
+        const double v2 = static_cast<double>(val[k + 2 * kLanes]), v3 = static_cast<double>(val[k + 3 * kLanes]);
+        a0 += v0 * x[c0];
+        a1 += v1 * x[c1];
+        a0 += v2 * x[c2];
+        a1 += v3 * x[c3];
+    }
+    for (; k < k1; k += kLanes) a0 += static_cast<double>(val[k]) * x[col[k]];
+    return a0 + a1;
+}
+
 }  // namespace oc

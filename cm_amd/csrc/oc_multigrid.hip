@@ -25,6 +25,7 @@
 #include <cstdint>
 #include <memory>
 
+#include "oc_fe_device.h"
 #include "oc_internal.h"
 
 namespace oc {
@@ -319,9 +320,7 @@ uu_residual_kernel(int nu, const int64_t *__restrict__ row_ptr, const int64_t *_
     if (row >= nu) return;
     const bool f = free_mask[row] != 0;
     double acc = 0.0;
-    if (f)
-        for (int64_t k = row_ptr[row] + lane; k < pstart[row]; k += kLanes)
-            acc += static_cast<double>(vals[k]) * x[col_idx[k]];
+    if (f) acc = csr_row_dot<VT, kLanes>(row_ptr[row], pstart[row], lane, col_idx, vals, x);
 #pragma unroll
     for (int o = kLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kLanes);
     if (lane == 0) r[row] = f ? b[row] - acc : 0.0;

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <vector>
 
+#include "oc_fe_device.h"
 #include "oc_internal.h"
 
 namespace oc {
@@ -147,9 +148,7 @@ __global__ void __launch_bounds__(256) spmv_kernel(int n, const int64_t *__restr
     if (row >= n) return;
     double acc = 0.0;
     const double wr = w == nullptr ? 1.0 : w[row];
-    if (wr != 0.0)
-        for (int64_t k = row_ptr[row] + lane; k < row_ptr[row + 1]; k += kLanes)
-            acc += static_cast<double>(vals[k]) * x[col_idx[k]];
+    if (wr != 0.0) acc = csr_row_dot<VT, kLanes>(row_ptr[row], row_ptr[row + 1], lane, col_idx, vals, x);
 #pragma unroll
     for (int o = kLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kLanes);
     if (lane == 0) y[row] = wr * acc;
@@ -287,7 +286,8 @@ __global__ void __launch_bounds__(256) triangular_u_kernel(int nq, const int64_t
     z[2 * nq + node] = o[6] * t[0] + o[7] * t[1] + o[8] * t[2];
 }
 
-// t = r_u - B^T z_p on the free displacement rows (pressure columns start at pstart[row])
+// t = r_u - B^T z_p on the free displacement rows (pressure columns start at
+// pstart[row]); 8 lanes per row, shuffle reduction in a fixed order
 template <typename VT>
 __global__ void __launch_bounds__(256) schur_rhs_kernel(int nu, const int64_t *__restrict__ row_ptr,
                                                        const int64_t *__restrict__ pstart,
@@ -296,14 +296,18 @@ __global__ void __launch_bounds__(256) schur_rhs_kernel(int nu, const int64_t *_
                                                        const double *__restrict__ r, const double *__restrict__ z,
                                                        double *__restrict__ t)
 {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int kLanes = 8;
+    const int lane = threadIdx.x % kLanes;
+    const int row = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes);
     if (row >= nu) return;
+    const bool f = free_mask[row] != 0;
     double acc = 0.0;
-    if (free_mask[row]) {
-        acc = r[row];
-        for (int64_t k = pstart[row]; k < row_ptr[row + 1]; ++k) acc -= static_cast<double>(vals[k]) * z[col_idx[k]];
-    }
-    t[row] = acc;
+    if (f)
+        for (int64_t k = pstart[row] + lane; k < row_ptr[row + 1]; k += kLanes)
+            acc += static_cast<double>(vals[k]) * z[col_idx[k]];
+#pragma unroll
+    for (int o = kLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kLanes);
+    if (lane == 0) t[row] = f ? r[row] - acc : 0.0;
 }
 
 __global__ void __launch_bounds__(256) pressure_start_kernel(int n_u_rows, int first_p_col,
@@ -461,12 +465,13 @@ struct Krylov {
                                                                          zout + 3 * static_cast<int64_t>(nq));
                 const int nu = 3 * nq;
                 double *t = s.mg->t0.ptr;
+                const unsigned gs = grid_rows(static_cast<int64_t>(nu) * 8);
                 if (fp32)
-                    schur_rhs_kernel<float><<<grid_rows(nu), 256, 0, rt.stream>>>(
-                        nu, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values_sp.ptr, s.free_mask.ptr, rin, zout, t);
+                    schur_rhs_kernel<float><<<gs, 256, 0, rt.stream>>>(nu, s.row_ptr.ptr, pstart, s.col_idx.ptr,
+                                                                        s.values_sp.ptr, s.free_mask.ptr, rin, zout, t);
                 else
-                    schur_rhs_kernel<double><<<grid_rows(nu), 256, 0, rt.stream>>>(
-                        nu, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values.ptr, s.free_mask.ptr, rin, zout, t);
+                    schur_rhs_kernel<double><<<gs, 256, 0, rt.stream>>>(nu, s.row_ptr.ptr, pstart, s.col_idx.ptr,
+                                                                         s.values.ptr, s.free_mask.ptr, rin, zout, t);
                 OC_HIP_CHECK(hipGetLastError());
                 OC_CHECK(mg_apply(s, fp32 ? OC_FP32 : OC_FP64, t, zout));
             } else if (precond == OC_PRECONDITIONER_BLOCK_TRIANGULAR) {
