@@ -281,7 +281,9 @@ def main():
         j32, l32 = cm.kernel_timing("mr_element_matrix_fp32")
         g32, _ = cm.kernel_timing("csr_row_gather_fp32")
         cm.kernel_timing_enable(False)
+        v64, v32 = eq.matrix_values(), eq.matrix_values_sp().astype(np.float64)
         secondary["mr_jacobian_fp32"] = {
+            "error_vs_fp64_rel_frobenius": float(np.linalg.norm(v32 - v64) / np.linalg.norm(v64)),
             "elements_per_s": E * args.steps / (ms32 / 1e3),
             "achieved_TFLOPs": MR_JACOBIAN_FLOP_PER_ELEMENT * E * args.steps / (j32 / 1e3) / 1e12,
             "peak_TFLOPs": FP32_MFMA_PEAK_TFLOPS,

@@ -130,15 +130,12 @@ template int fill<float>(float *, int64_t, float);
 // Laplace, standard subtype (Laplace_equations_routines.f90:474-571):
 // K_e(ms,ns) += sum_{mi,ni} dphi_ms/dxi_mi GU(mi,ni) dphi_ns/dxi_ni * J w.
 // Two phases, as the Jacobian: laplace_element_kernel writes every element
-// matrix to a buffer (eight lanes per element: lane g evaluates the metric of
-// Gauss point g, then lane ms owns element row ms and writes it as 64
-// contiguous bytes); laplace_row_gather_kernel gives each CSR row one thread,
+// matrix to a buffer (one thread per element); laplace_row_gather_kernel
+// gives each CSR row one thread,
 // which adds the element rows touching it in ascending element order (the
 // reference's element-loop order) into an LDS copy of its block's rows, and
 // the block writes its contiguous CSR segment once: no zero fill, no
 // read-modify-write, no colours.
-constexpr int kLapElemsPerBlock = 32;
-
 __global__ void __launch_bounds__(256)
 laplace_slot_offsets_kernel(int n_elements, const int *__restrict__ conn, const int64_t *__restrict__ row_ptr,
                             const int *__restrict__ col_idx, uint8_t *__restrict__ offs, int *__restrict__ n_bad)
@@ -159,67 +156,91 @@ laplace_slot_offsets_kernel(int n_elements, const int *__restrict__ conn, const 
     }
 }
 
-__global__ void __launch_bounds__(256)
+// One thread per element: the 36 upper-triangle entries of K_e stay in
+// registers over the 8 Gauss points (the basis table is read with uniform
+// addresses); each row goes to its place in the row-gather order, so the
+// gather reads every CSR row's contributions contiguously.
+constexpr int kLapThreads = 64;
+
+__global__ void __launch_bounds__(kLapThreads)
 laplace_element_kernel(int n_elements, const int *__restrict__ conn, const double *__restrict__ coords, int n_nodes,
-                       const BasisTables *__restrict__ T, double *__restrict__ kbuf)
+                       const BasisTables *__restrict__ T, const int *__restrict__ gpos, double *__restrict__ kbuf)
 {
-    __shared__ double s_x[kLapElemsPerBlock][3][8];
-    __shared__ double s_gu[kLapElemsPerBlock][8][10];
-    __shared__ double s_f[8 * 8 * 4];  // trilinear basis at the 2x2x2 Gauss points
-    const int local = threadIdx.x >> 3, lane = threadIdx.x & 7;
-    const int e = blockIdx.x * kLapElemsPerBlock + local;
-    const bool active = e < n_elements;
-    s_f[threadIdx.x] = T->lin8[threadIdx.x];  // 256 entries, one per thread
-    if (active) {
-        const int node = conn[static_cast<int64_t>(e) * 8 + lane];
-        for (int c = 0; c < 3; ++c) s_x[local][c][lane] = coords[static_cast<int64_t>(c) * n_nodes + node];
-    }
-    __syncthreads();
-    if (active) {
-        const int g = lane;
-        const double *f = s_f + g * 8 * 4;
-        double dx[9], gu[9];
-        for (int x = 0; x < 3; ++x)
-            for (int ni = 0; ni < 3; ++ni) {
-                double s = 0.0;
-                for (int n = 0; n < 8; ++n) s += f[n * 4 + 1 + ni] * s_x[local][x][n];
-                dx[x + 3 * ni] = s;
+    const int e = blockIdx.x * kLapThreads + threadIdx.x;
+    double K[36];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) K[i] = 0.0;
+    if (e < n_elements) {
+        double x[3][8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const int node = conn[static_cast<int64_t>(e) * 8 + n];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[c][n] = coords[static_cast<int64_t>(c) * n_nodes + node];
+        }
+#pragma unroll 1
+        for (int g = 0; g < 8; ++g) {
+            const double *f = T->lin8 + g * 8 * 4;
+            double dx[9], gu[9];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int ni = 0; ni < 3; ++ni) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int n = 0; n < 8; ++n) sum += f[n * 4 + 1 + ni] * x[c][n];
+                    dx[c + 3 * ni] = sum;
+                }
+            const double rwg = metric_jacobian(dx, gu) * T->w8[g];
+            int idx = 0;
+#pragma unroll
+            for (int ms = 0; ms < 8; ++ms) {
+                double v[3];
+#pragma unroll
+                for (int ni = 0; ni < 3; ++ni)
+                    v[ni] = f[ms * 4 + 1] * gu[0 + 3 * ni] + f[ms * 4 + 2] * gu[1 + 3 * ni] + f[ms * 4 + 3] * gu[2 + 3 * ni];
+#pragma unroll
+                for (int ns = ms; ns < 8; ++ns, ++idx) {
+                    const double sum = v[0] * f[ns * 4 + 1] + v[1] * f[ns * 4 + 2] + v[2] * f[ns * 4 + 3];
+                    K[idx] = K[idx] + sum * rwg;
+                }
             }
-        const double jac = metric_jacobian(dx, gu);
-        for (int i = 0; i < 9; ++i) s_gu[local][g][i] = gu[i];
-        s_gu[local][g][9] = jac * T->w8[g];
-    }
-    __syncthreads();
-    if (!active) return;
-    const int ms = lane;
-    double k[8];
-#pragma unroll
-    for (int ns = 0; ns < 8; ++ns) k[ns] = 0.0;
-#pragma unroll
-    for (int g = 0; g < 8; ++g) {
-        const double *f = s_f + g * 8 * 4;
-        const double *gu = s_gu[local][g];
-        double v[3];
-#pragma unroll
-        for (int ni = 0; ni < 3; ++ni)
-            v[ni] = f[ms * 4 + 1] * gu[0 + 3 * ni] + f[ms * 4 + 2] * gu[1 + 3 * ni] + f[ms * 4 + 3] * gu[2 + 3 * ni];
-        const double rwg = gu[9];
-#pragma unroll
-        for (int ns = 0; ns < 8; ++ns) {
-            const double sum = v[0] * f[ns * 4 + 1] + v[1] * f[ns * 4 + 2] + v[2] * f[ns * 4 + 3];
-            k[ns] = k[ns] + sum * rwg;
         }
     }
-    double2 *out = reinterpret_cast<double2 *>(kbuf + (static_cast<int64_t>(e) * 8 + ms) * 8);
+    if (e >= n_elements) return;
+    // row ms of this element goes to its position in the row-gather order
+    double row[64];
+    int idx = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) out[q] = make_double2(k[2 * q], k[2 * q + 1]);
+    for (int ms = 0; ms < 8; ++ms)
+#pragma unroll
+        for (int ns = ms; ns < 8; ++ns, ++idx) {
+            row[ms * 8 + ns] = K[idx];
+            row[ns * 8 + ms] = K[idx];
+        }
+#pragma unroll
+    for (int ms = 0; ms < 8; ++ms) {
+        double2 *out = reinterpret_cast<double2 *>(kbuf + static_cast<int64_t>(gpos[static_cast<int64_t>(e) * 8 + ms]) * 8);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q] = make_double2(row[ms * 8 + 2 * q], row[ms * 8 + 2 * q + 1]);
+    }
+}
+
+// slot offsets permuted into the row-gather order
+__global__ void __launch_bounds__(256) permute_offsets_kernel(int64_t n, const int *__restrict__ idx,
+                                                             const uint64_t *__restrict__ offs,
+                                                             uint64_t *__restrict__ out)
+{
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        out[p] = offs[idx[p]];
 }
 
 template <int kRows>
 __global__ void __launch_bounds__(kRows)
 laplace_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const int *__restrict__ gptr,
-                          const int *__restrict__ gidx, const double *__restrict__ kbuf,
-                          const uint8_t *__restrict__ offs, double *__restrict__ values)
+                          const double *__restrict__ kbuf, const uint64_t *__restrict__ offs,
+                          double *__restrict__ values)
 {
     extern __shared__ unsigned char s_raw[];
     double *s_seg = reinterpret_cast<double *>(s_raw);
@@ -231,10 +252,9 @@ laplace_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const
     __syncthreads();
     if (r < n_rows) {
         double *row = s_seg + (row_ptr[r] - base);
-        for (int c = gptr[r]; c < gptr[r + 1]; ++c) {
-            const int64_t ei = gidx[c];  // element * 8 + element row
-            const uint64_t packed = *reinterpret_cast<const uint64_t *>(offs + ei * 8);
-            const double2 *kr = reinterpret_cast<const double2 *>(kbuf + ei * 8);
+        for (int64_t c = gptr[r]; c < gptr[r + 1]; ++c) {  // element rows in ascending element order
+            const uint64_t packed = offs[c];
+            const double2 *kr = reinterpret_cast<const double2 *>(kbuf + c * 8);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const double2 v = kr[q];
@@ -292,8 +312,20 @@ static int laplace_prepare(EquationsSet &s)
     for (int d = 0; d < s.n_rows; ++d) ptr[d + 1] += ptr[d];
     std::vector<int32_t> fillp(ptr.begin(), ptr.end() - 1);
     for (size_t k = 0; k < hconn.size(); ++k) idx[fillp[hconn[k]]++] = static_cast<int32_t>(k);
+    std::vector<int32_t> gpos(idx.size());
+    for (size_t p = 0; p < idx.size(); ++p) gpos[idx[p]] = static_cast<int32_t>(p);
     OC_CHECK(s.gather_ptr.upload(ptr.data(), ptr.size()));
-    OC_CHECK(s.gather_idx.upload(idx.data(), idx.size()));
+    OC_CHECK(s.gather_idx.upload(gpos.data(), gpos.size()));  // element row -> gather position
+    DevBuf<int32_t> didx;
+    OC_CHECK(didx.upload(idx.data(), idx.size()));
+    DevBuf<uint8_t> permuted;
+    OC_CHECK(permuted.alloc(s.slot_offsets8.n));
+    const int64_t n = static_cast<int64_t>(idx.size());
+    permute_offsets_kernel<<<static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 65535)), 256, 0, rt.stream>>>(
+        n, didx.ptr, reinterpret_cast<const uint64_t *>(s.slot_offsets8.ptr), reinterpret_cast<uint64_t *>(permuted.ptr));
+    OC_HIP_CHECK(hipGetLastError());
+    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+    s.slot_offsets8 = std::move(permuted);
     OC_CHECK(s.kbuf64.alloc(static_cast<size_t>(s.n_elements) * 64));
     return OC_NO_ERROR;
 }
@@ -307,21 +339,21 @@ int launch_laplace_assemble(EquationsSet &s)
     OC_CHECK(fill<double>(s.rhs.ptr, s.n_rows, 0.0));  // RHS element vector is 0 (:542)
     {
         KernelTimer timer("laplace_element");
-        const unsigned blocks = static_cast<unsigned>((s.n_elements + kLapElemsPerBlock - 1) / kLapElemsPerBlock);
-        laplace_element_kernel<<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.coords.ptr, s.n_nodes, T,
-                                                             s.kbuf64.ptr);
+        const unsigned blocks = static_cast<unsigned>((s.n_elements + kLapThreads - 1) / kLapThreads);
+        laplace_element_kernel<<<blocks, kLapThreads, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.coords.ptr,
+                                                                     s.n_nodes, T, s.gather_idx.ptr, s.kbuf64.ptr);
         OC_HIP_CHECK(hipGetLastError());
     }
     KernelTimer timer("laplace_row_gather");
     const size_t lds = static_cast<size_t>(s.max_row_len) * sizeof(double);
     if (s.lap_rows_per_block == 256)
         laplace_row_gather_kernel<256><<<static_cast<unsigned>((s.n_rows + 255) / 256), 256, lds, rt.stream>>>(
-            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.gather_idx.ptr, s.kbuf64.ptr, s.slot_offsets8.ptr,
-            s.values.ptr);
+            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.kbuf64.ptr,
+            reinterpret_cast<const uint64_t *>(s.slot_offsets8.ptr), s.values.ptr);
     else
         laplace_row_gather_kernel<64><<<static_cast<unsigned>((s.n_rows + 63) / 64), 64, lds, rt.stream>>>(
-            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.gather_idx.ptr, s.kbuf64.ptr, s.slot_offsets8.ptr,
-            s.values.ptr);
+            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.kbuf64.ptr,
+            reinterpret_cast<const uint64_t *>(s.slot_offsets8.ptr), s.values.ptr);
     OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
 }

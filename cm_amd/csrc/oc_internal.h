@@ -180,8 +180,10 @@ struct EquationsSet {
     // element residuals (E*89) and the DOF -> element-entry gather lists
     DevBuf<double> relem;
     DevBuf<int32_t> gather_ptr, gather_idx;
-    // Laplace: 8-bit offsets, slot = row_ptr[node_i] + slot_offsets8[(e*8+i)*8+j];
-    // element matrices in kbuf64 (E*64) and rows per row-gather block
+    // Laplace: 8-bit offsets (in the row-gather order: entry p*8+j belongs to
+    // the p-th element row of the gather lists); gather_idx holds the gather
+    // position of element row e*8+i; element matrices in kbuf64 (E*64, rows
+    // in gather order) and rows per row-gather block
     DevBuf<uint8_t> slot_offsets8;
     int lap_rows_per_block = 256;
 
