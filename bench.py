@@ -147,6 +147,27 @@ def batched_line(cm, n_meshes, reps=3):
         out[name] = {"elements_per_s": E * reps / (ms / 1e3), "ms_per_residual_and_jacobian": ms / reps,
                      "graph_jacobian_ms": gms / reps, "kernel_ms": ks,
                      "element_matrix_TFLOPs": tflops, "element_matrix_frac": tflops / peak}
+    # one load step of every mesh: learned warm start, FP32 Jacobians, FP64
+    # residual, multigrid, the Newton-step graph; 1e-10 x each plain start
+    from cm_amd.train_warm_start import DEFAULT_WEIGHTS, solver_options
+    from cm_amd.warm_start import WarmStartMLP
+    net = WarmStartMLP.load(DEFAULT_WEIGHTS)
+    batch.set_multigrid()
+    batch.set_dofs(batch.undeformed_dofs())
+    batch.load_increment_apply(1, 5)
+    refs = batch.mesh_residual_norms()
+    batch.set_dofs(batch.undeformed_dofs())
+    cm.synchronise()
+    t0 = time.perf_counter()
+    net.warm_start(batch, np.array([c["stretch"] / 5 for c in cases]))
+    batch.load_increment_apply(1, 5)
+    st, finals = batch.newton_solve(solver_options(cm.OC_FP32), reference_norms=refs, raise_on_failure=False)
+    cm.synchronise()
+    out["learned_load_step"] = {"seconds": time.perf_counter() - t0, "converged_meshes": int(np.sum(finals <= 1e-10 * refs)),
+                                "meshes": n_meshes, "newton_iterations": st["newton_iterations"],
+                                "krylov_iterations": st["krylov_iterations"],
+                                "max_final_over_reference": float(np.max(finals / refs))}
+    net.destroy()
     batch.destroy()
     return out
 
