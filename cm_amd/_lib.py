@@ -111,6 +111,7 @@ SIGNATURES = {
     "oc_equations_set_assemble": [_H],
     "oc_equations_set_residual_evaluate": [_H],
     "oc_equations_set_jacobian_evaluate": [_H, _c_int],
+    "oc_equations_set_jacobian_evaluate_fd": [_H],
     "oc_matrix_values_get": [_H, _dp],
     "oc_matrix_values_get_sp": [_H, _fp],
     "oc_residual_get": [_H, _dp],

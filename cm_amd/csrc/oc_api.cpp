@@ -471,6 +471,14 @@ int oc_equations_set_jacobian_evaluate(oc_equations_set h, int precision)
     return launch_mr_jacobian(s, precision);
 }
 
+int oc_equations_set_jacobian_evaluate_fd(oc_equations_set h)
+{
+    OC_CHECK(check_handle(h));
+    if (h->s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "Jacobian evaluation is defined for Mooney-Rivlin sets only");
+    return launch_mr_jacobian_fd(h->s);
+}
+
 int oc_matrix_values_get(oc_equations_set h, double *values)
 {
     OC_CHECK(check_handle(h));

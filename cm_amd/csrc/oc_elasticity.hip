@@ -12,7 +12,9 @@
 // elements into a buffer and dof_gather_kernel sums them per DOF in
 // ascending element order.
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
+#include <vector>
 
 #include "oc_fe_device.h"
 #include "oc_internal.h"
@@ -179,11 +181,18 @@ int launch_mr_gauss(EquationsSet &s)
 // over the Gauss points in order into a per-element buffer.  A gather kernel
 // then adds the element contributions of every DOF in ascending element order
 // (the reference's element-loop order, equations_set_routines.f90:5535-5568).
+//
+// With kFD, "element" v of the launch is the pair (e = v / 89, column col =
+// v % 89): element e's residual with its DOF col raised by delta, written as
+// column col of e's element matrix (R(x + delta e_col) - R(x)) / delta, R(x)
+// from relem (EquationsSet_FiniteElementJacobianEvaluateFD,
+// equations_set_routines.f90:3032-3150).
+template <bool kFD>
 __global__ void __launch_bounds__(256)
 mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const int *__restrict__ conn_l,
                            const double *__restrict__ coords, int nq, const double *__restrict__ dofs,
                            const double *__restrict__ mat, const BasisTables *__restrict__ T,
-                           double *__restrict__ relem)
+                           double *__restrict__ relem, double delta, double *__restrict__ kfd)
 {
     __shared__ double s_fq[27 * 27 * 4];
     __shared__ double s_X[kGaussElemsPerBlock][81];
@@ -195,25 +204,31 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
     const int t = threadIdx.x;
     const int e0 = blockIdx.x * kGaussElemsPerBlock;
     for (int k = t; k < 27 * 27 * 4; k += 256) s_fq[k] = T->quad27[k];
+    // n_elements counts launch elements (element-column pairs with kFD)
     for (int k = t; k < kGaussElemsPerBlock * 81; k += 256) {
         const int le = k / 81, r = k % 81, c = r / 27, n = r % 27;
-        const int e = e0 + le;
-        if (e < n_elements) {
+        const int v = e0 + le;
+        if (v < n_elements) {
+            const int e = kFD ? v / 89 : v;
             const int node = conn_q[e * 27 + n];
             s_X[le][r] = coords[static_cast<int64_t>(c) * nq + node];
-            s_z[le][r] = dofs[static_cast<int64_t>(c) * nq + node];
+            s_z[le][r] = dofs[static_cast<int64_t>(c) * nq + node] + ((kFD && v % 89 == r) ? delta : 0.0);
         }
     }
     for (int k = t; k < kGaussElemsPerBlock * 8; k += 256) {
         const int le = k / 8, n = k % 8;
-        const int e = e0 + le;
-        if (e < n_elements) s_p[le][n] = dofs[static_cast<int64_t>(3) * nq + conn_l[e * 8 + n]];
+        const int v = e0 + le;
+        if (v < n_elements) {
+            const int e = kFD ? v / 89 : v;
+            s_p[le][n] = dofs[static_cast<int64_t>(3) * nq + conn_l[e * 8 + n]] +
+                         ((kFD && v % 89 == 81 + n) ? delta : 0.0);
+        }
     }
     __syncthreads();
     {
         const int le = t / 27, g = t % 27;
-        const int e = e0 + le;
-        if (t < kGaussElemsPerBlock * 27 && e < n_elements) {
+        const int v = e0 + le, e = kFD ? v / 89 : v;
+        if (t < kGaussElemsPerBlock * 27 && v < n_elements) {
             GaussKinematics k;
             gauss_kinematics(s_fq + g * 27 * 4, s_z[le], s_X[le], k);
             double P = 0.0;
@@ -230,8 +245,8 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
     __syncthreads();
     for (int w = t; w < kGaussElemsPerBlock * 89; w += 256) {
         const int le = w / 89, r = w % 89;
-        const int e = e0 + le;
-        if (e >= n_elements) continue;
+        const int v = e0 + le;
+        if (v >= n_elements) continue;
         double acc = 0.0;
         if (r < 81) {
             const int a = r / 27, m = r % 27;
@@ -252,7 +267,12 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
 #pragma unroll 1
             for (int g = 0; g < 27; ++g) acc = acc + s_rp[le][g] * T->lin27[(g * 8 + m) * 4];
         }
-        relem[static_cast<int64_t>(e) * 89 + r] = acc;
+        if (kFD) {
+            const int64_t e = v / 89, col = v % 89;
+            kfd[(e * 89 + r) * 89 + col] = (acc - relem[e * 89 + r]) / delta;
+        } else {
+            relem[static_cast<int64_t>(v) * 89 + r] = acc;
+        }
     }
 }
 
@@ -277,9 +297,9 @@ int launch_mr_residual(EquationsSet &s)
     const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
     {
         KernelTimer timer("mr_residual");
-        mr_residual_element_kernel<<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
-                                                                  s.coords.ptr, s.n_nodes, s.dofs.ptr, s.mat.ptr, T,
-                                                                  s.relem.ptr);
+        mr_residual_element_kernel<false><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
+                                                                         s.coords.ptr, s.n_nodes, s.dofs.ptr,
+                                                                         s.mat.ptr, T, s.relem.ptr, 0.0, nullptr);
         OC_HIP_CHECK(hipGetLastError());
     }
     {
@@ -566,6 +586,39 @@ int launch_mr_row_gather(EquationsSet &s, int precision, int chunk)
     }
     KernelTimer timer("csr_row_gather_fp32");
     return launch_row_gather<float>(s, s.kbuf32.ptr, s.values_sp.ptr, chunk);
+}
+
+// Forward-difference Jacobian: element residuals at x, then one launch over
+// all (element, column) pairs, then the same ordered CSR row gather.
+int launch_mr_jacobian_fd(EquationsSet &s)
+{
+    OC_CHECK(mr_jacobian_buffers(s, OC_FP64));
+    const BasisTables *T = nullptr;
+    OC_CHECK(ensure_tables(&T));
+    Runtime &rt = runtime();
+    // delta = (1 + ||x||_2) 1e-7 over the whole dependent vector (:3086-3087)
+    std::vector<double> x(s.n_rows);
+    OC_HIP_CHECK(hipMemcpyAsync(x.data(), s.dofs.ptr, sizeof(double) * x.size(), hipMemcpyDeviceToHost, rt.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+    double ss = 0.0;
+    for (double v : x) ss += v * v;
+    const double delta = (1.0 + std::sqrt(ss)) * 1e-7;
+    const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
+    mr_residual_element_kernel<false><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
+                                                                     s.coords.ptr, s.n_nodes, s.dofs.ptr, s.mat.ptr,
+                                                                     T, s.relem.ptr, 0.0, nullptr);
+    OC_HIP_CHECK(hipGetLastError());
+    const int64_t nv = static_cast<int64_t>(s.n_elements) * 89;
+    if (nv >= INT32_MAX) return set_error(OC_ERR_UNSUPPORTED, "too many elements for the finite-difference launch");
+    {
+        KernelTimer timer("mr_fd_columns");
+        const unsigned fb = static_cast<unsigned>((nv + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
+        mr_residual_element_kernel<true><<<fb, 256, 0, rt.stream>>>(static_cast<int>(nv), s.conn.ptr, s.conn_l.ptr,
+                                                                    s.coords.ptr, s.n_nodes, s.dofs.ptr, s.mat.ptr, T,
+                                                                    s.relem.ptr, delta, s.kbuf64.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+    }
+    return launch_mr_row_gather(s, OC_FP64, s.tuned_gather_chunk);
 }
 
 int launch_mr_jacobian(EquationsSet &s, int precision)

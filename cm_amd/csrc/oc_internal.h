@@ -235,6 +235,7 @@ int launch_laplace_assemble(EquationsSet &s);
 int mr_prepare(EquationsSet &s);  // builds the scatter map once
 int launch_mr_residual(EquationsSet &s);
 int launch_mr_jacobian(EquationsSet &s, int precision);
+int launch_mr_jacobian_fd(EquationsSet &s);  // forward differences of the element residuals
 int mr_node_materials(EquationsSet &s);  // fills node_mat from mat
 int mr_jacobian_buffers(EquationsSet &s, int precision);  // element-matrix and CSR value buffers
 int launch_mr_element_matrices(EquationsSet &s, int precision, int blocks_per_cu);

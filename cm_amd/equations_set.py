@@ -232,6 +232,10 @@ class MooneyRivlinEquationsSet(EquationsSet):
         call("oc_multigrid_structure_set", self._h, np.array(ns, np.int32))
         call("oc_multigrid_options_set", self._h, int(pre_sweeps), int(post_sweeps), float(omega))
 
+    def jacobian_evaluate_fd(self):
+        """Forward-difference Jacobian (FP64 values), the reference's rule."""
+        call("oc_equations_set_jacobian_evaluate_fd", self._h)
+
     def jacobian_evaluate_graph(self, precision=_lib.OC_FP64):
         """Jacobian assembly replayed from the set's recorded HIP graph."""
         call("oc_jacobian_evaluate_graph", self._h, int(precision))

@@ -155,6 +155,12 @@ int oc_equations_set_residual_evaluate(oc_equations_set equations_set);
  * equations_set_routines.f90:4694, 4878-5024).  precision OC_FP64 fills
  * OC_BUFFER_MATRIX_VALUES, OC_FP32 fills OC_BUFFER_MATRIX_VALUES_SP. */
 int oc_equations_set_jacobian_evaluate(oc_equations_set equations_set, int precision);
+/* Forward-difference Jacobian into the FP64 values: column k of each element
+ * matrix is (R_e(x + delta e_k) - R_e(x)) / delta with delta = (1 + ||x||_2)
+ * 1e-7 over the whole dependent vector, assembled like the analytic one
+ * (EquationsSet_FiniteElementJacobianEvaluateFD, equations_set_routines.f90:
+ * 3032-3150; the default for subtypes without an analytic Jacobian). */
+int oc_equations_set_jacobian_evaluate_fd(oc_equations_set equations_set);
 /* The same assembly replayed from a HIP graph recorded on the first call
  * (kernel nodes only; re-recorded when the precision or launch shapes
  * change).  Results are identical to oc_equations_set_jacobian_evaluate. */

@@ -137,3 +137,25 @@ def test_fortran_host_assembles_laplace(gpu):
     assert line[1] == "27" and line[3] == "343" and line[5] == "1"
     assert abs(float(line[7]) - 0.5 / 3) < 1e-15
     assert float(line[9]) < 1e-14
+
+
+def test_finite_difference_jacobian_against_oracle(gpu):
+    """The forward-difference Jacobian (the reference's rule for subtypes
+    without an analytic one) against the oracle's forward differences, and
+    against the analytic Jacobian to the truncation error of the scheme."""
+    cm = gpu
+    mesh = cm.RegularMesh.generate(2, 2, jitter=0.1, seed=6)
+    eq = cm.MooneyRivlinEquationsSet(mesh, 2.0, 6.0)
+    rp, ci = eq.matrix_structure(0)
+    d = _deformed_state(mesh, 0.1, 4)
+    eq.set_dofs(d)
+    eq.jacobian_evaluate_fd()
+    Jfd = eq.matrix_values()
+    oJfd = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, rp.astype(np.int64), ci,
+                           mode=1)
+    # both are (R(x + delta e) - R(x)) / delta in FP64: rounding of the two
+    # residuals (~1e-16 relative) is amplified by 1/delta ~ 1e7
+    assert rel(Jfd, oJfd) < 1e-7
+    # one-sided differences with delta = (1 + ||x||) 1e-7 ~ 1e-6 here: O(delta) truncation
+    eq.jacobian_evaluate(cm.OC_FP64)
+    assert rel(Jfd, eq.matrix_values()) < 1e-4
