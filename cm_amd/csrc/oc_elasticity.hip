@@ -243,13 +243,16 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
         }
     }
     __syncthreads();
-    for (int w = t; w < kGaussElemsPerBlock * 89; w += 256) {
-        const int le = w / 89, r = w % 89;
+    // one work item per (element, displacement node m): the DPHIDZ of node m
+    // is formed once per Gauss point for its three component rows; then one
+    // item per pressure row
+    for (int w = t; w < kGaussElemsPerBlock * 35; w += 256) {
+        const int le = w / 35, q = w % 35;
         const int v = e0 + le;
         if (v >= n_elements) continue;
-        double acc = 0.0;
-        if (r < 81) {
-            const int a = r / 27, m = r % 27;
+        double acc[3] = {0.0, 0.0, 0.0};
+        if (q < 27) {
+            const int m = q;
 #pragma unroll 1
             for (int g = 0; g < 27; ++g) {
                 const double *f = s_fq + (g * 27 + m) * 4;
@@ -258,20 +261,27 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
                 for (int i = 0; i < 3; ++i)
                     d[i] = s_dx[le][0 + 3 * i][g] * f[1] + s_dx[le][1 + 3 * i][g] * f[2] +
                            s_dx[le][2 + 3 * i][g] * f[3];
-                const double dot = d[0] * s_S[le][0 + 3 * a][g] + d[1] * s_S[le][1 + 3 * a][g] +
-                                   d[2] * s_S[le][2 + 3 * a][g];
-                acc = acc + dot;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double dot = d[0] * s_S[le][0 + 3 * a][g] + d[1] * s_S[le][1 + 3 * a][g] +
+                                       d[2] * s_S[le][2 + 3 * a][g];
+                    acc[a] = acc[a] + dot;
+                }
             }
         } else {
-            const int m = r - 81;
+            const int m = q - 27;
 #pragma unroll 1
-            for (int g = 0; g < 27; ++g) acc = acc + s_rp[le][g] * T->lin27[(g * 8 + m) * 4];
+            for (int g = 0; g < 27; ++g) acc[0] = acc[0] + s_rp[le][g] * T->lin27[(g * 8 + m) * 4];
         }
-        if (kFD) {
-            const int64_t e = v / 89, col = v % 89;
-            kfd[(e * 89 + r) * 89 + col] = (acc - relem[e * 89 + r]) / delta;
-        } else {
-            relem[static_cast<int64_t>(v) * 89 + r] = acc;
+        const int nr = q < 27 ? 3 : 1;
+        for (int a = 0; a < nr; ++a) {
+            const int r = q < 27 ? a * 27 + q : 81 + (q - 27);
+            if (kFD) {
+                const int64_t e = v / 89, col = v % 89;
+                kfd[(e * 89 + r) * 89 + col] = (acc[a] - relem[e * 89 + r]) / delta;
+            } else {
+                relem[static_cast<int64_t>(v) * 89 + r] = acc[a];
+            }
         }
     }
 }
