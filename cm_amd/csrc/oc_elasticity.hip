@@ -331,10 +331,13 @@ int launch_mr_residual(EquationsSet &s)
 //   U_g^{ab}[m][j] = sum_i D_g[i][m] W_g[(a,i),(b,j)],
 //   W_g[(a,i),(b,j)] = J_z w_g (c(V(i,a),V(j,b)) + delta_ab sigma_ij),
 // i.e. the reference's TEMPVEC / DOT_PRODUCT contraction (:1001-1037)
-// re-associated as a 27x81 by 81x27 GEMM on MFMA (16x16x4, one wave per 16x16
-// output tile); pressure rows K_(p,m),(b,n) = sum_g (J_z w_g psi_m) D_g[b][n]
-// (:1039-1054) as an 8x27 by 27x81 GEMM.  Lower triangle kept, mirrored
-// (:1149-1153); the element matrix goes to an HBM buffer, row-major.
+// re-associated as a 27x81 by 81x27 GEMM on MFMA 16x16x4; pressure rows
+// K_(p,m),(b,n) = sum_g (J_z w_g psi_m) D_g[b][n] (:1039-1054) as an 8x27 by
+// 27x81 GEMM.  One block of four waves per element: D lives in registers, U is
+// formed in registers from D and W (LDS), and the waves split the 21 tiles of
+// the lower displacement blocks and the pressure tiles with no barrier between
+// them.  Lower triangle kept, mirrored (:1149-1153); the element matrix goes
+// to an HBM buffer, row-major.
 // Phase B (csr_row_gather_kernel<Real>): one wavefront per CSR row adds the
 // element rows that touch it, in ascending element order, into an LDS copy of
 // the row and writes the row once (no zero fill, no read-modify-write).
@@ -359,124 +362,173 @@ struct Mfma<double> {
     __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
-constexpr int kKPad = 84;   // (g, j) = 81 padded to 21 MFMA k-steps of 4
-constexpr int kMPad = 32;   // 27 rows/columns padded to two 16-wide tiles
-constexpr int kPPad = 28;   // 27 Gauss points padded to 7 k-steps
+// Per-lane operand slices of D_g[j][n] = sum_mi DXI_DX(mi, j) dphi_n/dxi_mi
+// (:974-986).  The contraction index is k = (j, g) with g padded to 28, so
+// MFMA k-step ks covers j = ks / 7 and g = 4 (ks % 7) + kk (kk = lane / 16).
+// Lane (kk, l) keeps D_g[j][16 t + l] for its seven Gauss points g = 4 g' + kk
+// and j = 0..2 in D[t][g'][j] (t = 0, 1: the two 16-wide tiles of the 27 basis
+// functions).  These registers are the B operand of every displacement block
+// and, contracted with W, form its A operand
+//   U^{ab}_g[m][j] = sum_i D_g[i][m] W_g[(a,i),(b,j)]
+// on the fly, so U never goes through LDS.
+template <typename Real>
+struct DSlice {
+    Real v[2][7][3];
+};
+
+// One displacement block (a, b), row tile MT, column tiles 0 (and 1 if TWO):
+// 21 k-steps of mfma 16x16x4, then the tiles go to the element buffer, the
+// (b, a) block as the mirror image.  A diagonal block keeps its lower triangle
+// (:1149-1153).
+template <typename Real, int MT, bool TWO>
+__device__ __forceinline__ void mr_uu_block(int a, int b, int lane, const DSlice<Real> &D, const Real (*s_W)[81],
+                                            Real *__restrict__ Ke)
+{
+    using M = Mfma<Real>;
+    const int kk = lane >> 4, l = lane & 15;
+    typename M::acc_t acc0 = {Real(0), Real(0), Real(0), Real(0)};
+    typename M::acc_t acc1 = {Real(0), Real(0), Real(0), Real(0)};
+    const int wofs = a * 27 + b * 3;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+        for (int gp = 0; gp < 7; ++gp) {
+            const Real *w = &s_W[4 * gp + kk][wofs + j];
+            const Real u = D.v[MT][gp][0] * w[0] + D.v[MT][gp][1] * w[9] + D.v[MT][gp][2] * w[18];
+            acc0 = M::step(u, D.v[0][gp][j], acc0);
+            if (TWO) acc1 = M::step(u, D.v[1][gp][j], acc1);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int m = MT * 16 + M::row(lane, r);
+        if (m >= 27) continue;
+#pragma unroll
+        for (int nt = 0; nt < (TWO ? 2 : 1); ++nt) {
+            const int n = nt * 16 + l;
+            if (n >= 27 || (a == b && m < n)) continue;
+            const Real v = nt ? acc1[r] : acc0[r];
+            const int i = a * 27 + m, jj = b * 27 + n;
+            Ke[i * 89 + jj] = v;
+            Ke[jj * 89 + i] = v;
+        }
+    }
+}
+
+// Pressure rows K_(p,m),(b,n) = sum_g (J_z w_g psi_m(g)) D_g[b][n]
+// (:1039-1054): k = g only, so k-step ks is g = 4 ks + kk and the B operand
+// is D[t][ks][b]; six 16x16 tiles (b, t), rows 8..15 padding.  Also writes
+// the zero p-p block.
+template <typename Real>
+__device__ __forceinline__ void mr_pressure_rows(int lane, const DSlice<Real> &D, const Real *s_Pa,
+                                                 Real *__restrict__ Ke)
+{
+    using M = Mfma<Real>;
+    const int kk = lane >> 4, l = lane & 15;
+    Real pa[7];
+#pragma unroll
+    for (int ks = 0; ks < 7; ++ks) pa[ks] = s_Pa[(4 * ks + kk) * 16 + l];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            typename M::acc_t acc = {Real(0), Real(0), Real(0), Real(0)};
+#pragma unroll
+            for (int ks = 0; ks < 7; ++ks) acc = M::step(pa[ks], D.v[t][ks][b], acc);
+            const int n = t * 16 + l;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = M::row(lane, r);
+                if (m >= 8 || n >= 27) continue;
+                const int i = 81 + m, jj = b * 27 + n;
+                Ke[i * 89 + jj] = acc[r];
+                Ke[jj * 89 + i] = acc[r];
+            }
+        }
+    }
+    Ke[(81 + lane / 8) * 89 + 81 + lane % 8] = Real(0);
+}
 
 template <typename Real>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? 2 : 3)))
 mr_element_matrix_kernel(int n_elements, const double *__restrict__ rec, const BasisTables *__restrict__ T,
                          Real *__restrict__ kbuf)
 {
-    // The Gauss-point records are only read while D and W are formed, so they
-    // share storage with the A operand (whose padding is re-zeroed after).
-    constexpr int kA = kKPad * kMPad;
-    constexpr int kUnion = (52 * 27 * 8 > kA * static_cast<int>(sizeof(Real))) ? 52 * 27 * 8
-                                                                                 : kA * static_cast<int>(sizeof(Real));
-    __shared__ __align__(16) unsigned char s_union[kUnion];
-    double(*s_rec)[27] = reinterpret_cast<double(*)[27]>(s_union);
-    Real *s_A = reinterpret_cast<Real *>(s_union);  // A[(g,j)][m] = U_g^{ab}[m][j] (k-major)
-    __shared__ Real s_W[27][81];
-    // B[(g,j)][n] = D_g[j][n], n padded to 32.  Read with stride 96 it is also
-    // the pressure B operand B[g][(b,n)] = D_g[b][n].
-    __shared__ Real s_B[kA];
-    __shared__ Real s_Pa[kPPad * 16];     // pressure A[g][m] = J_z w_g psi_m(g)
-    using M = Mfma<Real>;
+    __shared__ double s_tab[27][3][27];  // dphi_n/dxi_mi at g: [g][mi][n]
+    __shared__ double s_rec[52][27];     // Gauss-point records of the element
+    __shared__ Real s_W[28][81];         // W_g[(a,i),(b,j)] at [g][(3a+i)*9 + 3b+j]; row 27 zero
+    __shared__ Real s_Pa[28 * 16];       // J_z w_g psi_m(g) at [g*16 + m]; m >= 8 and g = 27 zero
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int blk_a[6] = {0, 1, 1, 2, 2, 2}, blk_b[6] = {0, 0, 1, 0, 1, 2};
+    const int kk = lane >> 4, l = lane & 15;
     const int64_t n_gp = static_cast<int64_t>(n_elements) * 27;
-    // padding rows/columns of B and Pa stay zero for every element
-    for (int k = t; k < kA; k += 256) s_B[k] = Real(0);
-    for (int k = t; k < kPPad * 16; k += 256) s_Pa[k] = Real(0);
+    for (int k = t; k < 27 * 27 * 3; k += 256) {
+        const int g = k / 81, mi = (k / 27) % 3, n = k % 27;
+        s_tab[g][mi][n] = T->quad27[(g * 27 + n) * 4 + 1 + mi];
+    }
+    for (int k = t; k < 81; k += 256) s_W[27][k] = Real(0);
+    for (int k = t; k < 28 * 16; k += 256) s_Pa[k] = Real(0);
 
     for (int e = blockIdx.x; e < n_elements; e += gridDim.x) {
         const int64_t q0 = static_cast<int64_t>(e) * 27;
         Real *Ke = kbuf + static_cast<int64_t>(e) * 89 * 89;
-        __syncthreads();
+        __syncthreads();  // the previous element's W and Pa are consumed
         for (int k = t; k < 52 * 27; k += 256) s_rec[k / 27][k % 27] = rec[(k / 27) * n_gp + q0 + (k % 27)];
         __syncthreads();
-        for (int k = t; k < 27 * 81; k += 256) {
+        for (int k = t; k < 27 * 81; k += 256) {  // W_g[(a,i),(b,j)] = J_z w_g (c + delta_ab sigma_ij)
             const int g = k / 81, r = k % 81;
-            {  // D_g[j][n] = sum_mi DXI_DX(mi, j) dphi_n/dxi_mi (:974-986)
-                const int j = r / 27, n = r % 27;
-                const double *f = T->quad27 + (g * 27 + n) * 4;
-                const Real d = static_cast<Real>(s_rec[GP_DXIDZ + 0 + 3 * j][g] * f[1] +
-                                                 s_rec[GP_DXIDZ + 1 + 3 * j][g] * f[2] +
-                                                 s_rec[GP_DXIDZ + 2 + 3 * j][g] * f[3]);
-                s_B[(g * 3 + j) * kMPad + n] = d;
-            }
-            {  // W_g[(a,i),(b,j)]
-                const int row = r / 9, col = r % 9;
-                const int a = row / 3, i = row % 3, b = col / 3, j = col % 3;
-                double v = s_rec[GP_C + t2v(i, a) + 6 * t2v(j, b)][g];
-                if (a == b) v = v + s_rec[GP_SIGJ + t2v(i, j)][g];
-                s_W[g][r] = static_cast<Real>(s_rec[GP_JGW][g] * v);
-            }
+            const int row = r / 9, col = r % 9;
+            const int a = row / 3, i = row % 3, b = col / 3, j = col % 3;
+            double v = s_rec[GP_C + t2v(i, a) + 6 * t2v(j, b)][g];
+            if (a == b) v = v + s_rec[GP_SIGJ + t2v(i, j)][g];
+            s_W[g][r] = static_cast<Real>(s_rec[GP_JGW][g] * v);
         }
         for (int k = t; k < 27 * 8; k += 256) {
             const int g = k / 8, m = k % 8;
             s_Pa[g * 16 + m] = static_cast<Real>(s_rec[GP_JGW][g] * T->lin27[(g * 8 + m) * 4]);
         }
-        __syncthreads();
-        // records consumed: zero the padding of the A operand they overlapped
-        for (int k = t; k < kA; k += 256)
-            if (k >= 81 * kMPad || (k % kMPad) >= 27) s_A[k] = Real(0);
-        __syncthreads();
-        for (int bq = 0; bq < 6; ++bq) {
-            const int a = blk_a[bq], b = blk_b[bq];
-            // U_g^{ab}[m][j] = sum_i D_g[i][m] W_g[(a,i),(b,j)] into s_A[(g,j)][m]
-            for (int k = t; k < 27 * 81; k += 256) {
-                const int g = k / 81, r = k % 81, j = r / 27, m = r % 27;
-                const Real *Dm = s_B + (g * 3) * kMPad + m;
-                const Real *w = &s_W[g][(a * 3) * 9 + b * 3 + j];
-                s_A[(g * 3 + j) * kMPad + m] = Dm[0] * w[0] + Dm[kMPad] * w[9] + Dm[2 * kMPad] * w[18];
-            }
-            __syncthreads();
-            {   // wave -> 16x16 tile (mt, nt) of the 32x32 padded block
-                const int mt = wave >> 1, nt = wave & 1;
-                typename M::acc_t acc = {Real(0), Real(0), Real(0), Real(0)};
-                const int mi = mt * 16 + (lane & 15), ni = nt * 16 + (lane & 15), kk = lane >> 4;
-#pragma unroll 7
-                for (int ks = 0; ks < kKPad / 4; ++ks) {
-                    const int k = ks * 4 + kk;
-                    acc = M::step(s_A[k * kMPad + mi], s_B[k * kMPad + ni], acc);
-                }
+        DSlice<Real> D;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int m = mt * 16 + M::row(lane, r), n = ni;
-                    if (m >= 27 || n >= 27) continue;
-                    const int i = a * 27 + m, j = b * 27 + n;
-                    const Real v = acc[r];
-                    if (a != b) {
-                        Ke[i * 89 + j] = v;
-                        Ke[j * 89 + i] = v;
-                    } else if (m >= n) {  // lower triangle, mirrored (:1149-1153)
-                        Ke[i * 89 + j] = v;
-                        Ke[j * 89 + i] = v;
-                    }
+        for (int tt = 0; tt < 2; ++tt) {
+            const int n = tt * 16 + l, nc = n < 27 ? n : 26;
+#pragma unroll
+            for (int gp = 0; gp < 7; ++gp) {
+                const int g = 4 * gp + kk, gc = g < 27 ? g : 26;
+                const bool ok = n < 27 && g < 27;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double d = s_rec[GP_DXIDZ + 0 + 3 * j][gc] * s_tab[gc][0][nc] +
+                                     s_rec[GP_DXIDZ + 1 + 3 * j][gc] * s_tab[gc][1][nc] +
+                                     s_rec[GP_DXIDZ + 2 + 3 * j][gc] * s_tab[gc][2][nc];
+                    D.v[tt][gp][j] = ok ? static_cast<Real>(d) : Real(0);
                 }
             }
-            __syncthreads();
         }
-        // pressure rows: 16 (8 used) x 96 padded, K = 28; waves 0..3 take tiles nt = wave, wave + 4 (< 6)
-        for (int nt = wave; nt < 6; nt += 4) {
-            typename M::acc_t acc = {Real(0), Real(0), Real(0), Real(0)};
-            const int mi = lane & 15, ni = nt * 16 + (lane & 15), kk = lane >> 4;
-#pragma unroll
-            for (int ks = 0; ks < kPPad / 4; ++ks) {
-                const int k = ks * 4 + kk;
-                acc = M::step(s_Pa[k * 16 + mi], s_B[k * 96 + ni], acc);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = M::row(lane, r), bn = ni, bb = bn / 32, n = bn % 32;
-                if (m >= 8 || n >= 27) continue;
-                const int i = 81 + m, j = bb * 27 + n;
-                Ke[i * 89 + j] = acc[r];
-                Ke[j * 89 + i] = acc[r];
-            }
+        __syncthreads();
+        // the lane index is made opaque per element so that the W and store
+        // offsets derived from it are recomputed here rather than hoisted out
+        // of the element loop and held in registers across it
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        // 483 MFMA k-steps per element over four waves (126/126/126/105)
+        if (wave == 0) {
+            mr_uu_block<Real, 0, true>(1, 0, ln, D, s_W, Ke);
+            mr_uu_block<Real, 1, true>(1, 0, ln, D, s_W, Ke);
+            mr_uu_block<Real, 1, true>(0, 0, ln, D, s_W, Ke);
+        } else if (wave == 1) {
+            mr_uu_block<Real, 0, true>(2, 0, ln, D, s_W, Ke);
+            mr_uu_block<Real, 1, true>(2, 0, ln, D, s_W, Ke);
+            mr_uu_block<Real, 1, true>(1, 1, ln, D, s_W, Ke);
+        } else if (wave == 2) {
+            mr_uu_block<Real, 0, true>(2, 1, ln, D, s_W, Ke);
+            mr_uu_block<Real, 1, true>(2, 1, ln, D, s_W, Ke);
+            mr_uu_block<Real, 1, true>(2, 2, ln, D, s_W, Ke);
+        } else {
+            mr_pressure_rows<Real>(ln, D, s_Pa, Ke);
+            mr_uu_block<Real, 0, false>(0, 0, ln, D, s_W, Ke);
+            mr_uu_block<Real, 0, false>(1, 1, ln, D, s_W, Ke);
+            mr_uu_block<Real, 0, false>(2, 2, ln, D, s_W, Ke);
         }
-        if (t < 64) Ke[(81 + t / 8) * 89 + 81 + t % 8] = Real(0);  // no p-p block
     }
 }
 
@@ -573,7 +625,8 @@ int launch_mr_element_matrices(EquationsSet &s, int precision, int blocks_per_cu
     const BasisTables *T = nullptr;
     OC_CHECK(ensure_tables(&T));
     Runtime &rt = runtime();
-    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : 4;
+    // default: the resident blocks per CU (two FP64 / three FP32 waves per SIMD)
+    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : (precision == OC_FP64 ? 2 : 3);
     const unsigned blocks = static_cast<unsigned>(std::min(s.n_elements, rt.num_cus * bpc));
     if (precision == OC_FP64) {
         KernelTimer timer("mr_element_matrix_fp64");
