@@ -353,6 +353,7 @@ struct Mfma<float> {
     __device__ static acc_t step(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
     // output row of accumulator register r for this lane (column = lane % 16)
     __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+    static constexpr int kRowStep = 1;
 };
 
 template <>
@@ -360,6 +361,7 @@ struct Mfma<double> {
     using acc_t = mfma_f64x4;
     __device__ static acc_t step(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
     __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+    static constexpr int kRowStep = 4;
 };
 
 // Per-lane operand slices of D_g[j][n] = sum_mi DXI_DX(mi, j) dphi_n/dxi_mi
@@ -371,6 +373,24 @@ struct Mfma<double> {
 // and, contracted with W, form its A operand
 //   U^{ab}_g[m][j] = sum_i D_g[i][m] W_g[(a,i),(b,j)]
 // on the fly, so U never goes through LDS.
+// Element-matrix stores go through a buffer resource over the 89x89 block of
+// one element: the lane part of each offset is one register and the tile part
+// a constant, so a store costs no address arithmetic.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+template <typename Real>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t element_rsrc(Real *Ke)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(Ke, 0, 89 * 89 * static_cast<int>(sizeof(Real)), 0x00020000);
+}
+__device__ __forceinline__ void element_store(__amdgpu_buffer_rsrc_t rs, double v, int voff, int soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, voff, soff, 0);
+}
+__device__ __forceinline__ void element_store(__amdgpu_buffer_rsrc_t rs, float v, int voff, int soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, voff, soff, 0);
+}
+
 template <typename Real>
 struct DSlice {
     Real v[2][7][3];
@@ -388,30 +408,54 @@ __device__ __forceinline__ void mr_uu_block(int a, int b, int lane, const DSlice
     const int kk = lane >> 4, l = lane & 15;
     typename M::acc_t acc0 = {Real(0), Real(0), Real(0), Real(0)};
     typename M::acc_t acc1 = {Real(0), Real(0), Real(0), Real(0)};
-    const int wofs = a * 27 + b * 3;
+    // k-step s = 7 j + gp; software pipeline: the W loads of step s + 2 and
+    // the A operand of step s + 1 are issued behind the MFMAs of step s, and a
+    // scheduling barrier per step keeps that order (and the register count)
+    const Real *wbase = &s_W[kk][a * 27 + b * 3];
+    auto load_w = [&](int st, Real *w) {
+        const Real *p = wbase + (4 * (st % 7)) * 81 + st / 7;
+        w[0] = p[0];
+        w[1] = p[9];
+        w[2] = p[18];
+    };
+    auto a_operand = [&](int st, const Real *w) {
+        const int gp = st % 7;
+        return D.v[MT][gp][0] * w[0] + D.v[MT][gp][1] * w[1] + D.v[MT][gp][2] * w[2];
+    };
+    Real wq[2][3];
+    load_w(0, wq[0]);
+    load_w(1, wq[1]);
+    Real u = a_operand(0, wq[0]);
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-#pragma unroll
-        for (int gp = 0; gp < 7; ++gp) {
-            const Real *w = &s_W[4 * gp + kk][wofs + j];
-            const Real u = D.v[MT][gp][0] * w[0] + D.v[MT][gp][1] * w[9] + D.v[MT][gp][2] * w[18];
-            acc0 = M::step(u, D.v[0][gp][j], acc0);
-            if (TWO) acc1 = M::step(u, D.v[1][gp][j], acc1);
-        }
+    for (int st = 0; st < 21; ++st) {
+        const int j = st / 7, gp = st % 7;
+        if (st + 2 < 21) load_w(st + 2, wq[st & 1]);
+        acc0 = M::step(u, D.v[0][gp][j], acc0);
+        if (TWO) acc1 = M::step(u, D.v[1][gp][j], acc1);
+        if (st + 1 < 21) u = a_operand(st + 1, wq[(st + 1) & 1]);
         __builtin_amdgcn_sched_barrier(0);
     }
+    // entry (m, n) of the tile at (a*27 + m)*89 + b*27 + n and its mirror at
+    // (b*27 + n)*89 + a*27 + m, with m = MT*16 + row0 + kRowStep*r and
+    // n = nt*16 + l: lane part row0*89 + l (mirror l*89 + row0), the rest
+    // constant
+    const __amdgpu_buffer_rsrc_t rs = element_rsrc(Ke);
+    const int row0 = M::row(lane, 0);
+    const int vdir = (row0 * 89 + l) * static_cast<int>(sizeof(Real));
+    const int vmir = (l * 89 + row0) * static_cast<int>(sizeof(Real));
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int m = MT * 16 + M::row(lane, r);
+        const int m = MT * 16 + row0 + M::kRowStep * r;
         if (m >= 27) continue;
 #pragma unroll
         for (int nt = 0; nt < (TWO ? 2 : 1); ++nt) {
             const int n = nt * 16 + l;
             if (n >= 27 || (a == b && m < n)) continue;
             const Real v = nt ? acc1[r] : acc0[r];
-            const int i = a * 27 + m, jj = b * 27 + n;
-            Ke[i * 89 + jj] = v;
-            Ke[jj * 89 + i] = v;
+            const int cdir = ((a * 27 + MT * 16 + M::kRowStep * r) * 89 + b * 27 + nt * 16) * static_cast<int>(sizeof(Real));
+            const int cmir = ((b * 27 + nt * 16) * 89 + a * 27 + MT * 16 + M::kRowStep * r) * static_cast<int>(sizeof(Real));
+            element_store(rs, v, vdir, cdir);
+            element_store(rs, v, vmir, cmir);
         }
     }
 }
@@ -429,6 +473,10 @@ __device__ __forceinline__ void mr_pressure_rows(int lane, const DSlice<Real> &D
     Real pa[7];
 #pragma unroll
     for (int ks = 0; ks < 7; ++ks) pa[ks] = s_Pa[(4 * ks + kk) * 16 + l];
+    const __amdgpu_buffer_rsrc_t rs = element_rsrc(Ke);
+    const int row0 = M::row(lane, 0);
+    const int vdir = (row0 * 89 + l) * static_cast<int>(sizeof(Real));
+    const int vmir = (l * 89 + row0) * static_cast<int>(sizeof(Real));
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
 #pragma unroll
@@ -439,96 +487,139 @@ __device__ __forceinline__ void mr_pressure_rows(int lane, const DSlice<Real> &D
             const int n = t * 16 + l;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = M::row(lane, r);
+                const int m = row0 + M::kRowStep * r;
                 if (m >= 8 || n >= 27) continue;
-                const int i = 81 + m, jj = b * 27 + n;
-                Ke[i * 89 + jj] = acc[r];
-                Ke[jj * 89 + i] = acc[r];
+                const int cdir = ((81 + M::kRowStep * r) * 89 + b * 27 + t * 16) * static_cast<int>(sizeof(Real));
+                const int cmir = ((b * 27 + t * 16) * 89 + 81 + M::kRowStep * r) * static_cast<int>(sizeof(Real));
+                element_store(rs, acc[r], vdir, cdir);
+                element_store(rs, acc[r], vmir, cmir);
             }
         }
     }
     Ke[(81 + lane / 8) * 89 + 81 + lane % 8] = Real(0);
 }
 
+// Producer/consumer element-matrix kernel: one block of eight waves walks its
+// elements in a two-stage pipeline.  Waves 4..7 (producers) build, for
+// element i + 1, the operands D, W and the pressure A operand in one half of
+// double-buffered LDS and load the Gauss-point records of element i + 2;
+// waves 0..3 (consumers) meanwhile load their D slices of element i from the
+// other half and run its MFMA contraction.  One block barrier per element.
 template <typename Real>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? 2 : 3)))
+__global__ void __launch_bounds__(512)
 mr_element_matrix_kernel(int n_elements, const double *__restrict__ rec, const BasisTables *__restrict__ T,
                          Real *__restrict__ kbuf)
 {
     __shared__ double s_tab[27][3][27];  // dphi_n/dxi_mi at g: [g][mi][n]
-    __shared__ double s_rec[52][27];     // Gauss-point records of the element
-    __shared__ Real s_W[28][81];         // W_g[(a,i),(b,j)] at [g][(3a+i)*9 + 3b+j]; row 27 zero
-    __shared__ Real s_Pa[28 * 16];       // J_z w_g psi_m(g) at [g*16 + m]; m >= 8 and g = 27 zero
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int kk = lane >> 4, l = lane & 15;
+    __shared__ double s_psi[27][8];      // trilinear psi_m(g)
+    __shared__ double s_rec[2][52][27];  // Gauss-point records
+    __shared__ Real s_D[2][28][3][32];   // D_g[j][n]; g = 27 and n >= 27 zero
+    __shared__ Real s_W[2][28][81];      // W_g[(a,i),(b,j)] at [g][(3a+i)*9 + 3b+j]; g = 27 zero
+    __shared__ Real s_Pa[2][28 * 16];    // J_z w_g psi_m(g) at [g*16 + m]; m >= 8 and g = 27 zero
+    constexpr int kRec = 52 * 27, kRecPerThread = (kRec + 255) / 256;
+    const int t = threadIdx.x, lane = t & 63;
+    const bool producer = t >= 256;
+    const int pt = t - 256;
+    // producer thread pt < 243 owns column rq = pt % 81 of W_g (row (a,i),
+    // column (b,j)) and of D_g (j = rq / 27, n = rq % 27) for g = pg (mod 3)
+    const int rq = (pt >= 0 ? pt : 0) % 81, pg = (pt >= 0 ? pt : 0) / 81;
+    const int wa = rq / 27, wi = (rq / 9) % 3, wb = (rq % 9) / 3, wj = rq % 3;
+    const int wf1 = GP_C + t2v(wi, wa) + 6 * t2v(wj, wb), wf2 = GP_SIGJ + t2v(wi, wj);
+    const bool wdiag = wa == wb;
+    const int dj = rq / 27, dn = rq % 27;
     const int64_t n_gp = static_cast<int64_t>(n_elements) * 27;
-    for (int k = t; k < 27 * 27 * 3; k += 256) {
+    const int n_mine = static_cast<int>(blockIdx.x) < n_elements
+                           ? (n_elements - 1 - static_cast<int>(blockIdx.x)) / static_cast<int>(gridDim.x) + 1
+                           : 0;
+    for (int k = t; k < 27 * 27 * 3; k += 512) {
         const int g = k / 81, mi = (k / 27) % 3, n = k % 27;
         s_tab[g][mi][n] = T->quad27[(g * 27 + n) * 4 + 1 + mi];
     }
-    for (int k = t; k < 81; k += 256) s_W[27][k] = Real(0);
-    for (int k = t; k < 28 * 16; k += 256) s_Pa[k] = Real(0);
+    for (int k = t; k < 27 * 8; k += 512) s_psi[k / 8][k % 8] = T->lin27[k * 4];
+    for (int k = t; k < 2 * 28 * 3 * 32; k += 512) (&s_D[0][0][0][0])[k] = Real(0);
+    for (int k = t; k < 2 * 28 * 81; k += 512) (&s_W[0][0][0])[k] = Real(0);
+    for (int k = t; k < 2 * 28 * 16; k += 512) (&s_Pa[0][0])[k] = Real(0);
+    if (producer && n_mine > 0) {
+        const int64_t q0 = static_cast<int64_t>(blockIdx.x) * 27;
+        for (int k = pt; k < kRec; k += 256) s_rec[0][k / 27][k % 27] = rec[(k / 27) * n_gp + q0 + (k % 27)];
+    }
+    __syncthreads();
 
-    for (int e = blockIdx.x; e < n_elements; e += gridDim.x) {
-        const int64_t q0 = static_cast<int64_t>(e) * 27;
-        Real *Ke = kbuf + static_cast<int64_t>(e) * 89 * 89;
-        __syncthreads();  // the previous element's W and Pa are consumed
-        for (int k = t; k < 52 * 27; k += 256) s_rec[k / 27][k % 27] = rec[(k / 27) * n_gp + q0 + (k % 27)];
-        __syncthreads();
-        for (int k = t; k < 27 * 81; k += 256) {  // W_g[(a,i),(b,j)] = J_z w_g (c + delta_ab sigma_ij)
-            const int g = k / 81, r = k % 81;
-            const int row = r / 9, col = r % 9;
-            const int a = row / 3, i = row % 3, b = col / 3, j = col % 3;
-            double v = s_rec[GP_C + t2v(i, a) + 6 * t2v(j, b)][g];
-            if (a == b) v = v + s_rec[GP_SIGJ + t2v(i, j)][g];
-            s_W[g][r] = static_cast<Real>(s_rec[GP_JGW][g] * v);
-        }
-        for (int k = t; k < 27 * 8; k += 256) {
-            const int g = k / 8, m = k % 8;
-            s_Pa[g * 16 + m] = static_cast<Real>(s_rec[GP_JGW][g] * T->lin27[(g * 8 + m) * 4]);
-        }
-        DSlice<Real> D;
+    for (int it = 0; it <= n_mine; ++it) {
+        if (producer) {
+            if (it < n_mine) {
+                const int p = it & 1;
+                // records of the element after this one: issued first, stored last
+                const int e_next = static_cast<int>(blockIdx.x) + (it + 1) * static_cast<int>(gridDim.x);
+                double pre[kRecPerThread];
 #pragma unroll
-        for (int tt = 0; tt < 2; ++tt) {
-            const int n = tt * 16 + l, nc = n < 27 ? n : 26;
-#pragma unroll
-            for (int gp = 0; gp < 7; ++gp) {
-                const int g = 4 * gp + kk, gc = g < 27 ? g : 26;
-                const bool ok = n < 27 && g < 27;
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double d = s_rec[GP_DXIDZ + 0 + 3 * j][gc] * s_tab[gc][0][nc] +
-                                     s_rec[GP_DXIDZ + 1 + 3 * j][gc] * s_tab[gc][1][nc] +
-                                     s_rec[GP_DXIDZ + 2 + 3 * j][gc] * s_tab[gc][2][nc];
-                    D.v[tt][gp][j] = ok ? static_cast<Real>(d) : Real(0);
+                for (int i = 0; i < kRecPerThread; ++i) {
+                    const int k = pt + 256 * i;
+                    pre[i] = (it + 1 < n_mine && k < kRec)
+                                 ? rec[(k / 27) * n_gp + static_cast<int64_t>(e_next) * 27 + (k % 27)]
+                                 : 0.0;
                 }
+                const double(*R)[27] = s_rec[p];
+                if (pt < 243) {  // column rq of W_g and of D_g for g = pg, pg + 3, ...
+#pragma unroll 3
+                    for (int g = pg; g < 27; g += 3) {
+                        double v = R[wf1][g];
+                        if (wdiag) v = v + R[wf2][g];
+                        s_W[p][g][rq] = static_cast<Real>(R[GP_JGW][g] * v);
+                        const double d = R[GP_DXIDZ + 0 + 3 * dj][g] * s_tab[g][0][dn] +
+                                         R[GP_DXIDZ + 1 + 3 * dj][g] * s_tab[g][1][dn] +
+                                         R[GP_DXIDZ + 2 + 3 * dj][g] * s_tab[g][2][dn];
+                        s_D[p][g][dj][dn] = static_cast<Real>(d);
+                    }
+                }
+                if (pt < 27 * 8) s_Pa[p][pt / 8 * 16 + pt % 8] = static_cast<Real>(R[GP_JGW][pt / 8] * s_psi[pt / 8][pt % 8]);
+                if (it + 1 < n_mine) {
+#pragma unroll
+                    for (int i = 0; i < kRecPerThread; ++i) {
+                        const int k = pt + 256 * i;
+                        if (k < kRec) s_rec[p ^ 1][k / 27][k % 27] = pre[i];
+                    }
+                }
+            }
+        } else if (it >= 1) {
+            const int p = (it - 1) & 1;
+            const int e = static_cast<int>(blockIdx.x) + (it - 1) * static_cast<int>(gridDim.x);
+            Real *Ke = kbuf + static_cast<int64_t>(e) * 89 * 89;
+            // the lane index is made opaque per element so that the W and store
+            // offsets derived from it are recomputed here rather than hoisted
+            // out of the element loop and held in registers across it
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const int kk = ln >> 4, l = ln & 15, wave = t >> 6;
+            DSlice<Real> D;
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                for (int gp = 0; gp < 7; ++gp)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) D.v[tt][gp][j] = s_D[p][4 * gp + kk][j][16 * tt + l];
+            const Real(*W)[81] = s_W[p];
+            // 483 MFMA k-steps per element over four waves (126/126/126/105)
+            if (wave == 0) {
+                mr_uu_block<Real, 0, true>(1, 0, ln, D, W, Ke);
+                mr_uu_block<Real, 1, true>(1, 0, ln, D, W, Ke);
+                mr_uu_block<Real, 1, true>(0, 0, ln, D, W, Ke);
+            } else if (wave == 1) {
+                mr_uu_block<Real, 0, true>(2, 0, ln, D, W, Ke);
+                mr_uu_block<Real, 1, true>(2, 0, ln, D, W, Ke);
+                mr_uu_block<Real, 1, true>(1, 1, ln, D, W, Ke);
+            } else if (wave == 2) {
+                mr_uu_block<Real, 0, true>(2, 1, ln, D, W, Ke);
+                mr_uu_block<Real, 1, true>(2, 1, ln, D, W, Ke);
+                mr_uu_block<Real, 1, true>(2, 2, ln, D, W, Ke);
+            } else {
+                mr_pressure_rows<Real>(ln, D, s_Pa[p], Ke);
+                mr_uu_block<Real, 0, false>(0, 0, ln, D, W, Ke);
+                mr_uu_block<Real, 0, false>(1, 1, ln, D, W, Ke);
+                mr_uu_block<Real, 0, false>(2, 2, ln, D, W, Ke);
             }
         }
         __syncthreads();
-        // the lane index is made opaque per element so that the W and store
-        // offsets derived from it are recomputed here rather than hoisted out
-        // of the element loop and held in registers across it
-        int ln = lane;
-        asm volatile("" : "+v"(ln));
-        // 483 MFMA k-steps per element over four waves (126/126/126/105)
-        if (wave == 0) {
-            mr_uu_block<Real, 0, true>(1, 0, ln, D, s_W, Ke);
-            mr_uu_block<Real, 1, true>(1, 0, ln, D, s_W, Ke);
-            mr_uu_block<Real, 1, true>(0, 0, ln, D, s_W, Ke);
-        } else if (wave == 1) {
-            mr_uu_block<Real, 0, true>(2, 0, ln, D, s_W, Ke);
-            mr_uu_block<Real, 1, true>(2, 0, ln, D, s_W, Ke);
-            mr_uu_block<Real, 1, true>(1, 1, ln, D, s_W, Ke);
-        } else if (wave == 2) {
-            mr_uu_block<Real, 0, true>(2, 1, ln, D, s_W, Ke);
-            mr_uu_block<Real, 1, true>(2, 1, ln, D, s_W, Ke);
-            mr_uu_block<Real, 1, true>(2, 2, ln, D, s_W, Ke);
-        } else {
-            mr_pressure_rows<Real>(ln, D, s_Pa, Ke);
-            mr_uu_block<Real, 0, false>(0, 0, ln, D, s_W, Ke);
-            mr_uu_block<Real, 0, false>(1, 1, ln, D, s_W, Ke);
-            mr_uu_block<Real, 0, false>(2, 2, ln, D, s_W, Ke);
-        }
     }
 }
 
@@ -625,16 +716,16 @@ int launch_mr_element_matrices(EquationsSet &s, int precision, int blocks_per_cu
     const BasisTables *T = nullptr;
     OC_CHECK(ensure_tables(&T));
     Runtime &rt = runtime();
-    // default: the resident blocks per CU (two FP64 / three FP32 waves per SIMD)
-    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : (precision == OC_FP64 ? 2 : 3);
+    // default: one resident block of eight waves per CU
+    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : 1;
     const unsigned blocks = static_cast<unsigned>(std::min(s.n_elements, rt.num_cus * bpc));
     if (precision == OC_FP64) {
         KernelTimer timer("mr_element_matrix_fp64");
-        mr_element_matrix_kernel<double><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
+        mr_element_matrix_kernel<double><<<blocks, 512, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
                                                                        s.kbuf64.ptr);
     } else {
         KernelTimer timer("mr_element_matrix_fp32");
-        mr_element_matrix_kernel<float><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
+        mr_element_matrix_kernel<float><<<blocks, 512, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
                                                                       s.kbuf32.ptr);
     }
     OC_HIP_CHECK(hipGetLastError());
