@@ -623,6 +623,236 @@ mr_element_matrix_kernel(int n_elements, const double *__restrict__ rec, const B
     }
 }
 
+// FP64 element matrices on v_mfma_f64_4x4x4f64 (four 4x4x4 blocks per
+// instruction; lane = c + 4 blk + 16 h holds A_blk[c][h], B_blk[h][c] and the
+// result D_blk[h][c]).  The FP64 16x16x4 instruction runs at about 60% of the
+// 4x4x4 rate here, and the 4-wide tiles also pad 27 to 28 instead of 32.
+// Same producer/consumer pipeline as mr_element_matrix_kernel.  A consumer
+// instruction covers the row tiles I = 4t + blk of one displacement block
+// (rows m = 16t + 4blk + i) against one column tile J (n = 4J + j):
+//   A = U^{ab}_g[16t + l][j_dim] from the lane's D slice and W (l = c + 4 blk),
+//   B = D_g[j_dim][4J + c], shared by the four row tiles, read from LDS;
+// k-step s covers j_dim = s / 7 and g = 4 (s % 7) + h.
+constexpr int kF64Q[3][3] = {{0, 0, 0}, {1, 2, 0}, {3, 4, 5}};  // lower block (a, b) -> q
+
+__device__ __forceinline__ double2 lds_pair(const double *p) { return *reinterpret_cast<const double2 *>(p); }
+
+template <int Q, int T, int NJ>
+struct F64Unit {
+    static constexpr int q = Q, t = T, nj = NJ;
+    static constexpr int a = Q == 0 ? 0 : (Q <= 2 ? 1 : 2);
+    static constexpr int b = Q - (a * (a + 1)) / 2;
+};
+
+// stores the tiles of one unit: acc[J] at m = 16T + 4blk + h, n = 4J + c
+template <typename U>
+__device__ __forceinline__ void f64_unit_store(const double *acc, int blk, int h, int c, __amdgpu_buffer_rsrc_t rs)
+{
+    const int m = 16 * U::t + 4 * blk + h;
+    const int vdir = ((4 * blk + h) * 89 + c) * 8, vmir = (c * 89 + 4 * blk + h) * 8;
+#pragma unroll
+    for (int J = 0; J < U::nj; ++J) {
+        const int n = 4 * J + c;
+        if (m >= 27 || n >= 27 || (U::a == U::b && m < n)) continue;
+        element_store(rs, acc[J], vdir, ((U::a * 27 + 16 * U::t) * 89 + U::b * 27 + 4 * J) * 8);
+        element_store(rs, acc[J], vmir, ((U::b * 27 + 4 * J) * 89 + U::a * 27 + 16 * U::t) * 8);
+    }
+}
+
+// three units of one wave, all on row half T, sharing the B reads of every
+// k-step; the B and W operands of step s + 1 are loaded while the MFMAs of
+// step s run
+template <int T, typename U0, typename U1, typename U2>
+__device__ __forceinline__ void f64_units(int lane, const double (&DT)[7][3], const double *sD, const double *sW,
+                                          __amdgpu_buffer_rsrc_t rs)
+{
+    const int c = lane & 3, blk = (lane >> 2) & 3, h = lane >> 4;
+    double acc0[7], acc1[7], acc2[7];
+#pragma unroll
+    for (int J = 0; J < 7; ++J) acc0[J] = acc1[J] = acc2[J] = 0.0;
+    double B[2][7], W[2][3][3];
+    auto load = [&](int st, double (&b)[7], double (&w)[3][3]) {
+        const int jd = st / 7, g = 4 * (st % 7) + h;
+        const double *bp = sD + ((g * 3 + jd) * 4 + c) * 8;  // D_g[jd][4J + c], J = 0..6
+        const double2 b01 = lds_pair(bp), b23 = lds_pair(bp + 2), b45 = lds_pair(bp + 4);
+        b[0] = b01.x; b[1] = b01.y; b[2] = b23.x; b[3] = b23.y; b[4] = b45.x; b[5] = b45.y; b[6] = bp[6];
+        const int qs[3] = {U0::q, U1::q, U2::q};
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            const double *wp = sW + ((g * 6 + qs[x]) * 3 + jd) * 4;  // W_g[(a,i),(b,jd)], i = 0..2
+            const double2 w01 = lds_pair(wp);
+            w[x][0] = w01.x; w[x][1] = w01.y; w[x][2] = wp[2];
+        }
+    };
+    load(0, B[0], W[0]);
+#pragma unroll
+    for (int st = 0; st < 21; ++st) {
+        const int cur = st & 1, gp = st % 7;
+        if (st + 1 < 21) load(st + 1, B[cur ^ 1], W[cur ^ 1]);
+        double u[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+            u[x] = DT[gp][0] * W[cur][x][0] + DT[gp][1] * W[cur][x][1] + DT[gp][2] * W[cur][x][2];
+#pragma unroll
+        for (int J = 0; J < 7; ++J) {
+            if (J < U0::nj) acc0[J] = __builtin_amdgcn_mfma_f64_4x4x4f64(u[0], B[cur][J], acc0[J], 0, 0, 0);
+            if (J < U1::nj) acc1[J] = __builtin_amdgcn_mfma_f64_4x4x4f64(u[1], B[cur][J], acc1[J], 0, 0, 0);
+            if (J < U2::nj) acc2[J] = __builtin_amdgcn_mfma_f64_4x4x4f64(u[2], B[cur][J], acc2[J], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    f64_unit_store<U0>(acc0, blk, h, c, rs);
+    f64_unit_store<U1>(acc1, blk, h, c, rs);
+    f64_unit_store<U2>(acc2, blk, h, c, rs);
+}
+
+// pressure rows K_(p,m),(b,n), m = 4I + h, n = 16t + 4blk + c, for all b and
+// t; the B operand D_g[b][n] comes from LDS
+__device__ __forceinline__ void f64_pressure(int lane, const double *sD, const double *sPa, __amdgpu_buffer_rsrc_t rs)
+{
+    const int c = lane & 3, blk = (lane >> 2) & 3, h = lane >> 4;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+        for (int gp = 0; gp < 7; ++gp) {
+            const int g = 4 * gp + h;
+            const double *pp = sPa + g * 8 + c;
+            const double a0 = pp[0], a1 = pp[4];
+            const double *bp = sD + ((g * 3 + b) * 4 + c) * 8 + blk;  // D_g[b][16t + 4blk + c] at +4t
+            const double b0 = bp[0], b1 = bp[4];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_4x4x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_4x4x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_4x4x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_4x4x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        const int vdir = (h * 89 + 4 * blk + c) * 8, vmir = ((4 * blk + c) * 89 + h) * 8;
+#pragma unroll
+        for (int I = 0; I < 2; ++I)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                if (16 * t + 4 * blk + c >= 27) continue;
+                element_store(rs, acc[I][t], vdir, ((81 + 4 * I) * 89 + b * 27 + 16 * t) * 8);
+                element_store(rs, acc[I][t], vmir, ((b * 27 + 16 * t) * 89 + 81 + 4 * I) * 8);
+            }
+    }
+}
+
+__global__ void __launch_bounds__(512)
+mr_element_matrix_f64_kernel(int n_elements, const double *__restrict__ rec, const BasisTables *__restrict__ T,
+                             double *__restrict__ kbuf)
+{
+    __shared__ double s_tab[27][3][27];          // dphi_n/dxi_mi at g: [g][mi][n]
+    __shared__ double s_psi[27][8];              // trilinear psi_m(g)
+    __shared__ double s_rec[2][52][27];          // Gauss-point records
+    __shared__ __align__(16) double s_D[2][28][3][4][8];  // D_g[j][n] at [g][j][n % 4][n / 4]; padding zero
+    __shared__ __align__(16) double s_W[2][28][6][3][4];  // W_g[(a,i),(b,j)], lower block q, at [g][q][j][i]; padding zero
+    __shared__ double s_Pa[2][28][8];            // J_z w_g psi_m(g); g = 27 zero
+    constexpr int kRec = 52 * 27, kRecPerThread = (kRec + 255) / 256;
+    const int t = threadIdx.x, lane = t & 63;
+    const bool producer = t >= 256;
+    const int pt = producer ? t - 256 : 0;
+    // producer thread pt < 243 owns (j, n) = (rq / 27, rq % 27) of D_g for
+    // g = pg (mod 3); pt < 216 owns (q, j, i) = wq of W_g for g = pw (mod 4)
+    const int rq = pt % 81, pg = pt / 81, dj = rq / 27, dn = rq % 27;
+    const int wq = pt % 54, pw = pt / 54;
+    const int wqq = wq / 9, wj = (wq / 3) % 3, wi = wq % 3;
+    const int wa = wqq == 0 ? 0 : (wqq <= 2 ? 1 : 2), wb = wqq - (wa * (wa + 1)) / 2;
+    const int wf1 = GP_C + t2v(wi, wa) + 6 * t2v(wj, wb), wf2 = GP_SIGJ + t2v(wi, wj);
+    const bool wdiag = wa == wb;
+    const int64_t n_gp = static_cast<int64_t>(n_elements) * 27;
+    const int n_mine = static_cast<int>(blockIdx.x) < n_elements
+                           ? (n_elements - 1 - static_cast<int>(blockIdx.x)) / static_cast<int>(gridDim.x) + 1
+                           : 0;
+    for (int k = t; k < 27 * 27 * 3; k += 512) {
+        const int g = k / 81, mi = (k / 27) % 3, n = k % 27;
+        s_tab[g][mi][n] = T->quad27[(g * 27 + n) * 4 + 1 + mi];
+    }
+    for (int k = t; k < 27 * 8; k += 512) s_psi[k / 8][k % 8] = T->lin27[k * 4];
+    for (int k = t; k < 2 * 28 * 3 * 4 * 8; k += 512) (&s_D[0][0][0][0][0])[k] = 0.0;
+    for (int k = t; k < 2 * 28 * 6 * 3 * 4; k += 512) (&s_W[0][0][0][0][0])[k] = 0.0;
+    for (int k = t; k < 2 * 28 * 8; k += 512) (&s_Pa[0][0][0])[k] = 0.0;
+    if (producer && n_mine > 0) {
+        const int64_t q0 = static_cast<int64_t>(blockIdx.x) * 27;
+        for (int k = pt; k < kRec; k += 256) s_rec[0][k / 27][k % 27] = rec[(k / 27) * n_gp + q0 + (k % 27)];
+    }
+    __syncthreads();
+
+    for (int it = 0; it <= n_mine; ++it) {
+        if (producer) {
+            if (it < n_mine) {
+                const int p = it & 1;
+                const int e_next = static_cast<int>(blockIdx.x) + (it + 1) * static_cast<int>(gridDim.x);
+                double pre[kRecPerThread];
+#pragma unroll
+                for (int i = 0; i < kRecPerThread; ++i) {
+                    const int k = pt + 256 * i;
+                    pre[i] = (it + 1 < n_mine && k < kRec)
+                                 ? rec[(k / 27) * n_gp + static_cast<int64_t>(e_next) * 27 + (k % 27)]
+                                 : 0.0;
+                }
+                const double(*R)[27] = s_rec[p];
+                if (pt < 243) {  // D_g[j][n] = sum_mi DXI_DX(mi, j) dphi_n/dxi_mi (:974-986)
+#pragma unroll 3
+                    for (int g = pg; g < 27; g += 3)
+                        s_D[p][g][dj][dn % 4][dn / 4] = R[GP_DXIDZ + 0 + 3 * dj][g] * s_tab[g][0][dn] +
+                                                         R[GP_DXIDZ + 1 + 3 * dj][g] * s_tab[g][1][dn] +
+                                                         R[GP_DXIDZ + 2 + 3 * dj][g] * s_tab[g][2][dn];
+                }
+                if (pt < 216) {  // W_g[(a,i),(b,j)] = J_z w_g (c + delta_ab sigma_ij)
+#pragma unroll 7
+                    for (int g = pw; g < 27; g += 4) {
+                        double v = R[wf1][g];
+                        if (wdiag) v = v + R[wf2][g];
+                        s_W[p][g][wqq][wj][wi] = R[GP_JGW][g] * v;
+                    }
+                    s_Pa[p][pt / 8][pt % 8] = R[GP_JGW][pt / 8] * s_psi[pt / 8][pt % 8];
+                }
+                if (it + 1 < n_mine) {
+#pragma unroll
+                    for (int i = 0; i < kRecPerThread; ++i) {
+                        const int k = pt + 256 * i;
+                        if (k < kRec) s_rec[p ^ 1][k / 27][k % 27] = pre[i];
+                    }
+                }
+            }
+        } else if (it >= 1) {
+            const int p = (it - 1) & 1;
+            const int e = static_cast<int>(blockIdx.x) + (it - 1) * static_cast<int>(gridDim.x);
+            // the lane index is made opaque per element so that offsets derived
+            // from it are recomputed here rather than held across the loop
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const int c = ln & 3, blk = (ln >> 2) & 3, h = ln >> 4, wave = t >> 6;
+            // waves 0, 1: row half t = 0 (and the pressure rows); waves 2, 3: t = 1.
+            // The lane keeps D_g[i][16t + 4blk + c] for its Gauss points g = 4gp + h.
+            const int tw = wave >> 1;
+            double DT[7][3];
+#pragma unroll
+            for (int gp = 0; gp < 7; ++gp)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) DT[gp][j] = s_D[p][4 * gp + h][j][c][4 * tw + blk];
+            const double *sD = &s_D[p][0][0][0][0], *sW = &s_W[p][0][0][0][0];
+            const __amdgpu_buffer_rsrc_t rs = element_rsrc(kbuf + static_cast<int64_t>(e) * 89 * 89);
+            // 1659 k-steps per element: 378 / 399 / 441 / 441
+            if (wave == 0) {
+                f64_units<0, F64Unit<1, 0, 7>, F64Unit<3, 0, 7>, F64Unit<0, 0, 4>>(ln, DT, sD, sW, rs);
+            } else if (wave == 1) {
+                f64_units<0, F64Unit<4, 0, 7>, F64Unit<2, 0, 4>, F64Unit<5, 0, 4>>(ln, DT, sD, sW, rs);
+                f64_pressure(ln, sD, &s_Pa[p][0][0], rs);
+            } else if (wave == 2) {
+                f64_units<1, F64Unit<1, 1, 7>, F64Unit<3, 1, 7>, F64Unit<0, 1, 7>>(ln, DT, sD, sW, rs);
+            } else {
+                f64_units<1, F64Unit<4, 1, 7>, F64Unit<2, 1, 7>, F64Unit<5, 1, 7>>(ln, DT, sD, sW, rs);
+                // zero p-p block
+                double *Ke = kbuf + static_cast<int64_t>(e) * 89 * 89;
+                Ke[(81 + ln / 8) * 89 + 81 + ln % 8] = 0.0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <typename Real, int kChunk>
 __global__ void __launch_bounds__(64)
 csr_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const int *__restrict__ gptr,
@@ -721,8 +951,7 @@ int launch_mr_element_matrices(EquationsSet &s, int precision, int blocks_per_cu
     const unsigned blocks = static_cast<unsigned>(std::min(s.n_elements, rt.num_cus * bpc));
     if (precision == OC_FP64) {
         KernelTimer timer("mr_element_matrix_fp64");
-        mr_element_matrix_kernel<double><<<blocks, 512, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
-                                                                       s.kbuf64.ptr);
+        mr_element_matrix_f64_kernel<<<blocks, 512, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T, s.kbuf64.ptr);
     } else {
         KernelTimer timer("mr_element_matrix_fp32");
         mr_element_matrix_kernel<float><<<blocks, 512, 0, rt.stream>>>(s.n_elements, s.gp_records.ptr, T,
