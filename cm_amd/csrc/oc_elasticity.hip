@@ -560,16 +560,23 @@ mr_element_matrix_kernel(int n_elements, const double *__restrict__ rec, const B
                                  : 0.0;
                 }
                 const double(*R)[27] = s_rec[p];
-                if (pt < 243) {  // column rq of W_g and of D_g for g = pg, pg + 3, ...
-#pragma unroll 3
-                    for (int g = pg; g < 27; g += 3) {
+                // all LDS reads of the phase before its writes: one round trip
+                if (pt < 243) {  // column rq of W_g and of D_g for g = pg + 3k
+                    Real wv[9], dv[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        const int g = pg + 3 * k;
                         double v = R[wf1][g];
                         if (wdiag) v = v + R[wf2][g];
-                        s_W[p][g][rq] = static_cast<Real>(R[GP_JGW][g] * v);
-                        const double d = R[GP_DXIDZ + 0 + 3 * dj][g] * s_tab[g][0][dn] +
-                                         R[GP_DXIDZ + 1 + 3 * dj][g] * s_tab[g][1][dn] +
-                                         R[GP_DXIDZ + 2 + 3 * dj][g] * s_tab[g][2][dn];
-                        s_D[p][g][dj][dn] = static_cast<Real>(d);
+                        wv[k] = static_cast<Real>(R[GP_JGW][g] * v);
+                        dv[k] = static_cast<Real>(R[GP_DXIDZ + 0 + 3 * dj][g] * s_tab[g][0][dn] +
+                                                  R[GP_DXIDZ + 1 + 3 * dj][g] * s_tab[g][1][dn] +
+                                                  R[GP_DXIDZ + 2 + 3 * dj][g] * s_tab[g][2][dn]);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        s_W[p][pg + 3 * k][rq] = wv[k];
+                        s_D[p][pg + 3 * k][dj][dn] = dv[k];
                     }
                 }
                 if (pt < 27 * 8) s_Pa[p][pt / 8 * 16 + pt % 8] = static_cast<Real>(R[GP_JGW][pt / 8] * s_psi[pt / 8][pt % 8]);
@@ -792,21 +799,35 @@ mr_element_matrix_f64_kernel(int n_elements, const double *__restrict__ rec, con
                                  : 0.0;
                 }
                 const double(*R)[27] = s_rec[p];
-                if (pt < 243) {  // D_g[j][n] = sum_mi DXI_DX(mi, j) dphi_n/dxi_mi (:974-986)
-#pragma unroll 3
-                    for (int g = pg; g < 27; g += 3)
-                        s_D[p][g][dj][dn % 4][dn / 4] = R[GP_DXIDZ + 0 + 3 * dj][g] * s_tab[g][0][dn] +
-                                                         R[GP_DXIDZ + 1 + 3 * dj][g] * s_tab[g][1][dn] +
-                                                         R[GP_DXIDZ + 2 + 3 * dj][g] * s_tab[g][2][dn];
+                // every LDS read of a phase is issued before its writes (which
+                // the compiler may not move reads across), so a phase waits
+                // for one LDS round trip rather than one per Gauss point
+                if (pt < 243) {  // D_g[j][n] = sum_mi DXI_DX(mi, j) dphi_n/dxi_mi (:974-986), g = pg + 3k
+                    double dv[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        const int g = pg + 3 * k;
+                        dv[k] = R[GP_DXIDZ + 0 + 3 * dj][g] * s_tab[g][0][dn] +
+                                R[GP_DXIDZ + 1 + 3 * dj][g] * s_tab[g][1][dn] +
+                                R[GP_DXIDZ + 2 + 3 * dj][g] * s_tab[g][2][dn];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) s_D[p][pg + 3 * k][dj][dn % 4][dn / 4] = dv[k];
                 }
-                if (pt < 216) {  // W_g[(a,i),(b,j)] = J_z w_g (c + delta_ab sigma_ij)
-#pragma unroll 7
-                    for (int g = pw; g < 27; g += 4) {
+                if (pt < 216) {  // W_g[(a,i),(b,j)] = J_z w_g (c + delta_ab sigma_ij), g = pw + 4k < 27
+                    double wv[7];
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) {
+                        const int g = min(pw + 4 * k, 26);
                         double v = R[wf1][g];
                         if (wdiag) v = v + R[wf2][g];
-                        s_W[p][g][wqq][wj][wi] = R[GP_JGW][g] * v;
+                        wv[k] = R[GP_JGW][g] * v;
                     }
-                    s_Pa[p][pt / 8][pt % 8] = R[GP_JGW][pt / 8] * s_psi[pt / 8][pt % 8];
+                    const double pa = R[GP_JGW][pt / 8] * s_psi[pt / 8][pt % 8];
+#pragma unroll
+                    for (int k = 0; k < 7; ++k)
+                        if (pw + 4 * k < 27) s_W[p][pw + 4 * k][wqq][wj][wi] = wv[k];
+                    s_Pa[p][pt / 8][pt % 8] = pa;
                 }
                 if (it + 1 < n_mine) {
 #pragma unroll
