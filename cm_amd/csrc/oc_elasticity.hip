@@ -41,13 +41,16 @@ mr_gauss_kernel(int n_elements, const int *__restrict__ conn_q, const int *__res
                 const double *__restrict__ coords, int nq, const double *__restrict__ dofs,
                 const double *__restrict__ mat, const BasisTables *__restrict__ T, double *__restrict__ rec)
 {
-    __shared__ double s_fq[27 * 27 * 4];
+    __shared__ double s_tab[kTabSize];
     __shared__ double s_X[kGaussElemsPerBlock][81];
     __shared__ double s_z[kGaussElemsPerBlock][81];
     __shared__ double s_p[kGaussElemsPerBlock][8];
     const int t = threadIdx.x;
     const int e0 = blockIdx.x * kGaussElemsPerBlock;
-    for (int k = t; k < 27 * 27 * 4; k += 256) s_fq[k] = T->quad27[k];
+    for (int k = t; k < 27 * 27 * 3; k += 256) {  // [ni][n][g] <- quad27[(g*27 + n)*4 + 1 + ni]
+        const int ni = k / 729, n = (k / 27) % 27, g = k % 27;
+        s_tab[(ni * 27 + n) * kTabStride + g] = T->quad27[(g * 27 + n) * 4 + 1 + ni];
+    }
     for (int k = t; k < kGaussElemsPerBlock * 81; k += 256) {
         const int le = k / 81, r = k % 81, c = r / 27, n = r % 27;
         const int e = e0 + le;
@@ -67,7 +70,7 @@ mr_gauss_kernel(int n_elements, const int *__restrict__ conn_q, const int *__res
     const int e = e0 + le;
     if (t >= kGaussElemsPerBlock * 27 || e >= n_elements) return;
     GaussKinematics k;
-    gauss_kinematics(s_fq + g * 27 * 4, s_z[le], s_X[le], k);
+    gauss_kinematics(s_tab, g, s_z[le], s_X[le], k);
     double P = 0.0;
     for (int n = 0; n < 8; ++n) P += T->lin27[(g * 8 + n) * 4] * s_p[le][n];
     const int64_t n_gp = static_cast<int64_t>(n_elements) * 27;
@@ -194,7 +197,7 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
                            const double *__restrict__ mat, const BasisTables *__restrict__ T,
                            double *__restrict__ relem, double delta, double *__restrict__ kfd)
 {
-    __shared__ double s_fq[27 * 27 * 4];
+    __shared__ double s_tab[kTabSize];
     __shared__ double s_X[kGaussElemsPerBlock][81];
     __shared__ double s_z[kGaussElemsPerBlock][81];
     __shared__ double s_p[kGaussElemsPerBlock][8];
@@ -203,7 +206,10 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
     __shared__ double s_rp[kGaussElemsPerBlock][27];
     const int t = threadIdx.x;
     const int e0 = blockIdx.x * kGaussElemsPerBlock;
-    for (int k = t; k < 27 * 27 * 4; k += 256) s_fq[k] = T->quad27[k];
+    for (int k = t; k < 27 * 27 * 3; k += 256) {  // [ni][n][g] <- quad27[(g*27 + n)*4 + 1 + ni]
+        const int ni = k / 729, n = (k / 27) % 27, g = k % 27;
+        s_tab[(ni * 27 + n) * kTabStride + g] = T->quad27[(g * 27 + n) * 4 + 1 + ni];
+    }
     // n_elements counts launch elements (element-column pairs with kFD)
     for (int k = t; k < kGaussElemsPerBlock * 81; k += 256) {
         const int le = k / 81, r = k % 81, c = r / 27, n = r % 27;
@@ -230,7 +236,7 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
         const int v = e0 + le, e = kFD ? v / 89 : v;
         if (t < kGaussElemsPerBlock * 27 && v < n_elements) {
             GaussKinematics k;
-            gauss_kinematics(s_fq + g * 27 * 4, s_z[le], s_X[le], k);
+            gauss_kinematics(s_tab, g, s_z[le], s_X[le], k);
             double P = 0.0;
             for (int n = 0; n < 8; ++n) P += T->lin27[(g * 8 + n) * 4] * s_p[le][n];
             double sigma[6];
@@ -255,12 +261,12 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
             const int m = q;
 #pragma unroll 1
             for (int g = 0; g < 27; ++g) {
-                const double *f = s_fq + (g * 27 + m) * 4;
+                const double f1 = s_tab[(0 * 27 + m) * kTabStride + g], f2 = s_tab[(1 * 27 + m) * kTabStride + g],
+                             f3 = s_tab[(2 * 27 + m) * kTabStride + g];
                 double d[3];
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
-                    d[i] = s_dx[le][0 + 3 * i][g] * f[1] + s_dx[le][1 + 3 * i][g] * f[2] +
-                           s_dx[le][2 + 3 * i][g] * f[3];
+                    d[i] = s_dx[le][0 + 3 * i][g] * f1 + s_dx[le][1 + 3 * i][g] * f2 + s_dx[le][2 + 3 * i][g] * f3;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     const double dot = d[0] * s_S[le][0 + 3 * a][g] + d[1] * s_S[le][1 + 3 * a][g] +

@@ -83,8 +83,15 @@ struct GaussKinematics {
     double dxidz[9];
 };
 
-// ze, Xe: element parameters [c*27 + n]; fns: quad27 + g*27*4.
-__device__ __forceinline__ void gauss_kinematics(const double *fns, const double *ze, const double *Xe,
+// Triquadratic first derivatives at the 27 Gauss points, transposed for LDS:
+// dphi_n/dxi_ni at g is tab[(ni * 27 + n) * kTabStride + g].  The odd stride
+// keeps both access patterns nearly free of bank conflicts: lanes that differ
+// in g (one thread per Gauss point) and lanes that differ in n.
+constexpr int kTabStride = 29;
+constexpr int kTabSize = 3 * 27 * kTabStride;
+
+// ze, Xe: element parameters [c*27 + n]; tab: the transposed derivative table.
+__device__ __forceinline__ void gauss_kinematics(const double *tab, int g, const double *ze, const double *Xe,
                                                  GaussKinematics &k)
 {
     double dz[9], dX[9], gu[9], dxidnu[9];
@@ -92,7 +99,7 @@ __device__ __forceinline__ void gauss_kinematics(const double *fns, const double
         for (int ni = 0; ni < 3; ++ni) {
             double sz = 0.0, sX = 0.0;
             for (int n = 0; n < 27; ++n) {
-                const double f = fns[n * 4 + 1 + ni];
+                const double f = tab[(ni * 27 + n) * kTabStride + g];
                 sz += f * ze[x * 27 + n];
                 sX += f * Xe[x * 27 + n];
             }
