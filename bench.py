@@ -31,8 +31,22 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
-# dense per-element G^T C G count, 27*(2*81*9*9 + 2*81*81*9) + 27*2*81*8
-MR_JACOBIAN_FLOP_PER_ELEMENT = 3577932
+# Algorithmic FLOP per element of the re-associated Jacobian contraction the
+# element-matrix kernels compute (K_ab = sum_g D_g^T W_ab,g D_g over the six
+# lower 27x27 blocks, diagonal blocks as triangles, plus the pressure rows):
+#   D_g (27*81*3 MAC) 13,122 + W (27*(54 mul + 27 add)) 2,187
+#   + U = D^T W (6 blocks * 27*27*3*3 MAC) 78,732
+#   + U D (3 * 27*27*81 + 3 * 378*81 MAC) 538,002
+#   + pressure rows (8*81*27 MAC + 216 mul) 35,208            = 667,251
+# (the reference element loop does about 0.68 MFLOP per element).
+MR_JACOBIAN_FLOP_PER_ELEMENT = 667251
+# dense per-element G^T C G count, 27*(2*81*9*9 + 2*81*81*9) + 27*2*81*8, of a
+# dense implementation of the same Jacobian (for comparison only)
+MR_JACOBIAN_DENSE_FLOP_PER_ELEMENT = 3577932
+# MFMA work issued per element: FP64 1659 mfma_f64_4x4x4 (512 FLOP each),
+# FP32 483 mfma_f32_16x16x4 (2048 FLOP each), padding included
+MR_JACOBIAN_ISSUED_FLOP_F64 = 1659 * 512
+MR_JACOBIAN_ISSUED_FLOP_F32 = 483 * 2048
 
 
 def deformed_state(eq, mesh, stretch=1.10, seed=17):
@@ -61,11 +75,13 @@ def traffic_from_profile(kernel):
         return json.load(f).get("bytes_per_launch")
 
 
-def time_steps(fn, steps, warmup, cm):
+def time_steps(fn, steps, warmup, cm, only=None):
+    """Times `steps` calls of fn between two events; kernel timers on (all
+    kernels, or only kernel `only`: one event pair per launch of it)."""
     for _ in range(warmup):
         fn()
     cm.synchronise()
-    cm.kernel_timing_enable(True)
+    cm.kernel_timing_enable(True, only=only)
     cm.event_record(0)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -143,7 +159,7 @@ def batched_line(cm, n_meshes, reps=3):
         gms, _ = time_steps(lambda: batch.jacobian_evaluate_graph(prec), reps, 1, cm)
         cm.kernel_timing_enable(False)
         peak = FP32_MFMA_PEAK_TFLOPS if name == "fp32" else FP64_PEAK_TFLOPS
-        tflops = MR_JACOBIAN_FLOP_PER_ELEMENT * E / (em / reps / 1e3) / 1e12
+        tflops = MR_JACOBIAN_FLOP_PER_ELEMENT * E / (em / reps / 1e3) / 1e12  # one launch per step
         out[name] = {"elements_per_s": E * reps / (ms / 1e3), "ms_per_residual_and_jacobian": ms / reps,
                      "graph_jacobian_ms": gms / reps, "kernel_ms": ks,
                      "element_matrix_TFLOPs": tflops, "element_matrix_frac": tflops / peak}
@@ -268,18 +284,24 @@ def main():
         print(json.dumps({"profile": args.profile, "steps": args.steps, "elements": mesh.n_elements}))
         return
 
-    ms, wall = time_steps(step, args.steps, args.warmup, cm)
+    # timed region: only the element-matrix kernel carries events (its launch
+    # durations give the roofline figure); the per-kernel breakdown comes from
+    # a second pass with every kernel timed
+    ms, wall = time_steps(step, args.steps, args.warmup, cm, only="mr_element_matrix_fp64")
+    jac_ms, jac_launches = cm.kernel_timing("mr_element_matrix_fp64")
+    cm.kernel_timing_enable(False)
+    time_steps(step, args.steps, 0, cm)
     kernels = {name: cm.kernel_timing(name) for name in
                ("mr_gauss_jacobian", "mr_element_matrix_fp64", "csr_row_gather_fp64", "mr_residual", "dof_gather")}
-    jac_ms, jac_launches = kernels["mr_element_matrix_fp64"]
     cm.kernel_timing_enable(False)
     E = mesh.n_elements
     value = E * args.steps / (ms / 1e3)
     per_launch_elems = E * args.steps / jac_launches
     avg_launch_s = jac_ms / jac_launches / 1e3
     achieved = MR_JACOBIAN_FLOP_PER_ELEMENT * per_launch_elems / avg_launch_s / 1e12
+    issued = MR_JACOBIAN_ISSUED_FLOP_F64 * per_launch_elems / avg_launch_s / 1e12
     roofline = {
-        "kernel": "mr_element_matrix_kernel<double>",
+        "kernel": "mr_element_matrix_f64_kernel",
         "bound": "mfma",
         "achieved": achieved,
         "peak": FP64_PEAK_TFLOPS,
@@ -289,6 +311,9 @@ def main():
         "avg_launch_ms": avg_launch_s * 1e3,
         "elements_per_launch": per_launch_elems,
         "flop_per_element": MR_JACOBIAN_FLOP_PER_ELEMENT,
+        "issued_mfma_TFLOPs": issued,
+        "issued_mfma_frac": issued / FP64_PEAK_TFLOPS,
+        "dense_equivalent_TFLOPs": MR_JACOBIAN_DENSE_FLOP_PER_ELEMENT * per_launch_elems / avg_launch_s / 1e12,
     }
 
     secondary = {
@@ -306,7 +331,9 @@ def main():
         secondary["mr_jacobian_fp32"] = {
             "error_vs_fp64_rel_frobenius": float(np.linalg.norm(v32 - v64) / np.linalg.norm(v64)),
             "elements_per_s": E * args.steps / (ms32 / 1e3),
-            "achieved_TFLOPs": MR_JACOBIAN_FLOP_PER_ELEMENT * E * args.steps / (j32 / 1e3) / 1e12,
+            # E elements per launch, l32 launches in j32 ms
+            "achieved_TFLOPs": MR_JACOBIAN_FLOP_PER_ELEMENT * E * l32 / (j32 / 1e3) / 1e12,
+            "issued_mfma_TFLOPs": MR_JACOBIAN_ISSUED_FLOP_F32 * E * l32 / (j32 / 1e3) / 1e12,
             "peak_TFLOPs": FP32_MFMA_PEAK_TFLOPS,
             "kernel_ms_per_step": {"mr_element_matrix_fp32": j32 / args.steps, "csr_row_gather_fp32": g32 / args.steps},
         }

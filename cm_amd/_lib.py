@@ -151,6 +151,7 @@ SIGNATURES = {
     "oc_multigrid_options_set": [_H, _c_int, _c_int, _c_double],
     "oc_jacobian_evaluate_graph": [_H, _c_int],
     "oc_kernel_timing_enable": [_c_int],
+    "oc_kernel_timing_select": [ctypes.c_char_p],
     "oc_kernel_timing_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
 }
 

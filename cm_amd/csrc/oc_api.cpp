@@ -37,6 +37,7 @@ struct PendingTiming {
 };
 struct TimingState {
     bool enabled = false;
+    std::string only;  // when set, only this kernel is timed
     std::vector<PendingTiming> pending;
     std::vector<hipEvent_t> pool;
     std::vector<std::pair<std::string, std::pair<double, long long>>> totals;
@@ -87,6 +88,7 @@ bool capturing() { return g_capturing; }
 KernelTimer::KernelTimer(const char *name) : name_(name)
 {
     if (!g_timing.enabled || g_capturing) return;
+    if (!g_timing.only.empty() && g_timing.only != name) return;
     start_ = timing_event();
     if (start_) (void)hipEventRecord(start_, g_runtime.stream);
 }
@@ -573,6 +575,14 @@ int oc_kernel_timing_enable(int enable)
     OC_CHECK(timing_resolve());
     g_timing.totals.clear();
     g_timing.enabled = enable != 0;
+    return OC_NO_ERROR;
+}
+
+int oc_kernel_timing_select(const char *kernel_name)
+{
+    OC_CHECK(require_runtime());
+    OC_CHECK(timing_resolve());
+    g_timing.only = kernel_name ? kernel_name : "";
     return OC_NO_ERROR;
 }
 

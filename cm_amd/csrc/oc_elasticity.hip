@@ -45,26 +45,51 @@ mr_gauss_kernel(int n_elements, const int *__restrict__ conn_q, const int *__res
     __shared__ double s_X[kGaussElemsPerBlock][81];
     __shared__ double s_z[kGaussElemsPerBlock][81];
     __shared__ double s_p[kGaussElemsPerBlock][8];
+    constexpr int kTabPer = (27 * 27 * 3 + 255) / 256;
+    constexpr int kNodePer = (kGaussElemsPerBlock * 81 + 255) / 256;
     const int t = threadIdx.x;
     const int e0 = blockIdx.x * kGaussElemsPerBlock;
-    for (int k = t; k < 27 * 27 * 3; k += 256) {  // [ni][n][g] <- quad27[(g*27 + n)*4 + 1 + ni]
+    // all global loads are issued before the LDS writes: the table and the
+    // connectivity, then the nodal values they index
+    double tab[kTabPer];
+#pragma unroll
+    for (int i = 0; i < kTabPer; ++i) {  // [ni][n][g] <- quad27[(g*27 + n)*4 + 1 + ni]
+        const int k = t + 256 * i;
         const int ni = k / 729, n = (k / 27) % 27, g = k % 27;
-        s_tab[(ni * 27 + n) * kTabStride + g] = T->quad27[(g * 27 + n) * 4 + 1 + ni];
+        tab[i] = k < 27 * 27 * 3 ? T->quad27[(g * 27 + n) * 4 + 1 + ni] : 0.0;
     }
-    for (int k = t; k < kGaussElemsPerBlock * 81; k += 256) {
-        const int le = k / 81, r = k % 81, c = r / 27, n = r % 27;
-        const int e = e0 + le;
-        if (e < n_elements) {
-            const int node = conn_q[e * 27 + n];
-            s_X[le][r] = coords[static_cast<int64_t>(c) * nq + node];
-            s_z[le][r] = dofs[static_cast<int64_t>(c) * nq + node];
+    int node[kNodePer];
+#pragma unroll
+    for (int i = 0; i < kNodePer; ++i) {
+        const int k = t + 256 * i;
+        const int e = e0 + k / 81;
+        node[i] = (k < kGaussElemsPerBlock * 81 && e < n_elements) ? conn_q[e * 27 + k % 27] : -1;
+    }
+    const int pe = e0 + t / 8;
+    const int pnode = (t < kGaussElemsPerBlock * 8 && pe < n_elements) ? conn_l[pe * 8 + t % 8] : -1;
+    double xv[kNodePer], zv[kNodePer];
+#pragma unroll
+    for (int i = 0; i < kNodePer; ++i) {
+        const int k = t + 256 * i, c = (k % 81) / 27;
+        xv[i] = node[i] >= 0 ? coords[static_cast<int64_t>(c) * nq + node[i]] : 0.0;
+        zv[i] = node[i] >= 0 ? dofs[static_cast<int64_t>(c) * nq + node[i]] : 0.0;
+    }
+    const double pval = pnode >= 0 ? dofs[static_cast<int64_t>(3) * nq + pnode] : 0.0;
+#pragma unroll
+    for (int i = 0; i < kTabPer; ++i) {
+        const int k = t + 256 * i;
+        const int ni = k / 729, n = (k / 27) % 27, g = k % 27;
+        if (k < 27 * 27 * 3) s_tab[(ni * 27 + n) * kTabStride + g] = tab[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kNodePer; ++i) {
+        const int k = t + 256 * i;
+        if (node[i] >= 0) {
+            s_X[k / 81][k % 81] = xv[i];
+            s_z[k / 81][k % 81] = zv[i];
         }
     }
-    for (int k = t; k < kGaussElemsPerBlock * 8; k += 256) {
-        const int le = k / 8, n = k % 8;
-        const int e = e0 + le;
-        if (e < n_elements) s_p[le][n] = dofs[static_cast<int64_t>(3) * nq + conn_l[e * 8 + n]];
-    }
+    if (pnode >= 0) s_p[t / 8][t % 8] = pval;
     __syncthreads();
     const int le = t / 27, g = t % 27;
     const int e = e0 + le;
@@ -190,8 +215,12 @@ int launch_mr_gauss(EquationsSet &s)
 // column col of e's element matrix (R(x + delta e_col) - R(x)) / delta, R(x)
 // from relem (EquationsSet_FiniteElementJacobianEvaluateFD,
 // equations_set_routines.f90:3032-3150).
+// 320 threads: 243 evaluate the Gauss points, then each of the 9 x 35
+// residual work items has a thread of its own.
+constexpr int kResThreads = 320;
+
 template <bool kFD>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kResThreads)
 mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const int *__restrict__ conn_l,
                            const double *__restrict__ coords, int nq, const double *__restrict__ dofs,
                            const double *__restrict__ mat, const BasisTables *__restrict__ T,
@@ -204,32 +233,53 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
     __shared__ double s_dx[kGaussElemsPerBlock][9][27];
     __shared__ double s_S[kGaussElemsPerBlock][9][27];  // JGW * sigma(i, a): [i + 3a][g]
     __shared__ double s_rp[kGaussElemsPerBlock][27];
+    constexpr int kTabPer = (27 * 27 * 3 + kResThreads - 1) / kResThreads;
+    constexpr int kNodePer = (kGaussElemsPerBlock * 81 + kResThreads - 1) / kResThreads;
     const int t = threadIdx.x;
     const int e0 = blockIdx.x * kGaussElemsPerBlock;
-    for (int k = t; k < 27 * 27 * 3; k += 256) {  // [ni][n][g] <- quad27[(g*27 + n)*4 + 1 + ni]
+    // all global loads are issued before the LDS writes: the table and the
+    // connectivity, then the nodal values they index
+    double tab[kTabPer];
+#pragma unroll
+    for (int i = 0; i < kTabPer; ++i) {  // [ni][n][g] <- quad27[(g*27 + n)*4 + 1 + ni]
+        const int k = t + kResThreads * i;
         const int ni = k / 729, n = (k / 27) % 27, g = k % 27;
-        s_tab[(ni * 27 + n) * kTabStride + g] = T->quad27[(g * 27 + n) * 4 + 1 + ni];
+        tab[i] = k < 27 * 27 * 3 ? T->quad27[(g * 27 + n) * 4 + 1 + ni] : 0.0;
     }
     // n_elements counts launch elements (element-column pairs with kFD)
-    for (int k = t; k < kGaussElemsPerBlock * 81; k += 256) {
-        const int le = k / 81, r = k % 81, c = r / 27, n = r % 27;
-        const int v = e0 + le;
-        if (v < n_elements) {
-            const int e = kFD ? v / 89 : v;
-            const int node = conn_q[e * 27 + n];
-            s_X[le][r] = coords[static_cast<int64_t>(c) * nq + node];
-            s_z[le][r] = dofs[static_cast<int64_t>(c) * nq + node] + ((kFD && v % 89 == r) ? delta : 0.0);
+    int node[kNodePer];
+#pragma unroll
+    for (int i = 0; i < kNodePer; ++i) {
+        const int k = t + kResThreads * i;
+        const int le = k / 81, n = k % 27, v = e0 + le;
+        node[i] = (k < kGaussElemsPerBlock * 81 && v < n_elements) ? conn_q[(kFD ? v / 89 : v) * 27 + n] : -1;
+    }
+    const int pv = e0 + t / 8;
+    const int pnode = (t < kGaussElemsPerBlock * 8 && pv < n_elements) ? conn_l[(kFD ? pv / 89 : pv) * 8 + t % 8] : -1;
+    double xv[kNodePer], zv[kNodePer];
+#pragma unroll
+    for (int i = 0; i < kNodePer; ++i) {
+        const int k = t + kResThreads * i, c = (k % 81) / 27;
+        xv[i] = node[i] >= 0 ? coords[static_cast<int64_t>(c) * nq + node[i]] : 0.0;
+        zv[i] = node[i] >= 0 ? dofs[static_cast<int64_t>(c) * nq + node[i]] : 0.0;
+    }
+    const double pval = pnode >= 0 ? dofs[static_cast<int64_t>(3) * nq + pnode] : 0.0;
+#pragma unroll
+    for (int i = 0; i < kTabPer; ++i) {
+        const int k = t + kResThreads * i;
+        const int ni = k / 729, n = (k / 27) % 27, g = k % 27;
+        if (k < 27 * 27 * 3) s_tab[(ni * 27 + n) * kTabStride + g] = tab[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kNodePer; ++i) {
+        const int k = t + kResThreads * i;
+        const int le = k / 81, r = k % 81;
+        if (node[i] >= 0) {
+            s_X[le][r] = xv[i];
+            s_z[le][r] = zv[i] + ((kFD && (e0 + le) % 89 == r) ? delta : 0.0);
         }
     }
-    for (int k = t; k < kGaussElemsPerBlock * 8; k += 256) {
-        const int le = k / 8, n = k % 8;
-        const int v = e0 + le;
-        if (v < n_elements) {
-            const int e = kFD ? v / 89 : v;
-            s_p[le][n] = dofs[static_cast<int64_t>(3) * nq + conn_l[e * 8 + n]] +
-                         ((kFD && v % 89 == 81 + n) ? delta : 0.0);
-        }
-    }
+    if (pnode >= 0) s_p[t / 8][t % 8] = pval + ((kFD && pv % 89 == 81 + t % 8) ? delta : 0.0);
     __syncthreads();
     {
         const int le = t / 27, g = t % 27;
@@ -252,7 +302,7 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
     // one work item per (element, displacement node m): the DPHIDZ of node m
     // is formed once per Gauss point for its three component rows; then one
     // item per pressure row
-    for (int w = t; w < kGaussElemsPerBlock * 35; w += 256) {
+    for (int w = t; w < kGaussElemsPerBlock * 35; w += kResThreads) {
         const int le = w / 35, q = w % 35;
         const int v = e0 + le;
         if (v >= n_elements) continue;
@@ -313,7 +363,7 @@ int launch_mr_residual(EquationsSet &s)
     const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
     {
         KernelTimer timer("mr_residual");
-        mr_residual_element_kernel<false><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
+        mr_residual_element_kernel<false><<<blocks, kResThreads, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
                                                                          s.coords.ptr, s.n_nodes, s.dofs.ptr,
                                                                          s.mat.ptr, T, s.relem.ptr, 0.0, nullptr);
         OC_HIP_CHECK(hipGetLastError());
@@ -1014,7 +1064,7 @@ int launch_mr_jacobian_fd(EquationsSet &s)
     for (double v : x) ss += v * v;
     const double delta = (1.0 + std::sqrt(ss)) * 1e-7;
     const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
-    mr_residual_element_kernel<false><<<blocks, 256, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
+    mr_residual_element_kernel<false><<<blocks, kResThreads, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
                                                                      s.coords.ptr, s.n_nodes, s.dofs.ptr, s.mat.ptr,
                                                                      T, s.relem.ptr, 0.0, nullptr);
     OC_HIP_CHECK(hipGetLastError());
@@ -1023,7 +1073,7 @@ int launch_mr_jacobian_fd(EquationsSet &s)
     {
         KernelTimer timer("mr_fd_columns");
         const unsigned fb = static_cast<unsigned>((nv + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
-        mr_residual_element_kernel<true><<<fb, 256, 0, rt.stream>>>(static_cast<int>(nv), s.conn.ptr, s.conn_l.ptr,
+        mr_residual_element_kernel<true><<<fb, kResThreads, 0, rt.stream>>>(static_cast<int>(nv), s.conn.ptr, s.conn_l.ptr,
                                                                     s.coords.ptr, s.n_nodes, s.dofs.ptr, s.mat.ptr, T,
                                                                     s.relem.ptr, delta, s.kbuf64.ptr);
         OC_HIP_CHECK(hipGetLastError());

@@ -95,17 +95,27 @@ __device__ __forceinline__ void gauss_kinematics(const double *tab, int g, const
                                                  GaussKinematics &k)
 {
     double dz[9], dX[9], gu[9], dxidnu[9];
-    for (int x = 0; x < 3; ++x)
-        for (int ni = 0; ni < 3; ++ni) {
-            double sz = 0.0, sX = 0.0;
-            for (int n = 0; n < 27; ++n) {
-                const double f = tab[(ni * 27 + n) * kTabStride + g];
-                sz += f * ze[x * 27 + n];
-                sX += f * Xe[x * 27 + n];
+    // one derivative direction at a time, the three components together (each
+    // sum still runs over n in ascending order); the loops are kept rolled so
+    // that the unrolled loads do not take hundreds of registers
+#pragma unroll 1
+    for (int ni = 0; ni < 3; ++ni) {
+        double sz[3] = {0.0, 0.0, 0.0}, sX[3] = {0.0, 0.0, 0.0};
+#pragma unroll 3
+        for (int n = 0; n < 27; ++n) {
+            const double f = tab[(ni * 27 + n) * kTabStride + g];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                sz[x] += f * ze[x * 27 + n];
+                sX[x] += f * Xe[x * 27 + n];
             }
-            dz[x + 3 * ni] = sz;
-            dX[x + 3 * ni] = sX;
         }
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            dz[x + 3 * ni] = sz[x];
+            dX[x + 3 * ni] = sX[x];
+        }
+    }
     k.jz = metric_jacobian(dz, gu);
     k.jx = metric_jacobian(dX, gu);
     invert3(dz, k.dxidz);

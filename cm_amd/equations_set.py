@@ -46,8 +46,10 @@ def event_elapsed(start, end):
     return float(ms.value)
 
 
-def kernel_timing_enable(enable=True):
-    """Bracket every library kernel launch with events (clears totals)."""
+def kernel_timing_enable(enable=True, only=None):
+    """Bracket library kernel launches with events (clears totals): every
+    kernel, or only the launches of kernel name `only`."""
+    call("oc_kernel_timing_select", only.encode() if only else None)
     call("oc_kernel_timing_enable", 1 if enable else 0)
 
 

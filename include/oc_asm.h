@@ -308,9 +308,14 @@ int oc_synchronise(void);
  * 1332-1336,1366,1414) ------------------------------------------------------ */
 /* When enabled, every kernel launch is bracketed by events on the library
  * stream and its duration is accumulated under its kernel name
- * ("laplace_assemble", "mr_residual", "mr_jacobian_fp64", "mr_jacobian_fp32",
- * "fill", ...).  Enabling or resetting clears the accumulators. */
+ * ("laplace_element", "laplace_row_gather", "mr_residual", "dof_gather",
+ * "mr_gauss_jacobian", "mr_element_matrix_fp64", "mr_element_matrix_fp32",
+ * "csr_row_gather_fp64", "csr_row_gather_fp32", "fill", ...).  Enabling or
+ * resetting clears the accumulators. */
 int oc_kernel_timing_enable(int enable);
+/* Restricts the timing to the launches of one kernel name (NULL: all), so
+ * that a timed region carries one event pair per launch of that kernel. */
+int oc_kernel_timing_select(const char *kernel_name);
 /* Synchronises, then returns the total milliseconds and launch count of
  * `kernel_name` since the last enable/reset. */
 int oc_kernel_timing_get(const char *kernel_name, double *total_ms, long long *launches);
