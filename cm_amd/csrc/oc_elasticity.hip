@@ -120,31 +120,69 @@ __device__ __forceinline__ int element_dof(const int *__restrict__ conn_q, const
     return 3 * nq + conn_l[e * 8 + (i - 81)];
 }
 
-// Scatter map: offset of element column j inside CSR row dof_i.
-__global__ void __launch_bounds__(128)
-build_slot_offsets_kernel(int n_elements, const int *__restrict__ conn_q, const int *__restrict__ conn_l, int nq,
-                          const int64_t *__restrict__ row_ptr, const int *__restrict__ col_idx,
-                          uint16_t *__restrict__ offs, int *__restrict__ n_missing)
+// Scatter map.  A CSR row of node n (any of its three displacement rows, or
+// a pressure row) holds, in ascending DOF order, the component-0 block of its
+// nb displacement-node neighbours, the component-1 and component-2 blocks of
+// the same nodes, then its pressure-node neighbours.  Element column j of
+// row dof_i therefore sits at
+//   (j / 27) * nb + rank[e][k][j % 27]           (j < 81)
+//   3 * nb + rank[e][k][27 + j - 81]             (j >= 81),
+// where k is the element node of row i (0..26 displacement, 27..34 pressure)
+// and rank[e][k][m] is the position of element node m among the neighbours of
+// node k: 35 x 35 bytes per element instead of 89 x 89 offsets, shared by the
+// three displacement rows of a node.  nb is stored per row.
+constexpr int kRankNodes = 35;
+
+// nb of every row: the number of its columns below nq (component-0 block)
+__global__ void __launch_bounds__(256)
+row_neighbours_kernel(int n_rows, int nq, const int64_t *__restrict__ row_ptr, const int *__restrict__ col_idx,
+                      uint16_t *__restrict__ nb)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    int64_t lo = row_ptr[r], hi = row_ptr[r + 1];  // first column >= nq
+    const int64_t start = lo;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (col_idx[mid] < nq) lo = mid + 1;
+        else hi = mid;
+    }
+    nb[r] = static_cast<uint16_t>(lo - start);
+}
+
+// rank[e][k][m], checked by bisection against every component pair it stands for
+__global__ void __launch_bounds__(256)
+column_rank_kernel(int n_elements, const int *__restrict__ conn_q, const int *__restrict__ conn_l, int nq,
+                   const int64_t *__restrict__ row_ptr, const int *__restrict__ col_idx,
+                   const uint16_t *__restrict__ nb, uint8_t *__restrict__ rank, int *__restrict__ n_bad)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= static_cast<int64_t>(n_elements) * 89) return;
-    const int e = static_cast<int>(gid / 89), i = static_cast<int>(gid % 89);
-    const int row = element_dof(conn_q, conn_l, nq, e, i);
-    const int64_t base = row_ptr[row];
-    for (int j = 0; j < 89; ++j) {
-        const int64_t slot = csr_slot(row_ptr, col_idx, row, element_dof(conn_q, conn_l, nq, e, j));
-        if (slot < 0) {
-            atomicAdd(n_missing, 1);
-            offs[gid * 89 + j] = 0;
-        } else {
-            offs[gid * 89 + j] = static_cast<uint16_t>(slot - base);
+    if (gid >= static_cast<int64_t>(n_elements) * kRankNodes * kRankNodes) return;
+    const int e = static_cast<int>(gid / (kRankNodes * kRankNodes));
+    const int k = static_cast<int>((gid / kRankNodes) % kRankNodes), m = static_cast<int>(gid % kRankNodes);
+    const bool qrow = k < 27, qcol = m < 27;
+    const int rnode = qrow ? conn_q[e * 27 + k] : conn_l[e * 8 + k - 27];
+    const int cnode = qcol ? conn_q[e * 27 + m] : conn_l[e * 8 + m - 27];
+    const int row0 = qrow ? rnode : 3 * nq + rnode;
+    const int col0 = qcol ? cnode : 3 * nq + cnode;
+    const int64_t slot0 = csr_slot(row_ptr, col_idx, row0, col0);
+    const int64_t rk = slot0 - row_ptr[row0] - (qcol ? 0 : 3 * static_cast<int64_t>(nb[row0]));
+    bool ok = slot0 >= 0 && rk >= 0 && rk < 256;
+    for (int c = 0; c < (qrow ? 3 : 1) && ok; ++c) {
+        const int row = qrow ? c * nq + rnode : row0;
+        for (int cc = 0; cc < (qcol ? 3 : 1) && ok; ++cc) {
+            const int col = qcol ? cc * nq + cnode : col0;
+            const int64_t want = row_ptr[row] + (qcol ? cc : 3) * static_cast<int64_t>(nb[row]) + rk;
+            ok = csr_slot(row_ptr, col_idx, row, col) == want;
         }
     }
+    if (!ok) atomicAdd(n_bad, 1);
+    rank[gid] = ok ? static_cast<uint8_t>(rk) : 0;
 }
 
 int mr_prepare(EquationsSet &s)
 {
-    if (s.slot_offsets.ptr) return OC_NO_ERROR;
+    if (s.col_rank.ptr) return OC_NO_ERROR;
     int64_t max_row = 0;
     for (int r = 0; r < s.n_rows; ++r) max_row = std::max(max_row, s.h_row_ptr[r + 1] - s.h_row_ptr[r]);
     if (max_row > 8192)
@@ -153,20 +191,29 @@ int mr_prepare(EquationsSet &s)
     if (static_cast<int64_t>(s.n_elements) * 89 >= INT32_MAX)
         return set_error(OC_ERR_UNSUPPORTED, "%d elements exceed the int32 element-entry gather lists", s.n_elements);
     s.max_row_len = static_cast<int>(max_row);
-    OC_CHECK(s.slot_offsets.alloc(static_cast<size_t>(s.n_elements) * 89 * 89));
-    DevBuf<int> missing;
-    OC_CHECK(missing.alloc(1));
     Runtime &rt = runtime();
-    OC_HIP_CHECK(hipMemsetAsync(missing.ptr, 0, sizeof(int), rt.stream));
-    const int64_t nthreads = static_cast<int64_t>(s.n_elements) * 89;
-    build_slot_offsets_kernel<<<static_cast<unsigned>((nthreads + 127) / 128), 128, 0, rt.stream>>>(
-        s.n_elements, s.conn.ptr, s.conn_l.ptr, s.n_nodes, s.row_ptr.ptr, s.col_idx.ptr, s.slot_offsets.ptr,
-        missing.ptr);
+    OC_CHECK(s.row_nb.alloc(static_cast<size_t>(s.n_rows)));
+    row_neighbours_kernel<<<static_cast<unsigned>((s.n_rows + 255) / 256), 256, 0, rt.stream>>>(
+        s.n_rows, s.n_nodes, s.row_ptr.ptr, s.col_idx.ptr, s.row_nb.ptr);
     OC_HIP_CHECK(hipGetLastError());
-    int h_missing = 0;
-    OC_HIP_CHECK(hipMemcpyAsync(&h_missing, missing.ptr, sizeof(int), hipMemcpyDeviceToHost, rt.stream));
+    const int64_t nranks = static_cast<int64_t>(s.n_elements) * kRankNodes * kRankNodes;
+    OC_CHECK(s.col_rank.alloc(static_cast<size_t>(nranks)));
+    DevBuf<int> bad;
+    OC_CHECK(bad.alloc(1));
+    OC_HIP_CHECK(hipMemsetAsync(bad.ptr, 0, sizeof(int), rt.stream));
+    column_rank_kernel<<<static_cast<unsigned>((nranks + 255) / 256), 256, 0, rt.stream>>>(
+        s.n_elements, s.conn.ptr, s.conn_l.ptr, s.n_nodes, s.row_ptr.ptr, s.col_idx.ptr, s.row_nb.ptr,
+        s.col_rank.ptr, bad.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    int h_bad = 0;
+    OC_HIP_CHECK(hipMemcpyAsync(&h_bad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, rt.stream));
     OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
-    if (h_missing) return set_error(OC_ERR_PATTERN, "%d element entries missing from the CSR structure", h_missing);
+    if (h_bad) {
+        s.col_rank.release();
+        return set_error(OC_ERR_PATTERN,
+                         "%d element entries missing from the CSR structure, not in its component-block layout or past 255 neighbour nodes",
+                         h_bad);
+    }
     OC_CHECK(s.gp_records.alloc(static_cast<size_t>(s.n_elements) * 27 * GP_NFIELDS));
     OC_CHECK(s.relem.alloc(static_cast<size_t>(s.n_elements) * 89));
     // DOF -> element entries, ascending element order
@@ -934,7 +981,8 @@ template <typename Real, int kChunk>
 __global__ void __launch_bounds__(64)
 csr_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const int *__restrict__ gptr,
                       const int *__restrict__ gidx, const Real *__restrict__ kbuf,
-                      const uint16_t *__restrict__ offs, Real *__restrict__ values)
+                      const uint8_t *__restrict__ rank, const uint16_t *__restrict__ row_nb,
+                      Real *__restrict__ values)
 {
     extern __shared__ unsigned char s_raw[];
     Real *s_row = reinterpret_cast<Real *>(s_raw);
@@ -942,11 +990,16 @@ csr_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const int
     if (r >= n_rows) return;
     const int64_t r0 = row_ptr[r];
     const int len = static_cast<int>(row_ptr[r + 1] - r0);
+    const int nb = row_nb[r];
     for (int k = lane; k < len; k += 64) s_row[k] = Real(0);
     __syncthreads();
+    // lane j holds element columns j and j + 64: component block and rank slot
+    const int blk0 = (lane / 27) * nb, m0 = lane % 27;
+    const bool hi = lane + 64 < 89;
+    const int j1 = lane + 64;
+    const int blk1 = j1 < 81 ? (j1 / 27) * nb : 3 * nb, m1 = j1 < 81 ? j1 % 27 : 27 + j1 - 81;
     // contributions in chunks of kChunk element rows: all global loads of a
-    // chunk are issued before its LDS accumulation (lane j holds entries j and
-    // j + 64)
+    // chunk are issued before its LDS accumulation
     for (int c0 = gptr[r]; c0 < gptr[r + 1]; c0 += kChunk) {
         const int nc = min(kChunk, gptr[r + 1] - c0);
         Real v0[kChunk], v1[kChunk];
@@ -954,12 +1007,15 @@ csr_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const int
 #pragma unroll
         for (int c = 0; c < kChunk; ++c) {
             if (c < nc) {
-                const int64_t base = static_cast<int64_t>(gidx[c0 + c]) * 89;  // (e*89 + i) * 89
+                const int ei = gidx[c0 + c];  // e*89 + i
+                const int e = ei / 89, i = ei % 89;
+                const int64_t base = static_cast<int64_t>(ei) * 89;
+                const uint8_t *rk = rank + (static_cast<int64_t>(e) * kRankNodes + (i < 81 ? i % 27 : i - 54)) *
+                                               kRankNodes;
                 v0[c] = kbuf[base + lane];
-                o0[c] = offs[base + lane];
-                const bool hi = lane + 64 < 89;
-                v1[c] = hi ? kbuf[base + lane + 64] : Real(0);
-                o1[c] = hi ? offs[base + lane + 64] : -1;
+                o0[c] = blk0 + rk[m0];
+                v1[c] = hi ? kbuf[base + j1] : Real(0);
+                o1[c] = hi ? blk1 + rk[m1] : -1;
             }
         }
 #pragma unroll
@@ -983,17 +1039,17 @@ static int launch_row_gather(EquationsSet &s, const Real *kbuf, Real *values, in
     switch (chunk) {
     case 4:
         csr_row_gather_kernel<Real, 4><<<grid, 64, row_lds, rt.stream>>>(s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr,
-                                                                        s.gather_idx.ptr, kbuf, s.slot_offsets.ptr,
+                                                                        s.gather_idx.ptr, kbuf, s.col_rank.ptr, s.row_nb.ptr,
                                                                         values);
         break;
     case 16:
         csr_row_gather_kernel<Real, 16><<<grid, 64, row_lds, rt.stream>>>(s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr,
-                                                                         s.gather_idx.ptr, kbuf, s.slot_offsets.ptr,
+                                                                         s.gather_idx.ptr, kbuf, s.col_rank.ptr, s.row_nb.ptr,
                                                                          values);
         break;
     default:
         csr_row_gather_kernel<Real, 8><<<grid, 64, row_lds, rt.stream>>>(s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr,
-                                                                        s.gather_idx.ptr, kbuf, s.slot_offsets.ptr,
+                                                                        s.gather_idx.ptr, kbuf, s.col_rank.ptr, s.row_nb.ptr,
                                                                         values);
         break;
     }

@@ -170,10 +170,13 @@ struct EquationsSet {
     DevBuf<double> dofs, residual, rhs;
 
     // Mooney-Rivlin: per-Gauss-point records (field-major, see
-    // oc_elasticity.hip) and the CSR scatter map: for element e, element row
-    // i and element column j, slot = row_ptr[dof_i] + slot_offsets[(e*89+i)*89+j].
+    // oc_elasticity.hip) and the CSR scatter map: element column j of row
+    // dof_i of element e sits at row_ptr[dof_i] + (j / 27) * row_nb[dof_i] +
+    // col_rank[(e*35 + k)*35 + j % 27] for j < 81 (3 * row_nb + rank of
+    // pressure node j - 81 otherwise), k the element node of row i.
     DevBuf<double> gp_records;
-    DevBuf<uint16_t> slot_offsets;
+    DevBuf<uint8_t> col_rank;
+    DevBuf<uint16_t> row_nb;
     DevBuf<double> kbuf64;      // element matrices, E*89*89, row-major
     DevBuf<float> kbuf32;
     int max_row_len = 0;
