@@ -33,6 +33,12 @@ def test_python_binding_covers_header():
     assert set(declared_entry_points()) <= set(_lib.SIGNATURES)
 
 
+def test_fortran_module_covers_header():
+    text = open(os.path.join(ROOT, "include", "oc_asm.f90")).read()
+    bound = set(re.findall(r'BIND\(C,\s*NAME="(oc_\w+)"\)', text))
+    assert set(declared_entry_points()) <= bound, sorted(set(declared_entry_points()) - bound)
+
+
 def test_error_paths_without_device():
     lib = _lib.load()
     assert lib.oc_equations_set_info(None, None, None, None, None, None) == _lib.OC_ERR_INVALID_HANDLE
