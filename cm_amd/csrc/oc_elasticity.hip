@@ -111,15 +111,6 @@ mr_gauss_kernel(int n_elements, const int *__restrict__ conn_q, const int *__res
     for (int i = 0; i < 6; ++i) rec[(GP_SIGJ + i) * n_gp + q] = sigma[i];
 }
 
-// Element DOFs (component -> node order of the reference element matrix,
-// equations_matrices_routines.f90:684-941): 0..80 displacement, 81..88 p.
-__device__ __forceinline__ int element_dof(const int *__restrict__ conn_q, const int *__restrict__ conn_l, int nq,
-                                           int e, int i)
-{
-    if (i < 81) return (i / 27) * nq + conn_q[e * 27 + (i % 27)];
-    return 3 * nq + conn_l[e * 8 + (i - 81)];
-}
-
 // Scatter map.  A CSR row of node n (any of its three displacement rows, or
 // a pressure row) holds, in ascending DOF order, the component-0 block of its
 // nb displacement-node neighbours, the component-1 and component-2 blocks of
@@ -221,6 +212,9 @@ int mr_prepare(EquationsSet &s)
     OC_HIP_CHECK(hipMemcpy(hconn.data(), s.conn.ptr, hconn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     OC_HIP_CHECK(hipMemcpy(hconn_l.data(), s.conn_l.ptr, hconn_l.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     std::vector<int32_t> ptr(static_cast<size_t>(s.n_rows) + 1, 0), idx(static_cast<size_t>(s.n_elements) * 89);
+    // element DOF i of element e (component -> node order of the reference
+    // element matrix, equations_matrices_routines.f90:684-941): 0..80
+    // displacement, 81..88 pressure
     auto dof_of = [&](int64_t e, int i) {
         return i < 81 ? (i / 27) * s.n_nodes + hconn[e * 27 + i % 27] : 3 * s.n_nodes + hconn_l[e * 8 + i - 81];
     };
