@@ -90,6 +90,52 @@ def test_mooney_rivlin_parity(gpu, n, jitter):
     assert rel(eq.matrix_values_sp().astype(np.float64), oJ) < FP32_TOL
 
 
+def _permuted(cm, mesh, seed):
+    """The same mesh with its node numbering (both components) shuffled."""
+    rng = np.random.default_rng(seed)
+    pq = rng.permutation(mesh.n_nodes)
+    coords = np.empty_like(mesh.coords)
+    coords[:, pq] = mesh.coords
+    conn_l = None
+    if mesh.elem_nodes_l is not None:
+        pl = rng.permutation(mesh.n_nodes_l)
+        conn_l = pl[mesh.elem_nodes_l].astype(np.int32)
+    return cm.RegularMesh(mesh.order, mesh.n, pq[mesh.elem_nodes].astype(np.int32), coords, conn_l)
+
+
+def test_laplace_parity_shuffled_numbering(gpu):
+    """Scatter maps and gathers must not rely on the generated node order."""
+    cm = gpu
+    mesh = _permuted(cm, cm.RegularMesh.generate(1, (5, 4, 3), jitter=0.1, seed=4), 17)
+    eq = cm.LaplaceEquationsSet(mesh)
+    eq.assemble()
+    rp, ci = eq.matrix_structure(0)
+    orp, oci = _pattern_oracle_laplace(mesh)
+    assert np.array_equal(rp.astype(np.int64), orp) and np.array_equal(ci, oci)
+    ovals, _ = orc.laplace_assemble(mesh.elem_nodes, mesh.coords, orp, oci)
+    assert rel(eq.matrix_values(), ovals) < FP64_TOL
+
+
+def test_mooney_rivlin_parity_shuffled_numbering(gpu):
+    """The neighbour-rank scatter map on an arbitrary node numbering."""
+    cm = gpu
+    mesh = _permuted(cm, cm.RegularMesh.generate(2, (3, 2, 4), jitter=0.1, seed=6), 23)
+    c10, c01 = 3.0, 1.5
+    eq = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
+    rp, ci = eq.matrix_structure(0)
+    orp, oci = _pattern_oracle_mr(mesh)
+    assert np.array_equal(rp.astype(np.int64), orp) and np.array_equal(ci, oci)
+    d = _deformed_state(mesh, 0.1, 8)
+    eq.set_dofs(d)
+    eq.residual_evaluate()
+    assert rel(eq.residual(), orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, c10, c01)) < FP64_TOL
+    oJ = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, c10, c01, orp, oci)
+    eq.jacobian_evaluate(cm.OC_FP64)
+    assert rel(eq.matrix_values(), oJ) < FP64_TOL
+    eq.jacobian_evaluate(cm.OC_FP32)
+    assert rel(eq.matrix_values_sp().astype(np.float64), oJ) < FP32_TOL
+
+
 def test_mooney_rivlin_16_config(gpu):
     """Config 2 size: 16^3 triquadratic/trilinear, jitter 0.1h, c10=2, c01=6."""
     cm = gpu
