@@ -267,13 +267,19 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
                            const double *__restrict__ mat, const BasisTables *__restrict__ T,
                            double *__restrict__ relem, double delta, double *__restrict__ kfd)
 {
+    // The nodal values (X, z) are only read by the Gauss-point phase and the
+    // Gauss-point results (DXI_DX, JGW sigma, the pressure weight) only
+    // written after it, so they share LDS (the block stays under 64 KB).
+    constexpr int kNodal = 2 * kGaussElemsPerBlock * 81;
+    constexpr int kResults = 2 * kGaussElemsPerBlock * 9 * 27 + kGaussElemsPerBlock * 27;
     __shared__ double s_tab[kTabSize];
-    __shared__ double s_X[kGaussElemsPerBlock][81];
-    __shared__ double s_z[kGaussElemsPerBlock][81];
     __shared__ double s_p[kGaussElemsPerBlock][8];
-    __shared__ double s_dx[kGaussElemsPerBlock][9][27];
-    __shared__ double s_S[kGaussElemsPerBlock][9][27];  // JGW * sigma(i, a): [i + 3a][g]
-    __shared__ double s_rp[kGaussElemsPerBlock][27];
+    __shared__ double s_u[kNodal > kResults ? kNodal : kResults];
+    double(*s_X)[81] = reinterpret_cast<double(*)[81]>(s_u);
+    double(*s_z)[81] = reinterpret_cast<double(*)[81]>(s_u + kGaussElemsPerBlock * 81);
+    double(*s_dx)[9][27] = reinterpret_cast<double(*)[9][27]>(s_u);
+    double(*s_S)[9][27] = reinterpret_cast<double(*)[9][27]>(s_u + kGaussElemsPerBlock * 9 * 27);  // JGW sigma(i, a): [i + 3a][g]
+    double(*s_rp)[27] = reinterpret_cast<double(*)[27]>(s_u + 2 * kGaussElemsPerBlock * 9 * 27);
     constexpr int kTabPer = (27 * 27 * 3 + kResThreads - 1) / kResThreads;
     constexpr int kNodePer = (kGaussElemsPerBlock * 81 + kResThreads - 1) / kResThreads;
     const int t = threadIdx.x;
@@ -325,7 +331,9 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
     {
         const int le = t / 27, g = t % 27;
         const int v = e0 + le, e = kFD ? v / 89 : v;
-        if (t < kGaussElemsPerBlock * 27 && v < n_elements) {
+        const bool active = t < kGaussElemsPerBlock * 27 && v < n_elements;
+        double rdx[9], rS[9], rrp = 0.0;
+        if (active) {
             GaussKinematics k;
             gauss_kinematics(s_tab, g, s_z[le], s_X[le], k);
             double P = 0.0;
@@ -333,10 +341,16 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
             double sigma[6];
             mr_cauchy_stress(k.F, k.Jznu, P, mat[2 * e], mat[2 * e + 1], sigma);
             const double jgw = k.jz * T->w27[g];
-            for (int i = 0; i < 9; ++i) s_dx[le][i][g] = k.dxidz[i];
+            for (int i = 0; i < 9; ++i) rdx[i] = k.dxidz[i];
             for (int a = 0; a < 3; ++a)
-                for (int i = 0; i < 3; ++i) s_S[le][i + 3 * a][g] = jgw * sigma[t2v(i, a)];
-            s_rp[le][g] = k.jx * T->w27[g] * (k.Jznu - 1.0);
+                for (int i = 0; i < 3; ++i) rS[i + 3 * a] = jgw * sigma[t2v(i, a)];
+            rrp = k.jx * T->w27[g] * (k.Jznu - 1.0);
+        }
+        __syncthreads();  // X and z consumed: their LDS takes the results
+        if (active) {
+            for (int i = 0; i < 9; ++i) s_dx[le][i][g] = rdx[i];
+            for (int i = 0; i < 9; ++i) s_S[le][i][g] = rS[i];
+            s_rp[le][g] = rrp;
         }
     }
     __syncthreads();
