@@ -159,6 +159,27 @@ def test_mooney_rivlin_16_config(gpu):
     assert np.array_equal(eq.matrix_values(), J)
 
 
+def test_mooney_rivlin_32_config(gpu):
+    """Config 3 size: 32^3 (32,768 elements, 859,812 DOFs, 178,723,696 nonzeros)."""
+    cm = gpu
+    n = 32
+    mesh = cm.RegularMesh.generate(2, n, jitter=0.1, seed=2025)
+    eq = cm.MooneyRivlinEquationsSet(mesh, 2.0, 6.0)
+    assert eq.n_rows == 859812
+    assert eq.nnz == 9 * (8 * n + 1) ** 3 + 6 * (5 * n + 1) ** 3 + (3 * n + 1) ** 3 == 178723696
+    d = _deformed_state(mesh, 0.05, 10)
+    eq.set_dofs(d)
+    eq.residual_evaluate()
+    assert rel(eq.residual(), orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0)) < FP64_TOL
+    rp, ci = eq.matrix_structure(0)
+    oJ = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, rp.astype(np.int64), ci)
+    eq.jacobian_evaluate(cm.OC_FP64)
+    assert rel(eq.matrix_values(), oJ) < FP64_TOL
+    eq.jacobian_evaluate(cm.OC_FP32)
+    assert rel(eq.matrix_values_sp().astype(np.float64), oJ) < FP32_TOL
+    eq.destroy()
+
+
 def test_undeformed_residual_vanishes(gpu):
     cm = gpu
     mesh = cm.RegularMesh.generate(2, 5, jitter=0.1, seed=1)
