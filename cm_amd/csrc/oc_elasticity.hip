@@ -4,10 +4,12 @@
 //
 // Jacobian: mr_gauss_kernel (one thread per element Gauss point, all
 // elements) evaluates the FP64 kinematics and tangent into Gauss-point
-// records (field-major, see GpField); mr_element_matrix_kernel contracts the
-// records of every element into its 89x89 element matrix on MFMA; and
-// csr_row_gather_kernel assembles each CSR row from the element rows that
-// touch it in ascending element order (deterministic, write-once).
+// records (field-major, see GpField); mr_element_matrix_f64_kernel (FP64) or
+// mr_element_matrix_kernel<float> (FP32) contracts the records of every
+// element into its 89x89 element matrix on MFMA; and csr_row_gather_kernel
+// assembles each CSR row from the element rows that touch it in ascending
+// element order (deterministic, write-once), placing them through the
+// neighbour-rank scatter map.
 // Residual: mr_residual_element_kernel evaluates element residuals of all
 // elements into a buffer and dof_gather_kernel sums them per DOF in
 // ascending element order.
@@ -436,19 +438,20 @@ int launch_mr_residual(EquationsSet &s)
 // Analytic Jacobian (FiniteElasticity_FiniteElementJacobianEvaluate,
 // finite_elasticity_routines.f90:842-1175) in two phases.
 //
-// Phase A (mr_element_matrix_kernel<Real>, all elements, no colours): for each
-// of the six lower displacement blocks (a, b), b <= a,
+// Phase A (mr_element_matrix_f64_kernel for FP64, mr_element_matrix_kernel<float>
+// for FP32; all elements, no colours): for each of the six lower displacement
+// blocks (a, b), b <= a,
 //   K_(a,m),(b,n) = sum_(g,j) U_g^{ab}[m][j] D_g[j][n],
 //   U_g^{ab}[m][j] = sum_i D_g[i][m] W_g[(a,i),(b,j)],
 //   W_g[(a,i),(b,j)] = J_z w_g (c(V(i,a),V(j,b)) + delta_ab sigma_ij),
 // i.e. the reference's TEMPVEC / DOT_PRODUCT contraction (:1001-1037)
-// re-associated as a 27x81 by 81x27 GEMM on MFMA 16x16x4; pressure rows
+// re-associated as a 27x81 by 81x27 GEMM on MFMA; pressure rows
 // K_(p,m),(b,n) = sum_g (J_z w_g psi_m) D_g[b][n] (:1039-1054) as an 8x27 by
-// 27x81 GEMM.  One block of four waves per element: D lives in registers, U is
-// formed in registers from D and W (LDS), and the waves split the 21 tiles of
-// the lower displacement blocks and the pressure tiles with no barrier between
-// them.  Lower triangle kept, mirrored (:1149-1153); the element matrix goes
-// to an HBM buffer, row-major.
+// 27x81 GEMM.  Both kernels run a producer/consumer pipeline per block:
+// producer waves build D, W and the pressure operand of the next element in
+// double-buffered LDS while consumer waves contract the current one, with D in
+// registers and U formed in registers from D and W.  Lower triangle kept,
+// mirrored (:1149-1153); the element matrix goes to an HBM buffer, row-major.
 // Phase B (csr_row_gather_kernel<Real>): one wavefront per CSR row adds the
 // element rows that touch it, in ascending element order, into an LDS copy of
 // the row and writes the row once (no zero fill, no read-modify-write).
