@@ -131,11 +131,10 @@ template int fill<float>(float *, int64_t, float);
 // K_e(ms,ns) += sum_{mi,ni} dphi_ms/dxi_mi GU(mi,ni) dphi_ns/dxi_ni * J w.
 // Two phases, as the Jacobian: laplace_element_kernel writes every element
 // matrix to a buffer (one thread per element); laplace_row_gather_kernel
-// gives each CSR row one thread,
-// which adds the element rows touching it in ascending element order (the
-// reference's element-loop order) into an LDS copy of its block's rows, and
-// the block writes its contiguous CSR segment once: no zero fill, no
-// read-modify-write, no colours.
+// gives each CSR row eight lanes, which add the element rows touching it in
+// ascending element order (the reference's element-loop order) into an LDS
+// copy of its block's rows, and the block writes its contiguous CSR segment
+// once: no zero fill, no read-modify-write, no colours.
 __global__ void __launch_bounds__(256)
 laplace_slot_offsets_kernel(int n_elements, const int *__restrict__ conn, const int64_t *__restrict__ row_ptr,
                             const int *__restrict__ col_idx, uint8_t *__restrict__ offs, int *__restrict__ n_bad)
@@ -236,35 +235,47 @@ __global__ void __launch_bounds__(256) permute_offsets_kernel(int64_t n, const i
         out[p] = offs[idx[p]];
 }
 
-template <int kRows>
-__global__ void __launch_bounds__(kRows)
+// Eight lanes per CSR row (32 rows per block): lane j adds column j of each
+// of the row's element rows, in ascending element order, into the block's
+// LDS copy of its CSR segment; a chunk's global loads (eight element rows)
+// are issued before its LDS adds.  The block then writes the segment once.
+constexpr int kLapGatherRows = 32;
+
+__global__ void __launch_bounds__(8 * kLapGatherRows)
 laplace_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const int *__restrict__ gptr,
-                          const double *__restrict__ kbuf, const uint64_t *__restrict__ offs,
+                          const double *__restrict__ kbuf, const uint8_t *__restrict__ offs,
                           double *__restrict__ values)
 {
     extern __shared__ unsigned char s_raw[];
     double *s_seg = reinterpret_cast<double *>(s_raw);
-    const int r0 = blockIdx.x * kRows, t = threadIdx.x, r = r0 + t;
-    const int r_end = min(r0 + kRows, n_rows);
+    const int t = threadIdx.x, j = t & 7;
+    const int r0 = blockIdx.x * kLapGatherRows, r = r0 + (t >> 3);
+    const int r_end = min(r0 + kLapGatherRows, n_rows);
     const int64_t base = row_ptr[r0];
     const int seg = static_cast<int>(row_ptr[r_end] - base);
-    for (int k = t; k < seg; k += kRows) s_seg[k] = 0.0;
+    for (int k = t; k < seg; k += 8 * kLapGatherRows) s_seg[k] = 0.0;
     __syncthreads();
     if (r < n_rows) {
         double *row = s_seg + (row_ptr[r] - base);
-        for (int64_t c = gptr[r]; c < gptr[r + 1]; ++c) {  // element rows in ascending element order
-            const uint64_t packed = offs[c];
-            const double2 *kr = reinterpret_cast<const double2 *>(kbuf + c * 8);
+        const int c_end = gptr[r + 1];
+        for (int c0 = gptr[r]; c0 < c_end; c0 += 8) {
+            double v[8];
+            int o[8];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double2 v = kr[q];
-                row[(packed >> (16 * q)) & 0xff] += v.x;
-                row[(packed >> (16 * q + 8)) & 0xff] += v.y;
+            for (int c = 0; c < 8; ++c) {
+                if (c0 + c < c_end) {
+                    const int64_t p = static_cast<int64_t>(c0 + c) * 8 + j;
+                    v[c] = kbuf[p];
+                    o[c] = offs[p];
+                }
             }
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (c0 + c < c_end) row[o[c]] += v[c];
         }
     }
     __syncthreads();
-    for (int k = t; k < seg; k += kRows) values[base + k] = s_seg[k];
+    for (int k = t; k < seg; k += 8 * kLapGatherRows) values[base + k] = s_seg[k];
 }
 
 static int laplace_prepare(EquationsSet &s)
@@ -289,20 +300,13 @@ static int laplace_prepare(EquationsSet &s)
     }
     if (static_cast<int64_t>(s.n_elements) * 8 >= INT32_MAX)
         return set_error(OC_ERR_UNSUPPORTED, "%d elements exceed the int32 element-row gather lists", s.n_elements);
-    // rows per gather block: the block's CSR segment must fit the LDS
+    // the CSR segment of a row-gather block must fit the LDS
     int64_t max_seg = 0;
-    for (int r0 = 0; r0 < s.n_rows; r0 += 256)
-        max_seg = std::max(max_seg, s.h_row_ptr[std::min(r0 + 256, s.n_rows)] - s.h_row_ptr[r0]);
-    s.lap_rows_per_block = 256;
-    if (max_seg > 8192) {
-        max_seg = 0;
-        for (int r0 = 0; r0 < s.n_rows; r0 += 64)
-            max_seg = std::max(max_seg, s.h_row_ptr[std::min(r0 + 64, s.n_rows)] - s.h_row_ptr[r0]);
-        s.lap_rows_per_block = 64;
-        if (max_seg > 8192)
-            return set_error(OC_ERR_UNSUPPORTED, "Laplace CSR rows too long for the row gather (%lld entries in 64 rows)",
-                             static_cast<long long>(max_seg));
-    }
+    for (int r0 = 0; r0 < s.n_rows; r0 += kLapGatherRows)
+        max_seg = std::max(max_seg, s.h_row_ptr[std::min(r0 + kLapGatherRows, s.n_rows)] - s.h_row_ptr[r0]);
+    if (max_seg > 8192)
+        return set_error(OC_ERR_UNSUPPORTED, "Laplace CSR rows too long for the row gather (%lld entries in %d rows)",
+                         static_cast<long long>(max_seg), kLapGatherRows);
     s.max_row_len = static_cast<int>(max_seg);
     // DOF -> element rows, ascending element order
     std::vector<int32_t> hconn(static_cast<size_t>(s.n_elements) * 8);
@@ -346,14 +350,9 @@ int launch_laplace_assemble(EquationsSet &s)
     }
     KernelTimer timer("laplace_row_gather");
     const size_t lds = static_cast<size_t>(s.max_row_len) * sizeof(double);
-    if (s.lap_rows_per_block == 256)
-        laplace_row_gather_kernel<256><<<static_cast<unsigned>((s.n_rows + 255) / 256), 256, lds, rt.stream>>>(
-            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.kbuf64.ptr,
-            reinterpret_cast<const uint64_t *>(s.slot_offsets8.ptr), s.values.ptr);
-    else
-        laplace_row_gather_kernel<64><<<static_cast<unsigned>((s.n_rows + 63) / 64), 64, lds, rt.stream>>>(
-            s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr, s.kbuf64.ptr,
-            reinterpret_cast<const uint64_t *>(s.slot_offsets8.ptr), s.values.ptr);
+    laplace_row_gather_kernel<<<static_cast<unsigned>((s.n_rows + kLapGatherRows - 1) / kLapGatherRows),
+                                8 * kLapGatherRows, lds, rt.stream>>>(s.n_rows, s.row_ptr.ptr, s.gather_ptr.ptr,
+                                                                       s.kbuf64.ptr, s.slot_offsets8.ptr, s.values.ptr);
     OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
 }

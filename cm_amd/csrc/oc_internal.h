@@ -186,9 +186,8 @@ struct EquationsSet {
     // Laplace: 8-bit offsets (in the row-gather order: entry p*8+j belongs to
     // the p-th element row of the gather lists); gather_idx holds the gather
     // position of element row e*8+i; element matrices in kbuf64 (E*64, rows
-    // in gather order) and rows per row-gather block
+    // in gather order)
     DevBuf<uint8_t> slot_offsets8;
-    int lap_rows_per_block = 256;
 
     // Dirichlet conditions (0-based DOFs, full values) and the free-DOF mask
     int n_bc = 0;
