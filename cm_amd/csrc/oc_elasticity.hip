@@ -753,8 +753,8 @@ mr_element_matrix_kernel(int n_elements, const double *__restrict__ rec, const B
 // (rows m = 16t + 4blk + i) against one column tile J (n = 4J + j):
 //   A = U^{ab}_g[16t + l][j_dim] from the lane's D slice and W (l = c + 4 blk),
 //   B = D_g[j_dim][4J + c], shared by the four row tiles, read from LDS;
-// k-step s covers j_dim = s / 7 and g = 4 (s % 7) + h.
-constexpr int kF64Q[3][3] = {{0, 0, 0}, {1, 2, 0}, {3, 4, 5}};  // lower block (a, b) -> q
+// k-step s covers j_dim = s / 7 and g = 4 (s % 7) + h.  Lower block (a, b)
+// is numbered q = a (a + 1) / 2 + b (F64Unit).
 
 __device__ __forceinline__ double2 lds_pair(const double *p) { return *reinterpret_cast<const double2 *>(p); }
 
