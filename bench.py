@@ -114,6 +114,37 @@ def laplace_line(cm, n, steps):
     return out
 
 
+def solver_matrix_line(cm, eq, steps):
+    """Solver matrix and vector of the MR set under the uniaxial conditions:
+    the FP64 Jacobian restricted to the free DOFs, and the residual of the free
+    rows (SOLVER_MATRICES_STATIC_ASSEMBLE).  HBM-bound compaction."""
+    fixed = eq.uniaxial_boundary_conditions(1.1)
+    eq.residual_evaluate()
+    eq.jacobian_evaluate(cm.OC_FP64)
+    eq.solver_matrices_assemble(cm.OC_FP64)  # first call builds the structure
+    ms, _ = time_steps(lambda: eq.solver_matrices_assemble(cm.OC_FP64), steps, 2, cm, only="solver_matrix_fp64")
+    kms, launches = cm.kernel_timing("solver_matrix_fp64")
+    cm.kernel_timing_enable(False)
+    n, s_nnz = eq.solver_matrix_info()
+    rp, _ = eq.matrix_structure(0)
+    free = np.ones(eq.n_rows, bool)
+    free[fixed] = False
+    lens = np.diff(rp.astype(np.int64))
+    nnz_free_rows = int(lens[free].sum())
+    # column indices of the free rows, the free values read and written, the
+    # DOF -> solver map once, row metadata (two offsets, the row, the solver
+    # offset) and the vector (residual read, solver vector written)
+    nbytes = 4 * nnz_free_rows + 16 * s_nnz + 4 * eq.n_rows + 28 * n + 16 * n
+    avg_s = kms / launches / 1e3
+    return {
+        "workload": "16^3 Mooney-Rivlin FP64 Jacobian, uniaxial Dirichlet conditions",
+        "solver_rows": n, "solver_nnz": s_nnz, "equations_nnz": eq.nnz,
+        "ms_per_assembly": ms / steps, "kernel_ms_per_assembly": avg_s * 1e3,
+        "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / avg_s / 1e9,
+        "hbm_frac": nbytes / avg_s / 1e9 / HBM_PEAK_GBS,
+    }
+
+
 def newton_line(cm, n, load_steps):
     """Time to 1e-10 relative residual per load step, plain FP64 Newton versus
     learned warm start + FP32 Jacobian, 10% uniaxial stretch over `load_steps`."""
@@ -337,6 +368,7 @@ def main():
             "peak_TFLOPs": FP32_MFMA_PEAK_TFLOPS,
             "kernel_ms_per_step": {"mr_element_matrix_fp32": j32 / args.steps, "csr_row_gather_fp32": g32 / args.steps},
         }
+        secondary["solver_matrix"] = solver_matrix_line(cm, eq, 20)
         secondary["laplace"] = [laplace_line(cm, n, s) for n, s in ((8, 50), (64, 20), (128, 10))]
     if not args.no_newton:
         secondary["newton_per_load_step"] = newton_line(cm, args.n, args.load_steps)

@@ -8,7 +8,8 @@ from ._lib import (OC_PRECONDITIONER_BLOCK_DIAGONAL, OC_PRECONDITIONER_BLOCK_TRI
                    OC_PRECONDITIONER_JACOBI, OC_PRECONDITIONER_MULTIGRID)
 from ._lib import (CMISSError, OC_FP32, OC_FP64, OC_BUFFER_COL_IDX, OC_BUFFER_DOFS,  # noqa: F401
                    OC_BUFFER_MATRIX_VALUES, OC_BUFFER_MATRIX_VALUES_SP, OC_BUFFER_RESIDUAL, OC_BUFFER_RHS,
-                   OC_BUFFER_ROW_PTR, LIB_PATH)
+                   OC_BUFFER_ROW_PTR, OC_BUFFER_SOLVER_COL_IDX, OC_BUFFER_SOLVER_ROW_PTR, OC_BUFFER_SOLVER_VALUES,
+                   OC_BUFFER_SOLVER_VALUES_SP, OC_BUFFER_SOLVER_VECTOR, LIB_PATH)
 from .equations_set import (EquationsSet, LaplaceEquationsSet, MooneyRivlinBatch, MooneyRivlinEquationsSet,  # noqa: F401
                             event_elapsed, event_record, finalise, initialise, kernel_timing,
                             kernel_timing_enable, newton_options, synchronise)

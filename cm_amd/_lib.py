@@ -35,6 +35,11 @@ OC_BUFFER_COL_IDX = 3
 OC_BUFFER_DOFS = 4
 OC_BUFFER_RESIDUAL = 5
 OC_BUFFER_RHS = 6
+OC_BUFFER_SOLVER_VALUES = 7
+OC_BUFFER_SOLVER_VALUES_SP = 8
+OC_BUFFER_SOLVER_ROW_PTR = 9
+OC_BUFFER_SOLVER_COL_IDX = 10
+OC_BUFFER_SOLVER_VECTOR = 11
 
 
 class CMISSError(RuntimeError):
@@ -117,6 +122,12 @@ SIGNATURES = {
     "oc_residual_get": [_H, _dp],
     "oc_rhs_get": [_H, _dp],
     "oc_device_pointer_get": [_H, _c_int, ctypes.POINTER(ctypes.c_void_p)],
+    "oc_solver_matrices_assemble": [_H, _c_int],
+    "oc_solver_matrix_info": [_H, ctypes.POINTER(_c_int), ctypes.POINTER(ctypes.c_longlong)],
+    "oc_solver_matrix_structure_get": [_H, _c_int, _i32, _i32],
+    "oc_solver_matrix_values_get": [_H, _dp],
+    "oc_solver_matrix_values_get_sp": [_H, _fp],
+    "oc_solver_vector_get": [_H, _dp],
     "oc_event_record": [_c_int],
     "oc_event_elapsed": [_c_int, _c_int, ctypes.POINTER(_c_float)],
     "oc_synchronise": [],

@@ -537,10 +537,101 @@ int oc_device_pointer_get(oc_equations_set h, int buffer, void **device_pointer)
     case OC_BUFFER_DOFS: p = s.dofs.ptr; break;
     case OC_BUFFER_RESIDUAL: p = s.residual.ptr; break;
     case OC_BUFFER_RHS: p = s.rhs.ptr; break;
+    case OC_BUFFER_SOLVER_VALUES: p = s.solver.values.ptr; break;
+    case OC_BUFFER_SOLVER_VALUES_SP: p = s.solver.values_sp.ptr; break;
+    case OC_BUFFER_SOLVER_ROW_PTR: p = s.solver.row_ptr.ptr; break;
+    case OC_BUFFER_SOLVER_COL_IDX: p = s.solver.col_idx.ptr; break;
+    case OC_BUFFER_SOLVER_VECTOR: p = s.solver.vector.ptr; break;
     default: return set_error(OC_ERR_INVALID_ARGUMENT, "unknown buffer %d", buffer);
     }
     if (!p) return set_error(OC_ERR_INVALID_ARGUMENT, "buffer %d has not been allocated yet", buffer);
     *device_pointer = p;
+    return OC_NO_ERROR;
+}
+
+int oc_solver_matrices_assemble(oc_equations_set h, int precision)
+{
+    OC_CHECK(check_handle(h));
+    return solver_matrices_assemble(h->s, precision);
+}
+
+static int solver_ready(oc_equations_set h)
+{
+    OC_CHECK(check_handle(h));
+    if (!h->s.solver.built || h->s.solver.bc_version != h->s.bc_version)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "the solver matrices have not been assembled for the current "
+                                                  "boundary conditions (oc_solver_matrices_assemble)");
+    return OC_NO_ERROR;
+}
+
+int oc_solver_matrix_info(oc_equations_set h, int *n_rows, long long *nnz)
+{
+    OC_CHECK(solver_ready(h));
+    if (n_rows) *n_rows = h->s.solver.n;
+    if (nnz) *nnz = static_cast<long long>(h->s.solver.nnz);
+    return OC_NO_ERROR;
+}
+
+int oc_solver_matrix_structure_get(oc_equations_set h, int index_base, int *row_ptr, int *col_idx)
+{
+    OC_CHECK(solver_ready(h));
+    if (index_base != 0 && index_base != 1) return set_error(OC_ERR_INVALID_ARGUMENT, "index_base must be 0 or 1");
+    const SolverSystem &S = h->s.solver;
+    if (S.nnz + index_base > INT32_MAX)
+        return set_error(OC_ERR_UNSUPPORTED, "%lld solver nonzeros exceed the 4-byte integer structure arrays",
+                         static_cast<long long>(S.nnz));
+    if (row_ptr) {
+        std::vector<int64_t> p(static_cast<size_t>(S.n) + 1);
+        OC_HIP_CHECK(hipMemcpyAsync(p.data(), S.row_ptr.ptr, p.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                    g_runtime.stream));
+        OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+        for (size_t i = 0; i < p.size(); ++i) row_ptr[i] = static_cast<int>(p[i] + index_base);
+    }
+    if (col_idx && S.nnz > 0) {
+        OC_HIP_CHECK(hipMemcpyAsync(col_idx, S.col_idx.ptr, static_cast<size_t>(S.nnz) * sizeof(int),
+                                    hipMemcpyDeviceToHost, g_runtime.stream));
+        OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+        if (index_base)
+            for (int64_t k = 0; k < S.nnz; ++k) col_idx[k] += index_base;
+    }
+    return OC_NO_ERROR;
+}
+
+int oc_solver_matrix_values_get(oc_equations_set h, double *values)
+{
+    OC_CHECK(solver_ready(h));
+    if (!values) return set_error(OC_ERR_INVALID_ARGUMENT, "values is NULL");
+    const SolverSystem &S = h->s.solver;
+    if (!S.values.ptr && S.nnz) return set_error(OC_ERR_INVALID_ARGUMENT, "no FP64 solver matrix has been assembled");
+    if (S.nnz)
+        OC_HIP_CHECK(hipMemcpyAsync(values, S.values.ptr, static_cast<size_t>(S.nnz) * sizeof(double),
+                                    hipMemcpyDeviceToHost, g_runtime.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+int oc_solver_matrix_values_get_sp(oc_equations_set h, float *values)
+{
+    OC_CHECK(solver_ready(h));
+    if (!values) return set_error(OC_ERR_INVALID_ARGUMENT, "values is NULL");
+    const SolverSystem &S = h->s.solver;
+    if (!S.values_sp.ptr && S.nnz) return set_error(OC_ERR_INVALID_ARGUMENT, "no FP32 solver matrix has been assembled");
+    if (S.nnz)
+        OC_HIP_CHECK(hipMemcpyAsync(values, S.values_sp.ptr, static_cast<size_t>(S.nnz) * sizeof(float),
+                                    hipMemcpyDeviceToHost, g_runtime.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+    return OC_NO_ERROR;
+}
+
+int oc_solver_vector_get(oc_equations_set h, double *vector)
+{
+    OC_CHECK(solver_ready(h));
+    if (!vector) return set_error(OC_ERR_INVALID_ARGUMENT, "vector is NULL");
+    const SolverSystem &S = h->s.solver;
+    if (S.n)
+        OC_HIP_CHECK(hipMemcpyAsync(vector, S.vector.ptr, static_cast<size_t>(S.n) * sizeof(double),
+                                    hipMemcpyDeviceToHost, g_runtime.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
     return OC_NO_ERROR;
 }
 

@@ -130,6 +130,24 @@ struct Multigrid {
     int coarse_sweeps = 0;                // 0: dense solve of the coarsest level
 };
 
+// Solver system (oc_solver_matrices.hip): the equations matrix or Jacobian
+// restricted to the free DOFs.  index[d] = solver number of DOF d (-1 when
+// fixed), rows[i] = equations row of solver row i; CSR with 64-bit offsets.
+struct SolverSystem {
+    bool built = false;
+    bool columns_written = false;   // col_idx is written by the first values pass
+    uint64_t bc_version = 0;        // boundary conditions the structure was built for
+    int n = 0;
+    int64_t nnz = 0;
+    int precision = -1;             // precision of the last assembly
+    DevBuf<int32_t> index, rows;
+    DevBuf<int64_t> row_ptr;
+    DevBuf<int32_t> col_idx;
+    DevBuf<double> values;
+    DevBuf<float> values_sp;
+    DevBuf<double> vector;          // RHS with the fixed columns moved over (linear) or residual
+};
+
 // Per-equations-set state.  Element arrays are element-major; node and DOF
 // indices are 0-based; CSR col_idx 0-based int32, row offsets int64 on the
 // device (a batch of meshes can exceed 2^31 nonzeros).
@@ -189,8 +207,10 @@ struct EquationsSet {
     // in gather order)
     DevBuf<uint8_t> slot_offsets8;
 
-    // Dirichlet conditions (0-based DOFs, full values) and the free-DOF mask
+    // Dirichlet conditions (0-based DOFs, full values) and the free-DOF mask;
+    // bc_version counts oc_boundary_conditions_set calls
     int n_bc = 0;
+    uint64_t bc_version = 0;
     DevBuf<int32_t> bc_idx;
     DevBuf<double> bc_full;
     DevBuf<uint8_t> free_mask;
@@ -214,6 +234,7 @@ struct EquationsSet {
     hipGraphExec_t step_graph = nullptr;
     int step_graph_key[4] = {-1, -1, -1, -1};
 
+    SolverSystem solver;      // oc_solver_matrices_assemble
     Multigrid *mg = nullptr;  // set by oc_multigrid_structure_set
     ~EquationsSet()
     {
@@ -247,6 +268,8 @@ int launch_mr_gauss(EquationsSet &s);
 // `precision`; apply z_u = V-cycle(t) for the displacement block
 int mg_setup(EquationsSet &s, int precision);
 int mg_apply(EquationsSet &s, int precision, const double *t, double *z);
+// solver matrix and vector from the last assembled matrix (oc_solver_matrices.hip)
+int solver_matrices_assemble(EquationsSet &s, int precision);
 // y = A x with `lanes` lanes per CSR row (A in FP64 or FP32 values)
 int launch_spmv_lanes(EquationsSet &s, bool fp32, int lanes, const double *x, double *y);
 

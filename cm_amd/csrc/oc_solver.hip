@@ -946,6 +946,7 @@ int oc_boundary_conditions_set(oc_equations_set h, int n, const int *dof_indices
     OC_CHECK(s.bc_full.upload(full_values, static_cast<size_t>(n)));
     OC_CHECK(s.free_mask.upload(mask.data(), mask.size()));
     s.n_bc = n;
+    ++s.bc_version;
     return OC_NO_ERROR;
 }
 

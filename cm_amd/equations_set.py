@@ -137,6 +137,40 @@ class EquationsSet:
         call("oc_device_pointer_get", self._h, int(buffer), ctypes.byref(p))
         return p.value
 
+    # -- solver matrices ----------------------------------------------------
+    def solver_matrices_assemble(self, precision=_lib.OC_FP64):
+        """Solver matrix and vector from the last assembled matrix: the rows and
+        columns of fixed DOFs removed, free DOFs numbered in ascending order
+        (SOLVER_MATRICES_STATIC_ASSEMBLE, solver_routines.f90:13745-14771)."""
+        call("oc_solver_matrices_assemble", self._h, int(precision))
+
+    def solver_matrix_info(self):
+        n, nnz = ctypes.c_int(), ctypes.c_longlong()
+        call("oc_solver_matrix_info", self._h, ctypes.byref(n), ctypes.byref(nnz))
+        return n.value, nnz.value
+
+    def solver_matrix_structure(self, index_base=0):
+        n, nnz = self.solver_matrix_info()
+        row_ptr = np.zeros(n + 1, np.int32)
+        col_idx = np.zeros(nnz, np.int32)
+        call("oc_solver_matrix_structure_get", self._h, int(index_base), row_ptr, col_idx)
+        return row_ptr, col_idx
+
+    def solver_matrix_values(self):
+        out = np.zeros(self.solver_matrix_info()[1])
+        call("oc_solver_matrix_values_get", self._h, out)
+        return out
+
+    def solver_matrix_values_sp(self):
+        out = np.zeros(self.solver_matrix_info()[1], np.float32)
+        call("oc_solver_matrix_values_get_sp", self._h, out)
+        return out
+
+    def solver_vector(self):
+        out = np.zeros(self.solver_matrix_info()[0])
+        call("oc_solver_vector_get", self._h, out)
+        return out
+
     # -- boundary conditions and solution ------------------------------------
     def set_boundary_conditions(self, dof_indices, full_values):
         """Fixed(-incremented) Dirichlet DOFs (0-based) and their full values."""

@@ -19,7 +19,9 @@ MODULE OC_ASM
   INTEGER(C_INT), PARAMETER :: OC_FP64 = 0, OC_FP32 = 1
   INTEGER(C_INT), PARAMETER :: OC_EQUATIONS_SET_LAPLACE = 1, OC_EQUATIONS_SET_MOONEY_RIVLIN = 2
   INTEGER(C_INT), PARAMETER :: OC_BUFFER_MATRIX_VALUES = 0, OC_BUFFER_MATRIX_VALUES_SP = 1, &
-      & OC_BUFFER_ROW_PTR = 2, OC_BUFFER_COL_IDX = 3, OC_BUFFER_DOFS = 4, OC_BUFFER_RESIDUAL = 5, OC_BUFFER_RHS = 6
+      & OC_BUFFER_ROW_PTR = 2, OC_BUFFER_COL_IDX = 3, OC_BUFFER_DOFS = 4, OC_BUFFER_RESIDUAL = 5, OC_BUFFER_RHS = 6, &
+      & OC_BUFFER_SOLVER_VALUES = 7, OC_BUFFER_SOLVER_VALUES_SP = 8, OC_BUFFER_SOLVER_ROW_PTR = 9, &
+      & OC_BUFFER_SOLVER_COL_IDX = 10, OC_BUFFER_SOLVER_VECTOR = 11
   INTEGER(C_INT), PARAMETER :: OC_PRECONDITIONER_JACOBI = 0, OC_PRECONDITIONER_BLOCK_DIAGONAL = 1, &
       & OC_PRECONDITIONER_BLOCK_TRIANGULAR = 2, OC_PRECONDITIONER_MULTIGRID = 3
 
@@ -185,6 +187,48 @@ MODULE OC_ASM
       INTEGER(C_INT), VALUE :: buffer
       TYPE(C_PTR), INTENT(OUT) :: device_pointer
     END FUNCTION oc_device_pointer_get
+
+    ! solver matrices (SOLVER_MATRICES_STATIC_ASSEMBLE, solver_routines.f90:13745-14771)
+    INTEGER(C_INT) FUNCTION oc_solver_matrices_assemble(handle, precision) &
+        & BIND(C, NAME="oc_solver_matrices_assemble")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: handle
+      INTEGER(C_INT), VALUE :: precision
+    END FUNCTION oc_solver_matrices_assemble
+
+    INTEGER(C_INT) FUNCTION oc_solver_matrix_info(handle, n_rows, nnz) BIND(C, NAME="oc_solver_matrix_info")
+      IMPORT :: C_INT, C_LONG_LONG, C_PTR
+      TYPE(C_PTR), VALUE :: handle
+      INTEGER(C_INT), INTENT(OUT) :: n_rows
+      INTEGER(C_LONG_LONG), INTENT(OUT) :: nnz
+    END FUNCTION oc_solver_matrix_info
+
+    INTEGER(C_INT) FUNCTION oc_solver_matrix_structure_get(handle, index_base, row_ptr, col_idx) &
+        & BIND(C, NAME="oc_solver_matrix_structure_get")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: handle
+      INTEGER(C_INT), VALUE :: index_base
+      INTEGER(C_INT), DIMENSION(*), INTENT(OUT) :: row_ptr, col_idx
+    END FUNCTION oc_solver_matrix_structure_get
+
+    INTEGER(C_INT) FUNCTION oc_solver_matrix_values_get(handle, values) BIND(C, NAME="oc_solver_matrix_values_get")
+      IMPORT :: C_INT, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: handle
+      REAL(C_DOUBLE), DIMENSION(*), INTENT(OUT) :: values
+    END FUNCTION oc_solver_matrix_values_get
+
+    INTEGER(C_INT) FUNCTION oc_solver_matrix_values_get_sp(handle, values) &
+        & BIND(C, NAME="oc_solver_matrix_values_get_sp")
+      IMPORT :: C_INT, C_FLOAT, C_PTR
+      TYPE(C_PTR), VALUE :: handle
+      REAL(C_FLOAT), DIMENSION(*), INTENT(OUT) :: values
+    END FUNCTION oc_solver_matrix_values_get_sp
+
+    INTEGER(C_INT) FUNCTION oc_solver_vector_get(handle, vector) BIND(C, NAME="oc_solver_vector_get")
+      IMPORT :: C_INT, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: handle
+      REAL(C_DOUBLE), DIMENSION(*), INTENT(OUT) :: vector
+    END FUNCTION oc_solver_vector_get
 
     INTEGER(C_INT) FUNCTION oc_dofs_set_device(handle, dofs_device) BIND(C, NAME="oc_dofs_set_device")
       IMPORT :: C_INT, C_PTR

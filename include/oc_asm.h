@@ -61,6 +61,11 @@ extern "C" {
 #define OC_BUFFER_DOFS 4              /* double[n_rows]: dependent field values */
 #define OC_BUFFER_RESIDUAL 5          /* double[n_rows] */
 #define OC_BUFFER_RHS 6               /* double[n_rows] */
+#define OC_BUFFER_SOLVER_VALUES 7     /* double[solver nnz]: solver matrix (oc_solver_matrices_assemble) */
+#define OC_BUFFER_SOLVER_VALUES_SP 8  /* float[solver nnz]: solver matrix from the FP32 Jacobian */
+#define OC_BUFFER_SOLVER_ROW_PTR 9    /* int64_t[solver rows+1], 0-based */
+#define OC_BUFFER_SOLVER_COL_IDX 10   /* int[solver nnz], 0-based solver columns */
+#define OC_BUFFER_SOLVER_VECTOR 11    /* double[solver rows] */
 
 typedef struct oc_equations_set_s *oc_equations_set;
 
@@ -172,6 +177,26 @@ int oc_matrix_values_get_sp(oc_equations_set equations_set, float *values);
 int oc_residual_get(oc_equations_set equations_set, double *residual);
 int oc_rhs_get(oc_equations_set equations_set, double *rhs);
 int oc_device_pointer_get(oc_equations_set equations_set, int buffer, void **device_pointer);
+
+/* ---- solver matrices ------------------------------------------------------- */
+/* The solver system built from the last assembled equations matrix
+ * (SOLVER_MATRICES_STATIC_ASSEMBLE, solver_routines.f90:13745-14771, with
+ * SOLVER_MATRIX_JACOBIAN_MATRIX_ADD and its equations-matrix counterpart,
+ * solver_matrices_routines.f90:1258-1483).  Rows and columns of the DOFs fixed
+ * by oc_boundary_conditions_set are removed and the free DOFs are numbered in
+ * ascending order (INCLUDE_ROW / INCLUDE_COLUMN, solver_mapping_routines.f90:
+ * 434-520); the coupling coefficients are 1.  Laplace: matrix = the linear
+ * matrix (FP64), vector = RHS minus the fixed columns times the fixed DOFs'
+ * current values (solver_routines.f90:14380-14620).  Mooney-Rivlin: matrix =
+ * the Jacobian of `precision`, vector = the residual of the free rows.  The
+ * structure is built on the first call after a boundary-condition change. */
+int oc_solver_matrices_assemble(oc_equations_set equations_set, int precision);
+int oc_solver_matrix_info(oc_equations_set equations_set, int *n_rows, long long *nnz);
+/* 4-byte structure (OC_ERR_UNSUPPORTED when the solver nnz does not fit). */
+int oc_solver_matrix_structure_get(oc_equations_set equations_set, int index_base, int *row_ptr, int *col_idx);
+int oc_solver_matrix_values_get(oc_equations_set equations_set, double *values);
+int oc_solver_matrix_values_get_sp(oc_equations_set equations_set, float *values);
+int oc_solver_vector_get(oc_equations_set equations_set, double *vector);
 
 /* ---- boundary conditions and solution (the solver "judge") ----------------- */
 /* Fixed (or fixed-incremented) Dirichlet conditions on `n` DOFs with their
@@ -310,7 +335,8 @@ int oc_synchronise(void);
  * stream and its duration is accumulated under its kernel name
  * ("laplace_element", "laplace_row_gather", "mr_residual", "dof_gather",
  * "mr_gauss_jacobian", "mr_element_matrix_fp64", "mr_element_matrix_fp32",
- * "csr_row_gather_fp64", "csr_row_gather_fp32", "fill", ...).  Enabling or
+ * "csr_row_gather_fp64", "csr_row_gather_fp32", "solver_matrix_fp64",
+ * "solver_matrix_fp32", "fill", ...).  Enabling or
  * resetting clears the accumulators. */
 int oc_kernel_timing_enable(int enable);
 /* Restricts the timing to the launches of one kernel name (NULL: all), so

@@ -11,6 +11,7 @@
  */
 #include "oc_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -858,4 +859,112 @@ int oco_bc_increment(int ndof, const int32_t *dof_idx, const double *full, doubl
         else current[d] = current[d] + (full[i] - current[d]) / (double)(max_iterations - iteration + 1);
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solver matrices of one equations set on one rank.                         */
+
+/* ZERO_TOLERANCE, constants.f90:68 */
+#define OCO_ZERO_TOLERANCE (5.0 * DBL_EPSILON)
+
+int oco_solver_system(int n_rows, const int64_t *row_ptr, const int32_t *col_idx, const double *values,
+                      const uint8_t *free_mask, const double *vec, const double *dofs, int linear, int64_t *nnz,
+                      int64_t *out_row_ptr, int32_t *out_col_idx, double *out_values, double *out_vector)
+{
+    /* solver mapping: equations row/column d -> solver number, free DOFs in
+     * order (INCLUDE_ROW / INCLUDE_COLUMN, solver_mapping_routines.f90:434-520) */
+    int32_t *sidx = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n_rows > 0 ? n_rows : 1));
+    if (!sidx) return -1;
+    int n = 0;
+    for (int d = 0; d < n_rows; ++d) sidx[d] = free_mask[d] ? n++ : -1;
+    /* structure (SOLVER_MATRIX_STRUCTURE_CALCULATE): the mapped columns of every
+     * mapped row, ascending */
+    int64_t total = 0;
+    for (int r = 0; r < n_rows; ++r) {
+        if (sidx[r] < 0) continue;
+        for (int64_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) total += sidx[col_idx[k]] >= 0;
+    }
+    *nnz = total;
+    if (!out_row_ptr) {
+        free(sidx);
+        return n;
+    }
+    out_row_ptr[0] = 0;
+    for (int r = 0, i = 0; r < n_rows; ++r) {
+        if (sidx[r] < 0) continue;
+        int64_t p = out_row_ptr[i];
+        for (int64_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+            if (sidx[col_idx[k]] >= 0) out_col_idx[p++] = sidx[col_idx[k]];
+        out_row_ptr[++i] = p;
+    }
+    /* matrix: zero, then for every equations row and stored column
+     * VALUE = ALPHA * J * row coupling * column coupling added at the solver
+     * entry (CSR branch of SOLVER_MATRIX_JACOBIAN_MATRIX_ADD; the add locates
+     * the entry as MATRIX_VALUES_ADD_DP does) */
+    for (int64_t k = 0; k < total; ++k) out_values[k] = 0.0;
+    const double alpha = 1.0, row_c = 1.0, col_c = 1.0;
+    for (int r = 0; r < n_rows; ++r) {
+        if (sidx[r] < 0) continue;
+        for (int64_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+            const int32_t sc = sidx[col_idx[k]];
+            if (sc < 0) continue;
+            const double value = alpha * values[k] * row_c * col_c;
+            const int32_t sr = sidx[r];
+            int rc = oco_csr_add(out_row_ptr, out_col_idx, out_values, 1, &sr, 1, &sc, &value);
+            if (rc) {
+                free(sidx);
+                return rc;
+            }
+        }
+    }
+    /* vector */
+    for (int i = 0; i < n; ++i) out_vector[i] = 0.0;
+    if (!linear) {
+        for (int r = 0; r < n_rows; ++r)
+            if (sidx[r] >= 0) out_vector[sidx[r]] += vec[r] * row_c;
+        free(sidx);
+        return n;
+    }
+    /* rows storing each column, ascending (the Dirichlet sparsity indices) */
+    int64_t *cptr = (int64_t *)calloc((size_t)n_rows + 1, sizeof(int64_t));
+    int64_t nz = row_ptr[n_rows];
+    int32_t *crow = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nz > 0 ? nz : 1));
+    int64_t *cpos = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nz > 0 ? nz : 1));
+    if (!cptr || !crow || !cpos) {
+        free(sidx);
+        free(cptr);
+        free(crow);
+        free(cpos);
+        return -1;
+    }
+    for (int64_t k = 0; k < nz; ++k) cptr[col_idx[k] + 1]++;
+    for (int c = 0; c < n_rows; ++c) cptr[c + 1] += cptr[c];
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n_rows > 0 ? n_rows : 1));
+    for (int c = 0; c < n_rows; ++c) fill[c] = cptr[c];
+    for (int d = 0; d < n_rows; ++d)
+        for (int64_t k = row_ptr[d]; k < row_ptr[d + 1]; ++k) {
+            crow[fill[col_idx[k]]] = d;
+            cpos[fill[col_idx[k]]++] = k;
+        }
+    /* solver_routines.f90:14380-14620: in ascending equations row r, the RHS
+     * of a free row, then for a fixed DOF r the column r times its value moved
+     * to the right-hand side of every free row storing it */
+    for (int r = 0; r < n_rows; ++r) {
+        if (sidx[r] >= 0) out_vector[sidx[r]] += vec[r] * row_c;
+        if (sidx[r] >= 0) continue;
+        const double u = dofs[r];
+        if (!(fabs(u) >= OCO_ZERO_TOLERANCE)) continue;
+        for (int64_t q = cptr[r]; q < cptr[r + 1]; ++q) {
+            const int32_t d = crow[q];
+            const double K = values[cpos[q]];
+            if (!(fabs(K) >= OCO_ZERO_TOLERANCE) || sidx[d] < 0) continue;
+            out_vector[sidx[d]] += -1.0 * K * u * row_c;
+        }
+    }
+    free(fill);
+    free(cptr);
+    free(crow);
+    free(cpos);
+    free(sidx);
+    return n;
 }

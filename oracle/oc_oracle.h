@@ -100,6 +100,20 @@ int oco_mr_assemble_parallel(int nelem, const int32_t *elem_q, const int32_t *el
 int oco_bc_increment(int ndof, const int32_t *dof_idx, const double *full, double *current,
                      int iteration, int max_iterations);
 
+/* --- solver matrices (SOLVER_MATRICES_STATIC_ASSEMBLE, solver_routines.f90:
+ * 13745-14771; SOLVER_MATRIX_JACOBIAN_MATRIX_ADD, solver_matrices_routines.f90:
+ * 1258-1483) ----------------------------------------------------------------
+ * Solver rows/columns are the free DOFs (free_mask[d] != 0) in ascending
+ * order; coupling coefficients 1.  Returns the solver row count; the solver
+ * nnz goes to *nnz.  With out_row_ptr == NULL only the counts are returned;
+ * otherwise out_row_ptr[n+1], out_col_idx[nnz], out_values[nnz] and
+ * out_vector[n] are filled.  linear != 0: vector = RHS vec with the fixed
+ * columns times dofs moved over; linear == 0: vector = vec (the residual) on
+ * the free rows. */
+int oco_solver_system(int n_rows, const int64_t *row_ptr, const int32_t *col_idx, const double *values,
+                      const uint8_t *free_mask, const double *vec, const double *dofs, int linear, int64_t *nnz,
+                      int64_t *out_row_ptr, int32_t *out_col_idx, double *out_values, double *out_vector);
+
 /* --- splitmix64 uniform stream shared with the host generator ------------- */
 double oco_uniform(uint64_t seed, uint64_t index);
 

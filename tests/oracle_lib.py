@@ -14,6 +14,7 @@ _LIB_PATH = os.path.join(ROOT, "oracle", "liboc_oracle.so")
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
 def load():
@@ -44,6 +45,8 @@ def load():
                                              _i64, _i32, _dp, _dp, c_int]),
         "oco_bc_increment": (c_int, [c_int, _i32, _dp, _dp, c_int, c_int]),
         "oco_uniform": (c_double, [c_u64, c_u64]),
+        "oco_solver_system": (c_int, [c_int, _i64, _i32, _dp, _u8, _dp, _dp, c_int, ctypes.POINTER(c_i64),
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -219,3 +222,27 @@ def bc_increment(dof_idx, full, current, iteration, max_iterations):
                                   np.ascontiguousarray(full, np.float64), cur, iteration, max_iterations),
            "bc_increment")
     return cur
+
+
+def solver_system(row_ptr, col_idx, values, free_mask, vec, dofs, linear):
+    """Solver matrix (CSR, int64 offsets) and vector of one equations set:
+    SOLVER_MATRICES_STATIC_ASSEMBLE restated (oracle/oc_oracle.c)."""
+    row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+    col_idx = np.ascontiguousarray(col_idx, np.int32)
+    values = np.ascontiguousarray(values, np.float64)
+    mask = np.ascontiguousarray(free_mask, np.uint8)
+    vec = np.ascontiguousarray(vec, np.float64)
+    dofs = np.ascontiguousarray(dofs, np.float64)
+    n_rows = len(row_ptr) - 1
+    nnz = ctypes.c_int64()
+    n = lib().oco_solver_system(n_rows, row_ptr, col_idx, values, mask, vec, dofs, int(linear), ctypes.byref(nnz),
+                                None, None, None, None)
+    _check(0 if n >= 0 else n, "solver_system")
+    rp = np.zeros(n + 1, np.int64)
+    ci = np.zeros(max(nnz.value, 1), np.int32)
+    sv = np.zeros(max(nnz.value, 1))
+    sb = np.zeros(max(n, 1))
+    rc = lib().oco_solver_system(n_rows, row_ptr, col_idx, values, mask, vec, dofs, int(linear), ctypes.byref(nnz),
+                                 rp.ctypes.data, ci.ctypes.data, sv.ctypes.data, sb.ctypes.data)
+    _check(0 if rc >= 0 else rc, "solver_system")
+    return rp, ci[:nnz.value], sv[:nnz.value], sb[:n]

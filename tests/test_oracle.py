@@ -370,3 +370,86 @@ def test_parallel_baseline_matches_sequential_oracle():
     r_par, v_par = orc.mr_assemble_parallel(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, rp, ci, 4)
     assert np.linalg.norm(r_par - r_seq) <= 1e-13 * np.linalg.norm(r_seq)
     assert np.linalg.norm(v_par - v_seq) <= 1e-13 * np.linalg.norm(v_seq)
+
+
+# --- solver matrices (SOLVER_MATRICES_STATIC_ASSEMBLE) -----------------------
+def _laplace_system(n, jitter=0.0, seed=0):
+    mesh = RegularMesh.generate(1, n, jitter=jitter, seed=seed)
+    rp, ci = orc.pattern([(mesh.n_nodes, mesh.elem_nodes)])
+    vals, rhs = orc.laplace_assemble(mesh.elem_nodes, mesh.coords, rp, ci)
+    return mesh, rp, ci, vals, rhs
+
+
+def test_solver_system_patch_test():
+    """u = x on the config-1 BVP: the reduced system, with the x = 1 face moved
+    to the right-hand side, is solved by the x coordinates of the free nodes."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    mesh, rp, ci, vals, rhs = _laplace_system((4, 3, 2), jitter=0.15, seed=3)
+    fixed0, fixed1 = mesh.face_nodes(0, 0), mesh.face_nodes(0, 1)
+    mask = np.ones(mesh.n_nodes, np.uint8)
+    mask[fixed0] = 0
+    mask[fixed1] = 0
+    dofs = np.zeros(mesh.n_nodes)
+    dofs[fixed1] = 1.0
+    srp, sci, sv, sb = orc.solver_system(rp, ci, vals, mask, rhs, dofs, True)
+    n = int(mask.sum())
+    S = sp.csr_matrix((sv, sci, srp), shape=(n, n))
+    u = sla.spsolve(S.tocsc(), sb)
+    free = np.flatnonzero(mask)
+    assert np.abs(u - mesh.coords[0][free]).max() < 1e-12
+
+
+def test_solver_system_dense_restriction():
+    """Brute force: S = K[free][:, free] entry for entry (each solver entry
+    receives one contribution, so it is the equations value itself) and
+    b = RHS[free] - K[free][:, fixed] u[fixed]; structure = the nonzero pattern
+    of the restriction, columns ascending."""
+    rng = np.random.default_rng(5)
+    mesh, rp, ci, vals, _ = _laplace_system((2, 2, 1), jitter=0.1, seed=1)
+    nn = mesh.n_nodes
+    rhs = rng.standard_normal(nn)
+    mask = (rng.uniform(size=nn) > 0.35).astype(np.uint8)
+    dofs = rng.standard_normal(nn)
+    srp, sci, sv, sb = orc.solver_system(rp, ci, vals, mask, rhs, dofs, True)
+    K = np.zeros((nn, nn))
+    for r in range(nn):
+        K[r, ci[rp[r]:rp[r + 1]]] = vals[rp[r]:rp[r + 1]]
+    P = np.zeros((nn, nn), bool)
+    for r in range(nn):
+        P[r, ci[rp[r]:rp[r + 1]]] = True
+    free, fixed = np.flatnonzero(mask), np.flatnonzero(mask == 0)
+    n = free.size
+    Sd = np.zeros((n, n))
+    Pd = np.zeros((n, n), bool)
+    for i in range(n):
+        Sd[i, sci[srp[i]:srp[i + 1]]] = sv[srp[i]:srp[i + 1]]
+        Pd[i, sci[srp[i]:srp[i + 1]]] = True
+        assert np.all(np.diff(sci[srp[i]:srp[i + 1]]) > 0)
+    assert np.array_equal(Pd, P[np.ix_(free, free)])
+    assert np.array_equal(Sd, K[np.ix_(free, free)])
+    want = rhs[free] - K[np.ix_(free, fixed)] @ dofs[fixed]
+    assert np.abs(sb - want).max() < 1e-13 * max(1.0, np.abs(want).max())
+
+
+def test_solver_system_zero_tolerance_and_nonlinear():
+    """Fixed values below ZERO_TOLERANCE move nothing (solver_routines.f90:
+    14440-14445); a nonlinear set's vector is its residual on the free rows."""
+    mesh, rp, ci, vals, _ = _laplace_system((2, 1, 1))
+    nn = mesh.n_nodes
+    rhs = np.arange(nn, dtype=np.float64) + 0.5
+    mask = np.ones(nn, np.uint8)
+    mask[:3] = 0
+    dofs = np.zeros(nn)
+    dofs[:3] = 1e-17
+    _, _, _, sb = orc.solver_system(rp, ci, vals, mask, rhs, dofs, True)
+    assert np.array_equal(sb, rhs[3:])
+    dofs[:3] = 1.0
+    _, _, _, sb1 = orc.solver_system(rp, ci, vals, mask, rhs, dofs, True)
+    assert not np.array_equal(sb1, rhs[3:])
+    _, _, _, sr = orc.solver_system(rp, ci, vals, mask, rhs, dofs, False)
+    assert np.array_equal(sr, rhs[3:])
+    # no fixed DOFs: the solver system is the equations system
+    srp, sci, sv, sb2 = orc.solver_system(rp, ci, vals, np.ones(nn, np.uint8), rhs, dofs, True)
+    assert np.array_equal(srp, rp) and np.array_equal(sci, ci) and np.array_equal(sv, vals)
+    assert np.array_equal(sb2, rhs)
