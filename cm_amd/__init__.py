@@ -13,9 +13,9 @@ from ._lib import (CMISSError, OC_FP32, OC_FP64, OC_BUFFER_COL_IDX, OC_BUFFER_DO
 from .equations_set import (EquationsSet, LaplaceEquationsSet, MooneyRivlinBatch, MooneyRivlinEquationsSet,  # noqa: F401
                             event_elapsed, event_record, finalise, initialise, kernel_timing,
                             kernel_timing_enable, newton_options, synchronise)
-from .mesh import RegularMesh  # noqa: F401
+from .mesh import RegularMesh, UserMesh, as_user_mesh  # noqa: F401
 
 __all__ = ["CMISSError", "EquationsSet", "LaplaceEquationsSet", "MooneyRivlinBatch", "MooneyRivlinEquationsSet",
-           "RegularMesh",
+           "RegularMesh", "UserMesh", "as_user_mesh",
            "initialise", "finalise", "synchronise", "event_record", "event_elapsed", "OC_FP32", "OC_FP64",
            "newton_options"]

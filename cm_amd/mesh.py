@@ -114,3 +114,121 @@ class RegularMesh:
         idx = self.grid_index()[axis]
         last = self.order * self.n[axis]
         return np.nonzero(idx == (0 if side == 0 else last))[0].astype(np.int32)
+
+
+@dataclass
+class UserMesh:
+    """A mesh given by user-numbered nodes and elements, numbered as the
+    reference numbers a region's meshes.
+
+    * Region nodes (cmfe_Nodes) have global numbers in the order they are
+      given; their user numbers may be any distinct integers.
+    * Element nodes are given by user number in the basis' local node order
+      (cmfe_MeshElements_NodesSet) and translated to global numbers.
+    * A mesh component's nodes are the global nodes its elements use, sorted
+      ascending, and its local node number is the rank in that list
+      (MeshTopologyNodesCalculate, mesh_routines.f90:9006-9119).
+    * DOFs are separated by component over those local nodes
+      (FIELD_MAPPINGS_CALCULATE, field_routines.f90:10345-10628), which is the
+      layout the equations sets take.
+
+    ``order`` 1: trilinear (Laplace); 2: triquadratic geometry and displacement,
+    with the trilinear pressure component's elements in ``elem_nodes_l``.
+    """
+    order: int
+    elem_nodes: np.ndarray          # (E, 8 or 27), local node numbers of the main component
+    coords: np.ndarray              # (3, N) at the main component's nodes, local order
+    node_user_numbers: np.ndarray   # (N,) user number of each local node
+    elem_nodes_l: np.ndarray = None  # (E, 8) local numbers of the pressure component
+    node_user_numbers_l: np.ndarray = None
+    element_user_numbers: np.ndarray = None
+
+    @classmethod
+    def from_user_numbers(cls, order, node_user_numbers, coords, element_nodes, element_nodes_l=None,
+                          element_user_numbers=None):
+        """node_user_numbers (N_region,), coords (3, N_region) in the same
+        order, element_nodes (E, (order+1)^3) and, for order 2,
+        element_nodes_l (E, 8): user node numbers."""
+        user = np.asarray(node_user_numbers, np.int64)
+        xyz = np.asarray(coords, np.float64)
+        if user.ndim != 1 or xyz.shape != (3, user.size):
+            raise ValueError("coords must have shape (3, len(node_user_numbers))")
+        if np.unique(user).size != user.size:
+            raise ValueError("node user numbers must be distinct")
+        order_of = np.argsort(user, kind="stable")
+        sorted_user = user[order_of]
+
+        def to_global(e_user, what):
+            e_user = np.asarray(e_user, np.int64)
+            pos = np.searchsorted(sorted_user, e_user)
+            pos = np.minimum(pos, sorted_user.size - 1)
+            if not np.array_equal(sorted_user[pos], e_user):
+                bad = e_user[sorted_user[pos] != e_user].ravel()[0]
+                raise ValueError(f"{what}: user node {int(bad)} is not a region node")
+            return order_of[pos]  # 0-based global numbers
+
+        def component(e_user, nen, what):
+            e_user = np.asarray(e_user)
+            if e_user.ndim != 2 or e_user.shape[1] != nen:
+                raise ValueError(f"{what} must have shape (E, {nen})")
+            g = to_global(e_user, what)
+            used = np.unique(g)            # sorted global numbers of the component
+            local = np.searchsorted(used, g).astype(np.int32)
+            return used, local
+
+        nen = (order + 1) ** 3
+        used, local = component(element_nodes, nen, "element_nodes")
+        mesh = cls(order, local, np.ascontiguousarray(xyz[:, used]), user[used])
+        if order == 2:
+            if element_nodes_l is None:
+                raise ValueError("a triquadratic mesh needs the trilinear pressure component's element nodes")
+            used_l, local_l = component(element_nodes_l, 8, "element_nodes_l")
+            if local_l.shape[0] != local.shape[0]:
+                raise ValueError("both components must have the same elements")
+            mesh.elem_nodes_l = local_l
+            mesh.node_user_numbers_l = user[used_l]
+        n_el = mesh.elem_nodes.shape[0]
+        mesh.element_user_numbers = (np.arange(1, n_el + 1) if element_user_numbers is None
+                                     else np.asarray(element_user_numbers, np.int64))
+        return mesh
+
+    @property
+    def n_elements(self):
+        return self.elem_nodes.shape[0]
+
+    @property
+    def n_nodes(self):
+        return self.coords.shape[1]
+
+    @property
+    def n_nodes_l(self):
+        return 0 if self.node_user_numbers_l is None else self.node_user_numbers_l.size
+
+    def face_nodes(self, axis, side, tol=1e-12):
+        """Nodes of the main component on the plane x_axis = min (side 0) or
+        max (side 1) of the mesh."""
+        x = self.coords[axis]
+        target = x.min() if side == 0 else x.max()
+        return np.nonzero(np.abs(x - target) <= tol * max(1.0, abs(target)))[0].astype(np.int32)
+
+
+def as_user_mesh(mesh, region_order=None, user_numbers=None):
+    """A RegularMesh restated as user-numbered input: the region's nodes in
+    ``region_order`` (a permutation of the mesh nodes; default identity) with
+    ``user_numbers`` (default 1-based global), elements by user number."""
+    n = mesh.n_nodes
+    region_order = np.arange(n) if region_order is None else np.asarray(region_order)
+    user = np.arange(1, n + 1) if user_numbers is None else np.asarray(user_numbers)
+    user_of_node = np.empty(n, np.int64)
+    user_of_node[region_order] = user
+    e_user = user_of_node[mesh.elem_nodes]
+    e_user_l = None
+    if mesh.order == 2:
+        # corner (i, j, k) of the trilinear component is quadratic grid node (2i, 2j, 2k)
+        n1, n2, n3 = mesh.n
+        T1, T2 = 2 * n1 + 1, 2 * n2 + 1
+        i = np.arange(mesh.n_nodes_l)
+        ci, cj, ck = i % (n1 + 1), (i // (n1 + 1)) % (n2 + 1), i // ((n1 + 1) * (n2 + 1))
+        quad_of_corner = 2 * ci + 2 * cj * T1 + 2 * ck * T1 * T2
+        e_user_l = user_of_node[quad_of_corner[mesh.elem_nodes_l]]
+    return UserMesh.from_user_numbers(mesh.order, user, mesh.coords[:, region_order], e_user, e_user_l)
