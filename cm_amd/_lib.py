@@ -55,6 +55,13 @@ OC_PRECONDITIONER_BLOCK_DIAGONAL = 1
 OC_PRECONDITIONER_BLOCK_TRIANGULAR = 2
 OC_PRECONDITIONER_MULTIGRID = 3
 
+OC_CONVERGED_FNORM_ABS = 2
+OC_CONVERGED_FNORM_RELATIVE = 3
+OC_CONVERGED_SNORM_RELATIVE = 4
+OC_DIVERGED_FUNCTION_COUNT = -2
+OC_DIVERGED_FNORM_NAN = -4
+OC_DIVERGED_MAX_IT = -5
+
 
 class NewtonOptions(ctypes.Structure):
     """oc_newton_options (include/oc_asm.h)."""
@@ -70,6 +77,8 @@ class NewtonOptions(ctypes.Structure):
         ("line_search", ctypes.c_int),
         ("line_search_alpha", ctypes.c_double),
         ("preconditioner", ctypes.c_int),
+        ("solution_tolerance", ctypes.c_double),
+        ("maximum_function_evaluations", ctypes.c_int),
     ]
 
 
@@ -84,6 +93,7 @@ class NewtonStats(ctypes.Structure):
         ("initial_residual_norm", ctypes.c_double),
         ("final_residual_norm", ctypes.c_double),
         ("seconds", ctypes.c_double),
+        ("reason", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -825,8 +825,31 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
     stats->initial_residual_norm = plain_norm(nrm);
     double fnorm_prev = fnorm, eta_prev = 0.5;
     int it = 0;
-    while (!converged(nrm) && it < opt->maximum_iterations) {
-        if (!std::isfinite(fnorm)) break;
+    // PETSc's default convergence test (SNESConvergedDefault, the test the
+    // reference selects): a non-finite residual diverges; the residual test;
+    // the function-evaluation limit; after a step, ||step|| < stol ||x||.
+    // Deviation: the residual test (per mesh, abs or rel against ref_m) may
+    // also hold before the first step, so that a start already inside the
+    // tolerance (a warm start judged against the plain start) costs no step.
+    int reason = 0;
+    auto convergence_test = [&](double snorm, double xnorm) {
+        bool finite = true;
+        for (double v : nrm) finite = finite && std::isfinite(v);
+        if (!finite) {
+            reason = OC_DIVERGED_FNORM_NAN;
+        } else if (converged(nrm)) {
+            bool abs_all = true;
+            for (int m = 0; m < M; ++m) abs_all = abs_all && nrm[m] <= opt->absolute_tolerance;
+            reason = abs_all ? OC_CONVERGED_FNORM_ABS : OC_CONVERGED_FNORM_RELATIVE;
+        } else if (opt->maximum_function_evaluations >= 0 &&
+                   stats->residual_evaluations >= opt->maximum_function_evaluations) {
+            reason = OC_DIVERGED_FUNCTION_COUNT;
+        } else if (it > 0 && opt->solution_tolerance > 0.0 && snorm < opt->solution_tolerance * xnorm) {
+            reason = OC_CONVERGED_SNORM_RELATIVE;
+        }
+    };
+    convergence_test(0.0, 0.0);
+    while (reason == 0 && it < opt->maximum_iterations) {
         OC_CHECK(step_assembly(s, &K, opt->jacobian_precision));
         stats->jacobian_evaluations++;
         // forcing term: fixed, or Eisenstat-Walker (choice 2, gamma 0.9, alpha
@@ -856,7 +879,7 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
         stats->krylov_iterations += kit;
         // line search on f = 1/2 ||W F||^2 with slope -||W F||^2 (Newton direction)
         OC_HIP_CHECK(hipMemcpyAsync(x_prev, s.dofs.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
-        double lambda = 1.0, fnew = 0.0;
+        double lambda = 1.0, lambda_taken = 1.0, fnew = 0.0;
         const double f0 = 0.5 * fnorm * fnorm, slope = -fnorm * fnorm;
         double lambda_prev = 0.0, f_prev_trial = 0.0;
         std::vector<double> trial;
@@ -864,6 +887,7 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
             OC_HIP_CHECK(hipMemcpyAsync(s.dofs.ptr, x_prev, sizeof(double) * n, hipMemcpyDeviceToDevice,
                                         rt.stream));
             axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, lambda, dx, 1.0, s.dofs.ptr);
+            lambda_taken = lambda;
             OC_CHECK(residual_mesh_norms(s, trial));
             fnew = weighted_norm(trial, w);
             stats->residual_evaluations++;
@@ -893,11 +917,22 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
         fnorm_prev = fnorm;
         fnorm = fnew;
         ++it;
+        double snorm = 0.0, xnorm = 0.0;
+        if (opt->solution_tolerance > 0.0) {  // the step taken and the new iterate, free DOFs
+            std::vector<double> part;
+            OC_CHECK(mesh_norms(s, dx, part));
+            snorm = lambda_taken * plain_norm(part);
+            OC_CHECK(mesh_norms(s, s.dofs.ptr, part));
+            xnorm = plain_norm(part);
+        }
+        convergence_test(snorm, xnorm);
     }
+    if (reason == 0) reason = OC_DIVERGED_MAX_IT;
     OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
     stats->newton_iterations = it;
     stats->final_residual_norm = plain_norm(nrm);
-    stats->converged = converged(nrm) ? 1 : 0;
+    stats->reason = reason;
+    stats->converged = reason > 0 ? 1 : 0;
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (finals_out)
         for (int m = 0; m < M; ++m) finals_out[m] = nrm[m];
@@ -912,8 +947,8 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
             }
         }
         return set_error(OC_ERR_NOT_CONVERGED,
-                         "Newton did not converge after %d iterations: mesh %d ||F|| = %.3e > %.3e", it, worst,
-                         nrm[worst], target[worst]);
+                         "Newton did not converge after %d iterations (reason %d): mesh %d ||F|| = %.3e > %.3e", it,
+                         reason, worst, nrm[worst], target[worst]);
     }
     return OC_NO_ERROR;
 }
@@ -978,6 +1013,8 @@ int oc_newton_options_default(oc_newton_options *o)
     o->line_search = 1;
     o->line_search_alpha = 1e-4;
     o->preconditioner = OC_PRECONDITIONER_JACOBI;
+    o->solution_tolerance = 1e-5;
+    o->maximum_function_evaluations = 1000;
     return OC_NO_ERROR;
 }
 
@@ -1088,6 +1125,8 @@ int oc_linear_solve(oc_equations_set h, const oc_newton_options *opt, oc_newton_
     stats->newton_iterations = 1;
     stats->final_residual_norm = fres;
     stats->converged = fres <= std::max(target * 1.0001, opt->absolute_tolerance) ? 1 : 0;
+    stats->reason = stats->converged ? (fres <= opt->absolute_tolerance ? OC_CONVERGED_FNORM_ABS : OC_CONVERGED_FNORM_RELATIVE)
+                                     : OC_DIVERGED_MAX_IT;
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!stats->converged)
         return set_error(OC_ERR_NOT_CONVERGED, "linear solve did not converge: ||r|| = %.3e > %.3e", fres, target);

@@ -40,7 +40,9 @@ def solver_options(precision=_lib.OC_FP64, restart=100, reference_norm=0.0):
     return newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, reference_norm=reference_norm,
                           krylov_restart=restart, krylov_relative_tolerance=-1.0, jacobian_precision=precision,
                           preconditioner=_lib.OC_PRECONDITIONER_MULTIGRID, maximum_iterations=100,
-                          krylov_maximum_iterations=100000)
+                          krylov_maximum_iterations=100000,
+                          # judged by the residual alone: no step-size test, no evaluation limit
+                          solution_tolerance=0.0, maximum_function_evaluations=-1)
 
 
 def make_problem(case, jitter=0.1):

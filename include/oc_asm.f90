@@ -24,6 +24,9 @@ MODULE OC_ASM
       & OC_BUFFER_SOLVER_COL_IDX = 10, OC_BUFFER_SOLVER_VECTOR = 11
   INTEGER(C_INT), PARAMETER :: OC_PRECONDITIONER_JACOBI = 0, OC_PRECONDITIONER_BLOCK_DIAGONAL = 1, &
       & OC_PRECONDITIONER_BLOCK_TRIANGULAR = 2, OC_PRECONDITIONER_MULTIGRID = 3
+  INTEGER(C_INT), PARAMETER :: OC_CONVERGED_FNORM_ABS = 2, OC_CONVERGED_FNORM_RELATIVE = 3, &
+      & OC_CONVERGED_SNORM_RELATIVE = 4, OC_DIVERGED_FUNCTION_COUNT = -2, OC_DIVERGED_FNORM_NAN = -4, &
+      & OC_DIVERGED_MAX_IT = -5
 
   !> oc_newton_options (field order and types as in oc_asm.h)
   TYPE, BIND(C) :: OC_NEWTON_OPTIONS
@@ -33,12 +36,15 @@ MODULE OC_ASM
     INTEGER(C_INT) :: line_search
     REAL(C_DOUBLE) :: line_search_alpha
     INTEGER(C_INT) :: preconditioner
+    REAL(C_DOUBLE) :: solution_tolerance
+    INTEGER(C_INT) :: maximum_function_evaluations
   END TYPE OC_NEWTON_OPTIONS
 
   !> oc_newton_stats
   TYPE, BIND(C) :: OC_NEWTON_STATS
     INTEGER(C_INT) :: converged, newton_iterations, krylov_iterations, residual_evaluations, jacobian_evaluations
     REAL(C_DOUBLE) :: initial_residual_norm, final_residual_norm, seconds
+    INTEGER(C_INT) :: reason
   END TYPE OC_NEWTON_STATS
 
   INTERFACE

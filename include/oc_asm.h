@@ -229,7 +229,19 @@ typedef struct {
     int line_search;             /* 0: full steps, 1: backtracking (cubic/quadratic, alpha below) */
     double line_search_alpha;    /* sufficient-decrease constant (reference default 1e-4) */
     int preconditioner;          /* OC_PRECONDITIONER_* */
+    double solution_tolerance;   /* stol: converged when ||step|| < stol * ||x|| over the free DOFs; 0 disables */
+    int maximum_function_evaluations; /* residual evaluations allowed (< 0: no limit) */
 } oc_newton_options;
+
+/* Why a Newton solve stopped (oc_newton_stats.reason): the codes of PETSc's
+ * SNESConvergedReason for the default convergence test the reference selects
+ * (SOLVER_NEWTON_CONVERGENCE_PETSC_DEFAULT, solver_routines.f90:17780-17851). */
+#define OC_CONVERGED_FNORM_ABS 2       /* ||F|| <= absolute_tolerance */
+#define OC_CONVERGED_FNORM_RELATIVE 3  /* ||F|| <= relative_tolerance * ||F_ref|| */
+#define OC_CONVERGED_SNORM_RELATIVE 4  /* ||step|| < solution_tolerance * ||x|| */
+#define OC_DIVERGED_FUNCTION_COUNT -2  /* maximum_function_evaluations reached */
+#define OC_DIVERGED_FNORM_NAN -4       /* non-finite residual */
+#define OC_DIVERGED_MAX_IT -5          /* maximum_iterations reached */
 
 typedef struct {
     int converged;
@@ -240,11 +252,12 @@ typedef struct {
     double initial_residual_norm;
     double final_residual_norm;
     double seconds;
+    int reason;                  /* OC_CONVERGED_* (> 0) or OC_DIVERGED_* (< 0) */
 } oc_newton_stats;
 
 /* Reference defaults (solver_routines.f90:17780-17851, 10957-11006): abs tol
- * 1e-10, rel tol 1e-5, 50 iterations, line search alpha 1e-4, GMRES(30),
- * rtol 1e-5, Jacobi. */
+ * 1e-10, rel tol 1e-5, solution tol 1e-5, 50 iterations, 1000 function
+ * evaluations, line search alpha 1e-4, GMRES(30), rtol 1e-5, Jacobi. */
 int oc_newton_options_default(oc_newton_options *options);
 /* Newton solve of R(x) = 0 at the current Dirichlet values for a Mooney-
  * Rivlin set (SOLVER_NEWTON_LINESEARCH_SOLVE, solver_routines.f90:18871-

@@ -110,7 +110,7 @@ def test_batched_newton_meets_every_mesh_tolerance(gpu):
     batch = cm.MooneyRivlinBatch(meshes, [c["c10"] for c in cases], [c["c01"] for c in cases])
     batch.set_dofs(batch.undeformed_dofs())
     fixed = batch.uniaxial_boundary_conditions([c["s"] for c in cases])
-    opts = cm.newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=100,
+    opts = cm.newton_options(solution_tolerance=0.0, relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=100,
                              krylov_relative_tolerance=-1.0, preconditioner=cm.OC_PRECONDITIONER_BLOCK_TRIANGULAR,
                              maximum_iterations=30)
     for step in (1, 2):
@@ -163,7 +163,7 @@ def test_batched_newton_with_multigrid(gpu):
     batch.set_dofs(batch.undeformed_dofs())
     fixed = batch.uniaxial_boundary_conditions([c["s"] for c in cases])
     batch.set_multigrid()
-    opts = cm.newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=100,
+    opts = cm.newton_options(solution_tolerance=0.0, relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=100,
                              krylov_relative_tolerance=-1.0, preconditioner=cm.OC_PRECONDITIONER_MULTIGRID,
                              maximum_iterations=30)
     for step in (1, 2):

@@ -39,7 +39,7 @@ def test_mr_homogeneous_uniaxial_solution(gpu):
     eq = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
     eq.set_dofs(eq.undeformed_dofs())
     fixed = eq.uniaxial_boundary_conditions(lam)
-    opts = cm.newton_options(relative_tolerance=1e-12, absolute_tolerance=1e-13, krylov_restart=200,
+    opts = cm.newton_options(solution_tolerance=0.0, relative_tolerance=1e-12, absolute_tolerance=1e-13, krylov_restart=200,
                              krylov_relative_tolerance=-1.0, preconditioner=cm.OC_PRECONDITIONER_BLOCK_DIAGONAL)
     for step in (1, 2):
         eq.load_increment_apply(step, 2)
@@ -63,7 +63,7 @@ def test_mr_newton_jittered_judged_by_oracle(gpu, precision, precond):
     eq = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
     eq.set_dofs(eq.undeformed_dofs())
     fixed = eq.uniaxial_boundary_conditions(1.1)
-    opts = cm.newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=300,
+    opts = cm.newton_options(solution_tolerance=0.0, relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=300,
                              krylov_relative_tolerance=-1.0, jacobian_precision=precision, preconditioner=precond,
                              maximum_iterations=30)
     for step in (1, 2, 3):
@@ -87,11 +87,43 @@ def test_newton_failure_modes_forced_from_host(gpu):
     assert exc.value.code == -5 and "did not converge" in str(exc.value)
     st = eq.newton_solve(cm.newton_options(maximum_iterations=1, relative_tolerance=1e-12),
                          raise_on_failure=False)
-    assert not st["converged"]
+    assert not st["converged"] and st["reason"] == cm.OC_DIVERGED_MAX_IT
+    # function-evaluation limit (SNES_DIVERGED_FUNCTION_COUNT)
+    st = eq.newton_solve(cm.newton_options(maximum_function_evaluations=2, relative_tolerance=1e-12,
+                                           solution_tolerance=0.0), raise_on_failure=False)
+    assert not st["converged"] and st["reason"] == cm.OC_DIVERGED_FUNCTION_COUNT
+    assert st["residual_evaluations"] >= 2
     # NaN material constant (DIVERGED_FNORM_NAN analogue): stops, reports, never hangs
     eq.set_materials(float("nan"), 6.0)
     st = eq.newton_solve(cm.newton_options(maximum_iterations=5), raise_on_failure=False)
     assert not st["converged"] and not np.isfinite(st["final_residual_norm"])
+    assert st["reason"] == cm.OC_DIVERGED_FNORM_NAN
+
+
+def test_newton_petsc_default_convergence_tests(gpu):
+    """SNESConvergedDefault as the reference selects it: the step test
+    (||step|| < stol ||x||) ends a solve whose residual target is out of reach;
+    an absolute tolerance above the starting residual ends it before a step."""
+    cm = gpu
+    mesh = cm.RegularMesh.generate(2, 2, jitter=0.1, seed=8)
+    eq = cm.MooneyRivlinEquationsSet(mesh, 2.0, 6.0)
+    eq.set_dofs(eq.undeformed_dofs())
+    eq.uniaxial_boundary_conditions(1.05)
+    eq.load_increment_apply(1, 1)
+    start = eq.get_dofs()
+    ref = eq.residual_norm()
+    st = eq.newton_solve(cm.newton_options(relative_tolerance=1e-30, absolute_tolerance=0.0, solution_tolerance=1e-3,
+                                           krylov_restart=100))
+    assert st["converged"] and st["reason"] == cm.OC_CONVERGED_SNORM_RELATIVE
+    assert 1 <= st["newton_iterations"] < 50 and st["final_residual_norm"] > 1e-30 * ref
+    eq.set_dofs(start)
+    st = eq.newton_solve(cm.newton_options(absolute_tolerance=10.0 * ref))
+    assert st["converged"] and st["reason"] == cm.OC_CONVERGED_FNORM_ABS and st["newton_iterations"] == 0
+    eq.set_dofs(start)
+    st = eq.newton_solve(cm.newton_options(solution_tolerance=0.0, relative_tolerance=1e-8, absolute_tolerance=0.0,
+                                           krylov_restart=100))
+    assert st["converged"] and st["reason"] == cm.OC_CONVERGED_FNORM_RELATIVE
+    eq.destroy()
 
 
 def test_multigrid_preconditioner_judged_by_oracle(gpu):
@@ -107,7 +139,7 @@ def test_multigrid_preconditioner_judged_by_oracle(gpu):
         fixed = eq.uniaxial_boundary_conditions(1.1)
         if precond == cm.OC_PRECONDITIONER_MULTIGRID:
             eq.set_multigrid()
-        opts = cm.newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=100,
+        opts = cm.newton_options(solution_tolerance=0.0, relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=100,
                                  krylov_relative_tolerance=-1.0, preconditioner=precond, maximum_iterations=30)
         counts[precond] = 0
         for step in (1, 2):
