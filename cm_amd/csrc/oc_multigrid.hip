@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <memory>
+#include <utility>
 
 #include "oc_fe_device.h"
 #include "oc_internal.h"
@@ -127,45 +128,64 @@ q1_stencil_kernel(int M, const int *__restrict__ n_mesh, const int *__restrict__
     }
 }
 
-// Galerkin coarsening of a stencil level (masked by the fine level's free DOFs)
-__global__ void __launch_bounds__(256)
+// Galerkin coarsening of a stencil level (masked by the fine level's free
+// DOFs): coarse block (I, J) = sum over fine nodes i near I and j near J of
+// w_i w_j A_f(i, j).  A block of 243 threads takes nine (coarse node,
+// direction) pairs; thread (pair, s) sums fine node s of I's neighbourhood
+// against every j, and the 27 partial blocks are added in s order, so the
+// sum is deterministic and one pair's work is spread over 27 threads (the
+// coarse levels have few nodes).
+constexpr int kGalPairs = 9;
+
+__global__ void __launch_bounds__(27 * kGalPairs)
 galerkin_coarsen_kernel(int M, const int *__restrict__ dim_f, const int *__restrict__ off_f,
                         const double *__restrict__ Af, const uint8_t *__restrict__ freed_f,
                         const int *__restrict__ dim_c, const int *__restrict__ off_c, int nn_c,
                         double *__restrict__ Ac)
 {
+    __shared__ double s_part[kGalPairs][27][9];
+    const int p = threadIdx.x / 27, s = threadIdx.x % 27;
     const int64_t total = static_cast<int64_t>(nn_c) * 27;
-    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
-         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kGalPairs + p;
+    double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (t < total) {
         const int node = static_cast<int>(t / 27), d = static_cast<int>(t % 27);
         const int m = find_mesh(off_c, M, node);
         const int Nc = dim_c[m], Nf = dim_f[m], local = node - off_c[m];
         const int I[3] = {local % Nc, (local / Nc) % Nc, local / (Nc * Nc)};
         const int J[3] = {I[0] + d % 3 - 1, I[1] + (d / 3) % 3 - 1, I[2] + d / 9 - 1};
-        double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if (J[0] >= 0 && J[0] < Nc && J[1] >= 0 && J[1] < Nc && J[2] >= 0 && J[2] < Nc) {
-            for (int s = 0; s < 27; ++s) {
-                const int i[3] = {2 * I[0] + s % 3 - 1, 2 * I[1] + (s / 3) % 3 - 1, 2 * I[2] + s / 9 - 1};
-                if (i[0] < 0 || i[0] >= Nf || i[1] < 0 || i[1] >= Nf || i[2] < 0 || i[2] >= Nf) continue;
-                const double wi = lin_weight(s % 3 - 1) * lin_weight((s / 3) % 3 - 1) * lin_weight(s / 9 - 1);
-                const int64_t fi = off_f[m] + i[0] + static_cast<int64_t>(Nf) * (i[1] + static_cast<int64_t>(Nf) * i[2]);
-                for (int q = 0; q < 27; ++q) {
-                    const int j[3] = {2 * J[0] + q % 3 - 1, 2 * J[1] + (q / 3) % 3 - 1, 2 * J[2] + q / 9 - 1};
-                    if (j[0] < 0 || j[0] >= Nf || j[1] < 0 || j[1] >= Nf || j[2] < 0 || j[2] >= Nf) continue;
-                    const int e0 = j[0] - i[0], e1 = j[1] - i[1], e2 = j[2] - i[2];
-                    if (e0 < -1 || e0 > 1 || e1 < -1 || e1 > 1 || e2 < -1 || e2 > 1) continue;
-                    const double w = wi * lin_weight(q % 3 - 1) * lin_weight((q / 3) % 3 - 1) * lin_weight(q / 9 - 1);
-                    const int64_t fj = off_f[m] + j[0] + static_cast<int64_t>(Nf) * (j[1] + static_cast<int64_t>(Nf) * j[2]);
-                    const double *a = Af + (fi * 27 + (e0 + 1) + 3 * (e1 + 1) + 9 * (e2 + 1)) * 9;
-                    for (int c = 0; c < 3; ++c) {
-                        if (!freed_f[fi * 3 + c]) continue;
-                        for (int cp = 0; cp < 3; ++cp)
-                            if (freed_f[fj * 3 + cp]) blk[3 * c + cp] += w * a[3 * c + cp];
-                    }
+        const int i[3] = {2 * I[0] + s % 3 - 1, 2 * I[1] + (s / 3) % 3 - 1, 2 * I[2] + s / 9 - 1};
+        if (J[0] >= 0 && J[0] < Nc && J[1] >= 0 && J[1] < Nc && J[2] >= 0 && J[2] < Nc && i[0] >= 0 && i[0] < Nf &&
+            i[1] >= 0 && i[1] < Nf && i[2] >= 0 && i[2] < Nf) {
+            const double wi = lin_weight(s % 3 - 1) * lin_weight((s / 3) % 3 - 1) * lin_weight(s / 9 - 1);
+            const int64_t fi = off_f[m] + i[0] + static_cast<int64_t>(Nf) * (i[1] + static_cast<int64_t>(Nf) * i[2]);
+            for (int q = 0; q < 27; ++q) {
+                const int j[3] = {2 * J[0] + q % 3 - 1, 2 * J[1] + (q / 3) % 3 - 1, 2 * J[2] + q / 9 - 1};
+                if (j[0] < 0 || j[0] >= Nf || j[1] < 0 || j[1] >= Nf || j[2] < 0 || j[2] >= Nf) continue;
+                const int e0 = j[0] - i[0], e1 = j[1] - i[1], e2 = j[2] - i[2];
+                if (e0 < -1 || e0 > 1 || e1 < -1 || e1 > 1 || e2 < -1 || e2 > 1) continue;
+                const double w = wi * lin_weight(q % 3 - 1) * lin_weight((q / 3) % 3 - 1) * lin_weight(q / 9 - 1);
+                const int64_t fj = off_f[m] + j[0] + static_cast<int64_t>(Nf) * (j[1] + static_cast<int64_t>(Nf) * j[2]);
+                const double *a = Af + (fi * 27 + (e0 + 1) + 3 * (e1 + 1) + 9 * (e2 + 1)) * 9;
+                for (int c = 0; c < 3; ++c) {
+                    if (!freed_f[fi * 3 + c]) continue;
+                    for (int cp = 0; cp < 3; ++cp)
+                        if (freed_f[fj * 3 + cp]) blk[3 * c + cp] += w * a[3 * c + cp];
                 }
             }
         }
-        for (int k = 0; k < 9; ++k) Ac[t * 9 + k] = blk[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s_part[p][s][k] = blk[k];
+    __syncthreads();
+    if (threadIdx.x < kGalPairs * 9) {
+        const int pp = threadIdx.x / 9, k = threadIdx.x % 9;
+        const int64_t tt = static_cast<int64_t>(blockIdx.x) * kGalPairs + pp;
+        if (tt < total) {
+            double acc = 0.0;
+            for (int ss = 0; ss < 27; ++ss) acc += s_part[pp][ss][k];
+            Ac[tt * 9 + k] = acc;
+        }
     }
 }
 
@@ -404,35 +424,6 @@ __global__ void __launch_bounds__(256) prolong_q2_kernel(int M, const int *__res
     }
 }
 
-// levels >= 1: r = b - A x on free DOFs (27-point block stencil)
-__global__ void __launch_bounds__(256) stencil_residual_kernel(int M, const int *__restrict__ dim,
-                                                              const int *__restrict__ off, int nn,
-                                                              const double *__restrict__ A,
-                                                              const uint8_t *__restrict__ freed,
-                                                              const double *__restrict__ b,
-                                                              const double *__restrict__ x, double *__restrict__ r)
-{
-    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn; node += gridDim.x * blockDim.x) {
-        const int m = find_mesh(off, M, node);
-        const int N = dim[m], local = node - off[m];
-        const int I[3] = {local % N, (local / N) % N, local / (N * N)};
-        double acc[3] = {b[static_cast<int64_t>(node) * 3], b[static_cast<int64_t>(node) * 3 + 1],
-                         b[static_cast<int64_t>(node) * 3 + 2]};
-        for (int d = 0; d < 27; ++d) {
-            const int J[3] = {I[0] + d % 3 - 1, I[1] + (d / 3) % 3 - 1, I[2] + d / 9 - 1};
-            if (J[0] < 0 || J[0] >= N || J[1] < 0 || J[1] >= N || J[2] < 0 || J[2] >= N) continue;
-            const int64_t j = off[m] + J[0] + static_cast<int64_t>(N) * (J[1] + static_cast<int64_t>(N) * J[2]);
-            const double *a = A + (static_cast<int64_t>(node) * 27 + d) * 9;
-            const double x0 = x[j * 3], x1 = x[j * 3 + 1], x2 = x[j * 3 + 2];
-            for (int c = 0; c < 3; ++c) acc[c] -= a[3 * c] * x0 + a[3 * c + 1] * x1 + a[3 * c + 2] * x2;
-        }
-        for (int c = 0; c < 3; ++c) {
-            const int64_t k = static_cast<int64_t>(node) * 3 + c;
-            r[k] = freed[k] ? acc[c] : 0.0;
-        }
-    }
-}
-
 __global__ void __launch_bounds__(256) stencil_jacobi_kernel(int nn, const double *__restrict__ Dinv,
                                                             const double *__restrict__ r, double *__restrict__ x,
                                                             double omega, int first)
@@ -444,6 +435,61 @@ __global__ void __launch_bounds__(256) stencil_jacobi_kernel(int nn, const doubl
             const int64_t k = static_cast<int64_t>(node) * 3 + c;
             const double d = o[3 * c] * rr[0] + o[3 * c + 1] * rr[1] + o[3 * c + 2] * rr[2];
             x[k] = (first ? 0.0 : x[k]) + omega * d;
+        }
+    }
+}
+
+// levels >= 1, one 32-lane group per node (the coarse levels have few
+// nodes, so a node's 27 neighbour products run in parallel): lane d < 27
+// forms A(node, d) x(neighbour d), and a fixed butterfly adds the 27
+// products.  kSweep 0: out = r = b - A x on free DOFs.  kSweep 1: one Jacobi
+// sweep, out = x + omega Dinv r (out and x distinct buffers).
+template <int kSweep>
+__global__ void __launch_bounds__(256) stencil_apply_kernel(int M, const int *__restrict__ dim,
+                                                           const int *__restrict__ off, int nn,
+                                                           const double *__restrict__ A,
+                                                           const uint8_t *__restrict__ freed,
+                                                           const double *__restrict__ b,
+                                                           const double *__restrict__ x,
+                                                           const double *__restrict__ Dinv, double omega,
+                                                           double *__restrict__ out)
+{
+    const int node = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 5);
+    const int d = threadIdx.x & 31;
+    if (node >= nn) return;  // whole 32-lane groups
+    const int m = find_mesh(off, M, node);
+    const int N = dim[m], local = node - off[m];
+    double p[3] = {0.0, 0.0, 0.0};
+    if (d < 27) {
+        const int J[3] = {local % N + d % 3 - 1, (local / N) % N + (d / 3) % 3 - 1, local / (N * N) + d / 9 - 1};
+        if (J[0] >= 0 && J[0] < N && J[1] >= 0 && J[1] < N && J[2] >= 0 && J[2] < N) {
+            const int64_t j = off[m] + J[0] + static_cast<int64_t>(N) * (J[1] + static_cast<int64_t>(N) * J[2]);
+            const double *a = A + (static_cast<int64_t>(node) * 27 + d) * 9;
+            const double x0 = x[j * 3], x1 = x[j * 3 + 1], x2 = x[j * 3 + 2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p[c] = a[3 * c] * x0 + a[3 * c + 1] * x1 + a[3 * c + 2] * x2;
+        }
+    }
+#pragma unroll
+    for (int sh = 16; sh > 0; sh >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] += __shfl_xor(p[c], sh, 32);
+    if (d != 0) return;
+    double r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int64_t k = static_cast<int64_t>(node) * 3 + c;
+        r[c] = freed[k] ? b[k] - p[c] : 0.0;
+    }
+    if (kSweep == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[static_cast<int64_t>(node) * 3 + c] = r[c];
+    } else {
+        const double *o = Dinv + static_cast<int64_t>(node) * 9;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int64_t k = static_cast<int64_t>(node) * 3 + c;
+            out[k] = x[k] + omega * (o[3 * c] * r[0] + o[3 * c + 1] * r[1] + o[3 * c + 2] * r[2]);
         }
     }
 }
@@ -556,7 +602,8 @@ int mg_setup(EquationsSet &s, int precision)
     const int nl = static_cast<int>(g.levels.size());
     for (int l = 1; l + 1 < nl; ++l) {
         MgLevel &F = g.levels[l], &C = g.levels[l + 1];
-        galerkin_coarsen_kernel<<<grid1(static_cast<int64_t>(C.nn) * 27), 256, 0, rt.stream>>>(
+        galerkin_coarsen_kernel<<<static_cast<unsigned>((static_cast<int64_t>(C.nn) * 27 + kGalPairs - 1) / kGalPairs),
+                                  27 * kGalPairs, 0, rt.stream>>>(
             M, F.dim.ptr, F.off.ptr, F.A.ptr, F.freed.ptr, C.dim.ptr, C.off.ptr, C.nn, C.A.ptr);
         coincident_kernel<<<grid1(C.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.freed.ptr, C.dim.ptr,
                                                              C.off.ptr, C.nn, C.freed.ptr);
@@ -569,6 +616,20 @@ int mg_setup(EquationsSet &s, int precision)
     coarse_inverse_kernel<<<static_cast<unsigned>(M), 256, 0, rt.stream>>>(M, Lc.dim.ptr, Lc.off.ptr, Lc.A.ptr,
                                                                          g.coarse_stride, g.coarse_inv.ptr);
     OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
+
+static unsigned group_grid(int nn) { return static_cast<unsigned>((static_cast<int64_t>(nn) * 32 + 255) / 256); }
+
+// one Jacobi sweep of a stencil level: x <- x + omega Dinv (b - A x), through
+// the level's r buffer (the two buffers trade places)
+static int stencil_sweep(int M, MgLevel &F, double omega)
+{
+    stencil_apply_kernel<1><<<group_grid(F.nn), 256, 0, runtime().stream>>>(
+        M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr, F.freed.ptr, F.b.ptr, F.x.ptr, F.Dinv.ptr, omega, F.r.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    std::swap(F.x, F.r);
     return OC_NO_ERROR;
 }
 
@@ -605,13 +666,10 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     for (int l = 1; l + 1 < nl; ++l) {
         MgLevel &F = g.levels[l], &C = g.levels[l + 1];
         stencil_jacobi_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(F.nn, F.Dinv.ptr, F.b.ptr, F.x.ptr, g.omega, 1);
-        for (int k = 1; k < g.pre; ++k) {
-            stencil_residual_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
-                                                                       F.freed.ptr, F.b.ptr, F.x.ptr, F.r.ptr);
-            stencil_jacobi_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(F.nn, F.Dinv.ptr, F.r.ptr, F.x.ptr, g.omega, 0);
-        }
-        stencil_residual_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
-                                                                   F.freed.ptr, F.b.ptr, F.x.ptr, F.r.ptr);
+        for (int k = 1; k < g.pre; ++k) OC_CHECK(stencil_sweep(M, F, g.omega));
+        stencil_apply_kernel<0><<<group_grid(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
+                                                                        F.freed.ptr, F.b.ptr, F.x.ptr, nullptr, 0.0,
+                                                                        F.r.ptr);
         restrict_stencil_kernel<<<grid1(C.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, C.dim.ptr, C.off.ptr,
                                                                    C.nn, C.freed.ptr, F.r.ptr, C.b.ptr);
         OC_HIP_CHECK(hipGetLastError());
@@ -625,11 +683,7 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
         MgLevel &F = g.levels[l], &C = g.levels[l + 1];
         prolong_stencil_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, C.dim.ptr,
                                                                   C.off.ptr, F.freed.ptr, C.x.ptr, F.x.ptr);
-        for (int k = 0; k < g.post; ++k) {
-            stencil_residual_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
-                                                                       F.freed.ptr, F.b.ptr, F.x.ptr, F.r.ptr);
-            stencil_jacobi_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(F.nn, F.Dinv.ptr, F.r.ptr, F.x.ptr, g.omega, 0);
-        }
+        for (int k = 0; k < g.post; ++k) OC_CHECK(stencil_sweep(M, F, g.omega));
         OC_HIP_CHECK(hipGetLastError());
     }
     prolong_q2_kernel<<<grid1(nq), 256, 0, rt.stream>>>(M, g.n.ptr, s.q_off.ptr, nq, L1.off.ptr, s.free_mask.ptr,
