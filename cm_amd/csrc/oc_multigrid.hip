@@ -370,22 +370,33 @@ __global__ void __launch_bounds__(256) restrict_q2_kernel(int M, const int *__re
                                                          const uint8_t *__restrict__ freed1,
                                                          const double *__restrict__ r0, double *__restrict__ b1)
 {
-    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn1; node += gridDim.x * blockDim.x) {
-        const int m = find_mesh(off1, M, node);
-        const int n = n_mesh[m], N = n + 1, T = 2 * n + 1, local = node - off1[m];
-        const int I[3] = {local % N, (local / N) % N, local / (N * N)};
-        double acc[3] = {0.0, 0.0, 0.0};
-        for (int s = 0; s < 27; ++s) {
-            const int i[3] = {2 * I[0] + s % 3 - 1, 2 * I[1] + (s / 3) % 3 - 1, 2 * I[2] + s / 9 - 1};
-            if (i[0] < 0 || i[0] >= T || i[1] < 0 || i[1] >= T || i[2] < 0 || i[2] >= T) continue;
+    // one 32-lane group per coarse node: lane s < 27 takes fine node s of its
+    // neighbourhood, a fixed butterfly adds them
+    const int node = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 5);
+    const int s = threadIdx.x & 31;
+    if (node >= nn1) return;  // whole 32-lane groups
+    const int m = find_mesh(off1, M, node);
+    const int n = n_mesh[m], N = n + 1, T = 2 * n + 1, local = node - off1[m];
+    const int I[3] = {local % N, (local / N) % N, local / (N * N)};
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (s < 27) {
+        const int i[3] = {2 * I[0] + s % 3 - 1, 2 * I[1] + (s / 3) % 3 - 1, 2 * I[2] + s / 9 - 1};
+        if (i[0] >= 0 && i[0] < T && i[1] >= 0 && i[1] < T && i[2] >= 0 && i[2] < T) {
             const double w = lin_weight(s % 3 - 1) * lin_weight((s / 3) % 3 - 1) * lin_weight(s / 9 - 1);
             const int64_t q = q_off[m] + i[0] + static_cast<int64_t>(T) * (i[1] + static_cast<int64_t>(T) * i[2]);
-            for (int c = 0; c < 3; ++c) acc[c] += w * r0[static_cast<int64_t>(c) * nq_total + q];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c] = w * r0[static_cast<int64_t>(c) * nq_total + q];
         }
-        for (int c = 0; c < 3; ++c) {
-            const int64_t k = static_cast<int64_t>(node) * 3 + c;
-            b1[k] = freed1[k] ? acc[c] : 0.0;
-        }
+    }
+#pragma unroll
+    for (int sh = 16; sh > 0; sh >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] += __shfl_xor(acc[c], sh, 32);
+    if (s != 0) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int64_t k = static_cast<int64_t>(node) * 3 + c;
+        b1[k] = freed1[k] ? acc[c] : 0.0;
     }
 }
 
@@ -501,22 +512,32 @@ __global__ void __launch_bounds__(256) restrict_stencil_kernel(int M, const int 
                                                               const uint8_t *__restrict__ freed_c,
                                                               const double *__restrict__ rf, double *__restrict__ bc)
 {
-    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn_c; node += gridDim.x * blockDim.x) {
-        const int m = find_mesh(off_c, M, node);
-        const int Nc = dim_c[m], Nf = dim_f[m], local = node - off_c[m];
-        const int I[3] = {local % Nc, (local / Nc) % Nc, local / (Nc * Nc)};
-        double acc[3] = {0.0, 0.0, 0.0};
-        for (int s = 0; s < 27; ++s) {
-            const int i[3] = {2 * I[0] + s % 3 - 1, 2 * I[1] + (s / 3) % 3 - 1, 2 * I[2] + s / 9 - 1};
-            if (i[0] < 0 || i[0] >= Nf || i[1] < 0 || i[1] >= Nf || i[2] < 0 || i[2] >= Nf) continue;
+    // one 32-lane group per coarse node, as restrict_q2_kernel
+    const int node = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 5);
+    const int s = threadIdx.x & 31;
+    if (node >= nn_c) return;  // whole 32-lane groups
+    const int m = find_mesh(off_c, M, node);
+    const int Nc = dim_c[m], Nf = dim_f[m], local = node - off_c[m];
+    const int I[3] = {local % Nc, (local / Nc) % Nc, local / (Nc * Nc)};
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (s < 27) {
+        const int i[3] = {2 * I[0] + s % 3 - 1, 2 * I[1] + (s / 3) % 3 - 1, 2 * I[2] + s / 9 - 1};
+        if (i[0] >= 0 && i[0] < Nf && i[1] >= 0 && i[1] < Nf && i[2] >= 0 && i[2] < Nf) {
             const double w = lin_weight(s % 3 - 1) * lin_weight((s / 3) % 3 - 1) * lin_weight(s / 9 - 1);
             const int64_t f = off_f[m] + i[0] + static_cast<int64_t>(Nf) * (i[1] + static_cast<int64_t>(Nf) * i[2]);
-            for (int c = 0; c < 3; ++c) acc[c] += w * rf[f * 3 + c];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c] = w * rf[f * 3 + c];
         }
-        for (int c = 0; c < 3; ++c) {
-            const int64_t k = static_cast<int64_t>(node) * 3 + c;
-            bc[k] = freed_c[k] ? acc[c] : 0.0;
-        }
+    }
+#pragma unroll
+    for (int sh = 16; sh > 0; sh >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] += __shfl_xor(acc[c], sh, 32);
+    if (s != 0) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int64_t k = static_cast<int64_t>(node) * 3 + c;
+        bc[k] = freed_c[k] ? acc[c] : 0.0;
     }
 }
 
@@ -560,17 +581,22 @@ __global__ void __launch_bounds__(256) coarse_apply_kernel(int M, const int *__r
                                                           const double *__restrict__ inv, int stride,
                                                           const double *__restrict__ b, double *__restrict__ x)
 {
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 3 * nn; k += gridDim.x * blockDim.x) {
-        const int node = k / 3;
-        const int m = find_mesh(off, M, node);
-        const int N = dim[m], nd = 3 * N * N * N;
-        const int r = k - 3 * off[m];
-        const double *row = inv + static_cast<int64_t>(m) * stride + static_cast<int64_t>(r) * nd;
-        const double *bm = b + static_cast<int64_t>(off[m]) * 3;
-        double acc = 0.0;
-        for (int j = 0; j < nd; ++j) acc += row[j] * bm[j];
-        x[k] = acc;
-    }
+    // one wavefront per coarse DOF: its row of the dense inverse against b,
+    // lanes over the row, a fixed butterfly for the sum
+    const int k = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= 3 * nn) return;  // whole wavefronts
+    const int node = k / 3;
+    const int m = find_mesh(off, M, node);
+    const int N = dim[m], nd = 3 * N * N * N;
+    const int r = k - 3 * off[m];
+    const double *row = inv + static_cast<int64_t>(m) * stride + static_cast<int64_t>(r) * nd;
+    const double *bm = b + static_cast<int64_t>(off[m]) * 3;
+    double acc = 0.0;
+    for (int j = lane; j < nd; j += 64) acc += row[j] * bm[j];
+#pragma unroll
+    for (int sh = 32; sh > 0; sh >>= 1) acc += __shfl_xor(acc, sh, 64);
+    if (lane == 0) x[k] = acc;
 }
 
 unsigned grid1(int64_t n) { return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384))); }
@@ -659,7 +685,7 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     }
     OC_CHECK(residual0(t, z, g.r0.ptr));
     MgLevel &L1 = g.levels[1];
-    restrict_q2_kernel<<<grid1(L1.nn), 256, 0, rt.stream>>>(M, g.n.ptr, s.q_off.ptr, nq, L1.off.ptr, L1.nn,
+    restrict_q2_kernel<<<group_grid(L1.nn), 256, 0, rt.stream>>>(M, g.n.ptr, s.q_off.ptr, nq, L1.off.ptr, L1.nn,
                                                             L1.freed.ptr, g.r0.ptr, L1.b.ptr);
     OC_HIP_CHECK(hipGetLastError());
     // down the stencil levels
@@ -670,12 +696,12 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
         stencil_apply_kernel<0><<<group_grid(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
                                                                         F.freed.ptr, F.b.ptr, F.x.ptr, nullptr, 0.0,
                                                                         F.r.ptr);
-        restrict_stencil_kernel<<<grid1(C.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, C.dim.ptr, C.off.ptr,
+        restrict_stencil_kernel<<<group_grid(C.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, C.dim.ptr, C.off.ptr,
                                                                    C.nn, C.freed.ptr, F.r.ptr, C.b.ptr);
         OC_HIP_CHECK(hipGetLastError());
     }
     MgLevel &Lc = g.levels[nl - 1];
-    coarse_apply_kernel<<<grid1(3 * static_cast<int64_t>(Lc.nn)), 256, 0, rt.stream>>>(
+    coarse_apply_kernel<<<static_cast<unsigned>((3 * static_cast<int64_t>(Lc.nn) * 64 + 255) / 256), 256, 0, rt.stream>>>(
         M, Lc.dim.ptr, Lc.off.ptr, Lc.nn, g.coarse_inv.ptr, g.coarse_stride, Lc.b.ptr, Lc.x.ptr);
     OC_HIP_CHECK(hipGetLastError());
     // up
