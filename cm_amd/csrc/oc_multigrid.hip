@@ -317,10 +317,10 @@ __global__ void __launch_bounds__(256) coarse_inverse_kernel(int M, const int *_
         for (int j = t; j < nd; j += 256) a[k][j] = (j == k ? 1.0 : a[k][j]) / p;
         for (int i = t; i < nd; i += 256) fcol[i] = (i == k) ? 0.0 : a[i][k];
         __syncthreads();
-        for (int q = t; q < nd * nd; q += 256) {
-            const int i = q / nd, j = q % nd;
+        for (int i = t >> 5; i < nd; i += 8) {  // 8 x 32 threads over rows and columns
             if (i == k) continue;
-            a[i][j] = (j == k ? 0.0 : a[i][j]) - fcol[i] * a[k][j];
+            const double f = fcol[i];
+            for (int j = t & 31; j < nd; j += 32) a[i][j] = (j == k ? 0.0 : a[i][j]) - f * a[k][j];
         }
         __syncthreads();
     }

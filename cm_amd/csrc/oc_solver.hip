@@ -233,18 +233,23 @@ __global__ void __launch_bounds__(256) schur_setup_kernel(int nq, int nl, const 
                                                          const uint8_t *__restrict__ free_mask,
                                                          const double *__restrict__ diag_u, double *__restrict__ inv)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nl) return;
+    // one wavefront per pressure row: lanes over the row's displacement
+    // columns, a fixed butterfly for the sum
+    const int p = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= nl) return;  // whole wavefronts
     const int r = 3 * nq + p;
     double s = 0.0;
-    for (int64_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+    for (int64_t k = row_ptr[r] + lane; k < row_ptr[r + 1]; k += 64) {
         const int c = col_idx[k];
         if (c >= 3 * nq || !free_mask[c]) continue;
         const double b = static_cast<double>(vals[k]);
         const double kd = diag_u[c];
         if (kd != 0.0) s += b * b / kd;
     }
-    inv[p] = s > 0.0 ? -1.0 / s : -1.0;
+#pragma unroll
+    for (int sh = 32; sh > 0; sh >>= 1) s += __shfl_xor(s, sh, 64);
+    if (lane == 0) inv[p] = s > 0.0 ? -1.0 / s : -1.0;
 }
 
 __global__ void __launch_bounds__(256) apply_jacobi_kernel(int n, const double *__restrict__ inv,
@@ -435,14 +440,14 @@ struct Krylov {
             block_setup_kernel<float><<<grid_rows(nq), 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.col_idx.ptr,
                                                                             s.values_sp.ptr, s.free_mask.ptr, inv_u,
                                                                             diag_u);
-            schur_setup_kernel<float><<<grid_rows(nl), 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.col_idx.ptr,
+            schur_setup_kernel<float><<<grid_rows(static_cast<int64_t>(nl) * 64), 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.col_idx.ptr,
                                                                             s.values_sp.ptr, s.free_mask.ptr, diag_u,
                                                                             inv_p);
         } else {
             block_setup_kernel<double><<<grid_rows(nq), 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.col_idx.ptr,
                                                                              s.values.ptr, s.free_mask.ptr, inv_u,
                                                                              diag_u);
-            schur_setup_kernel<double><<<grid_rows(nl), 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.col_idx.ptr,
+            schur_setup_kernel<double><<<grid_rows(static_cast<int64_t>(nl) * 64), 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.col_idx.ptr,
                                                                              s.values.ptr, s.free_mask.ptr, diag_u,
                                                                              inv_p);
         }
