@@ -32,7 +32,14 @@ struct Runtime {
     hipStream_t stream = nullptr;
     hipEvent_t events[64] = {};
     int num_cus = 0;
+    // Krylov progress read back without a stream synchronisation: pinned,
+    // mapped, coherent host memory the Arnoldi kernels write (kArnoldiHostSlots
+    // doubles), and two timing-free events marking alternate iterations.
+    double *arnoldi_host = nullptr;
+    double *arnoldi_host_dev = nullptr;
+    hipEvent_t arnoldi_events[2] = {};
 };
+constexpr int kArnoldiHostSlots = 2 * 1024;
 Runtime &runtime();
 int require_runtime();
 
@@ -128,6 +135,11 @@ struct Multigrid {
     int pre = 2, post = 2;
     double omega = 0.6;
     int coarse_sweeps = 0;                // 0: dense solve of the coarsest level
+    // 1 when the three displacement rows of every node share one column
+    // pattern of three component blocks (checked on the device at the first
+    // V-cycle), 0 when not, -1 before the check
+    int node_rows = -1;
+    DevBuf<int32_t> check;
 };
 
 // Solver system (oc_solver_matrices.hip): the equations matrix or Jacobian
@@ -222,6 +234,7 @@ struct EquationsSet {
     DevBuf<double> work;        // vectors
     DevBuf<double> precond;     // preconditioner storage
     DevBuf<double> partials;    // reduction partial sums
+    DevBuf<double> arnoldi;     // Hessenberg factor, rotations and coefficients of the Arnoldi process
     DevBuf<int64_t> pstart;     // first pressure column of each displacement row
     int krylov_capacity = 0;
 

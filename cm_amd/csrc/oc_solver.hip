@@ -103,8 +103,10 @@ __global__ void __launch_bounds__(256) mesh_norms_kernel(const int *__restrict__
 // partial[j * gridDim.x + blockIdx.x] = sum over this block's slice of V_j . w
 // for j = 0..k-1 (blockIdx.y = j).  Deterministic: fixed grid, fixed tree.
 __global__ void __launch_bounds__(256) multi_dot_kernel(int n, const double *__restrict__ V, int64_t ldv,
-                                                       const double *__restrict__ w, double *__restrict__ partial)
+                                                       const double *__restrict__ w, double *__restrict__ partial,
+                                                       const double *__restrict__ gate)
 {
+    if (gate && *gate == 0.0) return;  // a pass the Arnoldi process did not ask for
     __shared__ double s[256];
     const double *v = V + static_cast<int64_t>(blockIdx.y) * ldv;
     double acc = 0.0;
@@ -132,6 +134,122 @@ __global__ void __launch_bounds__(256) combine_kernel(int n, int k, const double
         for (int j = 0; j < k; ++j) acc += coef.c[j] * V[static_cast<int64_t>(j) * ldv + i];
         w[i] += sign * acc;
     }
+}
+
+// ---- Arnoldi process on the device ---------------------------------------------
+// Classical Gram-Schmidt with one reorthogonalisation pass when the projection
+// removes more than 3/4 of |w|^2 (CGS2), and Givens rotations of the
+// Hessenberg column, all without a host round trip.  Layout of the state for
+// restart m: R (m x m, row-major, column j written by step j), h and h2 (m+2),
+// cs, sn (m), g (m+1), then flag, hn and 1/hn.
+struct Arnoldi {
+    double *R, *h, *h2, *cs, *sn, *g, *flag, *hn, *inv;
+    static size_t size(int m) { return static_cast<size_t>(m) * m + 2 * (m + 2) + 2 * m + (m + 1) + 3; }
+    static Arnoldi at(double *base, int m)
+    {
+        Arnoldi a;
+        a.R = base;
+        a.h = a.R + static_cast<size_t>(m) * m;
+        a.h2 = a.h + m + 2;
+        a.cs = a.h2 + m + 2;
+        a.sn = a.cs + m;
+        a.g = a.sn + m;
+        a.flag = a.g + m + 1;
+        a.hn = a.flag + 1;
+        a.inv = a.hn + 1;
+        return a;
+    }
+};
+
+// sum of the kReduceBlocks partials of dot product i, in block order
+__device__ __forceinline__ double partial_sum(const double *__restrict__ partial, int i)
+{
+    double acc = 0.0;
+    for (int b = 0; b < kReduceBlocks; ++b) acc += partial[static_cast<size_t>(i) * kReduceBlocks + b];
+    return acc;
+}
+
+// g = (beta, 0, ..., 0)
+__global__ void __launch_bounds__(256) arnoldi_start_kernel(int m, double beta, Arnoldi a)
+{
+    for (int i = threadIdx.x; i <= m; i += blockDim.x) a.g[i] = i == 0 ? beta : 0.0;
+}
+
+// step j, first pass: h_i = V_i . w (i <= j), |w|^2 = h_{j+1}; asks for the
+// second pass or takes |w - V h| by Pythagoras
+__global__ void __launch_bounds__(256) arnoldi_project_kernel(int j, const double *__restrict__ partial, Arnoldi a)
+{
+    for (int i = threadIdx.x; i < j + 2; i += blockDim.x) a.h[i] = partial_sum(partial, i);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double wn2 = a.h[j + 1];
+        double proj2 = 0.0;
+        for (int i = 0; i <= j; ++i) proj2 += a.h[i] * a.h[i];
+        const bool again = wn2 - proj2 < 0.25 * wn2;
+        *a.flag = again ? 1.0 : 0.0;
+        *a.hn = again ? 0.0 : sqrt(wn2 - proj2);
+    }
+}
+
+// step j, second pass (only with the flag set): h2_i = V_i . w, h_i += h2_i
+__global__ void __launch_bounds__(256) arnoldi_correct_kernel(int j, const double *__restrict__ partial, Arnoldi a)
+{
+    if (*a.flag == 0.0) return;
+    for (int i = threadIdx.x; i <= j; i += blockDim.x) {
+        const double c = partial_sum(partial, i);
+        a.h2[i] = c;
+        a.h[i] += c;
+    }
+}
+
+// w += sign * sum_{j<k} coef_j V_j, coefficients in device memory
+__global__ void __launch_bounds__(256) combine_device_kernel(int n, int k, const double *__restrict__ V, int64_t ldv,
+                                                            const double *__restrict__ coef, double sign,
+                                                            double *__restrict__ w, const double *__restrict__ gate)
+{
+    if (gate && *gate == 0.0) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int j = 0; j < k; ++j) acc += coef[j] * V[static_cast<int64_t>(j) * ldv + i];
+        w[i] += sign * acc;
+    }
+}
+
+// step j, last: the norm of the orthogonalised vector (from the partials of
+// w . w after a second pass), the rotations, R's column j, g, and the
+// residual estimate |g_{j+1}| and hn into the host-visible slots 2j, 2j+1
+__global__ void __launch_bounds__(64) arnoldi_finalize_kernel(int j, int m, const double *__restrict__ partial,
+                                                            Arnoldi a, double *__restrict__ host)
+{
+    if (threadIdx.x != 0) return;
+    double hn = *a.hn;
+    if (*a.flag != 0.0) hn = sqrt(fmax(partial_sum(partial, 0), 0.0));
+    *a.inv = hn > 0.0 ? 1.0 / hn : 0.0;
+    double hi = a.h[0];
+    for (int i = 0; i < j; ++i) {
+        const double hi1 = a.h[i + 1];
+        a.R[static_cast<size_t>(i) * m + j] = a.cs[i] * hi + a.sn[i] * hi1;
+        hi = -a.sn[i] * hi + a.cs[i] * hi1;
+    }
+    // hi is the rotated h_j; h_{j+1} = hn
+    const double den = hypot(hi, hn);
+    const double c = den > 0.0 ? hi / den : 1.0, sgn = den > 0.0 ? hn / den : 0.0;
+    a.cs[j] = c;
+    a.sn[j] = sgn;
+    a.R[static_cast<size_t>(j) * m + j] = den;
+    const double gj = a.g[j];
+    a.g[j + 1] = -sgn * gj;
+    a.g[j] = c * gj;
+    host[2 * j] = fabs(a.g[j + 1]);
+    host[2 * j + 1] = hn;
+}
+
+// v *= 1/hn unless hn is zero
+__global__ void __launch_bounds__(256) arnoldi_scale_kernel(int n, const double *__restrict__ inv, double *__restrict__ v)
+{
+    const double a = *inv;
+    if (a == 0.0) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = a * v[i];
 }
 
 // ---- CSR SpMV of the weighted Dirichlet operator: y_r = w_r sum_k v_k x_c
@@ -391,6 +509,7 @@ struct Krylov {
     const int64_t *pstart = nullptr;
     int precond = OC_PRECONDITIONER_BLOCK_DIAGONAL;
     bool fp32 = false;
+    Arnoldi ar{};
     std::vector<double> hpart;
 
     // y = W (A x)
@@ -504,7 +623,7 @@ struct Krylov {
     {
         Runtime &rt = runtime();
         dim3 grid(kReduceBlocks, k);
-        multi_dot_kernel<<<grid, 256, 0, rt.stream>>>(n, Vb, n, wv, s.partials.ptr);
+        multi_dot_kernel<<<grid, 256, 0, rt.stream>>>(n, Vb, n, wv, s.partials.ptr, nullptr);
         OC_HIP_CHECK(hipGetLastError());
         hpart.resize(static_cast<size_t>(k) * kReduceBlocks);
         OC_HIP_CHECK(hipMemcpyAsync(hpart.data(), s.partials.ptr, sizeof(double) * hpart.size(),
@@ -540,17 +659,47 @@ struct Krylov {
         return OC_NO_ERROR;
     }
 
+    // One Arnoldi step j on the device: V_{j+1} = orthonormalised W A P^-1 W^-1 V_j,
+    // R's column j and g rotated, |g_{j+1}| and hn into the host slots 2j, 2j+1.
+    int arnoldi_step(int j)
+    {
+        Runtime &rt = runtime();
+        double *vj = V + static_cast<int64_t>(j) * n, *vn = V + static_cast<int64_t>(j + 1) * n;
+        OC_CHECK(apply_preconditioner(vj, z));
+        OC_CHECK(spmv(z, vn));
+        // classical Gram-Schmidt: one fused reduction gives V_0..V_j . w and
+        // w . w (V_{j+1} is w itself); the norm after projection follows by
+        // Pythagoras unless cancellation asks for a second pass and an
+        // explicit norm, whose kernels then run (they return at once otherwise)
+        multi_dot_kernel<<<dim3(kReduceBlocks, j + 2), 256, 0, rt.stream>>>(n, V, n, vn, s.partials.ptr, nullptr);
+        arnoldi_project_kernel<<<1, 256, 0, rt.stream>>>(j, s.partials.ptr, ar);
+        combine_device_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, j + 1, V, n, ar.h, -1.0, vn, nullptr);
+        multi_dot_kernel<<<dim3(kReduceBlocks, j + 1), 256, 0, rt.stream>>>(n, V, n, vn, s.partials.ptr, ar.flag);
+        arnoldi_correct_kernel<<<1, 256, 0, rt.stream>>>(j, s.partials.ptr, ar);
+        combine_device_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, j + 1, V, n, ar.h2, -1.0, vn, ar.flag);
+        multi_dot_kernel<<<dim3(kReduceBlocks, 1), 256, 0, rt.stream>>>(n, vn, n, vn, s.partials.ptr, ar.flag);
+        arnoldi_finalize_kernel<<<1, 64, 0, rt.stream>>>(j, m, s.partials.ptr, ar, rt.arnoldi_host_dev);
+        arnoldi_scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, ar.inv, vn);
+        OC_HIP_CHECK(hipGetLastError());
+        return OC_NO_ERROR;
+    }
+
     // Solves W A x = b (x starts at 0) to ||b - W A x|| <= tol_abs; b is zero
-    // on fixed DOFs.  Right preconditioning: x = W^-1 P^-1 V y.
+    // on fixed DOFs.  Right preconditioning: x = W^-1 P^-1 V y.  The host
+    // stays one Arnoldi step ahead of the device: it queues step j, then
+    // reads step j-1's residual estimate; a converged j-1 leaves step j's
+    // work unused (R's column j and g_j, g_{j+1} are not read).
     int gmres(const double *b, double *x, double tol_abs, int max_iter, int *iters, double *res_out)
     {
         Runtime &rt = runtime();
+        if (2 * m > kArnoldiHostSlots) return set_error(OC_ERR_INVALID_ARGUMENT, "krylov restart %d too large", m);
         OC_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, rt.stream));
         int total = 0;
         double beta = 0.0;
         OC_CHECK(norm(b, &beta));
         double res = beta;
-        std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 2), h2(m + 2);
+        std::vector<double> R(static_cast<size_t>(m) * m), g(m + 1), y(m);
+        const volatile double *slots = rt.arnoldi_host;
         bool first = true;
         while (total < max_iter && res > tol_abs) {
             // r = b - W A x
@@ -566,68 +715,37 @@ struct Krylov {
                 if (res <= tol_abs) break;
             }
             copy_scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0 / beta, r, V);
-            std::fill(g.begin(), g.end(), 0.0);
-            g[0] = beta;
-            int j = 0;
-            for (; j < m && total < max_iter; ++j, ++total) {
-                double *vj = V + static_cast<int64_t>(j) * n, *vn = V + static_cast<int64_t>(j + 1) * n;
-                OC_CHECK(apply_preconditioner(vj, z));
-                OC_CHECK(spmv(z, vn));
-                // classical Gram-Schmidt: one fused reduction gives V_0..V_j . w
-                // and w . w (V_{j+1} is w itself); the norm after projection
-                // follows by Pythagoras unless cancellation asks for a second
-                // pass (CGS2) and an explicit norm.
-                OC_CHECK(multi_dot(V, j + 2, vn, h.data()));
-                const double wn2 = h[j + 1];
-                std::vector<double> hc(h.begin(), h.begin() + j + 1);
-                OC_CHECK(combine(V, j + 1, hc, -1.0, vn));
-                double proj2 = 0.0;
-                for (int i = 0; i <= j; ++i) proj2 += h[i] * h[i];
-                double hn = 0.0;
-                if (wn2 - proj2 < 0.25 * wn2) {
-                    OC_CHECK(multi_dot(V, j + 1, vn, h2.data()));
-                    std::vector<double> hc2(h2.begin(), h2.begin() + j + 1);
-                    OC_CHECK(combine(V, j + 1, hc2, -1.0, vn));
-                    for (int i = 0; i <= j; ++i) h[i] += h2[i];
-                    OC_CHECK(norm(vn, &hn));
-                } else {
-                    hn = std::sqrt(wn2 - proj2);
-                }
-                h[j + 1] = hn;
-                if (hn > 0.0) copy_scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0 / hn, vn, vn);
-                // Givens
-                for (int i = 0; i < j; ++i) {
-                    const double t = cs[i] * h[i] + sn[i] * h[i + 1];
-                    h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
-                    h[i] = t;
-                }
-                const double den = std::hypot(h[j], h[j + 1]);
-                cs[j] = den > 0.0 ? h[j] / den : 1.0;
-                sn[j] = den > 0.0 ? h[j + 1] / den : 0.0;
-                h[j] = den;
-                h[j + 1] = 0.0;
-                g[j + 1] = -sn[j] * g[j];
-                g[j] = cs[j] * g[j];
-                for (int i = 0; i <= j; ++i) H[static_cast<size_t>(i) * m + j] = h[i];
-                res = std::fabs(g[j + 1]);
-                if (res <= tol_abs || hn == 0.0) {
-                    ++j;
-                    ++total;
-                    break;
+            arnoldi_start_kernel<<<1, 256, 0, rt.stream>>>(m, beta, ar);
+            const int steps = std::min(m, max_iter - total);
+            int cols = 0;
+            for (int j = 0; j < steps && cols == 0; ++j) {
+                OC_CHECK(arnoldi_step(j));
+                OC_HIP_CHECK(hipEventRecord(rt.arnoldi_events[j & 1], rt.stream));
+                // read step j-1; the cycle's last step is read as soon as it is queued
+                for (int k = j - 1; k <= j && cols == 0; ++k) {
+                    if (k < 0 || (k == j && j != steps - 1)) continue;
+                    OC_HIP_CHECK(hipEventSynchronize(rt.arnoldi_events[k & 1]));
+                    const double rk = slots[2 * k], hk = slots[2 * k + 1];
+                    res = rk;
+                    if (rk <= tol_abs || hk == 0.0 || k == steps - 1) cols = k + 1;
                 }
             }
-            // y = H^-1 g ; x += W^-1 P^-1 (V y)
-            for (int i = j - 1; i >= 0; --i) {
+            total += cols;
+            // y = R^-1 g over the first cols columns; x += W^-1 P^-1 (V y)
+            OC_HIP_CHECK(hipMemcpyAsync(R.data(), ar.R, sizeof(double) * static_cast<size_t>(cols) * m,
+                                        hipMemcpyDeviceToHost, rt.stream));
+            OC_HIP_CHECK(hipMemcpyAsync(g.data(), ar.g, sizeof(double) * cols, hipMemcpyDeviceToHost, rt.stream));
+            OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+            for (int i = cols - 1; i >= 0; --i) {
                 double t = g[i];
-                for (int k = i + 1; k < j; ++k) t -= H[static_cast<size_t>(i) * m + k] * y[k];
-                y[i] = t / H[static_cast<size_t>(i) * m + i];
+                for (int k = i + 1; k < cols; ++k) t -= R[static_cast<size_t>(i) * m + k] * y[k];
+                y[i] = t / R[static_cast<size_t>(i) * m + i];
             }
             OC_CHECK(fill<double>(w, n, 0.0));
-            std::vector<double> yc(y.begin(), y.begin() + j);
-            OC_CHECK(combine(V, j, yc, 1.0, w));
+            std::vector<double> yc(y.begin(), y.begin() + cols);
+            OC_CHECK(combine(V, cols, yc, 1.0, w));
             OC_CHECK(apply_preconditioner(w, z));
             axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, 1.0, z, 1.0, x);
-            if (j == 0) break;
         }
         *iters = total;
         *res_out = res;
@@ -658,6 +776,7 @@ static int solver_prepare(EquationsSet &s, int restart, Krylov &K)
         OC_CHECK(s.krylov.alloc(static_cast<size_t>(restart + 1) * n));
         s.krylov_capacity = restart;
     }
+    if (s.arnoldi.n < Arnoldi::size(restart)) OC_CHECK(s.arnoldi.alloc(Arnoldi::size(restart)));
     if (!s.work.ptr) OC_CHECK(s.work.alloc(static_cast<size_t>(8) * n));
     if (!s.precond.ptr) OC_CHECK(s.precond.alloc(static_cast<size_t>(9) * s.n_nodes + 3 * static_cast<size_t>(n)));
     const size_t npart = std::max<size_t>(static_cast<size_t>(kReduceBlocks) * (1024 + 1),
@@ -666,6 +785,7 @@ static int solver_prepare(EquationsSet &s, int restart, Krylov &K)
     K.n = n;
     K.m = restart;
     K.V = s.krylov.ptr;
+    K.ar = Arnoldi::at(s.arnoldi.ptr, restart);
     K.r = s.work.ptr + 3 * static_cast<int64_t>(n);
     K.z = s.work.ptr + 4 * static_cast<int64_t>(n);
     K.w = s.work.ptr + 5 * static_cast<int64_t>(n);
