@@ -135,11 +135,6 @@ struct Multigrid {
     int pre = 2, post = 2;
     double omega = 0.6;
     int coarse_sweeps = 0;                // 0: dense solve of the coarsest level
-    // 1 when the three displacement rows of every node share one column
-    // pattern of three component blocks (checked on the device at the first
-    // V-cycle), 0 when not, -1 before the check
-    int node_rows = -1;
-    DevBuf<int32_t> check;
 };
 
 // Solver system (oc_solver_matrices.hip): the equations matrix or Jacobian
@@ -235,6 +230,11 @@ struct EquationsSet {
     DevBuf<double> precond;     // preconditioner storage
     DevBuf<double> partials;    // reduction partial sums
     DevBuf<double> arnoldi;     // Hessenberg factor, rotations and coefficients of the Arnoldi process
+    // Mooney-Rivlin sets: 1 when the displacement rows c*nq + i (c < 3) of
+    // every node i hold three blocks of the same sorted neighbour nodes
+    // (components 0, 1, 2) and then one common list of pressure columns, so
+    // kernels can read one column list per node; 0 when not; -1 unchecked
+    int node_rows = -1;
     DevBuf<int64_t> pstart;     // first pressure column of each displacement row
     int krylov_capacity = 0;
 

@@ -346,39 +346,9 @@ uu_residual_kernel(int nu, const int64_t *__restrict__ row_ptr, const int64_t *_
     if (lane == 0) r[row] = f ? b[row] - acc : 0.0;
 }
 
-// The displacement rows c*nq + i (c < 3) of node i: [row_ptr, pstart) holds
-// three blocks of deg columns, block c' being c'*nq + j over the same sorted
-// neighbour nodes j.  bad = 1 when any node breaks that layout.
-__global__ void __launch_bounds__(256) node_pattern_check_kernel(int nq, const int64_t *__restrict__ row_ptr,
-                                                                const int64_t *__restrict__ pstart,
-                                                                const int *__restrict__ col_idx, int *__restrict__ bad)
-{
-    const int i = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 5);
-    const int lane = threadIdx.x & 31;
-    if (i >= nq) return;
-    const int64_t d0 = pstart[i] - row_ptr[i];
-    bool ok = d0 % 3 == 0;
-    for (int c = 1; c < 3; ++c) {
-        const int64_t r = static_cast<int64_t>(c) * nq + i;
-        ok = ok && pstart[r] - row_ptr[r] == d0;
-    }
-    if (ok) {
-        const int64_t deg = d0 / 3;
-        for (int64_t p = lane; p < deg; p += 32) {
-            const int j = col_idx[row_ptr[i] + p];
-            ok = ok && j >= 0 && j < nq;
-            for (int c = 0; c < 3; ++c)
-                for (int cc = 0; cc < 3; ++cc)
-                    ok = ok && col_idx[row_ptr[static_cast<int64_t>(c) * nq + i] + cc * deg + p] ==
-                                   static_cast<int64_t>(cc) * nq + j;
-        }
-    }
-    if (!ok) *bad = 1;
-}
-
-// level 0 residual by nodes (the layout above): one group of kLanes lanes per
-// node reads the neighbour list once and the nine value blocks of its three
-// rows; r = b - K_uu x on free rows, 0 on fixed ones
+// level 0 residual by nodes (EquationsSet::node_rows): one group of kLanes
+// lanes per node reads the neighbour list once and the nine value blocks of
+// its three rows; r = b - K_uu x on free rows, 0 on fixed ones
 template <typename VT, int kLanes>
 __global__ void __launch_bounds__(256)
 uu_node_residual_kernel(int nq, const int64_t *__restrict__ row_ptr, const int64_t *__restrict__ pstart,
@@ -737,24 +707,13 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     const int M = s.n_meshes, nq = s.n_nodes, nu = 3 * nq, nl = static_cast<int>(g.levels.size());
     const double *inv_u = s.precond.ptr;  // nodal block inverses of the set's block preconditioner
     const unsigned gu = static_cast<unsigned>((static_cast<int64_t>(nu) * 32 + 255) / 256);
-    if (g.node_rows < 0) {
-        if (!g.check.ptr) OC_CHECK(g.check.alloc(1));
-        OC_HIP_CHECK(hipMemsetAsync(g.check.ptr, 0, sizeof(int32_t), rt.stream));
-        node_pattern_check_kernel<<<group_grid(nq), 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
-                                                                          s.col_idx.ptr, g.check.ptr);
-        OC_HIP_CHECK(hipGetLastError());
-        int32_t bad = 1;
-        OC_HIP_CHECK(hipMemcpyAsync(&bad, g.check.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, rt.stream));
-        OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
-        g.node_rows = bad ? 0 : 1;
-    }
     const unsigned gn = group_grid(nq);
     auto residual0 = [&](const double *b, const double *x, double *r) -> int {
-        if (g.node_rows && precision == OC_FP32)
+        if (s.node_rows == 1 && precision == OC_FP32)
             uu_node_residual_kernel<float, 32><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
                                                                          s.col_idx.ptr, s.values_sp.ptr,
                                                                          s.free_mask.ptr, b, x, r);
-        else if (g.node_rows)
+        else if (s.node_rows == 1)
             uu_node_residual_kernel<double, 32><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
                                                                           s.col_idx.ptr, s.values.ptr,
                                                                           s.free_mask.ptr, b, x, r);

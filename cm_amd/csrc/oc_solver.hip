@@ -272,6 +272,138 @@ __global__ void __launch_bounds__(256) spmv_kernel(int n, const int64_t *__restr
     if (lane == 0) y[row] = wr * acc;
 }
 
+// ---- Mooney-Rivlin rows by nodes ------------------------------------------------------
+// Node i's displacement rows c*nq + i (c < 3): [row_ptr, pstart) holds three
+// blocks of deg columns, block c' being c'*nq + j over the same sorted
+// neighbour nodes j, and [pstart, row_ptr + 1) the same pressure columns in
+// all three rows.  bad = 1 when any node breaks that layout.
+__global__ void __launch_bounds__(256) node_pattern_check_kernel(int nq, const int64_t *__restrict__ row_ptr,
+                                                                const int64_t *__restrict__ pstart,
+                                                                const int *__restrict__ col_idx, int *__restrict__ bad)
+{
+    const int i = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 5);
+    const int lane = threadIdx.x & 31;
+    if (i >= nq) return;
+    const int64_t d0 = pstart[i] - row_ptr[i], dp = row_ptr[i + 1] - pstart[i];
+    bool ok = d0 % 3 == 0;
+    for (int c = 1; c < 3; ++c) {
+        const int64_t r = static_cast<int64_t>(c) * nq + i;
+        ok = ok && pstart[r] - row_ptr[r] == d0 && row_ptr[r + 1] - pstart[r] == dp;
+    }
+    if (ok) {
+        const int64_t deg = d0 / 3;
+        for (int64_t p = lane; p < deg; p += 32) {
+            const int j = col_idx[row_ptr[i] + p];
+            ok = ok && j >= 0 && j < nq;
+            for (int c = 0; c < 3; ++c)
+                for (int cc = 0; cc < 3; ++cc)
+                    ok = ok && col_idx[row_ptr[static_cast<int64_t>(c) * nq + i] + cc * deg + p] ==
+                                   static_cast<int64_t>(cc) * nq + j;
+        }
+        for (int64_t p = lane; p < dp; p += 32)
+            for (int c = 1; c < 3; ++c)
+                ok = ok && col_idx[pstart[static_cast<int64_t>(c) * nq + i] + p] == col_idx[pstart[i] + p];
+    }
+    if (!ok) *bad = 1;
+}
+
+// y = W (A x) (or A x when w is NULL) of a Mooney-Rivlin set with node rows:
+// groups g < nq take node g's three displacement rows (one read of the
+// neighbour and pressure column lists), groups nq + l take pressure row l
+template <typename VT, int kLanes>
+__global__ void __launch_bounds__(256) spmv_node_kernel(int nq, int nl, const int64_t *__restrict__ row_ptr,
+                                                       const int64_t *__restrict__ pstart,
+                                                       const int *__restrict__ col_idx, const VT *__restrict__ vals,
+                                                       const double *__restrict__ w, const double *__restrict__ x,
+                                                       double *__restrict__ y)
+{
+    const int lane = threadIdx.x % kLanes;
+    const int g = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes);
+    const int64_t nq64 = nq;
+    if (g >= nq + nl) return;  // whole groups
+    if (g >= nq) {
+        const int64_t row = 3 * nq64 + (g - nq);
+        const double wr = w == nullptr ? 1.0 : w[row];
+        double acc = 0.0;
+        if (wr != 0.0) acc = csr_row_dot<VT, kLanes>(row_ptr[row], row_ptr[row + 1], lane, col_idx, vals, x);
+#pragma unroll
+        for (int o = kLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kLanes);
+        if (lane == 0) y[row] = wr * acc;
+        return;
+    }
+    const int i = g;
+    const int64_t s0 = row_ptr[i], s1 = row_ptr[nq64 + i], s2 = row_ptr[2 * nq64 + i];
+    const int64_t p0 = pstart[i], p1 = pstart[nq64 + i], p2 = pstart[2 * nq64 + i];
+    const int deg = static_cast<int>((p0 - s0) / 3), dp = static_cast<int>(row_ptr[i + 1] - p0);
+    const double w0 = w == nullptr ? 1.0 : w[i], w1 = w == nullptr ? 1.0 : w[nq64 + i],
+                 w2 = w == nullptr ? 1.0 : w[2 * nq64 + i];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (w0 != 0.0 || w1 != 0.0 || w2 != 0.0) {
+        for (int p = lane; p < deg; p += kLanes) {
+            const int j = col_idx[s0 + p];
+            const double x0 = x[j], x1 = x[nq64 + j], x2 = x[2 * nq64 + j];
+            a0 += static_cast<double>(vals[s0 + p]) * x0 + static_cast<double>(vals[s0 + deg + p]) * x1 +
+                  static_cast<double>(vals[s0 + 2 * deg + p]) * x2;
+            a1 += static_cast<double>(vals[s1 + p]) * x0 + static_cast<double>(vals[s1 + deg + p]) * x1 +
+                  static_cast<double>(vals[s1 + 2 * deg + p]) * x2;
+            a2 += static_cast<double>(vals[s2 + p]) * x0 + static_cast<double>(vals[s2 + deg + p]) * x1 +
+                  static_cast<double>(vals[s2 + 2 * deg + p]) * x2;
+        }
+        for (int p = lane; p < dp; p += kLanes) {
+            const double xp = x[col_idx[p0 + p]];
+            a0 += static_cast<double>(vals[p0 + p]) * xp;
+            a1 += static_cast<double>(vals[p1 + p]) * xp;
+            a2 += static_cast<double>(vals[p2 + p]) * xp;
+        }
+    }
+#pragma unroll
+    for (int o = kLanes / 2; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, kLanes);
+        a1 += __shfl_xor(a1, o, kLanes);
+        a2 += __shfl_xor(a2, o, kLanes);
+    }
+    if (lane < 3) {
+        const int64_t row = lane * nq64 + i;
+        y[row] = lane == 0 ? w0 * a0 : (lane == 1 ? w1 * a1 : w2 * a2);
+    }
+}
+
+// t = r_u - B^T z_p on free displacement rows, 0 on fixed ones, by nodes
+template <typename VT, int kLanes>
+__global__ void __launch_bounds__(256) schur_rhs_node_kernel(int nq, const int64_t *__restrict__ row_ptr,
+                                                            const int64_t *__restrict__ pstart,
+                                                            const int *__restrict__ col_idx,
+                                                            const VT *__restrict__ vals,
+                                                            const uint8_t *__restrict__ free_mask,
+                                                            const double *__restrict__ r,
+                                                            const double *__restrict__ z, double *__restrict__ t)
+{
+    const int lane = threadIdx.x % kLanes;
+    const int i = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes);
+    if (i >= nq) return;  // whole groups
+    const int64_t nq64 = nq;
+    const int64_t p0 = pstart[i], p1 = pstart[nq64 + i], p2 = pstart[2 * nq64 + i];
+    const int dp = static_cast<int>(row_ptr[i + 1] - p0);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int p = lane; p < dp; p += kLanes) {
+        const double zp = z[col_idx[p0 + p]];
+        a0 += static_cast<double>(vals[p0 + p]) * zp;
+        a1 += static_cast<double>(vals[p1 + p]) * zp;
+        a2 += static_cast<double>(vals[p2 + p]) * zp;
+    }
+#pragma unroll
+    for (int o = kLanes / 2; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, kLanes);
+        a1 += __shfl_xor(a1, o, kLanes);
+        a2 += __shfl_xor(a2, o, kLanes);
+    }
+    if (lane < 3) {
+        const int64_t row = lane * nq64 + i;
+        const double a = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+        t[row] = free_mask[row] ? r[row] - a : 0.0;
+    }
+}
+
 // ---- preconditioners ----------------------------------------------------------------
 template <typename VT>
 __device__ __forceinline__ double csr_entry_t(const int64_t *__restrict__ row_ptr, const int *__restrict__ col_idx,
@@ -526,6 +658,21 @@ struct Krylov {
     int launch_spmv(const VT *vals, int lanes, const double *wv, const double *x, double *y)
     {
         Runtime &rt = runtime();
+        if (s.cls == OC_EQUATIONS_SET_MOONEY_RIVLIN && s.node_rows == 1 && lanes >= 16) {
+            const int nq = s.n_nodes, nl = s.n_nodes_l;
+            const unsigned gn = static_cast<unsigned>((static_cast<int64_t>(nq + nl) * lanes + 255) / 256);
+            if (lanes == 16)
+                spmv_node_kernel<VT, 16><<<gn, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.pstart.ptr,
+                                                                    s.col_idx.ptr, vals, wv, x, y);
+            else if (lanes == 64)
+                spmv_node_kernel<VT, 64><<<gn, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.pstart.ptr,
+                                                                    s.col_idx.ptr, vals, wv, x, y);
+            else
+                spmv_node_kernel<VT, 32><<<gn, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.pstart.ptr,
+                                                                    s.col_idx.ptr, vals, wv, x, y);
+            OC_HIP_CHECK(hipGetLastError());
+            return OC_NO_ERROR;
+        }
         const unsigned g = static_cast<unsigned>((static_cast<int64_t>(n) * lanes + 255) / 256);
         switch (lanes) {
         case 8: spmv_kernel<VT, 8><<<g, 256, 0, rt.stream>>>(n, s.row_ptr.ptr, s.col_idx.ptr, vals, wv, x, y); break;
@@ -590,7 +737,14 @@ struct Krylov {
                 const int nu = 3 * nq;
                 double *t = s.mg->t0.ptr;
                 const unsigned gs = grid_rows(static_cast<int64_t>(nu) * 8);
-                if (fp32)
+                const unsigned gsn = grid_rows(static_cast<int64_t>(nq) * 16);
+                if (s.node_rows == 1 && fp32)
+                    schur_rhs_node_kernel<float, 16><<<gsn, 256, 0, rt.stream>>>(
+                        nq, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values_sp.ptr, s.free_mask.ptr, rin, zout, t);
+                else if (s.node_rows == 1)
+                    schur_rhs_node_kernel<double, 16><<<gsn, 256, 0, rt.stream>>>(
+                        nq, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values.ptr, s.free_mask.ptr, rin, zout, t);
+                else if (fp32)
                     schur_rhs_kernel<float><<<gs, 256, 0, rt.stream>>>(nu, s.row_ptr.ptr, pstart, s.col_idx.ptr,
                                                                         s.values_sp.ptr, s.free_mask.ptr, rin, zout, t);
                 else
@@ -808,6 +962,20 @@ static int solver_prepare(EquationsSet &s, int restart, Krylov &K)
             pressure_start_kernel<<<static_cast<unsigned>((nu + 255) / 256), 256, 0, runtime().stream>>>(
                 nu, nu, s.row_ptr.ptr, s.col_idx.ptr, s.pstart.ptr);
             OC_HIP_CHECK(hipGetLastError());
+        }
+        if (s.node_rows < 0) {
+            DevBuf<int32_t> bad_d;
+            OC_CHECK(bad_d.alloc(1));
+            Runtime &rt = runtime();
+            OC_HIP_CHECK(hipMemsetAsync(bad_d.ptr, 0, sizeof(int32_t), rt.stream));
+            node_pattern_check_kernel<<<static_cast<unsigned>((static_cast<int64_t>(s.n_nodes) * 32 + 255) / 256), 256,
+                                        0, rt.stream>>>(s.n_nodes, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr,
+                                                        bad_d.ptr);
+            OC_HIP_CHECK(hipGetLastError());
+            int32_t bad = 1;
+            OC_HIP_CHECK(hipMemcpyAsync(&bad, bad_d.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, rt.stream));
+            OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+            s.node_rows = bad ? 0 : 1;
         }
         K.pstart = s.pstart.ptr;
     }
