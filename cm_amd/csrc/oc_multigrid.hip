@@ -348,13 +348,16 @@ uu_residual_kernel(int nu, const int64_t *__restrict__ row_ptr, const int64_t *_
 
 // level 0 residual by nodes (EquationsSet::node_rows): one group of kLanes
 // lanes per node reads the neighbour list once and the nine value blocks of
-// its three rows; r = b - K_uu x on free rows, 0 on fixed ones
-template <typename VT, int kLanes>
+// its three rows; r = b - K_uu x on free rows, 0 on fixed ones.  kSweep 0
+// writes r to out; kSweep 1 writes the block Jacobi update
+// x + omega Kblock^-1 r to out (not x: neighbours still read the old x).
+template <typename VT, int kLanes, int kSweep>
 __global__ void __launch_bounds__(256)
 uu_node_residual_kernel(int nq, const int64_t *__restrict__ row_ptr, const int64_t *__restrict__ pstart,
                         const int *__restrict__ col_idx, const VT *__restrict__ vals,
                         const uint8_t *__restrict__ free_mask, const double *__restrict__ b,
-                        const double *__restrict__ x, double *__restrict__ r)
+                        const double *__restrict__ x, const double *__restrict__ inv_u, double omega,
+                        double *__restrict__ out)
 {
     const int lane = threadIdx.x % kLanes;
     const int i = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes);
@@ -379,10 +382,19 @@ uu_node_residual_kernel(int nq, const int64_t *__restrict__ row_ptr, const int64
         a1 += __shfl_xor(a1, o, kLanes);
         a2 += __shfl_xor(a2, o, kLanes);
     }
-    if (lane < 3) {
+    if (kSweep == 0) {
+        if (lane < 3) {
+            const int64_t row = lane * nq64 + i;
+            const double a = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+            out[row] = free_mask[row] ? b[row] - a : 0.0;
+        }
+    } else if (lane < 3) {
+        const double r0 = free_mask[i] ? b[i] - a0 : 0.0;
+        const double r1 = free_mask[nq64 + i] ? b[nq64 + i] - a1 : 0.0;
+        const double r2 = free_mask[2 * nq64 + i] ? b[2 * nq64 + i] - a2 : 0.0;
+        const double *o = inv_u + static_cast<int64_t>(i) * 9 + 3 * lane;
         const int64_t row = lane * nq64 + i;
-        const double a = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
-        r[row] = free_mask[row] ? b[row] - a : 0.0;
+        out[row] = x[row] + omega * (o[0] * r0 + o[1] * r1 + o[2] * r2);
     }
 }
 
@@ -710,13 +722,13 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     const unsigned gn = group_grid(nq);
     auto residual0 = [&](const double *b, const double *x, double *r) -> int {
         if (s.node_rows == 1 && precision == OC_FP32)
-            uu_node_residual_kernel<float, 32><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
-                                                                         s.col_idx.ptr, s.values_sp.ptr,
-                                                                         s.free_mask.ptr, b, x, r);
+            uu_node_residual_kernel<float, 32, 0><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
+                                                                            s.col_idx.ptr, s.values_sp.ptr,
+                                                                            s.free_mask.ptr, b, x, nullptr, 0.0, r);
         else if (s.node_rows == 1)
-            uu_node_residual_kernel<double, 32><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
-                                                                          s.col_idx.ptr, s.values.ptr,
-                                                                          s.free_mask.ptr, b, x, r);
+            uu_node_residual_kernel<double, 32, 0><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
+                                                                             s.col_idx.ptr, s.values.ptr,
+                                                                             s.free_mask.ptr, b, x, nullptr, 0.0, r);
         else if (precision == OC_FP32)
             uu_residual_kernel<float, 32><<<gu, 256, 0, rt.stream>>>(nu, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr,
                                                                     s.values_sp.ptr, s.free_mask.ptr, b, x, r);
@@ -726,13 +738,37 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
         OC_HIP_CHECK(hipGetLastError());
         return OC_NO_ERROR;
     };
-    // level 0 pre-smoothing from x = 0
-    q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, t, z, g.omega, 1);
+    // level 0 pre-smoothing from x = 0.  With node rows a sweep is one fused
+    // kernel writing the other of z and x0; the first Jacobi step picks the
+    // buffer that leaves the last post-smoothing sweep in z.
+    const bool fused = s.node_rows == 1;
+    double *cur = z, *other = g.x0.ptr;
+    if (fused && ((std::max(g.pre - 1, 0) + g.post) & 1)) std::swap(cur, other);
+    auto sweep0 = [&]() -> int {
+        if (precision == OC_FP32)
+            uu_node_residual_kernel<float, 32, 1><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
+                                                                            s.col_idx.ptr, s.values_sp.ptr,
+                                                                            s.free_mask.ptr, t, cur, inv_u, g.omega,
+                                                                            other);
+        else
+            uu_node_residual_kernel<double, 32, 1><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
+                                                                             s.col_idx.ptr, s.values.ptr,
+                                                                             s.free_mask.ptr, t, cur, inv_u, g.omega,
+                                                                             other);
+        OC_HIP_CHECK(hipGetLastError());
+        std::swap(cur, other);
+        return OC_NO_ERROR;
+    };
+    q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, t, cur, g.omega, 1);
     for (int k = 1; k < g.pre; ++k) {
-        OC_CHECK(residual0(t, z, g.r0.ptr));
-        q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, g.r0.ptr, z, g.omega, 0);
+        if (fused) {
+            OC_CHECK(sweep0());
+            continue;
+        }
+        OC_CHECK(residual0(t, cur, g.r0.ptr));
+        q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, g.r0.ptr, cur, g.omega, 0);
     }
-    OC_CHECK(residual0(t, z, g.r0.ptr));
+    OC_CHECK(residual0(t, cur, g.r0.ptr));
     MgLevel &L1 = g.levels[1];
     restrict_q2_kernel<<<group_grid(L1.nn), 256, 0, rt.stream>>>(M, g.n.ptr, s.q_off.ptr, nq, L1.off.ptr, L1.nn,
                                                             L1.freed.ptr, g.r0.ptr, L1.b.ptr);
@@ -762,11 +798,15 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
         OC_HIP_CHECK(hipGetLastError());
     }
     prolong_q2_kernel<<<grid1(nq), 256, 0, rt.stream>>>(M, g.n.ptr, s.q_off.ptr, nq, L1.off.ptr, s.free_mask.ptr,
-                                                        L1.x.ptr, z);
+                                                        L1.x.ptr, cur);
     OC_HIP_CHECK(hipGetLastError());
     for (int k = 0; k < g.post; ++k) {
-        OC_CHECK(residual0(t, z, g.r0.ptr));
-        q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, g.r0.ptr, z, g.omega, 0);
+        if (fused) {
+            OC_CHECK(sweep0());
+            continue;
+        }
+        OC_CHECK(residual0(t, cur, g.r0.ptr));
+        q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, g.r0.ptr, cur, g.omega, 0);
     }
     OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
@@ -848,6 +888,7 @@ extern "C" int oc_multigrid_structure_set(oc_equations_set h, const int *n_per_m
     OC_CHECK(g->ke1.alloc(static_cast<size_t>(s.n_elements) * 576));
     OC_CHECK(g->r0.alloc(static_cast<size_t>(3) * s.n_nodes));
     OC_CHECK(g->t0.alloc(static_cast<size_t>(3) * s.n_nodes));
+    OC_CHECK(g->x0.alloc(static_cast<size_t>(3) * s.n_nodes));
     delete s.mg;
     s.mg = guard.release();
     if (s.step_graph) {  // recorded without the multigrid setup

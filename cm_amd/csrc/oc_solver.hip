@@ -368,25 +368,30 @@ __global__ void __launch_bounds__(256) spmv_node_kernel(int nq, int nl, const in
     }
 }
 
-// t = r_u - B^T z_p on free displacement rows, 0 on fixed ones, by nodes
+// Pressure Jacobi z_p = inv_p .* r_p (written by the groups i < nl) and
+// t = r_u - B^T z_p on free displacement rows, 0 on fixed ones, by nodes; z_p
+// is formed again where B^T reads it, so one launch does both (nl <= nq)
 template <typename VT, int kLanes>
-__global__ void __launch_bounds__(256) schur_rhs_node_kernel(int nq, const int64_t *__restrict__ row_ptr,
+__global__ void __launch_bounds__(256) schur_rhs_node_kernel(int nq, int nl, const int64_t *__restrict__ row_ptr,
                                                             const int64_t *__restrict__ pstart,
                                                             const int *__restrict__ col_idx,
                                                             const VT *__restrict__ vals,
                                                             const uint8_t *__restrict__ free_mask,
-                                                            const double *__restrict__ r,
-                                                            const double *__restrict__ z, double *__restrict__ t)
+                                                            const double *__restrict__ inv_p,
+                                                            const double *__restrict__ r, double *__restrict__ z,
+                                                            double *__restrict__ t)
 {
     const int lane = threadIdx.x % kLanes;
     const int i = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes);
     if (i >= nq) return;  // whole groups
     const int64_t nq64 = nq;
+    if (i < nl && lane == 0) z[3 * nq64 + i] = inv_p[i] * r[3 * nq64 + i];
     const int64_t p0 = pstart[i], p1 = pstart[nq64 + i], p2 = pstart[2 * nq64 + i];
     const int dp = static_cast<int>(row_ptr[i + 1] - p0);
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int p = lane; p < dp; p += kLanes) {
-        const double zp = z[col_idx[p0 + p]];
+        const int c = col_idx[p0 + p];
+        const double zp = inv_p[c - 3 * nq64] * r[c];
         a0 += static_cast<double>(vals[p0 + p]) * zp;
         a1 += static_cast<double>(vals[p1 + p]) * zp;
         a2 += static_cast<double>(vals[p2 + p]) * zp;
@@ -732,18 +737,22 @@ struct Krylov {
         } else {
             const int nq = s.n_nodes, nl = s.n_nodes_l;
             if (precond == OC_PRECONDITIONER_MULTIGRID) {
-                apply_jacobi_kernel<<<grid_for(nl), 256, 0, rt.stream>>>(nl, inv_p, rin + 3 * static_cast<int64_t>(nq),
-                                                                         zout + 3 * static_cast<int64_t>(nq));
+                const bool nodes = s.node_rows == 1 && nl <= nq;
+                if (!nodes)
+                    apply_jacobi_kernel<<<grid_for(nl), 256, 0, rt.stream>>>(
+                        nl, inv_p, rin + 3 * static_cast<int64_t>(nq), zout + 3 * static_cast<int64_t>(nq));
                 const int nu = 3 * nq;
                 double *t = s.mg->t0.ptr;
                 const unsigned gs = grid_rows(static_cast<int64_t>(nu) * 8);
                 const unsigned gsn = grid_rows(static_cast<int64_t>(nq) * 16);
-                if (s.node_rows == 1 && fp32)
+                if (nodes && fp32)
                     schur_rhs_node_kernel<float, 16><<<gsn, 256, 0, rt.stream>>>(
-                        nq, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values_sp.ptr, s.free_mask.ptr, rin, zout, t);
-                else if (s.node_rows == 1)
+                        nq, nl, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values_sp.ptr, s.free_mask.ptr, inv_p, rin,
+                        zout, t);
+                else if (nodes)
                     schur_rhs_node_kernel<double, 16><<<gsn, 256, 0, rt.stream>>>(
-                        nq, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values.ptr, s.free_mask.ptr, rin, zout, t);
+                        nq, nl, s.row_ptr.ptr, pstart, s.col_idx.ptr, s.values.ptr, s.free_mask.ptr, inv_p, rin,
+                        zout, t);
                 else if (fp32)
                     schur_rhs_kernel<float><<<gs, 256, 0, rt.stream>>>(nu, s.row_ptr.ptr, pstart, s.col_idx.ptr,
                                                                         s.values_sp.ptr, s.free_mask.ptr, rin, zout, t);
