@@ -23,6 +23,24 @@ def test_laplace_config1_patch_solution(gpu):
     assert np.abs(u - mesh.coords[0]).max() < 1e-10
 
 
+@pytest.mark.parametrize("restart", [2, 3, 5])
+def test_laplace_restarted_gmres(gpu, restart):
+    """Restarted GMRES(m): many restart cycles (each ends on its m-th Arnoldi
+    step, read at once) still reach the patch solution u = x (trilinear
+    elements reproduce linear fields on jittered meshes too)."""
+    cm = gpu
+    mesh = cm.RegularMesh.generate(1, 8, jitter=0.2, seed=5)
+    eq = cm.LaplaceEquationsSet(mesh)
+    left, right = mesh.face_nodes(0, 0), mesh.face_nodes(0, 1)
+    eq.set_boundary_conditions(np.concatenate([left, right]),
+                               np.concatenate([np.zeros(len(left)), np.ones(len(right))]))
+    st = eq.solve(cm.newton_options(krylov_relative_tolerance=1e-12, krylov_restart=restart, absolute_tolerance=0.0,
+                                    krylov_maximum_iterations=20000))
+    assert st["converged"]
+    assert st["krylov_iterations"] > restart  # at least one restart
+    assert np.abs(eq.get_dofs() - mesh.coords[0]).max() < 1e-9
+
+
 def _oracle_free_residual(eq, mesh, fixed, c10, c01):
     d = eq.get_dofs()
     r = orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, c10, c01)
@@ -152,6 +170,32 @@ def test_multigrid_preconditioner_judged_by_oracle(gpu):
         eq.destroy()
     print("Krylov iterations", counts)
     assert counts[cm.OC_PRECONDITIONER_MULTIGRID] * 3 < counts[cm.OC_PRECONDITIONER_BLOCK_TRIANGULAR]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_multigrid_restarted_gmres_judged_by_oracle(gpu, precision):
+    """GMRES(4) under the multigrid preconditioner: every Newton step's linear
+    solve runs through several restart cycles and the answer still meets the
+    FP64 oracle residual."""
+    cm = gpu
+    c10, c01 = 2.0, 6.0
+    mesh = cm.RegularMesh.generate(2, 4, jitter=0.1, seed=31)
+    eq = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
+    eq.set_dofs(eq.undeformed_dofs())
+    fixed = eq.uniaxial_boundary_conditions(1.1)
+    eq.set_multigrid()
+    opts = cm.newton_options(solution_tolerance=0.0, relative_tolerance=1e-10, absolute_tolerance=0.0, krylov_restart=4,
+                             krylov_relative_tolerance=-1.0, jacobian_precision=precision,
+                             preconditioner=cm.OC_PRECONDITIONER_MULTIGRID, maximum_iterations=30,
+                             krylov_maximum_iterations=20000)
+    for step in (1, 2):
+        eq.load_increment_apply(step, 4)
+        ref = eq.residual_norm()
+        st = eq.newton_solve(opts)
+        assert st["converged"], st
+        assert st["krylov_iterations"] > 4 * st["newton_iterations"]
+        assert _oracle_free_residual(eq, mesh, fixed, c10, c01) <= 1.05e-10 * ref
+    eq.destroy()
 
 
 def test_multigrid_rejects_irregular_numbering(gpu):
