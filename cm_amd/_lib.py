@@ -27,6 +27,7 @@ OC_EQUATIONS_SET_LAPLACE = 1
 OC_EQUATIONS_SET_MOONEY_RIVLIN = 2
 OC_FP64 = 0
 OC_FP32 = 1
+OC_BF16 = 2  # MLP inference only
 
 OC_BUFFER_MATRIX_VALUES = 0
 OC_BUFFER_MATRIX_VALUES_SP = 1
@@ -154,6 +155,7 @@ SIGNATURES = {
     "oc_mlp_parameters_set": [_H, _fp],
     "oc_mlp_normalisation_set": [_H, _fp, _fp],
     "oc_mlp_forward": [_H, _c_int, _fp, _fp],
+    "oc_mlp_inference_precision_set": [_H, _c_int],
     "oc_mlp_gradient": [_H, _c_int, _fp, _fp, ctypes.POINTER(_c_float), _fp],
     "oc_mlp_dataset_set": [_H, _c_int, _fp, _fp],
     "oc_mlp_train": [_H, _c_int, _c_int, _c_float, ctypes.c_ulonglong, ctypes.POINTER(_c_float)],

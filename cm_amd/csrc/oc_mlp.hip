@@ -82,6 +82,91 @@ __global__ void __launch_bounds__(256) gemm_f32_kernel(int M, int N, int K, int 
 }
 
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+
+// ---- BF16 inference ------------------------------------------------------------------
+// bf16 = the upper 16 bits of an IEEE float; conversion rounds to nearest even
+// (NaN stays a quiet NaN).
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+// W (rows x cols, FP32, row-major) -> Wb (rows_p x cols_p, bf16, zero padded)
+__global__ void __launch_bounds__(256) weights_bf16_kernel(int rows, int cols, int rows_p, int cols_p,
+                                                          const float *__restrict__ W, uint16_t *__restrict__ Wb)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < rows_p * cols_p; q += gridDim.x * blockDim.x) {
+        const int r = q / cols_p, c = q % cols_p;
+        Wb[q] = (r < rows && c < cols) ? f32_to_bf16(W[r * cols + c]) : static_cast<uint16_t>(0);
+    }
+}
+
+// x (n x cols, FP32) -> xb (n x cols_p, bf16, zero padded)
+__global__ void __launch_bounds__(256) rows_bf16_kernel(int64_t n, int cols, int cols_p, const float *__restrict__ x,
+                                                       uint16_t *__restrict__ xb)
+{
+    for (int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; q < n * cols_p;
+         q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t r = q / cols_p;
+        const int c = static_cast<int>(q % cols_p);
+        xb[q] = c < cols ? f32_to_bf16(x[r * cols + c]) : static_cast<uint16_t>(0);
+    }
+}
+
+// One layer Y = A W^T + b on mfma_f32_16x16x16bf16_1k with FP32 accumulation.
+// A: n x K bf16, W: N x K bf16 (K, N multiples of 16, at most 128), staged in
+// LDS.  A block is four waves of 16 rows; lane l holds A[l%16][k0 + 4(l/16) ..
+// +3] and W[16j + l%16][same k], and D[4(l/16) + r][16j + l%16].  Hidden
+// layers (out_bf) store bf16(GELU(y)) for all N columns; the last layer stores
+// y in FP32 for its n_valid columns.
+constexpr int kBfMax = 128;
+__global__ void __launch_bounds__(256) mlp_layer_bf16_kernel(int n, int K, int N, int n_valid,
+                                                            const uint16_t *__restrict__ A,
+                                                            const uint16_t *__restrict__ Wb,
+                                                            const float *__restrict__ bias,
+                                                            uint16_t *__restrict__ out_bf, float *__restrict__ out_f)
+{
+    __shared__ __align__(16) uint16_t Ws[kBfMax][kBfMax + 8];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int q = t; q < N * K; q += 256) Ws[q / K][q % K] = Wb[q];
+    __syncthreads();
+    const int64_t row0 = static_cast<int64_t>(blockIdx.x) * 64 + wave * 16;
+    const int64_t arow = row0 + (lane & 15);
+    const int kq = 4 * (lane >> 4);
+    f32x4 acc[kBfMax / 16];
+#pragma unroll
+    for (int j = 0; j < kBfMax / 16; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x4 a = bf16x4{0, 0, 0, 0};
+        if (arow < n) a = *reinterpret_cast<const bf16x4 *>(A + arow * K + k0 + kq);
+#pragma unroll
+        for (int j = 0; j < kBfMax / 16; ++j) {
+            if (16 * j < N) {
+                const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&Ws[16 * j + (lane & 15)][k0 + kq]);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kBfMax / 16; ++j) {
+        if (16 * j >= N) continue;
+        const int col = 16 * j + (lane & 15);
+        const float bc = col < n_valid ? bias[col] : 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t row = row0 + 4 * (lane >> 4) + r;
+            if (row >= n) continue;
+            const float v = acc[j][r] + bc;
+            if (out_bf) out_bf[row * N + col] = f32_to_bf16(gelu(v));
+            else if (col < n_valid) out_f[row * n_valid + col] = v;
+        }
+    }
+}
 __device__ __forceinline__ float gelu_grad(float x)
 {
     return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
@@ -295,6 +380,14 @@ struct Mlp {
     DevBuf<float> data_x, data_t;
     DevBuf<double> dload;  // per-mesh load increments of the last warm start
     int n_data = 0;
+    // inference precision (OC_FP32 or OC_BF16); bf16 weights are converted
+    // from params whenever params changed since the last conversion
+    int infer_precision = OC_FP32;
+    uint64_t params_version = 1, bf16_version = 0;
+    DevBuf<uint16_t> wbf, xb, hb0, hb1;
+    std::vector<int64_t> wbf_off;
+    int bf_capacity = 0;
+    static int pad16(int v) { return (v + 15) / 16 * 16; }
 
     int layers() const { return static_cast<int>(in_dim.size()); }
 
@@ -345,6 +438,52 @@ struct Mlp {
         return OC_NO_ERROR;
     }
 
+    // act[0] must hold the normalised input; result in out (FP32).  BF16
+    // operands and activations, FP32 accumulation, bias and GELU.
+    int forward_bf16(int n)
+    {
+        KernelTimer timer("mlp_forward_bf16");
+        const int L = layers();
+        Runtime &rt = runtime();
+        if (bf16_version != params_version) {
+            wbf_off.assign(L + 1, 0);
+            for (int l = 0; l < L; ++l)
+                wbf_off[l + 1] = wbf_off[l] + static_cast<int64_t>(pad16(out_dim[l])) * pad16(in_dim[l]);
+            if (wbf.n < static_cast<size_t>(wbf_off[L])) OC_CHECK(wbf.alloc(static_cast<size_t>(wbf_off[L])));
+            for (int l = 0; l < L; ++l) {
+                const int rp = pad16(out_dim[l]), cp = pad16(in_dim[l]);
+                weights_bf16_kernel<<<blocks_for(static_cast<int64_t>(rp) * cp), 256, 0, rt.stream>>>(
+                    out_dim[l], in_dim[l], rp, cp, params.ptr + w_off[l], wbf.ptr + wbf_off[l]);
+                OC_HIP_CHECK(hipGetLastError());
+            }
+            bf16_version = params_version;
+        }
+        if (n > bf_capacity) {
+            OC_CHECK(xb.alloc(static_cast<size_t>(n) * pad16(n_in)));
+            OC_CHECK(hb0.alloc(static_cast<size_t>(n) * pad16(n_hidden)));
+            OC_CHECK(hb1.alloc(static_cast<size_t>(n) * pad16(n_hidden)));
+            bf_capacity = n;
+        }
+        rows_bf16_kernel<<<blocks_for(static_cast<int64_t>(n) * pad16(n_in)), 256, 0, rt.stream>>>(
+            n, n_in, pad16(n_in), act[0].ptr, xb.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+        const uint16_t *a = xb.ptr;
+        uint16_t *h = hb0.ptr, *spare = hb1.ptr;
+        const unsigned blocks = static_cast<unsigned>((static_cast<int64_t>(n) + 63) / 64);
+        for (int l = 0; l < L; ++l) {
+            const bool last = l + 1 == L;
+            mlp_layer_bf16_kernel<<<blocks, 256, 0, rt.stream>>>(n, pad16(in_dim[l]), pad16(out_dim[l]), out_dim[l], a,
+                                                                 wbf.ptr + wbf_off[l], params.ptr + b_off[l],
+                                                                 last ? nullptr : h, last ? out.ptr : nullptr);
+            OC_HIP_CHECK(hipGetLastError());
+            a = h;
+            std::swap(h, spare);
+        }
+        return OC_NO_ERROR;
+    }
+
+    int infer(int n) { return infer_precision == OC_BF16 ? forward_bf16(n) : forward(n); }
+
     // after forward(n) with targets T on device: gradients into grads, loss partials into partial
     int backward(int n, const float *T, int *n_partials)
     {
@@ -379,6 +518,7 @@ struct Mlp {
     int adam(float lr)
     {
         ++adam_t;
+        ++params_version;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.f - std::pow(b1, static_cast<float>(adam_t)), c2 = 1.f - std::pow(b2, static_cast<float>(adam_t));
         adam_kernel<<<static_cast<unsigned>((n_params + 255) / 256), 256, 0, runtime().stream>>>(
@@ -489,6 +629,19 @@ int oc_mlp_parameters_set(oc_mlp mlp, const float *parameters)
     if (!mlp || !parameters) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
     OC_HIP_CHECK(hipStreamSynchronize(runtime().stream));
     OC_HIP_CHECK(hipMemcpy(mlp->m.params.ptr, parameters, sizeof(float) * mlp->m.n_params, hipMemcpyHostToDevice));
+    ++mlp->m.params_version;
+    return OC_NO_ERROR;
+}
+
+int oc_mlp_inference_precision_set(oc_mlp mlp, int precision)
+{
+    if (!mlp) return set_error(OC_ERR_INVALID_HANDLE, "MLP handle is NULL");
+    if (precision != OC_FP32 && precision != OC_BF16)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "inference precision must be OC_FP32 or OC_BF16");
+    const Mlp &m = mlp->m;
+    if (precision == OC_BF16 && (m.n_in > kBfMax || m.n_hidden > kBfMax || m.n_out > kBfMax))
+        return set_error(OC_ERR_UNSUPPORTED, "BF16 inference needs layer widths of at most %d", kBfMax);
+    mlp->m.infer_precision = precision;
     return OC_NO_ERROR;
 }
 
@@ -513,7 +666,7 @@ int oc_mlp_forward(oc_mlp mlp, int n, const float *x, float *y)
     OC_CHECK(raw.upload(x, static_cast<size_t>(n) * m.n_in));
     normalise_kernel<<<blocks_for(n), 256, 0, rt.stream>>>(n, m.n_in, raw.ptr, m.shift.ptr, m.scale.ptr, m.act[0].ptr);
     OC_HIP_CHECK(hipGetLastError());
-    OC_CHECK(m.forward(n));
+    OC_CHECK(m.infer(n));
     OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
     OC_HIP_CHECK(hipMemcpy(y, m.out.ptr, sizeof(float) * n * m.n_out, hipMemcpyDeviceToHost));
     return OC_NO_ERROR;
@@ -618,7 +771,7 @@ int oc_mlp_warm_start_batch(oc_mlp mlp, oc_equations_set h, const double *load_i
                                                                 m.dload.ptr, s.node_mat.ptr, m.shift.ptr, m.scale.ptr,
                                                                 m.act[0].ptr);
     OC_HIP_CHECK(hipGetLastError());
-    OC_CHECK(m.forward(nq));
+    OC_CHECK(m.infer(nq));
     warm_apply_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, m.out.ptr, m.n_out, M, s.q_off.ptr, m.dload.ptr,
                                                              s.free_mask.ptr, s.dofs.ptr);
     OC_HIP_CHECK(hipGetLastError());

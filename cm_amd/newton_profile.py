@@ -1,7 +1,7 @@
 """One load step of the 32^3 Mooney-Rivlin case (20% stretch in 10 steps),
 plain FP64 Newton or learned warm start + FP32 Jacobian, for kernel traces.
 
-Usage: python -m cm_amd.newton_profile [--learned] [--n 32] [--steps 1]
+Usage: python -m cm_amd.newton_profile [--learned [--bf16]] [--n 32] [--steps 1]
 """
 import argparse
 import json
@@ -18,11 +18,14 @@ def main():
     ap.add_argument("--learned", action="store_true")
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--bf16", action="store_true", help="BF16 inference of the warm start")
     args = ap.parse_args()
     initialise(0)
     case = dict(n=args.n, c10=2.0, c01=6.0, stretch=0.20, seed=2024)
     mesh, eq = make_problem(case)
     net = WarmStartMLP.load(DEFAULT_WEIGHTS) if args.learned else None
+    if net is not None and args.bf16:
+        net.set_inference_precision(_lib.OC_BF16)
     dload = case["stretch"] / 10
     for s in range(1, args.steps + 1):
         d0 = eq.get_dofs()
@@ -37,7 +40,8 @@ def main():
             prec = _lib.OC_FP32
         st = eq.newton_solve(solver_options(prec, reference_norm=ref))
         synchronise()
-        print(json.dumps({"step": s, "learned": net is not None, "seconds": time.perf_counter() - t0,
+        print(json.dumps({"step": s, "learned": net is not None, "bf16": bool(args.bf16 and net is not None),
+                          "seconds": time.perf_counter() - t0,
                           "newton": st["newton_iterations"], "krylov": st["krylov_iterations"],
                           "final_over_ref": st["final_residual_norm"] / ref}), flush=True)
     eq.destroy()

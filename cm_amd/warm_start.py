@@ -70,6 +70,10 @@ class WarmStartMLP:
             raise ValueError(f"expected {self.n_parameters} parameters, got {p.size}")
         call("oc_mlp_parameters_set", self._h, p)
 
+    def set_inference_precision(self, precision):
+        """OC_FP32 (default) or OC_BF16 for forward() and the warm starts."""
+        call("oc_mlp_inference_precision_set", self._h, int(precision))
+
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float32)
         if x.ndim != 2 or x.shape[1] != N_FEATURES:

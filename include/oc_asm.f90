@@ -16,7 +16,7 @@ MODULE OC_ASM
   INTEGER(C_INT), PARAMETER :: OC_ERR_PATTERN = -6
   INTEGER(C_INT), PARAMETER :: OC_ERR_NOT_INITIALISED = -7
   INTEGER(C_INT), PARAMETER :: OC_ERR_UNSUPPORTED = -8
-  INTEGER(C_INT), PARAMETER :: OC_FP64 = 0, OC_FP32 = 1
+  INTEGER(C_INT), PARAMETER :: OC_FP64 = 0, OC_FP32 = 1, OC_BF16 = 2
   INTEGER(C_INT), PARAMETER :: OC_EQUATIONS_SET_LAPLACE = 1, OC_EQUATIONS_SET_MOONEY_RIVLIN = 2
   INTEGER(C_INT), PARAMETER :: OC_BUFFER_MATRIX_VALUES = 0, OC_BUFFER_MATRIX_VALUES_SP = 1, &
       & OC_BUFFER_ROW_PTR = 2, OC_BUFFER_COL_IDX = 3, OC_BUFFER_DOFS = 4, OC_BUFFER_RESIDUAL = 5, OC_BUFFER_RHS = 6, &
@@ -423,6 +423,13 @@ MODULE OC_ASM
       REAL(C_FLOAT), DIMENSION(*), INTENT(IN) :: x
       REAL(C_FLOAT), DIMENSION(*), INTENT(OUT) :: y
     END FUNCTION oc_mlp_forward
+
+    INTEGER(C_INT) FUNCTION oc_mlp_inference_precision_set(mlp, precision) &
+        BIND(C, NAME="oc_mlp_inference_precision_set")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: mlp
+      INTEGER(C_INT), VALUE :: precision
+    END FUNCTION oc_mlp_inference_precision_set
 
     INTEGER(C_INT) FUNCTION oc_mlp_gradient(mlp, n, x, t, loss, gradient) BIND(C, NAME="oc_mlp_gradient")
       IMPORT :: C_INT, C_FLOAT, C_PTR

@@ -52,6 +52,7 @@ extern "C" {
 /* ---- precision flags ----------------------------------------------------- */
 #define OC_FP64 0
 #define OC_FP32 1
+#define OC_BF16 2   /* MLP inference only */
 
 /* ---- device buffers that can be borrowed --------------------------------- */
 #define OC_BUFFER_MATRIX_VALUES 0     /* double[nnz]: linear matrix or FP64 Jacobian */
@@ -317,6 +318,11 @@ int oc_mlp_parameters_set(oc_mlp mlp, const float *parameters);
 int oc_mlp_normalisation_set(oc_mlp mlp, const float *shift, const float *scale);
 /* y = MLP(x) for n samples (host arrays, x: n*n_inputs, y: n*n_outputs). */
 int oc_mlp_forward(oc_mlp mlp, int n, const float *x, float *y);
+/* Inference precision of oc_mlp_forward and the warm starts: OC_FP32 (the
+ * default) or OC_BF16 (bf16 weights and activations on MFMA with FP32
+ * accumulation, bias and GELU; layer widths up to 128).  Training stays FP32.
+ * No counterpart in the reference: the warm start is this library's own. */
+int oc_mlp_inference_precision_set(oc_mlp mlp, int precision);
 /* Loss 1/(n*n_out) sum (y - t)^2 and its gradient w.r.t. the parameters
  * (host arrays; gradient in parameter order). */
 int oc_mlp_gradient(oc_mlp mlp, int n, const float *x, const float *t, float *loss, float *gradient);

@@ -88,3 +88,28 @@ def test_committed_network_on_heldout_mesh(gpu):
     assert np.mean(ratios) < 0.3
     assert sum(r["newton"] for r in learned) < sum(r["newton"] for r in plain)
     net.destroy()
+
+
+def test_committed_network_bf16_inference(gpu):
+    """BF16 inference of the shipped network as the warm start: the accepted
+    answers meet the same oracle residual.  Its start is several times further
+    from the solution than the FP32-inference start (bf16 keeps 8 significant
+    bits of every hidden activation) but still far inside the plain start, and
+    costs at most one more Newton iteration per step."""
+    from cm_amd import _lib
+    from cm_amd.train_warm_start import DEFAULT_WEIGHTS
+    from cm_amd.warm_start import WarmStartMLP
+    net = WarmStartMLP.load(DEFAULT_WEIGHTS)
+    case = dict(n=5, c10=3.0, c01=2.0, stretch=0.15, seed=31337)
+    plain = run_steps(case, 3, None, _lib.OC_FP64)
+    f32 = run_steps(case, 3, net, _lib.OC_FP32)
+    net.set_inference_precision(_lib.OC_BF16)
+    b16 = run_steps(case, 3, net, _lib.OC_FP32)
+    check_accepted(b16)
+    ratios = [b["start"] / f["start"] for b, f in zip(b16, f32)]
+    print("start residual bf16/fp32 inference per step:", ratios)
+    print("Newton iterations fp32", [r["newton"] for r in f32], "bf16", [r["newton"] for r in b16])
+    assert max(ratios) < 20.0
+    assert max(b["start"] / p["start"] for b, p in zip(b16, plain)) < 0.1
+    assert all(b["newton"] <= f["newton"] + 1 for b, f in zip(b16, f32))
+    net.destroy()
