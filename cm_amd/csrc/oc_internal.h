@@ -121,6 +121,7 @@ struct MgLevel {
     DevBuf<int32_t> dim, off;
     DevBuf<double> A;                     // nn*27*9: A[(node*27 + d)*9 + 3c + c']
     DevBuf<double> Dinv;                  // nn*9 inverse diagonal blocks
+    DevBuf<double> As;                    // A(node, d) Dinv(neighbour d), levels with a coarser level
     DevBuf<uint8_t> freed;                // nn*3: DOF not fixed
     DevBuf<double> x, b, r;               // nn*3 each
 };

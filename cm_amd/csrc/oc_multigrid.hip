@@ -487,6 +487,75 @@ __global__ void __launch_bounds__(256) prolong_q2_kernel(int M, const int *__res
     }
 }
 
+// As(node, d) = A(node, d) Dinv(neighbour d): the first Jacobi step from
+// x = 0 (x1 = omega Dinv b) folded into the stencil, so r = b - A x1 =
+// b - omega sum_d As(node, d) b(neighbour d) needs no x1 from other nodes
+__global__ void __launch_bounds__(256) scaled_stencil_kernel(int M, const int *__restrict__ dim,
+                                                            const int *__restrict__ off, int nn,
+                                                            const double *__restrict__ A,
+                                                            const double *__restrict__ Dinv, double *__restrict__ As)
+{
+    const int64_t total = static_cast<int64_t>(nn) * 27;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int node = static_cast<int>(t / 27), d = static_cast<int>(t % 27);
+        const int m = find_mesh(off, M, node);
+        const int N = dim[m], local = node - off[m];
+        const int J[3] = {local % N + d % 3 - 1, (local / N) % N + (d / 3) % 3 - 1, local / (N * N) + d / 9 - 1};
+        double *o = As + t * 9;
+        if (J[0] < 0 || J[0] >= N || J[1] < 0 || J[1] >= N || J[2] < 0 || J[2] >= N) {
+            for (int k = 0; k < 9; ++k) o[k] = 0.0;
+            continue;
+        }
+        const int64_t j = off[m] + J[0] + static_cast<int64_t>(N) * (J[1] + static_cast<int64_t>(N) * J[2]);
+        const double *a = A + t * 9, *q = Dinv + j * 9;
+        for (int c = 0; c < 3; ++c)
+            for (int cp = 0; cp < 3; ++cp) o[3 * c + cp] = a[3 * c] * q[cp] + a[3 * c + 1] * q[3 + cp] + a[3 * c + 2] * q[6 + cp];
+    }
+}
+
+// First smoothing step of a stencil level and its residual in one launch:
+// x = omega Dinv b, r = b - omega sum_d As(node, d) b(neighbour d) on free
+// DOFs (0 on fixed ones); one 32-lane group per node as in stencil_apply_kernel
+__global__ void __launch_bounds__(256) stencil_first_residual_kernel(int M, const int *__restrict__ dim,
+                                                                    const int *__restrict__ off, int nn,
+                                                                    const double *__restrict__ As,
+                                                                    const uint8_t *__restrict__ freed,
+                                                                    const double *__restrict__ b,
+                                                                    const double *__restrict__ Dinv, double omega,
+                                                                    double *__restrict__ x, double *__restrict__ r)
+{
+    const int node = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 5);
+    const int d = threadIdx.x & 31;
+    if (node >= nn) return;  // whole 32-lane groups
+    const int m = find_mesh(off, M, node);
+    const int N = dim[m], local = node - off[m];
+    double p[3] = {0.0, 0.0, 0.0};
+    if (d < 27) {
+        const int J[3] = {local % N + d % 3 - 1, (local / N) % N + (d / 3) % 3 - 1, local / (N * N) + d / 9 - 1};
+        if (J[0] >= 0 && J[0] < N && J[1] >= 0 && J[1] < N && J[2] >= 0 && J[2] < N) {
+            const int64_t j = off[m] + J[0] + static_cast<int64_t>(N) * (J[1] + static_cast<int64_t>(N) * J[2]);
+            const double *a = As + (static_cast<int64_t>(node) * 27 + d) * 9;
+            const double b0 = b[j * 3], b1 = b[j * 3 + 1], b2 = b[j * 3 + 2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p[c] = a[3 * c] * b0 + a[3 * c + 1] * b1 + a[3 * c + 2] * b2;
+        }
+    }
+#pragma unroll
+    for (int sh = 16; sh > 0; sh >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] += __shfl_xor(p[c], sh, 32);
+    if (d != 0) return;
+    const double *o = Dinv + static_cast<int64_t>(node) * 9;
+    const int64_t k0 = static_cast<int64_t>(node) * 3;
+    const double bn[3] = {b[k0], b[k0 + 1], b[k0 + 2]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r[k0 + c] = freed[k0 + c] ? bn[c] - omega * p[c] : 0.0;
+        x[k0 + c] = omega * (o[3 * c] * bn[0] + o[3 * c + 1] * bn[1] + o[3 * c + 2] * bn[2]);
+    }
+}
+
 __global__ void __launch_bounds__(256) stencil_jacobi_kernel(int nn, const double *__restrict__ Dinv,
                                                             const double *__restrict__ r, double *__restrict__ x,
                                                             double omega, int first)
@@ -677,6 +746,9 @@ int mg_setup(EquationsSet &s, int precision)
     level_mask_kernel<<<grid1(static_cast<int64_t>(L1.nn) * 27), 256, 0, rt.stream>>>(
         M, L1.dim.ptr, L1.off.ptr, L1.nn, L1.A.ptr, L1.freed.ptr, L1.Dinv.ptr);
     OC_HIP_CHECK(hipGetLastError());
+    if (L1.As.ptr)
+        scaled_stencil_kernel<<<grid1(static_cast<int64_t>(L1.nn) * 27), 256, 0, rt.stream>>>(
+            M, L1.dim.ptr, L1.off.ptr, L1.nn, L1.A.ptr, L1.Dinv.ptr, L1.As.ptr);
     const int nl = static_cast<int>(g.levels.size());
     for (int l = 1; l + 1 < nl; ++l) {
         MgLevel &F = g.levels[l], &C = g.levels[l + 1];
@@ -688,6 +760,9 @@ int mg_setup(EquationsSet &s, int precision)
         level_freed_kernel<<<grid1(C.nn), 256, 0, rt.stream>>>(C.nn, C.A.ptr, C.freed.ptr);
         level_mask_kernel<<<grid1(static_cast<int64_t>(C.nn) * 27), 256, 0, rt.stream>>>(
             M, C.dim.ptr, C.off.ptr, C.nn, C.A.ptr, C.freed.ptr, C.Dinv.ptr);
+        if (C.As.ptr)
+            scaled_stencil_kernel<<<grid1(static_cast<int64_t>(C.nn) * 27), 256, 0, rt.stream>>>(
+                M, C.dim.ptr, C.off.ptr, C.nn, C.A.ptr, C.Dinv.ptr, C.As.ptr);
         OC_HIP_CHECK(hipGetLastError());
     }
     MgLevel &Lc = g.levels[nl - 1];
@@ -776,11 +851,16 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     // down the stencil levels
     for (int l = 1; l + 1 < nl; ++l) {
         MgLevel &F = g.levels[l], &C = g.levels[l + 1];
-        stencil_jacobi_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(F.nn, F.Dinv.ptr, F.b.ptr, F.x.ptr, g.omega, 1);
-        for (int k = 1; k < g.pre; ++k) OC_CHECK(stencil_sweep(M, F, g.omega));
-        stencil_apply_kernel<0><<<group_grid(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
-                                                                        F.freed.ptr, F.b.ptr, F.x.ptr, nullptr, 0.0,
-                                                                        F.r.ptr);
+        if (g.pre <= 1 && F.As.ptr) {
+            stencil_first_residual_kernel<<<group_grid(F.nn), 256, 0, rt.stream>>>(
+                M, F.dim.ptr, F.off.ptr, F.nn, F.As.ptr, F.freed.ptr, F.b.ptr, F.Dinv.ptr, g.omega, F.x.ptr, F.r.ptr);
+        } else {
+            stencil_jacobi_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(F.nn, F.Dinv.ptr, F.b.ptr, F.x.ptr, g.omega, 1);
+            for (int k = 1; k < g.pre; ++k) OC_CHECK(stencil_sweep(M, F, g.omega));
+            stencil_apply_kernel<0><<<group_grid(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
+                                                                            F.freed.ptr, F.b.ptr, F.x.ptr, nullptr,
+                                                                            0.0, F.r.ptr);
+        }
         restrict_stencil_kernel<<<group_grid(C.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, C.dim.ptr, C.off.ptr,
                                                                    C.nn, C.freed.ptr, F.r.ptr, C.b.ptr);
         OC_HIP_CHECK(hipGetLastError());
@@ -877,6 +957,7 @@ extern "C" int oc_multigrid_structure_set(oc_equations_set h, const int *n_per_m
         OC_CHECK(L.b.alloc(static_cast<size_t>(L.nn) * 3));
         OC_CHECK(L.r.alloc(static_cast<size_t>(L.nn) * 3));
         if (*std::max_element(dims.begin(), dims.end()) <= 3) break;
+        OC_CHECK(L.As.alloc(static_cast<size_t>(L.nn) * 27 * 9));
         for (int m = 0; m < M; ++m) dims[m] = (dims[m] - 1 + 1) / 2 + 1;  // n -> ceil(n/2)
         g->levels.emplace_back();
     }
