@@ -193,6 +193,7 @@ struct EquationsSet {
     DevBuf<int32_t> col_idx;
     DevBuf<double> values;
     DevBuf<float> values_sp;
+    DevBuf<uint16_t> values_bf;  // bf16 copy of values_sp for the multigrid smoother (FP32 Jacobians)
     DevBuf<double> dofs, residual, rhs;
 
     // Mooney-Rivlin: per-Gauss-point records (field-major, see

@@ -346,6 +346,31 @@ uu_residual_kernel(int nu, const int64_t *__restrict__ row_ptr, const int64_t *_
     if (lane == 0) r[row] = f ? b[row] - acc : 0.0;
 }
 
+// bf16 matrix values (the upper 16 bits of an FP32 value, rounded to nearest
+// even): the level-0 smoother's copy of an FP32 Jacobian
+struct Bf16 {
+    uint16_t bits;
+    __device__ explicit operator double() const
+    {
+        return static_cast<double>(__uint_as_float(static_cast<uint32_t>(bits) << 16));
+    }
+};
+
+__global__ void __launch_bounds__(256) values_bf16_kernel(int64_t n, const float *__restrict__ v,
+                                                         uint16_t *__restrict__ o)
+{
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        uint32_t u = __float_as_uint(v[i]);
+        if ((u & 0x7fffffffu) > 0x7f800000u) {
+            o[i] = static_cast<uint16_t>((u >> 16) | 0x40u);
+            continue;
+        }
+        u += 0x7fffu + ((u >> 16) & 1u);
+        o[i] = static_cast<uint16_t>(u >> 16);
+    }
+}
+
 // level 0 residual by nodes (EquationsSet::node_rows): one group of kLanes
 // lanes per node reads the neighbour list once and the nine value blocks of
 // its three rows; r = b - K_uu x on free rows, 0 on fixed ones.  kSweep 0
@@ -737,6 +762,12 @@ int mg_setup(EquationsSet &s, int precision)
         q1_galerkin_element_kernel<float><<<eb, 256, 0, rt.stream>>>(s.n_elements, s.kbuf32.ptr, s.conn.ptr,
                                                                     s.n_nodes, s.free_mask.ptr, g.ke1.ptr);
     OC_HIP_CHECK(hipGetLastError());
+    if (precision == OC_FP32 && s.values_bf.ptr) {
+        const int64_t nnz = static_cast<int64_t>(s.values_bf.n);
+        values_bf16_kernel<<<static_cast<unsigned>(std::min<int64_t>((nnz + 255) / 256, 16384)), 256, 0, rt.stream>>>(
+            nnz, s.values_sp.ptr, s.values_bf.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+    }
     MgLevel &L1 = g.levels[1];
     q1_stencil_kernel<<<grid1(static_cast<int64_t>(L1.nn) * 27), 256, 0, rt.stream>>>(
         M, g.n.ptr, g.e_off.ptr, L1.off.ptr, L1.nn, g.ke1.ptr, L1.A.ptr);
@@ -795,8 +826,14 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     const double *inv_u = s.precond.ptr;  // nodal block inverses of the set's block preconditioner
     const unsigned gu = static_cast<unsigned>((static_cast<int64_t>(nu) * 32 + 255) / 256);
     const unsigned gn = group_grid(nq);
+    const bool bf = precision == OC_FP32 && s.node_rows == 1 && s.values_bf.ptr;
+    const Bf16 *vbf = reinterpret_cast<const Bf16 *>(s.values_bf.ptr);
     auto residual0 = [&](const double *b, const double *x, double *r) -> int {
-        if (s.node_rows == 1 && precision == OC_FP32)
+        if (bf)
+            uu_node_residual_kernel<Bf16, 32, 0><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
+                                                                           s.col_idx.ptr, vbf, s.free_mask.ptr, b, x,
+                                                                           nullptr, 0.0, r);
+        else if (s.node_rows == 1 && precision == OC_FP32)
             uu_node_residual_kernel<float, 32, 0><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
                                                                             s.col_idx.ptr, s.values_sp.ptr,
                                                                             s.free_mask.ptr, b, x, nullptr, 0.0, r);
@@ -820,7 +857,11 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     double *cur = z, *other = g.x0.ptr;
     if (fused && ((std::max(g.pre - 1, 0) + g.post) & 1)) std::swap(cur, other);
     auto sweep0 = [&]() -> int {
-        if (precision == OC_FP32)
+        if (bf)
+            uu_node_residual_kernel<Bf16, 32, 1><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
+                                                                           s.col_idx.ptr, vbf, s.free_mask.ptr, t,
+                                                                           cur, inv_u, g.omega, other);
+        else if (precision == OC_FP32)
             uu_node_residual_kernel<float, 32, 1><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
                                                                             s.col_idx.ptr, s.values_sp.ptr,
                                                                             s.free_mask.ptr, t, cur, inv_u, g.omega,
