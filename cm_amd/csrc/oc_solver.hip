@@ -1091,10 +1091,17 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
     K.precond = opt->preconditioner;
     K.fp32 = opt->jacobian_precision == OC_FP32;
     OC_CHECK(solver_prepare(s, opt->krylov_restart, K));
-    // an FP32 Jacobian's multigrid smoother reads a bf16 copy (made in the
-    // preconditioner setup, so allocated here, outside the step graph)
-    if (K.fp32 && K.precond == OC_PRECONDITIONER_MULTIGRID && s.node_rows == 1 && !s.values_bf.ptr)
-        OC_CHECK(s.values_bf.alloc(s.values_sp.n));
+    // an FP32 Jacobian's multigrid smoother reads a bf16 copy, made in the
+    // preconditioner setup: allocated here, outside the step graph, and a
+    // graph recorded before the copy existed is recorded again
+    if (K.fp32 && K.precond == OC_PRECONDITIONER_MULTIGRID && s.node_rows == 1 && !s.values_bf.ptr) {
+        if (!s.values_sp.ptr) OC_CHECK(s.values_sp.alloc(s.nnz));
+        OC_CHECK(s.values_bf.alloc(s.nnz));
+        if (s.step_graph) {
+            (void)hipGraphExecDestroy(s.step_graph);
+            s.step_graph = nullptr;
+        }
+    }
     OC_CHECK(mr_jacobian_buffers(s, opt->jacobian_precision));
     const BasisTables *T = nullptr;
     OC_CHECK(ensure_tables(&T));
