@@ -151,7 +151,8 @@ def newton_line(cm, n, load_steps):
     from cm_amd.train_warm_start import DEFAULT_WEIGHTS, solve_steps
     from cm_amd.warm_start import WarmStartMLP
     case = dict(n=n, c10=2.0, c01=6.0, stretch=0.10, seed=2024)
-    plain = solve_steps(case, load_steps)
+    plain = solve_steps(case, load_steps)  # the reference's forcing: GMRES to 1e-5 relative per Newton step
+    plain_ew = solve_steps(case, load_steps, krylov_rtol=-1.0)  # plain start with the learned form's forcing
     net = WarmStartMLP.load(DEFAULT_WEIGHTS)
     learned = solve_steps(case, load_steps, net=net, precision=cm.OC_FP32)
     net.destroy()
@@ -161,10 +162,12 @@ def newton_line(cm, n, load_steps):
                 "seconds_per_step": float(np.mean([r["seconds"] for r in rows])),
                 "newton_iterations": [r["newton"] for r in rows], "krylov_iterations": [r["krylov"] for r in rows],
                 "start_residual_over_plain_start": [r["start_norm"] / r["reference_norm"] for r in rows]}
-    p, l = summary(plain), summary(learned)
+    p, pe, l = summary(plain), summary(plain_ew), summary(learned)
     return {"workload": f"{n}^3 Mooney-Rivlin, c10=2, c01=6, 10% stretch in {load_steps} load steps, "
                         "1e-10 x plain start residual, GMRES with block-triangular multigrid preconditioner",
-            "plain_fp64": p, "learned_fp32": l, "speedup": p["seconds_per_step"] / l["seconds_per_step"]}
+            "plain_fp64": p, "plain_fp64_eisenstat_walker": pe, "learned_fp32": l,
+            "speedup": p["seconds_per_step"] / l["seconds_per_step"],
+            "speedup_vs_plain_eisenstat_walker": pe["seconds_per_step"] / l["seconds_per_step"]}
 
 
 def batched_line(cm, n_meshes, reps=3):

@@ -9,7 +9,7 @@ import time
 
 from . import _lib
 from .equations_set import initialise, synchronise
-from .train_warm_start import DEFAULT_WEIGHTS, make_problem, solver_options
+from .train_warm_start import DEFAULT_WEIGHTS, PLAIN_KRYLOV_RTOL, make_problem, solver_options
 from .warm_start import WarmStartMLP
 
 
@@ -38,7 +38,8 @@ def main():
             net.warm_start(eq, dload)
             eq.load_increment_apply(s, 10)
             prec = _lib.OC_FP32
-        st = eq.newton_solve(solver_options(prec, reference_norm=ref))
+        st = eq.newton_solve(solver_options(prec, reference_norm=ref,
+                                            krylov_rtol=-1.0 if net is not None else PLAIN_KRYLOV_RTOL))
         synchronise()
         print(json.dumps({"step": s, "learned": net is not None, "bf16": bool(args.bf16 and net is not None),
                           "seconds": time.perf_counter() - t0,

@@ -36,9 +36,16 @@ def draw_case(rng, n_min=4, n_max=16):
                 stretch=float(rng.uniform(0.05, 0.20)), seed=int(rng.integers(0, 2 ** 31)))
 
 
-def solver_options(precision=_lib.OC_FP64, restart=100, reference_norm=0.0):
+# Forcing of the plain method: every Newton step's linear system solved to the
+# reference's default GMRES relative tolerance (solver_routines.f90:10983; the
+# reference sets no Eisenstat-Walker forcing).  The learned form uses
+# Eisenstat-Walker forcing capped by ||F||/||F_ref|| (krylov rtol -1).
+PLAIN_KRYLOV_RTOL = 1e-5
+
+
+def solver_options(precision=_lib.OC_FP64, restart=100, reference_norm=0.0, krylov_rtol=-1.0):
     return newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, reference_norm=reference_norm,
-                          krylov_restart=restart, krylov_relative_tolerance=-1.0, jacobian_precision=precision,
+                          krylov_restart=restart, krylov_relative_tolerance=krylov_rtol, jacobian_precision=precision,
                           preconditioner=_lib.OC_PRECONDITIONER_MULTIGRID, maximum_iterations=100,
                           krylov_maximum_iterations=100000,
                           # judged by the residual alone: no step-size test, no evaluation limit
@@ -148,9 +155,13 @@ def train(net, X, T, steps, batch=4096, lr=1e-3, seed=1, log=print):
     return loss
 
 
-def solve_steps(case, steps, net=None, precision=_lib.OC_FP64, max_steps=None, restart=100, log=None):
+def solve_steps(case, steps, net=None, precision=_lib.OC_FP64, max_steps=None, restart=100, log=None,
+                krylov_rtol=None):
     """Solves the load steps; with `net`, each step starts from the MLP guess.
-    Returns per-step dicts (iterations, seconds, residual norms)."""
+    Krylov forcing: Eisenstat-Walker with `net`, else PLAIN_KRYLOV_RTOL, unless
+    krylov_rtol is given.  Returns per-step dicts (iterations, seconds, residual norms)."""
+    if krylov_rtol is None:
+        krylov_rtol = -1.0 if net is not None else PLAIN_KRYLOV_RTOL
     mesh, eq = make_problem(case)
     dload = case["stretch"] / steps
     out = []
@@ -164,7 +175,8 @@ def solve_steps(case, steps, net=None, precision=_lib.OC_FP64, max_steps=None, r
             eq.set_dofs(d0)
             net.warm_start(eq, dload)
             eq.load_increment_apply(s, steps)
-        st = eq.newton_solve(solver_options(precision, restart, reference_norm=ref), raise_on_failure=False)
+        st = eq.newton_solve(solver_options(precision, restart, reference_norm=ref, krylov_rtol=krylov_rtol),
+                             raise_on_failure=False)
         synchronise()
         wall = time.perf_counter() - t0
         out.append(dict(step=s, converged=bool(st["converged"]), newton=st["newton_iterations"],

@@ -23,7 +23,7 @@ def free_residual_norm(mesh, eq, dofs):
 
 def run_steps(case, steps, net, precision):
     from cm_amd import _lib
-    from cm_amd.train_warm_start import make_problem, solver_options
+    from cm_amd.train_warm_start import PLAIN_KRYLOV_RTOL, make_problem, solver_options
     mesh, eq = make_problem(case)
     dload = case["stretch"] / steps
     rows = []
@@ -35,7 +35,8 @@ def run_steps(case, steps, net, precision):
             eq.set_dofs(d0)
             net.warm_start(eq, dload)
             eq.load_increment_apply(s, steps)
-        st = eq.newton_solve(solver_options(precision, reference_norm=ref))
+        st = eq.newton_solve(solver_options(precision, reference_norm=ref,
+                                            krylov_rtol=-1.0 if net is not None else PLAIN_KRYLOV_RTOL))
         rows.append(dict(ref=ref, start=st["initial_residual_norm"], final=st["final_residual_norm"],
                          newton=st["newton_iterations"], oracle=free_residual_norm(mesh, eq, eq.get_dofs())))
     eq.destroy()
