@@ -173,10 +173,6 @@ int oc_initialise(int device)
     rt.num_cus = prop.multiProcessorCount;
     OC_HIP_CHECK(hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking));
     for (auto &e : rt.events) OC_HIP_CHECK(hipEventCreate(&e));
-    for (auto &e : rt.arnoldi_events) OC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    OC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&rt.arnoldi_host), sizeof(double) * kArnoldiHostSlots,
-                               hipHostMallocMapped | hipHostMallocCoherent));
-    OC_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&rt.arnoldi_host_dev), rt.arnoldi_host, 0));
     rt.device = device;
     return OC_NO_ERROR;
 }
@@ -191,12 +187,6 @@ int oc_finalise(void)
         if (e) (void)hipEventDestroy(e);
         e = nullptr;
     }
-    for (auto &e : rt.arnoldi_events) {
-        if (e) (void)hipEventDestroy(e);
-        e = nullptr;
-    }
-    if (rt.arnoldi_host) (void)hipHostFree(rt.arnoldi_host);
-    rt.arnoldi_host = rt.arnoldi_host_dev = nullptr;
     (void)hipStreamDestroy(rt.stream);
     rt.stream = nullptr;
     rt.device = -1;

@@ -32,12 +32,6 @@ struct Runtime {
     hipStream_t stream = nullptr;
     hipEvent_t events[64] = {};
     int num_cus = 0;
-    // Krylov progress read back without a stream synchronisation: pinned,
-    // mapped, coherent host memory the Arnoldi kernels write (kArnoldiHostSlots
-    // doubles), and two timing-free events marking alternate iterations.
-    double *arnoldi_host = nullptr;
-    double *arnoldi_host_dev = nullptr;
-    hipEvent_t arnoldi_events[2] = {};
 };
 constexpr int kArnoldiHostSlots = 2 * 1024;
 Runtime &runtime();
@@ -232,6 +226,13 @@ struct EquationsSet {
     DevBuf<double> precond;     // preconditioner storage
     DevBuf<double> partials;    // reduction partial sums
     DevBuf<double> arnoldi;     // Hessenberg factor, rotations and coefficients of the Arnoldi process
+    // Krylov progress read back without a stream synchronisation: pinned,
+    // mapped, coherent host memory the Arnoldi kernels write (kArnoldiHostSlots
+    // doubles, host and device addresses), and two timing-free events marking
+    // alternate iterations; made at the first solve, per set
+    double *arnoldi_host = nullptr;
+    double *arnoldi_host_dev = nullptr;
+    hipEvent_t arnoldi_events[2] = {};
     // Mooney-Rivlin sets: 1 when the displacement rows c*nq + i (c < 3) of
     // every node i hold three blocks of the same sorted neighbour nodes
     // (components 0, 1, 2) and then one common list of pressure columns, so
@@ -254,6 +255,9 @@ struct EquationsSet {
     ~EquationsSet()
     {
         if (step_graph) (void)hipGraphExecDestroy(step_graph);
+        for (hipEvent_t e : arnoldi_events)
+            if (e) (void)hipEventDestroy(e);
+        if (arnoldi_host) (void)hipHostFree(arnoldi_host);
         delete mg;
     }
 };

@@ -841,7 +841,7 @@ struct Krylov {
         arnoldi_correct_kernel<<<1, 256, 0, rt.stream>>>(j, s.partials.ptr, ar);
         combine_device_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, j + 1, V, n, ar.h2, -1.0, vn, ar.flag);
         multi_dot_kernel<<<dim3(kReduceBlocks, 1), 256, 0, rt.stream>>>(n, vn, n, vn, s.partials.ptr, ar.flag);
-        arnoldi_finalize_kernel<<<1, 64, 0, rt.stream>>>(j, m, s.partials.ptr, ar, rt.arnoldi_host_dev);
+        arnoldi_finalize_kernel<<<1, 64, 0, rt.stream>>>(j, m, s.partials.ptr, ar, s.arnoldi_host_dev);
         arnoldi_scale_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, ar.inv, vn);
         OC_HIP_CHECK(hipGetLastError());
         return OC_NO_ERROR;
@@ -862,7 +862,7 @@ struct Krylov {
         OC_CHECK(norm(b, &beta));
         double res = beta;
         std::vector<double> R(static_cast<size_t>(m) * m), g(m + 1), y(m);
-        const volatile double *slots = rt.arnoldi_host;
+        const volatile double *slots = s.arnoldi_host;
         bool first = true;
         while (total < max_iter && res > tol_abs) {
             // r = b - W A x
@@ -883,11 +883,11 @@ struct Krylov {
             int cols = 0;
             for (int j = 0; j < steps && cols == 0; ++j) {
                 OC_CHECK(arnoldi_step(j));
-                OC_HIP_CHECK(hipEventRecord(rt.arnoldi_events[j & 1], rt.stream));
+                OC_HIP_CHECK(hipEventRecord(s.arnoldi_events[j & 1], rt.stream));
                 // read step j-1; the cycle's last step is read as soon as it is queued
                 for (int k = j - 1; k <= j && cols == 0; ++k) {
                     if (k < 0 || (k == j && j != steps - 1)) continue;
-                    OC_HIP_CHECK(hipEventSynchronize(rt.arnoldi_events[k & 1]));
+                    OC_HIP_CHECK(hipEventSynchronize(s.arnoldi_events[k & 1]));
                     const double rk = slots[2 * k], hk = slots[2 * k + 1];
                     res = rk;
                     if (rk <= tol_abs || hk == 0.0 || k == steps - 1) cols = k + 1;
@@ -940,6 +940,18 @@ static int solver_prepare(EquationsSet &s, int restart, Krylov &K)
         s.krylov_capacity = restart;
     }
     if (s.arnoldi.n < Arnoldi::size(restart)) OC_CHECK(s.arnoldi.alloc(Arnoldi::size(restart)));
+    if (!s.arnoldi_host) {
+        for (auto &e : s.arnoldi_events)
+            if (!e) OC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        OC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&s.arnoldi_host), sizeof(double) * kArnoldiHostSlots,
+                                   hipHostMallocMapped | hipHostMallocCoherent));
+        const hipError_t e = hipHostGetDevicePointer(reinterpret_cast<void **>(&s.arnoldi_host_dev), s.arnoldi_host, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(s.arnoldi_host);
+            s.arnoldi_host = nullptr;
+            return set_error(OC_ERR_HIP, "hipHostGetDevicePointer failed: %s", hipGetErrorString(e));
+        }
+    }
     if (!s.work.ptr) OC_CHECK(s.work.alloc(static_cast<size_t>(8) * n));
     if (!s.precond.ptr) OC_CHECK(s.precond.alloc(static_cast<size_t>(9) * s.n_nodes + 3 * static_cast<size_t>(n)));
     const size_t npart = std::max<size_t>(static_cast<size_t>(kReduceBlocks) * (1024 + 1),
