@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256)
 q1_galerkin_element_kernel(int n_elements, const Real *__restrict__ kbuf, const int *__restrict__ conn_q, int nq,
                            const uint8_t *__restrict__ free_mask, double *__restrict__ ke1)
 {
-    __shared__ double sK[81][82];
+    __shared__ Real sK[81][82];  // K_e in its own precision: an FP32 Jacobian halves the block's LDS
     __shared__ double sT[81][25];
     __shared__ double sW[27][8];
     __shared__ uint8_t sf[81];
@@ -73,13 +73,13 @@ q1_galerkin_element_kernel(int n_elements, const Real *__restrict__ kbuf, const 
         const Real *Ke = kbuf + static_cast<int64_t>(e) * 89 * 89;
         for (int k = t; k < 81 * 81; k += 256) {
             const int i = k / 81, j = k % 81;
-            sK[i][j] = (sf[i] && sf[j]) ? static_cast<double>(Ke[i * 89 + j]) : 0.0;
+            sK[i][j] = (sf[i] && sf[j]) ? Ke[i * 89 + j] : static_cast<Real>(0);
         }
         __syncthreads();
         for (int k = t; k < 81 * 24; k += 256) {
             const int i = k / 24, col = k % 24, cp = col / 8, kk = col % 8;
             double acc = 0.0;
-            for (int a = 0; a < 27; ++a) acc += sK[i][cp * 27 + a] * sW[a][kk];
+            for (int a = 0; a < 27; ++a) acc += static_cast<double>(sK[i][cp * 27 + a]) * sW[a][kk];
             sT[i][col] = acc;
         }
         __syncthreads();
@@ -754,7 +754,7 @@ int mg_setup(EquationsSet &s, int precision)
     Multigrid &g = *s.mg;
     Runtime &rt = runtime();
     const int M = s.n_meshes;
-    const unsigned eb = static_cast<unsigned>(std::min(s.n_elements, rt.num_cus * 2));
+    const unsigned eb = static_cast<unsigned>(std::min(s.n_elements, rt.num_cus * (precision == OC_FP32 ? 3 : 2)));
     if (precision == OC_FP64)
         q1_galerkin_element_kernel<double><<<eb, 256, 0, rt.stream>>>(s.n_elements, s.kbuf64.ptr, s.conn.ptr,
                                                                      s.n_nodes, s.free_mask.ptr, g.ke1.ptr);
