@@ -2,20 +2,21 @@
 """Benchmark of the MI355X finite element assembly path.
 
 Workload (one "step"): Mooney-Rivlin residual + FP64 analytic Jacobian
-assembly of the 16x16x16 triquadratic/trilinear unit-cube mesh (4,096
-elements, 112,724 DOFs, 22,626,496 Jacobian nonzeros), interior nodes
+assembly of the 32x32x32 triquadratic/trilinear unit-cube mesh (32,768
+elements, 859,812 DOFs, 178,723,696 Jacobian nonzeros), interior nodes
 jittered by U[-0.1h, 0.1h], c10 = 2, c01 = 6, at a deformed state (10%
 uniaxial stretch plus a seeded perturbation).  Inputs are resident in HBM
 before the timed region.  Prints one JSON line.
 
-Also reported (``secondary``): Laplace assembly throughput at 8^3 / 64^3 /
-128^3 trilinear, the FP32-Jacobian variant, time to a 1e-10 relative
-residual per load step (plain FP64 Newton versus learned warm start + FP32
-Jacobian), and the batched workload (256 meshes of uneven size assembled in
-one launch per kernel).  ``cpu_baseline`` times the
-C oracle (an FP64 restatement of the reference element loop with the
-reference's per-entry CSR bisection) on this host, single thread, on a
-bounded sample of the same workload.
+Also reported (``secondary``): the FP32-Jacobian variant; time to a 1e-10
+relative residual per load step on the 32^3 case (20% stretch in 10 load
+steps) for plain and learned starts with FP64 and FP32 Jacobians, and the
+same on the 16^3 case (10% in 10 steps); the solver-matrix assembly; Laplace
+assembly throughput at 8^3 / 64^3 / 128^3 trilinear; and the batched
+workload (256 meshes of uneven size assembled in one launch per kernel).
+``cpu_baseline`` times the C oracle (an FP64 restatement of the reference
+element loop with the reference's per-entry CSR bisection) on this host,
+single thread, on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -40,9 +41,6 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 #   + pressure rows (8*81*27 MAC + 216 mul) 35,208            = 667,251
 # (the reference element loop does about 0.68 MFLOP per element).
 MR_JACOBIAN_FLOP_PER_ELEMENT = 667251
-# dense per-element G^T C G count, 27*(2*81*9*9 + 2*81*81*9) + 27*2*81*8, of a
-# dense implementation of the same Jacobian (for comparison only)
-MR_JACOBIAN_DENSE_FLOP_PER_ELEMENT = 3577932
 # MFMA work issued per element: FP64 1659 mfma_f64_4x4x4 (512 FLOP each),
 # FP32 483 mfma_f32_16x16x4 (2048 FLOP each), padding included
 MR_JACOBIAN_ISSUED_FLOP_F64 = 1659 * 512
@@ -65,14 +63,16 @@ def laplace_bytes(n_elements, n_nodes, nnz):
     return 32 * n_elements + 24 * n_nodes + 8 * nnz + 8 * n_nodes
 
 
-def traffic_from_profile(kernel):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
-    WRITE_SIZE, scaled by a copy-kernel calibration); None if not measured."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.json")
+def traffic_from_profile(kernel, workload):
+    """(HBM bytes per launch, source file): measured with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) in a
+    separate profiling run and kept under profiles/; (None, None) if that
+    kernel and workload have not been measured."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        return json.load(f).get("bytes_per_launch"), os.path.relpath(path, ROOT)
 
 
 def time_steps(fn, steps, warmup, cm, only=None):
@@ -145,29 +145,38 @@ def solver_matrix_line(cm, eq, steps):
     }
 
 
-def newton_line(cm, n, load_steps):
-    """Time to 1e-10 relative residual per load step, plain FP64 Newton versus
-    learned warm start + FP32 Jacobian, 10% uniaxial stretch over `load_steps`."""
+def newton_line(cm, n, stretch, load_steps, krylov_rtol):
+    """Time to 1e-10 x the plain start's residual per load step (uniaxial
+    stretch in `load_steps` steps): plain start (previous solution + boundary
+    increment) and learned warm start, each with FP64 and FP32 Jacobians, all
+    four with the same forcing rule and preconditioner."""
     from cm_amd.train_warm_start import DEFAULT_WEIGHTS, solve_steps
     from cm_amd.warm_start import WarmStartMLP
-    case = dict(n=n, c10=2.0, c01=6.0, stretch=0.10, seed=2024)
-    plain = solve_steps(case, load_steps)  # the reference's forcing: GMRES to 1e-5 relative per Newton step
-    plain_ew = solve_steps(case, load_steps, krylov_rtol=-1.0)  # plain start with the learned form's forcing
+    case = dict(n=n, c10=2.0, c01=6.0, stretch=stretch, seed=2024)
     net = WarmStartMLP.load(DEFAULT_WEIGHTS)
-    learned = solve_steps(case, load_steps, net=net, precision=cm.OC_FP32)
+    runs = {
+        "plain_fp64": solve_steps(case, load_steps, krylov_rtol=krylov_rtol),
+        "plain_fp32": solve_steps(case, load_steps, precision=cm.OC_FP32, krylov_rtol=krylov_rtol),
+        "learned_fp64": solve_steps(case, load_steps, net=net, krylov_rtol=krylov_rtol),
+        "learned_fp32": solve_steps(case, load_steps, net=net, precision=cm.OC_FP32, krylov_rtol=krylov_rtol),
+    }
     net.destroy()
 
     def summary(rows):
         return {"converged_steps": sum(r["converged"] for r in rows), "steps": len(rows),
                 "seconds_per_step": float(np.mean([r["seconds"] for r in rows])),
+                "seconds": [round(r["seconds"], 4) for r in rows],
                 "newton_iterations": [r["newton"] for r in rows], "krylov_iterations": [r["krylov"] for r in rows],
-                "start_residual_over_plain_start": [r["start_norm"] / r["reference_norm"] for r in rows]}
-    p, pe, l = summary(plain), summary(plain_ew), summary(learned)
-    return {"workload": f"{n}^3 Mooney-Rivlin, c10=2, c01=6, 10% stretch in {load_steps} load steps, "
-                        "1e-10 x plain start residual, GMRES with block-triangular multigrid preconditioner",
-            "plain_fp64": p, "plain_fp64_eisenstat_walker": pe, "learned_fp32": l,
-            "speedup": p["seconds_per_step"] / l["seconds_per_step"],
-            "speedup_vs_plain_eisenstat_walker": pe["seconds_per_step"] / l["seconds_per_step"]}
+                "start_residual_over_plain_start": [r["start_norm"] / r["reference_norm"] for r in rows],
+                "final_residual_over_plain_start": [r["final_norm"] / r["reference_norm"] for r in rows]}
+    out = {"workload": f"{n}^3 Mooney-Rivlin, c10=2, c01=6, {round(100 * stretch)}% stretch in {load_steps} load "
+                       "steps, each step to 1e-10 x the plain start's FP64 residual; GMRES(100), block-triangular "
+                       "multigrid preconditioner, "
+                       + ("Eisenstat-Walker forcing" if krylov_rtol <= 0 else f"forcing rtol {krylov_rtol:g}")}
+    out.update({k: summary(v) for k, v in runs.items()})
+    base = out["plain_fp64"]["seconds_per_step"]
+    out["speedup_vs_plain_fp64"] = {k: base / out[k]["seconds_per_step"] for k in runs}
+    return out
 
 
 def batched_line(cm, n_meshes, reps=3):
@@ -256,22 +265,59 @@ def cpu_baseline(mesh, dofs, c10, c01, seconds):
     }
 
 
+def progress(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_roofline(name, kind, per_launch, avg_launch_s, traffic_key, workload_key):
+    """Roofline object of one kernel from its event-timed average launch."""
+    traffic, source = traffic_from_profile(traffic_key, workload_key)
+    if kind == "mfma64" or kind == "mfma32":
+        peak = FP64_PEAK_TFLOPS if kind == "mfma64" else FP32_MFMA_PEAK_TFLOPS
+        achieved = MR_JACOBIAN_FLOP_PER_ELEMENT * per_launch / avg_launch_s / 1e12
+        issued = (MR_JACOBIAN_ISSUED_FLOP_F64 if kind == "mfma64" else MR_JACOBIAN_ISSUED_FLOP_F32) * per_launch
+        out = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+               "frac": achieved / peak, "traffic": traffic,
+               "algorithmic_count": f"{MR_JACOBIAN_FLOP_PER_ELEMENT} FLOP per element x {per_launch} elements",
+               "issued_mfma_TFLOPs": issued / avg_launch_s / 1e12,
+               "issued_mfma_frac": issued / avg_launch_s / 1e12 / peak}
+    else:  # hbm: per_launch is the algorithmic bytes of the launch
+        achieved = per_launch / avg_launch_s / 1e9
+        out = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_count": f"{per_launch} bytes"}
+    out["avg_launch_ms"] = avg_launch_s * 1e3
+    out["traffic_source"] = (f"{source}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of a separate profiling run, "
+                             "scaled by a copy-kernel calibration") if source else None
+    return out
+
+
+def gather_bytes(eq, E, value_bytes):
+    """Algorithmic bytes of one CSR row gather: the element matrices read
+    (E*89*89 values), the neighbour-rank map (E*35*35 bytes) and the CSR values
+    written once."""
+    return E * 89 * 89 * value_bytes + E * 35 * 35 + eq.nnz * value_bytes
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=16, help="elements per direction of the MR workload")
+    ap.add_argument("--n", type=int, default=32, help="elements per direction of the MR workload")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--load-steps", type=int, default=10)
+    ap.add_argument("--newton-rtol", type=float, default=1e-5,
+                    help="GMRES forcing of every Newton step in the load-step comparison (<= 0: Eisenstat-Walker)")
     ap.add_argument("--batched-meshes", type=int, default=256, help="0 skips the batched-meshes line")
-    ap.add_argument("--profile", choices=["jacobian", "jacobian32", "laplace", "copy"], default=None,
-                    help="run only one kernel sequence for rocprofv3 counter collection: the FP64 (or FP32) "
-                         "Jacobian assembly of the MR workload, the 128^3 Laplace assembly (--steps times), or a "
-                         "device copy of 512 MiB (the FETCH_SIZE/WRITE_SIZE calibration)")
+    ap.add_argument("--profile", choices=["jacobian", "jacobian32", "laplace", "copy", "newton"], default=None,
+                    help="run only one kernel sequence for rocprofv3: the FP64 (or FP32) Jacobian assembly of the "
+                         "MR workload (--steps times), the 128^3 Laplace assembly, a device copy of 512 MiB (the "
+                         "FETCH_SIZE/WRITE_SIZE calibration), or --steps load steps of the --n case (20%% in 10 "
+                         "steps) with --profile-start learned or plain")
+    ap.add_argument("--profile-start", choices=["learned", "plain"], default="learned")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
@@ -300,6 +346,15 @@ def main():
         print(json.dumps({"profile": "laplace", "steps": args.steps, "elements": mesh.n_elements, "nnz": eq.nnz,
                           "algorithmic_bytes": laplace_bytes(mesh.n_elements, mesh.n_nodes, eq.nnz)}))
         return
+    if args.profile == "newton":
+        from cm_amd.train_warm_start import DEFAULT_WEIGHTS, solve_steps
+        from cm_amd.warm_start import WarmStartMLP
+        case = dict(n=args.n, c10=2.0, c01=6.0, stretch=0.20, seed=2024)
+        net = WarmStartMLP.load(DEFAULT_WEIGHTS) if args.profile_start == "learned" else None
+        rows = solve_steps(case, args.load_steps, net=net, precision=cm.OC_FP32 if net else cm.OC_FP64,
+                           max_steps=args.steps, krylov_rtol=args.newton_rtol, log=progress)
+        print(json.dumps({"profile": "newton", "start": args.profile_start, "n": args.n, "steps": rows}))
+        return
     c10, c01 = 2.0, 6.0
     mesh = cm.RegularMesh.generate(2, args.n, jitter=0.1, seed=2024)
     eq = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
@@ -318,65 +373,76 @@ def main():
         print(json.dumps({"profile": args.profile, "steps": args.steps, "elements": mesh.n_elements}))
         return
 
-    # timed region: only the element-matrix kernel carries events (its launch
-    # durations give the roofline figure); the per-kernel breakdown comes from
-    # a second pass with every kernel timed
-    ms, wall = time_steps(step, args.steps, args.warmup, cm, only="mr_element_matrix_fp64")
-    jac_ms, jac_launches = cm.kernel_timing("mr_element_matrix_fp64")
+    E = mesh.n_elements
+    wl = f"mr{args.n}"
+    # untimed pass with every kernel timed: which kernel dominates a step
+    progress(f"MR {args.n}^3: {E} elements, {eq.n_rows} DOFs, {eq.nnz} nonzeros")
+    time_steps(step, 2, args.warmup, cm)
+    pre = {k: cm.kernel_timing(k)[0] for k in ("mr_element_matrix_fp64", "csr_row_gather_fp64")}
+    cm.kernel_timing_enable(False)
+    dominant = max(pre, key=pre.get)
+    # timed region: only the dominant kernel carries events (one pair per launch);
+    # the per-kernel breakdown comes from a second pass with every kernel timed
+    ms, wall = time_steps(step, args.steps, 0, cm, only=dominant)
+    dom_ms, dom_launches = cm.kernel_timing(dominant)
     cm.kernel_timing_enable(False)
     time_steps(step, args.steps, 0, cm)
     kernels = {name: cm.kernel_timing(name) for name in
                ("mr_gauss_jacobian", "mr_element_matrix_fp64", "csr_row_gather_fp64", "mr_residual", "dof_gather")}
     cm.kernel_timing_enable(False)
-    E = mesh.n_elements
     value = E * args.steps / (ms / 1e3)
-    per_launch_elems = E * args.steps / jac_launches
-    avg_launch_s = jac_ms / jac_launches / 1e3
-    achieved = MR_JACOBIAN_FLOP_PER_ELEMENT * per_launch_elems / avg_launch_s / 1e12
-    issued = MR_JACOBIAN_ISSUED_FLOP_F64 * per_launch_elems / avg_launch_s / 1e12
-    roofline = {
-        "kernel": "mr_element_matrix_f64_kernel",
-        "bound": "mfma",
-        "achieved": achieved,
-        "peak": FP64_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": achieved / FP64_PEAK_TFLOPS,
-        "traffic": traffic_from_profile("mr_element_matrix_fp64"),
-        "avg_launch_ms": avg_launch_s * 1e3,
-        "elements_per_launch": per_launch_elems,
-        "flop_per_element": MR_JACOBIAN_FLOP_PER_ELEMENT,
-        "issued_mfma_TFLOPs": issued,
-        "issued_mfma_frac": issued / FP64_PEAK_TFLOPS,
-        "dense_equivalent_TFLOPs": MR_JACOBIAN_DENSE_FLOP_PER_ELEMENT * per_launch_elems / avg_launch_s / 1e12,
-    }
-
+    em, el = kernels["mr_element_matrix_fp64"]
+    gm, gl = kernels["csr_row_gather_fp64"]
+    rl_elem = kernel_roofline("mr_element_matrix_f64_kernel", "mfma64", E,
+                              (dom_ms / dom_launches if dominant == "mr_element_matrix_fp64" else em / el) / 1e3,
+                              "mr_element_matrix_fp64", wl)
+    rl_gather = kernel_roofline("csr_row_gather_kernel<double>", "hbm", gather_bytes(eq, E, 8),
+                                (dom_ms / dom_launches if dominant == "csr_row_gather_fp64" else gm / gl) / 1e3,
+                                "csr_row_gather_fp64", wl)
+    roofline = rl_elem if dominant == "mr_element_matrix_fp64" else rl_gather
+    other = rl_gather if dominant == "mr_element_matrix_fp64" else rl_elem
     secondary = {
         "kernel_ms_per_step": {name: v[0] / args.steps for name, v in kernels.items()},
         "kernel_launches_per_step": {name: v[1] / args.steps for name, v in kernels.items()},
         "wall_ms_per_step": wall * 1e3 / args.steps,
+        "roofline_other_assembly_kernel": other,
     }
+    progress(f"assembly: {value:.4g} elements/s, {ms / args.steps:.3f} ms per step; dominant {dominant}")
     if not args.no_secondary:
         eq.jacobian_evaluate(cm.OC_FP32)
         ms32, _ = time_steps(lambda: eq.jacobian_evaluate(cm.OC_FP32), args.steps, 1, cm)
         j32, l32 = cm.kernel_timing("mr_element_matrix_fp32")
-        g32, _ = cm.kernel_timing("csr_row_gather_fp32")
+        g32, k32 = cm.kernel_timing("csr_row_gather_fp32")
         cm.kernel_timing_enable(False)
         v64, v32 = eq.matrix_values(), eq.matrix_values_sp().astype(np.float64)
         secondary["mr_jacobian_fp32"] = {
             "error_vs_fp64_rel_frobenius": float(np.linalg.norm(v32 - v64) / np.linalg.norm(v64)),
             "elements_per_s": E * args.steps / (ms32 / 1e3),
-            # E elements per launch, l32 launches in j32 ms
-            "achieved_TFLOPs": MR_JACOBIAN_FLOP_PER_ELEMENT * E * l32 / (j32 / 1e3) / 1e12,
-            "issued_mfma_TFLOPs": MR_JACOBIAN_ISSUED_FLOP_F32 * E * l32 / (j32 / 1e3) / 1e12,
-            "peak_TFLOPs": FP32_MFMA_PEAK_TFLOPS,
+            "element_matrix": kernel_roofline("mr_element_matrix_kernel<float>", "mfma32", E, j32 / l32 / 1e3,
+                                              "mr_element_matrix_fp32", wl),
+            "row_gather": kernel_roofline("csr_row_gather_kernel<float>", "hbm", gather_bytes(eq, E, 4),
+                                          g32 / k32 / 1e3, "csr_row_gather_fp32", wl),
             "kernel_ms_per_step": {"mr_element_matrix_fp32": j32 / args.steps, "csr_row_gather_fp32": g32 / args.steps},
         }
-        secondary["solver_matrix"] = solver_matrix_line(cm, eq, 20)
-        secondary["laplace"] = [laplace_line(cm, n, s) for n, s in ((8, 50), (64, 20), (128, 10))]
+        del v64, v32
+        progress("FP32 Jacobian done")
     if not args.no_newton:
-        secondary["newton_per_load_step"] = newton_line(cm, args.n, args.load_steps)
+        progress(f"load steps, {args.n}^3, 20% in {args.load_steps} steps")
+        secondary["newton_per_load_step"] = newton_line(cm, args.n, 0.20, args.load_steps, args.newton_rtol)
+        if not args.no_secondary:
+            progress("load steps, 16^3, 10% in 10 steps")
+            secondary["newton_per_load_step_16"] = newton_line(cm, 16, 0.10, 10, args.newton_rtol)
+    if not args.no_secondary:
+        m16 = cm.RegularMesh.generate(2, 16, jitter=0.1, seed=2024)
+        e16 = cm.MooneyRivlinEquationsSet(m16, c10, c01)
+        e16.set_dofs(deformed_state(e16, m16))
+        secondary["solver_matrix"] = solver_matrix_line(cm, e16, 20)
+        e16.destroy()
+        secondary["laplace"] = [laplace_line(cm, n, s) for n, s in ((8, 50), (64, 20), (128, 10))]
+        progress("solver matrix and Laplace done")
     if args.batched_meshes > 0 and not args.no_secondary:
         secondary["batched"] = batched_line(cm, args.batched_meshes)
+        progress("batched meshes done")
 
     line = {
         "metric": "elements/s assembled; time-to-1e-10-residual per load step, plain vs learned",
@@ -399,6 +465,7 @@ def main():
         "secondary": secondary,
     }
     if not args.no_cpu_baseline:
+        progress("CPU baseline")
         line["cpu_baseline"] = cpu_baseline(mesh, dofs, c10, c01, args.cpu_seconds)
         line["cpu_baseline"]["host_cpus"] = os.cpu_count()
     print(json.dumps(line))

@@ -60,8 +60,12 @@ OC_CONVERGED_FNORM_ABS = 2
 OC_CONVERGED_FNORM_RELATIVE = 3
 OC_CONVERGED_SNORM_RELATIVE = 4
 OC_DIVERGED_FUNCTION_COUNT = -2
+OC_DIVERGED_LINEAR_SOLVE = -3
 OC_DIVERGED_FNORM_NAN = -4
 OC_DIVERGED_MAX_IT = -5
+OC_DIVERGED_LINE_SEARCH = -6
+OC_SOLVER_NO_OUTPUT = 0
+OC_SOLVER_PROGRESS_OUTPUT = 1
 
 
 class NewtonOptions(ctypes.Structure):
@@ -80,6 +84,11 @@ class NewtonOptions(ctypes.Structure):
         ("preconditioner", ctypes.c_int),
         ("solution_tolerance", ctypes.c_double),
         ("maximum_function_evaluations", ctypes.c_int),
+        ("maximum_linear_solve_failures", ctypes.c_int),
+        ("maximum_step_failures", ctypes.c_int),
+        ("output_type", ctypes.c_int),
+        ("fallback", ctypes.c_int),
+        ("fallback_maximum_iterations", ctypes.c_int),
     ]
 
 
@@ -95,6 +104,10 @@ class NewtonStats(ctypes.Structure):
         ("final_residual_norm", ctypes.c_double),
         ("seconds", ctypes.c_double),
         ("reason", ctypes.c_int),
+        ("linear_solve_failures", ctypes.c_int),
+        ("step_failures", ctypes.c_int),
+        ("fallback_used", ctypes.c_int),
+        ("warm_start_reason", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -164,6 +177,8 @@ SIGNATURES = {
     "oc_mooney_rivlin_batch_create": [_c_int, _i32, _i32, _i32, _dp, _i32, _i32, _dp, _dp, _c_int,
                                       ctypes.POINTER(_H)],
     "oc_mesh_count": [_H, ctypes.POINTER(_c_int)],
+    "oc_mesh_matrix_get": [_H, _c_int, _c_int, _c_int, ctypes.POINTER(_c_int), ctypes.POINTER(ctypes.c_longlong),
+                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "oc_equations_set_mesh_get": [_H, _c_int] + [ctypes.POINTER(_c_int)] * 6,
     "oc_mesh_materials_set": [_H, _dp, _dp],
     "oc_newton_solve_batch": [_H, ctypes.POINTER(NewtonOptions), ctypes.c_void_p, ctypes.c_void_p,

@@ -318,6 +318,7 @@ int oc_mooney_rivlin_batch_create(int n_meshes, const int *n_nodes_q, const int 
     if (rc == OC_NO_ERROR) rc = s.mat.upload(mat.data(), mat.size());
     if (rc == OC_NO_ERROR) rc = s.q_off.upload(qo.data(), qo.size());
     if (rc == OC_NO_ERROR) rc = s.l_off.upload(lo.data(), lo.size());
+    if (rc == OC_NO_ERROR) rc = mr_prepare(s);  // scatter map and gather lists
     if (rc != OC_NO_ERROR) {
         delete h;
         return rc;
@@ -340,6 +341,57 @@ int oc_equations_set_mesh_get(oc_equations_set h, int mesh, int *node_offset_q, 
     if (n_nodes_l) *n_nodes_l = s.h_l_off[mesh + 1] - s.h_l_off[mesh];
     if (element_offset) *element_offset = s.h_e_off[mesh];
     if (n_elements) *n_elements = s.h_e_off[mesh + 1] - s.h_e_off[mesh];
+    return OC_NO_ERROR;
+}
+
+int oc_mesh_matrix_get(oc_equations_set h, int mesh, int precision, int index_base, int *n_rows, long long *nnz,
+                       int *row_ptr, int *col_idx, void *values)
+{
+    OC_CHECK(check_handle(h));
+    const EquationsSet &s = h->s;
+    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN || mesh < 0 || mesh >= s.n_meshes)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "mesh %d of a set with %d Mooney-Rivlin meshes", mesh,
+                         s.cls == OC_EQUATIONS_SET_MOONEY_RIVLIN ? s.n_meshes : 0);
+    if ((precision != OC_FP64 && precision != OC_FP32) || (index_base != 0 && index_base != 1))
+        return set_error(OC_ERR_INVALID_ARGUMENT, "oc_mesh_matrix_get: precision or index_base");
+    const int NQ = s.n_nodes;
+    const int q0 = s.h_q_off[mesh], nq = s.h_q_off[mesh + 1] - q0;
+    const int l0 = s.h_l_off[mesh], nl = s.h_l_off[mesh + 1] - l0;
+    // the mesh's rows: four contiguous ranges of union rows (z1, z2, z3, p)
+    const int first[4] = {q0, NQ + q0, 2 * NQ + q0, 3 * NQ + l0}, count[4] = {nq, nq, nq, nl};
+    int64_t total = 0;
+    for (int c = 0; c < 4; ++c) total += s.h_row_ptr[first[c] + count[c]] - s.h_row_ptr[first[c]];
+    if (n_rows) *n_rows = 3 * nq + nl;
+    if (nnz) *nnz = total;
+    if ((row_ptr || col_idx) && total + index_base > INT32_MAX)
+        return set_error(OC_ERR_UNSUPPORTED, "mesh %d has %lld nonzeros, beyond the 4-byte indices", mesh,
+                         static_cast<long long>(total));
+    const void *dev = precision == OC_FP64 ? static_cast<const void *>(s.values.ptr)
+                                           : static_cast<const void *>(s.values_sp.ptr);
+    if (values && !dev) return set_error(OC_ERR_INVALID_ARGUMENT, "no Jacobian of that precision assembled");
+    const size_t vsize = precision == OC_FP64 ? sizeof(double) : sizeof(float);
+    if (values) OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+    int64_t pos = 0;
+    int row = 0;
+    for (int c = 0; c < 4; ++c) {
+        const int64_t k0 = s.h_row_ptr[first[c]], k1 = s.h_row_ptr[first[c] + count[c]];
+        if (row_ptr)
+            for (int i = 0; i < count[c]; ++i)
+                row_ptr[row + i] = static_cast<int>(pos + s.h_row_ptr[first[c] + i] - k0 + index_base);
+        if (col_idx)  // union column -> mesh-local column (the block is diagonal: every column is the mesh's own)
+            for (int64_t k = k0; k < k1; ++k) {
+                const int col = s.h_col_idx[k];
+                const int local = col < 3 * NQ ? (col / NQ) * nq + (col % NQ - q0) : 3 * nq + (col - 3 * NQ - l0);
+                col_idx[pos + k - k0] = local + index_base;
+            }
+        if (values && k1 > k0)
+            OC_HIP_CHECK(hipMemcpy(static_cast<char *>(values) + pos * vsize,
+                                   static_cast<const char *>(dev) + k0 * vsize, (k1 - k0) * vsize,
+                                   hipMemcpyDeviceToHost));
+        pos += k1 - k0;
+        row += count[c];
+    }
+    if (row_ptr) row_ptr[row] = static_cast<int>(pos + index_base);
     return OC_NO_ERROR;
 }
 
@@ -419,6 +471,7 @@ int oc_dofs_set(oc_equations_set h, const double *dofs)
 {
     OC_CHECK(check_handle(h));
     if (!dofs) return set_error(OC_ERR_INVALID_ARGUMENT, "dofs is NULL");
+    h->s.plain_start_valid = false;
     OC_HIP_CHECK(hipMemcpyAsync(h->s.dofs.ptr, dofs, sizeof(double) * h->s.n_rows, hipMemcpyHostToDevice,
                                 g_runtime.stream));
     OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
@@ -429,6 +482,7 @@ int oc_dofs_set_device(oc_equations_set h, const double *dofs_device)
 {
     OC_CHECK(check_handle(h));
     if (!dofs_device) return set_error(OC_ERR_INVALID_ARGUMENT, "dofs_device is NULL");
+    h->s.plain_start_valid = false;
     OC_HIP_CHECK(hipMemcpyAsync(h->s.dofs.ptr, dofs_device, sizeof(double) * h->s.n_rows, hipMemcpyDeviceToDevice,
                                 g_runtime.stream));
     return OC_NO_ERROR;

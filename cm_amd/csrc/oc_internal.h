@@ -246,9 +246,17 @@ struct EquationsSet {
     int tuned_spmv_lanes = 0;
     int tuned_gather_chunk = 0;
 
-    // Newton-step graph: Jacobian assembly + preconditioner setup
+    // Newton-step graph: Jacobian assembly + preconditioner setup, recorded for
+    // {precision, preconditioner, element blocks per CU, gather chunk,
+    // boundary-condition version}
     hipGraphExec_t step_graph = nullptr;
-    int step_graph_key[4] = {-1, -1, -1, -1};
+    long long step_graph_key[5] = {-1, -1, -1, -1, -1};
+
+    // The plain start of a warm-started load step: the DOFs before
+    // oc_mlp_warm_start, advanced by the same load increments; the Newton
+    // fallback restarts from it (valid until the next solve or DOF upload)
+    DevBuf<double> plain_start;
+    bool plain_start_valid = false;
 
     SolverSystem solver;      // oc_solver_matrices_assemble
     Multigrid *mg = nullptr;  // set by oc_multigrid_structure_set

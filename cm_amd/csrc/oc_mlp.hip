@@ -765,6 +765,11 @@ int oc_mlp_warm_start_batch(oc_mlp mlp, oc_equations_set h, const double *load_i
     Runtime &rt = runtime();
     OC_HIP_CHECK(hipStreamSynchronize(rt.stream));  // the increments buffer may still be read by a previous call
     OC_CHECK(m.dload.upload(load_increments, static_cast<size_t>(M)));
+    // the plain start (for the Newton fallback) is the DOFs before the warm start
+    if (!s.plain_start.ptr) OC_CHECK(s.plain_start.alloc(static_cast<size_t>(s.n_rows)));
+    OC_HIP_CHECK(hipMemcpyAsync(s.plain_start.ptr, s.dofs.ptr, sizeof(double) * s.n_rows, hipMemcpyDeviceToDevice,
+                                rt.stream));
+    s.plain_start_valid = true;
     KernelTimer timer("mlp_warm_start");
     if (!s.node_mat.ptr) OC_CHECK(mr_node_materials(s));
     warm_features_kernel<<<blocks_for(nq), 256, 0, rt.stream>>>(nq, s.coords.ptr, s.dofs.ptr, M, s.q_off.ptr,

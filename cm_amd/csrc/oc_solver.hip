@@ -10,6 +10,7 @@
 // in every Krylov vector, which is the same eliminated system.
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <vector>
 
@@ -1045,8 +1046,11 @@ static int residual_mesh_norms(EquationsSet &s, std::vector<double> &out)
 static int step_assembly(EquationsSet &s, Krylov *K, int precision)
 {
     Runtime &rt = runtime();
-    const int key[4] = {precision, K ? K->precond : -1, s.tuned_element_blocks_per_cu, s.tuned_gather_chunk};
-    if (!s.step_graph || !std::equal(key, key + 4, s.step_graph_key)) {
+    // the preconditioner kernels read the free-DOF mask, so a graph belongs to
+    // one set of boundary conditions
+    const long long key[5] = {precision, K ? K->precond : -1, s.tuned_element_blocks_per_cu, s.tuned_gather_chunk,
+                              static_cast<long long>(s.bc_version)};
+    if (!s.step_graph || !std::equal(key, key + 5, s.step_graph_key)) {
         if (s.step_graph) {
             (void)hipGraphExecDestroy(s.step_graph);
             s.step_graph = nullptr;
@@ -1066,7 +1070,7 @@ static int step_assembly(EquationsSet &s, Krylov *K, int precision)
             s.step_graph = nullptr;
             return set_error(OC_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
         }
-        std::copy(key, key + 4, s.step_graph_key);
+        std::copy(key, key + 5, s.step_graph_key);
     }
     KernelTimer timer("step_graph");
     OC_HIP_CHECK(hipGraphLaunch(s.step_graph, rt.stream));
@@ -1089,8 +1093,8 @@ static double plain_norm(const std::vector<double> &nrm)
 
 // Inexact Newton with backtracking line search on every mesh of the set at
 // once; mesh m has converged when ||F_m|| <= max(atol, rtol * ref_m).
-static int newton_core(EquationsSet &s, const oc_newton_options *opt, const double *refs_in, double *finals_out,
-                       oc_newton_stats *stats)
+static int newton_core(EquationsSet &s, const oc_newton_options *opt, const double *refs_in,
+                       std::vector<double> &refs_out, std::vector<double> &finals_out, oc_newton_stats *stats)
 {
     if (opt->krylov_restart < 1 || opt->maximum_iterations < 0)
         return set_error(OC_ERR_INVALID_ARGUMENT, "invalid Newton options");
@@ -1200,15 +1204,38 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
         OC_HIP_CHECK(hipGetLastError());
         int kit = 0;
         double kres = 0.0;
-        OC_CHECK(K.gmres(rhs, dx, eta * fnorm, opt->krylov_maximum_iterations, &kit, &kres));
+        const double ktol = eta * fnorm;
+        OC_CHECK(K.gmres(rhs, dx, ktol, opt->krylov_maximum_iterations, &kit, &kres));
         stats->krylov_iterations += kit;
-        // line search on f = 1/2 ||W F||^2 with slope -||W F||^2 (Newton direction)
+        // a linear solve that ended above its tolerance (KSP_DIVERGED_ITS):
+        // PETSc's SNES stops with SNES_DIVERGED_LINEAR_SOLVE once
+        // maxLinearSolveFailures of them have happened, before the line search
+        if (!(kres <= ktol)) {
+            ++stats->linear_solve_failures;
+            if (stats->linear_solve_failures >= std::max(1, opt->maximum_linear_solve_failures)) {
+                reason = OC_DIVERGED_LINEAR_SOLVE;
+                if (opt->output_type >= OC_SOLVER_PROGRESS_OUTPUT)
+                    std::fprintf(stderr, "  newton %3d  ||WF|| %.6e  gmres %d its ended at %.3e > %.3e: diverged linear solve\n",
+                                 it, fnorm, kit, kres, ktol);
+                break;
+            }
+        }
+        // line search on f = 1/2 ||W F||^2; initial slope (W F) . (W J dx), as
+        // PETSc's bt line search takes it (made negative if it is not)
+        OC_CHECK(K.spmv(dx, K.u));
+        double fjdx = 0.0;
+        OC_CHECK(K.multi_dot(rhs, 1, K.u, &fjdx));  // rhs = -W F
+        double slope = -fjdx;
+        if (slope > 0.0) slope = -slope;
+        if (slope == 0.0) slope = -1.0;
         OC_HIP_CHECK(hipMemcpyAsync(x_prev, s.dofs.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
         double lambda = 1.0, lambda_taken = 1.0, fnew = 0.0;
-        const double f0 = 0.5 * fnorm * fnorm, slope = -fnorm * fnorm;
+        const double f0 = 0.5 * fnorm * fnorm;
         double lambda_prev = 0.0, f_prev_trial = 0.0;
         std::vector<double> trial;
-        for (int ls = 0; ls < 20; ++ls) {
+        bool decreased = !opt->line_search;
+        int ls = 0;
+        for (; ls < 20; ++ls) {
             OC_HIP_CHECK(hipMemcpyAsync(s.dofs.ptr, x_prev, sizeof(double) * n, hipMemcpyDeviceToDevice,
                                         rt.stream));
             axpby_kernel<<<grid_for(n), 256, 0, rt.stream>>>(n, lambda, dx, 1.0, s.dofs.ptr);
@@ -1217,8 +1244,11 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
             fnew = weighted_norm(trial, w);
             stats->residual_evaluations++;
             const double f1 = 0.5 * fnew * fnew;
-            if (!opt->line_search || (std::isfinite(f1) && f1 <= f0 + opt->line_search_alpha * lambda * slope))
+            if (!opt->line_search) break;
+            if (std::isfinite(f1) && f1 <= f0 + opt->line_search_alpha * lambda * slope) {
+                decreased = true;
                 break;
+            }
             // quadratic (first backtrack) or cubic interpolation, safeguarded to [0.1, 0.5] lambda
             double lnew;
             if (ls == 0 || !std::isfinite(f1)) {
@@ -1239,9 +1269,19 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
             lambda = std::min(0.5 * lambda, std::max(0.1 * lambda, lnew));
         }
         nrm = trial;
+        if (opt->output_type >= OC_SOLVER_PROGRESS_OUTPUT)
+            std::fprintf(stderr, "  newton %3d  ||WF|| %.6e -> %.6e  eta %.2e  gmres %4d its %.3e/%.3e  slope/f0 %.3f  lambda %.3g (%d trials)\n",
+                         it, fnorm, fnew, eta, kit, kres, ktol, slope / (2.0 * f0), lambda_taken, ls + (decreased ? 1 : 0));
         fnorm_prev = fnorm;
         fnorm = fnew;
         ++it;
+        if (!decreased) {  // backtracking ended without sufficient decrease; the last trial is kept
+            ++stats->step_failures;
+            if (stats->step_failures >= std::max(1, opt->maximum_step_failures)) {
+                reason = OC_DIVERGED_LINE_SEARCH;
+                break;
+            }
+        }
         double snorm = 0.0, xnorm = 0.0;
         if (opt->solution_tolerance > 0.0) {  // the step taken and the new iterate, free DOFs
             std::vector<double> part;
@@ -1259,8 +1299,11 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
     stats->reason = reason;
     stats->converged = reason > 0 ? 1 : 0;
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (finals_out)
-        for (int m = 0; m < M; ++m) finals_out[m] = nrm[m];
+    refs_out = ref;
+    finals_out = nrm;
+    if (opt->output_type >= OC_SOLVER_PROGRESS_OUTPUT)
+        std::fprintf(stderr, "  newton: reason %d after %d iterations, %d Krylov, ||F|| %.6e\n", reason, it,
+                     stats->krylov_iterations, stats->final_residual_norm);
     if (!stats->converged) {
         int worst = 0;
         double worst_ratio = -1.0;
@@ -1276,6 +1319,78 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
                          reason, worst, nrm[worst], target[worst]);
     }
     return OC_NO_ERROR;
+}
+
+// dst = src on the DOFs of the meshes with redo[m] set (quadratic nodes of
+// mesh m are [q_off[m], q_off[m+1]) in each of the three components, its
+// pressure nodes [l_off[m], l_off[m+1]))
+__global__ void __launch_bounds__(256) restore_meshes_kernel(int n_rows, int nq, int n_meshes,
+                                                            const int *__restrict__ q_off,
+                                                            const int *__restrict__ l_off,
+                                                            const uint8_t *__restrict__ redo,
+                                                            const double *__restrict__ src, double *__restrict__ dst)
+{
+    for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < n_rows; d += gridDim.x * blockDim.x) {
+        const bool disp = d < 3 * nq;
+        const int node = disp ? d % nq : d - 3 * nq;
+        const int *off = disp ? q_off : l_off;
+        int lo = 0, hi = n_meshes - 1;  // mesh with off[m] <= node < off[m + 1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= node) lo = mid;
+            else hi = mid - 1;
+        }
+        if (redo[lo]) dst[d] = src[d];
+    }
+}
+
+// Newton solve with the plain-start fallback: when the attempt fails and the
+// load step was warm-started, the meshes that did not converge restart from
+// the plain start with an FP64 Jacobian, judged against the same targets.
+static int newton_solve_set(EquationsSet &s, const oc_newton_options *opt, const double *refs_in, double *finals_out,
+                            oc_newton_stats *stats)
+{
+    const bool can_fall_back = opt->fallback != 0 && s.plain_start_valid;
+    std::vector<double> refs, finals;
+    int rc = newton_core(s, opt, refs_in, refs, finals, stats);
+    if (rc == OC_ERR_NOT_CONVERGED && can_fall_back) {
+        const int M = s.n_meshes;
+        std::vector<uint8_t> redo(M, 0);
+        for (int m = 0; m < M; ++m) {
+            const double target = std::max(opt->absolute_tolerance, opt->relative_tolerance * refs[m]);
+            redo[m] = (finals[m] <= target) ? 0 : 1;
+        }
+        DevBuf<uint8_t> redo_d;
+        OC_CHECK(redo_d.upload(redo.data(), redo.size()));
+        Runtime &rt = runtime();
+        restore_meshes_kernel<<<grid_for(s.n_rows), 256, 0, rt.stream>>>(
+            s.n_rows, s.n_nodes, M, s.q_off.ptr, s.l_off.ptr, redo_d.ptr, s.plain_start.ptr, s.dofs.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+        OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+        oc_newton_options plain = *opt;
+        plain.jacobian_precision = OC_FP64;
+        plain.fallback = 0;
+        if (opt->fallback_maximum_iterations > 0) plain.maximum_iterations = opt->fallback_maximum_iterations;
+        if (opt->output_type >= OC_SOLVER_PROGRESS_OUTPUT)
+            std::fprintf(stderr, "  newton: warm-started attempt failed (reason %d); plain FP64 restart\n", stats->reason);
+        oc_newton_stats first = *stats, second{};
+        rc = newton_core(s, &plain, refs.data(), refs, finals, &second);
+        *stats = second;
+        stats->newton_iterations += first.newton_iterations;
+        stats->krylov_iterations += first.krylov_iterations;
+        stats->residual_evaluations += first.residual_evaluations;
+        stats->jacobian_evaluations += first.jacobian_evaluations;
+        stats->linear_solve_failures += first.linear_solve_failures;
+        stats->step_failures += first.step_failures;
+        stats->initial_residual_norm = first.initial_residual_norm;
+        stats->seconds += first.seconds;
+        stats->fallback_used = 1;
+        stats->warm_start_reason = first.reason;
+    }
+    s.plain_start_valid = false;
+    if (finals_out && finals.size() == static_cast<size_t>(s.n_meshes))
+        std::copy(finals.begin(), finals.end(), finals_out);
+    return rc;
 }
 
 }  // namespace oc
@@ -1302,9 +1417,20 @@ int oc_boundary_conditions_set(oc_equations_set h, int n, const int *dof_indices
         mask[d] = 0;
         idx[i] = d;
     }
-    OC_CHECK(s.bc_idx.upload(idx.data(), idx.size()));
-    OC_CHECK(s.bc_full.upload(full_values, static_cast<size_t>(n)));
-    OC_CHECK(s.free_mask.upload(mask.data(), mask.size()));
+    // the free-DOF mask keeps its device address (a recorded Newton-step graph
+    // holds it; that graph is also re-recorded, its key has the BC version)
+    if (s.bc_idx.n != static_cast<size_t>(n)) {
+        OC_CHECK(s.bc_idx.alloc(static_cast<size_t>(n)));
+        OC_CHECK(s.bc_full.alloc(static_cast<size_t>(n)));
+    }
+    if (!s.free_mask.ptr) OC_CHECK(s.free_mask.alloc(mask.size()));
+    Runtime &rt = runtime();
+    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));  // kernels in flight may read the old values
+    if (n) {
+        OC_HIP_CHECK(hipMemcpy(s.bc_idx.ptr, idx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        OC_HIP_CHECK(hipMemcpy(s.bc_full.ptr, full_values, sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    OC_HIP_CHECK(hipMemcpy(s.free_mask.ptr, mask.data(), mask.size(), hipMemcpyHostToDevice));
     s.n_bc = n;
     ++s.bc_version;
     return OC_NO_ERROR;
@@ -1321,6 +1447,11 @@ int oc_load_increment_apply(oc_equations_set h, int iteration, int max_iteration
     bc_increment_kernel<<<static_cast<unsigned>((s.n_bc + 255) / 256), 256, 0, runtime().stream>>>(
         s.n_bc, s.bc_idx.ptr, s.bc_full.ptr, s.dofs.ptr, iteration, max_iterations);
     OC_HIP_CHECK(hipGetLastError());
+    if (s.plain_start_valid) {  // the plain start of a warm-started step takes the same increment
+        bc_increment_kernel<<<static_cast<unsigned>((s.n_bc + 255) / 256), 256, 0, runtime().stream>>>(
+            s.n_bc, s.bc_idx.ptr, s.bc_full.ptr, s.plain_start.ptr, iteration, max_iterations);
+        OC_HIP_CHECK(hipGetLastError());
+    }
     return OC_NO_ERROR;
 }
 
@@ -1340,6 +1471,11 @@ int oc_newton_options_default(oc_newton_options *o)
     o->preconditioner = OC_PRECONDITIONER_JACOBI;
     o->solution_tolerance = 1e-5;
     o->maximum_function_evaluations = 1000;
+    o->maximum_linear_solve_failures = 1;
+    o->maximum_step_failures = 1;
+    o->output_type = OC_SOLVER_NO_OUTPUT;
+    o->fallback = 0;
+    o->fallback_maximum_iterations = 0;
     return OC_NO_ERROR;
 }
 
@@ -1381,7 +1517,7 @@ int oc_newton_solve(oc_equations_set h, const oc_newton_options *opt, oc_newton_
     if (h->s.n_meshes != 1 && opt->reference_norm > 0.0)
         return set_error(OC_ERR_INVALID_ARGUMENT, "a batch takes per-mesh reference norms (oc_newton_solve_batch)");
     std::vector<double> refs(h->s.n_meshes, opt->reference_norm);
-    return newton_core(h->s, opt, opt->reference_norm > 0.0 ? refs.data() : nullptr, nullptr, stats);
+    return newton_solve_set(h->s, opt, opt->reference_norm > 0.0 ? refs.data() : nullptr, nullptr, stats);
 }
 
 int oc_newton_solve_batch(oc_equations_set h, const oc_newton_options *opt, const double *reference_norms,
@@ -1389,7 +1525,7 @@ int oc_newton_solve_batch(oc_equations_set h, const oc_newton_options *opt, cons
 {
     if (!opt || !stats) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
     OC_CHECK(mr_solver_ready(h));
-    return newton_core(h->s, opt, reference_norms, final_norms, stats);
+    return newton_solve_set(h->s, opt, reference_norms, final_norms, stats);
 }
 
 int oc_jacobian_evaluate_graph(oc_equations_set h, int precision)

@@ -351,6 +351,19 @@ class MooneyRivlinBatch(MooneyRivlinEquationsSet):
         q0, q1, l0, l1 = self.q_off[m], self.q_off[m + 1], self.l_off[m], self.l_off[m + 1]
         return np.concatenate([c * NQ + np.arange(q0, q1) for c in range(3)] + [3 * NQ + np.arange(l0, l1)])
 
+    def mesh_matrix(self, m, precision=_lib.OC_FP64, index_base=0):
+        """(row_ptr, col_idx, values) of mesh m's block of the last Jacobian of
+        `precision`, in the DOF order of a single-mesh set."""
+        n, nnz = ctypes.c_int(), ctypes.c_longlong()
+        call("oc_mesh_matrix_get", self._h, int(m), int(precision), int(index_base), ctypes.byref(n),
+             ctypes.byref(nnz), None, None, None)
+        rp = np.zeros(n.value + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        vals = np.zeros(nnz.value, np.float64 if precision == _lib.OC_FP64 else np.float32)
+        call("oc_mesh_matrix_get", self._h, int(m), int(precision), int(index_base), ctypes.byref(n),
+             ctypes.byref(nnz), rp.ctypes.data, ci.ctypes.data, vals.ctypes.data)
+        return rp, ci, vals
+
     def set_materials(self, c10, c01):
         c10 = np.ascontiguousarray(np.broadcast_to(np.asarray(c10, np.float64), (self.n_meshes,)))
         c01 = np.ascontiguousarray(np.broadcast_to(np.asarray(c01, np.float64), (self.n_meshes,)))

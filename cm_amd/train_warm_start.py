@@ -43,15 +43,15 @@ def draw_case(rng, n_min=4, n_max=16):
 PLAIN_KRYLOV_RTOL = 1e-5
 
 
-def solver_options(precision=_lib.OC_FP64, restart=100, reference_norm=0.0, krylov_rtol=-1.0):
-    return newton_options(relative_tolerance=1e-10, absolute_tolerance=0.0, reference_norm=reference_norm,
-                          krylov_restart=restart, krylov_relative_tolerance=krylov_rtol, jacobian_precision=precision,
-                          preconditioner=_lib.OC_PRECONDITIONER_MULTIGRID, maximum_iterations=100,
-                          # a GMRES solve that stalls on a distorted first Jacobian returns its best
-                          # iterate after 3000 iterations instead of running for minutes
-                          krylov_maximum_iterations=3000,
-                          # judged by the residual alone: no step-size test, no evaluation limit
-                          solution_tolerance=0.0, maximum_function_evaluations=-1)
+def solver_options(precision=_lib.OC_FP64, restart=100, reference_norm=0.0, krylov_rtol=-1.0, **extra):
+    opts = dict(relative_tolerance=1e-10, absolute_tolerance=0.0, reference_norm=reference_norm,
+                krylov_restart=restart, krylov_relative_tolerance=krylov_rtol, jacobian_precision=precision,
+                preconditioner=_lib.OC_PRECONDITIONER_MULTIGRID, maximum_iterations=100,
+                krylov_maximum_iterations=3000,
+                # judged by the residual alone: no step-size test, no evaluation limit
+                solution_tolerance=0.0, maximum_function_evaluations=-1)
+    opts.update(extra)
+    return newton_options(**opts)
 
 
 def make_problem(case, jitter=0.1):

@@ -25,8 +25,9 @@ MODULE OC_ASM
   INTEGER(C_INT), PARAMETER :: OC_PRECONDITIONER_JACOBI = 0, OC_PRECONDITIONER_BLOCK_DIAGONAL = 1, &
       & OC_PRECONDITIONER_BLOCK_TRIANGULAR = 2, OC_PRECONDITIONER_MULTIGRID = 3
   INTEGER(C_INT), PARAMETER :: OC_CONVERGED_FNORM_ABS = 2, OC_CONVERGED_FNORM_RELATIVE = 3, &
-      & OC_CONVERGED_SNORM_RELATIVE = 4, OC_DIVERGED_FUNCTION_COUNT = -2, OC_DIVERGED_FNORM_NAN = -4, &
-      & OC_DIVERGED_MAX_IT = -5
+      & OC_CONVERGED_SNORM_RELATIVE = 4, OC_DIVERGED_FUNCTION_COUNT = -2, OC_DIVERGED_LINEAR_SOLVE = -3, &
+      & OC_DIVERGED_FNORM_NAN = -4, OC_DIVERGED_MAX_IT = -5, OC_DIVERGED_LINE_SEARCH = -6
+  INTEGER(C_INT), PARAMETER :: OC_SOLVER_NO_OUTPUT = 0, OC_SOLVER_PROGRESS_OUTPUT = 1
 
   !> oc_newton_options (field order and types as in oc_asm.h)
   TYPE, BIND(C) :: OC_NEWTON_OPTIONS
@@ -38,6 +39,8 @@ MODULE OC_ASM
     INTEGER(C_INT) :: preconditioner
     REAL(C_DOUBLE) :: solution_tolerance
     INTEGER(C_INT) :: maximum_function_evaluations
+    INTEGER(C_INT) :: maximum_linear_solve_failures, maximum_step_failures, output_type
+    INTEGER(C_INT) :: fallback, fallback_maximum_iterations
   END TYPE OC_NEWTON_OPTIONS
 
   !> oc_newton_stats
@@ -45,6 +48,7 @@ MODULE OC_ASM
     INTEGER(C_INT) :: converged, newton_iterations, krylov_iterations, residual_evaluations, jacobian_evaluations
     REAL(C_DOUBLE) :: initial_residual_norm, final_residual_norm, seconds
     INTEGER(C_INT) :: reason
+    INTEGER(C_INT) :: linear_solve_failures, step_failures, fallback_used, warm_start_reason
   END TYPE OC_NEWTON_STATS
 
   INTERFACE
@@ -62,6 +66,15 @@ MODULE OC_ASM
       CHARACTER(KIND=C_CHAR), DIMENSION(*) :: buffer
       INTEGER(C_INT), VALUE :: buffer_size
     END FUNCTION oc_last_error
+
+    INTEGER(C_INT) FUNCTION oc_mesh_matrix_get(handle, mesh, precision, index_base, n_rows, nnz, row_ptr, col_idx, &
+        & values) BIND(C, NAME="oc_mesh_matrix_get")
+      IMPORT :: C_INT, C_LONG_LONG, C_PTR
+      TYPE(C_PTR), VALUE :: handle, row_ptr, col_idx, values
+      INTEGER(C_INT), VALUE :: mesh, precision, index_base
+      INTEGER(C_INT), INTENT(OUT) :: n_rows
+      INTEGER(C_LONG_LONG), INTENT(OUT) :: nnz
+    END FUNCTION oc_mesh_matrix_get
 
     INTEGER(C_INT) FUNCTION oc_laplace_create(n_nodes, coords, n_elements, elem_nodes, index_base, handle) &
         & BIND(C, NAME="oc_laplace_create")

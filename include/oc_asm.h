@@ -126,6 +126,16 @@ int oc_mesh_count(oc_equations_set equations_set, int *n_meshes);
  * pressure node i: 3 * n_nodes_total + node_offset_l + i. */
 int oc_equations_set_mesh_get(oc_equations_set equations_set, int mesh, int *node_offset_q, int *n_nodes_q,
                               int *node_offset_l, int *n_nodes_l, int *element_offset, int *n_elements);
+/* The equations-matrix block of mesh `mesh` of a batch as a CSR matrix of its
+ * own, in the DOF order of a single-mesh set (z1 of the mesh's nodes, z2, z3,
+ * then p): n_rows and nnz receive its sizes; row_ptr[n_rows+1] and
+ * col_idx[nnz] (index_base 0 or 1) and values[nnz] (double for OC_FP64, float
+ * for OC_FP32: the last Jacobian of that precision) are filled when not NULL.
+ * No counterpart in the reference, where each mesh is its own equations set
+ * and this is cmfe_DistributedMatrix_StorageLocationsGetObj / DataGetDPObj
+ * (opencmiss_iron.f90:44843, 44920) of that set. */
+int oc_mesh_matrix_get(oc_equations_set equations_set, int mesh, int precision, int index_base, int *n_rows,
+                       long long *nnz, int *row_ptr, int *col_idx, void *values);
 int oc_equations_set_destroy(oc_equations_set equations_set);
 /* class (OC_EQUATIONS_SET_*), number of rows (= DOFs), number of nonzeros,
  * number of elements, number of element colours (0: the assembly needs none). */
@@ -232,7 +242,27 @@ typedef struct {
     int preconditioner;          /* OC_PRECONDITIONER_* */
     double solution_tolerance;   /* stol: converged when ||step|| < stol * ||x|| over the free DOFs; 0 disables */
     int maximum_function_evaluations; /* residual evaluations allowed (< 0: no limit) */
+    /* GMRES solves allowed to end above their tolerance (at krylov_maximum_iterations) before the solve stops
+     * with OC_DIVERGED_LINEAR_SOLVE; a failed solve below the limit still gives the step (PETSc's
+     * SNESSetMaxLinearSolveFailures, default 1: the first failure stops) */
+    int maximum_linear_solve_failures;
+    /* line searches allowed to end without sufficient decrease before the solve stops with
+     * OC_DIVERGED_LINE_SEARCH (PETSc's SNESSetMaxNonlinearStepFailures, default 1) */
+    int maximum_step_failures;
+    /* OC_SOLVER_NO_OUTPUT or OC_SOLVER_PROGRESS_OUTPUT: one monitor line per Newton iteration on stderr
+     * (SOLVER_PROGRESS_OUTPUT and the nonlinear monitor, solver_routines.f90:350, 18628-18631, 20520-20635) */
+    int output_type;
+    /* 1: when a solve started from a learned warm start (oc_mlp_warm_start) fails, the meshes that did not
+     * converge restart from the plain start (the DOFs before the warm start, with the same load increments
+     * applied) with an FP64 Jacobian and fallback_maximum_iterations Newton iterations (0: maximum_iterations);
+     * oc_newton_stats.fallback_used reports it.  No counterpart in the reference: its divergence handling
+     * stops with an error (solver_routines.f90:20161-20268). */
+    int fallback;
+    int fallback_maximum_iterations;
 } oc_newton_options;
+
+#define OC_SOLVER_NO_OUTPUT 0
+#define OC_SOLVER_PROGRESS_OUTPUT 1
 
 /* Why a Newton solve stopped (oc_newton_stats.reason): the codes of PETSc's
  * SNESConvergedReason for the default convergence test the reference selects
@@ -241,8 +271,10 @@ typedef struct {
 #define OC_CONVERGED_FNORM_RELATIVE 3  /* ||F|| <= relative_tolerance * ||F_ref|| */
 #define OC_CONVERGED_SNORM_RELATIVE 4  /* ||step|| < solution_tolerance * ||x|| */
 #define OC_DIVERGED_FUNCTION_COUNT -2  /* maximum_function_evaluations reached */
+#define OC_DIVERGED_LINEAR_SOLVE -3    /* maximum_linear_solve_failures GMRES solves ended above their tolerance */
 #define OC_DIVERGED_FNORM_NAN -4       /* non-finite residual */
 #define OC_DIVERGED_MAX_IT -5          /* maximum_iterations reached */
+#define OC_DIVERGED_LINE_SEARCH -6     /* maximum_step_failures line searches ended without sufficient decrease */
 
 typedef struct {
     int converged;
@@ -254,11 +286,16 @@ typedef struct {
     double final_residual_norm;
     double seconds;
     int reason;                  /* OC_CONVERGED_* (> 0) or OC_DIVERGED_* (< 0) */
+    int linear_solve_failures;   /* GMRES solves that ended above their tolerance */
+    int step_failures;           /* line searches that ended without sufficient decrease */
+    int fallback_used;           /* 1: the answer comes from the plain-start fallback (options.fallback) */
+    int warm_start_reason;       /* reason of the warm-started attempt when the fallback ran, else 0 */
 } oc_newton_stats;
 
 /* Reference defaults (solver_routines.f90:17780-17851, 10957-11006): abs tol
  * 1e-10, rel tol 1e-5, solution tol 1e-5, 50 iterations, 1000 function
- * evaluations, line search alpha 1e-4, GMRES(30), rtol 1e-5, Jacobi. */
+ * evaluations, line search alpha 1e-4, GMRES(30), rtol 1e-5, Jacobi; PETSc's
+ * one linear-solve failure and one step failure; no output; no fallback. */
 int oc_newton_options_default(oc_newton_options *options);
 /* Newton solve of R(x) = 0 at the current Dirichlet values for a Mooney-
  * Rivlin set (SOLVER_NEWTON_LINESEARCH_SOLVE, solver_routines.f90:18871-

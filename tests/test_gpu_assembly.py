@@ -226,3 +226,45 @@ def test_finite_difference_jacobian_against_oracle(gpu):
     # one-sided differences with delta = (1 + ||x||) 1e-7 ~ 1e-6 here: O(delta) truncation
     eq.jacobian_evaluate(cm.OC_FP64)
     assert rel(Jfd, eq.matrix_values()) < 1e-4
+
+
+def test_fortran_host_mooney_rivlin_residual_and_jacobians(gpu, tmp_path):
+    """A Fortran host (tests/fortran/drive_mooney_rivlin.f90, the shape of the
+    Iron-side residual and Jacobian evaluation drivers) passes 1-based
+    connectivity and its dependent field through ISO_C_BINDING and gets the
+    residual and its own 1-based CSR (ROW_INDICES, COLUMN_INDICES, DATA_DP) of
+    the FP64 and FP32 Jacobians: structure bit-exact, FP64 values and residual
+    within 1e-12, FP32 within 2e-6 of the oracle."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran", "drive_mooney_rivlin")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (no amdflang)")
+    cm = gpu
+    mesh = cm.RegularMesh.generate(2, (3, 2, 4), jitter=0.1, seed=21)
+    c10, c01 = 2.5, 4.0
+    d = _deformed_state(mesh, 0.1, 22)
+    fin, fout = os.path.join(tmp_path, "in.bin"), os.path.join(tmp_path, "out.bin")
+    with open(fin, "wb") as f:
+        np.array([mesh.n_nodes, mesh.n_nodes_l, mesh.n_elements], np.int32).tofile(f)
+        np.array([c10, c01], np.float64).tofile(f)
+        np.ascontiguousarray(mesh.coords, np.float64).tofile(f)
+        (np.ascontiguousarray(mesh.elem_nodes, np.int32) + 1).tofile(f)
+        (np.ascontiguousarray(mesh.elem_nodes_l, np.int32) + 1).tofile(f)
+        d.astype(np.float64).tofile(f)
+    out = subprocess.run([exe, fin, fout], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    with open(fout, "rb") as f:
+        n, nnz = np.fromfile(f, np.int32, 2)
+        res = np.fromfile(f, np.float64, n)
+        rp = np.fromfile(f, np.int32, n + 1)
+        ci = np.fromfile(f, np.int32, nnz)
+        v64 = np.fromfile(f, np.float64, nnz)
+        v32 = np.fromfile(f, np.float64, nnz)
+    orp, oci = _pattern_oracle_mr(mesh)
+    assert np.array_equal(rp.astype(np.int64), orp + 1) and np.array_equal(ci, oci + 1)
+    oR = orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, c10, c01)
+    oJ = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, c10, c01, orp, oci)
+    assert rel(res, oR) < FP64_TOL
+    assert rel(v64, oJ) < FP64_TOL
+    assert rel(v32, oJ) < FP32_TOL

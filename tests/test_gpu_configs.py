@@ -1,0 +1,192 @@
+"""GPU tests at the workloads of the benchmark configurations that the smaller
+parity tests do not reach, judged by the C oracle:
+
+* the batch of 256 Mooney-Rivlin meshes (n ~ U{4..16}, seeded materials,
+  stretches and jitter): sampled meshes' CSR structure bit-exact and values
+  within 1e-12 (FP64) / 2e-6 (FP32) of the oracle, read back per mesh; one
+  learned and one plain load step of every mesh through the batched Newton
+  solve on its recorded graphs, every mesh judged by the oracle residual;
+* the learned warm start on the 32^3 case (20% stretch in 10 load steps);
+* the fixed-incremented boundary-condition schedule against the oracle's;
+* the forward-difference Jacobian on a jittered 4^3 mesh.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 2e-6
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _mr_pattern(mesh):
+    return orc.pattern([(mesh.n_nodes, mesh.elem_nodes)] * 3 + [(mesh.n_nodes_l, mesh.elem_nodes_l)])
+
+
+def _perturbed(mesh, rng, amp=0.05):
+    """z = X + amp h U[-1, 1], p ~ N(0, 1) (h = 1/n of the mesh)."""
+    z = mesh.coords + amp / max(mesh.n) * rng.uniform(-1, 1, mesh.coords.shape)
+    return np.concatenate([z.ravel(), rng.standard_normal(mesh.n_nodes_l)])
+
+
+def _oracle_free_norm(mesh, c, dofs, fixed_local):
+    r = orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, dofs, c["c10"], c["c01"])
+    r[fixed_local] = 0.0
+    return float(np.linalg.norm(r))
+
+
+def test_batch_of_256_meshes_against_oracle(gpu):
+    cm = gpu
+    from cm_amd import _lib
+    from cm_amd.batch_workload import build, draw_cases
+    from cm_amd.train_warm_start import DEFAULT_WEIGHTS, solver_options
+    from cm_amd.warm_start import WarmStartMLP
+
+    cases = draw_cases(256)
+    meshes, batch, _ = build(cases)
+    assert batch.n_meshes == 256 and batch.n_elements == sum(m.n_elements for m in meshes)
+    print(f"batch: {batch.n_elements} elements, {batch.n_rows} DOFs, {batch.nnz} nonzeros", flush=True)
+    rng = np.random.default_rng(77)
+    d = np.zeros(batch.n_rows)
+    local = []
+    for m, mesh in enumerate(meshes):
+        dm = _perturbed(mesh, rng)
+        d[batch.mesh_dofs(m)] = dm
+        local.append(dm)
+    batch.set_dofs(d)
+    batch.residual_evaluate()
+    batch.jacobian_evaluate(cm.OC_FP64)
+    batch.jacobian_evaluate(cm.OC_FP32)
+    r = batch.residual()
+    # eight sampled meshes, one of them of the largest size (n = 16)
+    big = [m for m, c in enumerate(cases) if c["n"] == 16]
+    assert big
+    pick = sorted(set([big[0]] + list(np.random.default_rng(5).choice(256, 7, replace=False))))
+    for m in pick:
+        mesh, c = meshes[m], cases[m]
+        orp, oci = _mr_pattern(mesh)
+        rp, ci, v64 = batch.mesh_matrix(m, cm.OC_FP64)
+        assert np.array_equal(rp.astype(np.int64), orp) and np.array_equal(ci, oci), m
+        _, _, v32 = batch.mesh_matrix(m, cm.OC_FP32)
+        oJ = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, local[m], c["c10"], c["c01"], orp, oci)
+        oR = orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, local[m], c["c10"], c["c01"])
+        e64, e32, er = rel(v64, oJ), rel(v32.astype(np.float64), oJ), rel(r[batch.mesh_dofs(m)], oR)
+        print(f"mesh {m} n={c['n']}: FP64 {e64:.2e} FP32 {e32:.2e} residual {er:.2e}", flush=True)
+        assert e64 < FP64_TOL and e32 < FP32_TOL and er < FP64_TOL, m
+
+    # one load step of every mesh, learned (FP32 Jacobians) then plain (FP64),
+    # each through the batched Newton solve replaying its recorded graphs
+    batch.set_multigrid()
+    fixed_local = [np.isin(batch.mesh_dofs(m), batch.fixed_dofs) for m in range(256)]
+    dload = np.array([c["stretch"] / 5 for c in cases])
+    net = WarmStartMLP.load(DEFAULT_WEIGHTS)
+
+    def judge(refs, label):
+        dd = batch.get_dofs()
+        worst = 0.0
+        for m, mesh in enumerate(meshes):
+            worst = max(worst, _oracle_free_norm(mesh, cases[m], dd[batch.mesh_dofs(m)], fixed_local[m]) / refs[m])
+        print(f"{label}: worst oracle residual / reference {worst:.3e}", flush=True)
+        assert worst <= 1.05e-10, label
+
+    batch.set_dofs(batch.undeformed_dofs())
+    batch.load_increment_apply(1, 5)
+    refs1 = batch.mesh_residual_norms()
+    batch.set_dofs(batch.undeformed_dofs())
+    net.warm_start(batch, dload)
+    batch.load_increment_apply(1, 5)
+    st, finals = batch.newton_solve(solver_options(_lib.OC_FP32), reference_norms=refs1)
+    print("learned step 1:", {k: st[k] for k in ("newton_iterations", "krylov_iterations", "seconds", "fallback_used")},
+          flush=True)
+    assert st["converged"] and np.all(finals <= 1e-10 * refs1)
+    judge(refs1, "learned step 1")
+    # step 2 from the step-1 solutions with the plain start and FP64 Jacobians
+    batch.load_increment_apply(2, 5)
+    refs2 = batch.mesh_residual_norms()
+    st, finals = batch.newton_solve(solver_options(_lib.OC_FP64, krylov_rtol=1e-5), reference_norms=refs2)
+    print("plain step 2:", {k: st[k] for k in ("newton_iterations", "krylov_iterations", "seconds")}, flush=True)
+    assert st["converged"] and np.all(finals <= 1e-10 * refs2)
+    judge(refs2, "plain step 2")
+    net.destroy()
+    batch.destroy()
+
+
+def test_learned_steps_on_the_32_cube(gpu):
+    """Three load steps of the 32^3 case (20% stretch in 10 steps) from the
+    learned warm start with FP32 Jacobians: each accepted answer meets 1e-10 x
+    the plain start's residual by the oracle's FP64 residual, and the learned
+    start is far inside the plain start."""
+    cm = gpu
+    from cm_amd import _lib
+    from cm_amd.train_warm_start import DEFAULT_WEIGHTS, make_problem, solver_options
+    from cm_amd.warm_start import WarmStartMLP
+    case = dict(n=32, c10=2.0, c01=6.0, stretch=0.20, seed=2024)
+    mesh, eq = make_problem(case)
+    net = WarmStartMLP.load(DEFAULT_WEIGHTS)
+    for s in (1, 2, 3):
+        d0 = eq.get_dofs()
+        eq.load_increment_apply(s, 10)
+        ref = eq.residual_norm()
+        eq.set_dofs(d0)
+        net.warm_start(eq, case["stretch"] / 10)
+        eq.load_increment_apply(s, 10)
+        st = eq.newton_solve(solver_options(_lib.OC_FP32, reference_norm=ref))
+        orr = orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, eq.get_dofs(), 2.0, 6.0)
+        orr[eq.fixed_dofs] = 0.0
+        print(f"step {s}: start/ref {st['initial_residual_norm'] / ref:.2e} Newton {st['newton_iterations']} "
+              f"Krylov {st['krylov_iterations']} {st['seconds']:.3f} s, oracle/ref {np.linalg.norm(orr) / ref:.2e}",
+              flush=True)
+        assert st["converged"] and not st["fallback_used"]
+        assert st["initial_residual_norm"] < 0.05 * ref
+        assert np.linalg.norm(orr) <= 1.05e-10 * ref
+    net.destroy()
+    eq.destroy()
+
+
+def test_load_increment_schedule_against_oracle(gpu):
+    """oc_load_increment_apply over a 10-step schedule, bit for bit against the
+    oracle's fixed-incremented rule (equations_set_routines.f90:6491-6610)."""
+    cm = gpu
+    mesh = cm.RegularMesh.generate(2, 3, jitter=0.1, seed=2)
+    eq = cm.MooneyRivlinEquationsSet(mesh, 2.0, 6.0)
+    rng = np.random.default_rng(3)
+    d = _perturbed(mesh, rng)
+    eq.set_dofs(d)
+    nq = mesh.n_nodes
+    idx = np.concatenate([mesh.face_nodes(0, 0), nq + mesh.face_nodes(1, 0), 2 * nq + mesh.face_nodes(2, 0),
+                          mesh.face_nodes(0, 1)]).astype(np.int32)
+    full = rng.uniform(-0.5, 1.5, idx.size)
+    eq.set_boundary_conditions(idx, full)
+    expect = d.copy()
+    for it in range(1, 11):
+        eq.load_increment_apply(it, 10)
+        expect = orc.bc_increment(idx, full, expect, it, 10)
+        np.testing.assert_array_equal(eq.get_dofs(), expect)
+    assert np.array_equal(expect[idx], full)
+    eq.destroy()
+
+
+def test_finite_difference_jacobian_jittered_4_cube(gpu):
+    """Forward-difference Jacobian of a jittered 4^3 mesh at a deformed state
+    against the oracle's (same delta rule) and the analytic Jacobian."""
+    cm = gpu
+    mesh = cm.RegularMesh.generate(2, 4, jitter=0.1, seed=12)
+    eq = cm.MooneyRivlinEquationsSet(mesh, 2.0, 6.0)
+    d = _perturbed(mesh, np.random.default_rng(13), amp=0.1)
+    eq.set_dofs(d)
+    orp, oci = _mr_pattern(mesh)
+    eq.jacobian_evaluate_fd()
+    oJfd = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, orp, oci, mode=1)
+    e_fd = rel(eq.matrix_values(), oJfd)
+    eq.jacobian_evaluate(cm.OC_FP64)
+    e_an = rel(eq.matrix_values(), oJfd)
+    print(f"FD vs oracle FD {e_fd:.2e}, analytic vs oracle FD {e_an:.2e}")
+    assert e_fd < 1e-7 and e_an < 1e-4
+    eq.destroy()

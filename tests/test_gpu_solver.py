@@ -209,3 +209,138 @@ def test_multigrid_rejects_irregular_numbering(gpu):
     with pytest.raises(cm.CMISSError):
         eq.newton_solve(opts)
     eq.destroy()
+
+
+def test_boundary_conditions_changed_after_a_solve(gpu):
+    """Solve, set different Dirichlet conditions (another DOF count), solve
+    again: the set's recorded Newton-step graph (Jacobian assembly and
+    multigrid setup, which read the free-DOF mask) follows the new conditions,
+    and the answer matches a fresh set's, judged by the oracle residual."""
+    cm = gpu
+    c10, c01 = 2.0, 6.0
+    mesh = cm.RegularMesh.generate(2, 3, jitter=0.1, seed=9)
+    opts = cm.newton_options(solution_tolerance=0.0, relative_tolerance=1e-10, absolute_tolerance=0.0,
+                             krylov_restart=100, krylov_relative_tolerance=-1.0,
+                             preconditioner=cm.OC_PRECONDITIONER_MULTIGRID, maximum_iterations=30)
+    nq = mesh.n_nodes
+    x0 = mesh.face_nodes(0, 0)
+    clamp = np.concatenate([x0, nq + x0, 2 * nq + x0, mesh.face_nodes(0, 1)])
+    clamp_vals = np.concatenate([mesh.coords[0, x0], mesh.coords[1, x0], mesh.coords[2, x0],
+                                 np.full(len(mesh.face_nodes(0, 1)), 1.08)])
+
+    def solve(eq):
+        eq.set_dofs(eq.undeformed_dofs())
+        eq.load_increment_apply(1, 1)
+        ref = eq.residual_norm()
+        st = eq.newton_solve(opts)
+        assert st["converged"], st
+        return ref
+
+    eq = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
+    eq.uniaxial_boundary_conditions(1.05)
+    eq.set_multigrid()
+    solve(eq)
+    eq.set_boundary_conditions(clamp, clamp_vals)
+    ref = solve(eq)
+    assert _oracle_free_residual(eq, mesh, clamp, c10, c01) <= 1.05e-10 * ref
+    fresh = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
+    fresh.set_boundary_conditions(clamp, clamp_vals)
+    fresh.set_multigrid()
+    solve(fresh)
+    assert np.abs(eq.get_dofs() - fresh.get_dofs()).max() < 1e-8
+    eq.destroy()
+    fresh.destroy()
+
+
+def test_plain_fallback_after_a_failed_warm_start(gpu):
+    """A learned start that fails (the shipped network's weights scaled by 100
+    from the host, then a learned attempt allowed no Newton iteration) restarts
+    from the plain start with FP64 Jacobians when the options ask for the
+    fallback; the answer meets the plain method's FP64 residual by the oracle."""
+    cm = gpu
+    from cm_amd import _lib
+    from cm_amd.train_warm_start import DEFAULT_WEIGHTS, make_problem, solver_options
+    from cm_amd.warm_start import WarmStartMLP
+    net = WarmStartMLP.load(DEFAULT_WEIGHTS)
+    good = net.parameters()
+    case = dict(n=4, c10=2.0, c01=6.0, stretch=0.15, seed=4242)
+    mesh, eq = make_problem(case)
+    dload = case["stretch"] / 5
+    for s, (weights, max_it) in enumerate([(good * 100.0, 8), (good, 0)], start=1):
+        net.set_parameters(weights)
+        d0 = eq.get_dofs()
+        eq.load_increment_apply(s, 5)
+        ref = eq.residual_norm()
+        eq.set_dofs(d0)
+        net.warm_start(eq, dload)
+        eq.load_increment_apply(s, 5)
+        start = eq.get_dofs()
+        st = eq.newton_solve(solver_options(_lib.OC_FP32, reference_norm=ref, maximum_iterations=max_it),
+                             raise_on_failure=False)
+        assert not st["converged"] and st["fallback_used"] == 0
+        eq.set_dofs(start)  # a DOF upload drops the saved plain start: no fallback possible
+        st = eq.newton_solve(solver_options(_lib.OC_FP32, reference_norm=ref, maximum_iterations=max_it, fallback=1),
+                             raise_on_failure=False)
+        assert not st["converged"] and st["fallback_used"] == 0
+        eq.set_dofs(d0)
+        net.warm_start(eq, dload)
+        eq.load_increment_apply(s, 5)
+        st = eq.newton_solve(solver_options(_lib.OC_FP32, reference_norm=ref, maximum_iterations=max_it, fallback=1,
+                                            fallback_maximum_iterations=50))
+        print(s, st)
+        assert st["converged"] and st["fallback_used"] == 1 and st["warm_start_reason"] < 0
+        assert _oracle_free_residual(eq, mesh, eq.fixed_dofs, case["c10"], case["c01"]) <= 1.05e-10 * ref
+    net.destroy()
+    eq.destroy()
+
+
+def test_solution_export_and_ndiff(gpu, tmp_path):
+    """The GPU solution of the 8^3 Laplace case (u = 0 on x = 0, u = 1 on x = 1)
+    exported as exnode/exelem agrees with the export of the exact field u = x
+    within 1e-10 under the ndiff rule, and the homogeneous uniaxial
+    Mooney-Rivlin solution with the export of its closed form; a perturbed
+    field does not."""
+    import os
+    from cm_amd import field_io as fio
+
+    class Exact:
+        def __init__(self, mesh, dofs):
+            self.mesh, self._d = mesh, dofs
+
+        def get_dofs(self):
+            return self._d
+
+    cm = gpu
+    mesh = cm.RegularMesh.generate(1, 8)
+    eq = cm.LaplaceEquationsSet(mesh)
+    left, right = mesh.face_nodes(0, 0), mesh.face_nodes(0, 1)
+    eq.set_boundary_conditions(np.concatenate([left, right]),
+                               np.concatenate([np.zeros(len(left)), np.ones(len(right))]))
+    assert eq.solve(cm.newton_options(krylov_relative_tolerance=1e-13, krylov_restart=60,
+                                      absolute_tolerance=0.0))["converged"]
+    a, b, c = (os.path.join(tmp_path, k) for k in ("gpu", "exact", "off"))
+    fio.export_laplace(a, eq)
+    fio.export_laplace(b, Exact(mesh, mesh.coords[0].copy()))
+    fio.export_laplace(c, Exact(mesh, mesh.coords[0] + 1e-6))
+    assert fio.ndiff(a + ".part0.exnode", b + ".part0.exnode", "1e-10") == []
+    assert fio.ndiff(a + ".part0.exelem", b + ".part0.exelem", "0") == []
+    assert len(fio.ndiff(a + ".part0.exnode", c + ".part0.exnode", "1e-10")) > 0
+    eq.destroy()
+
+    c10, c01, lam = 2.0, 6.0, 1.1
+    mesh = cm.RegularMesh.generate(2, 2)
+    mr = cm.MooneyRivlinEquationsSet(mesh, c10, c01)
+    mr.set_dofs(mr.undeformed_dofs())
+    mr.uniaxial_boundary_conditions(lam)
+    mr.load_increment_apply(1, 1)
+    assert mr.newton_solve(cm.newton_options(solution_tolerance=0.0, relative_tolerance=1e-13, absolute_tolerance=1e-14,
+                                             krylov_restart=200, krylov_relative_tolerance=-1.0,
+                                             preconditioner=cm.OC_PRECONDITIONER_BLOCK_DIAGONAL))["converged"]
+    s0 = orc.mr_stress(np.diag([lam, lam ** -0.5, lam ** -0.5]), 0.0, c10, c01)
+    z = mesh.coords * np.array([[lam], [lam ** -0.5], [lam ** -0.5]])
+    exact = np.concatenate([z.ravel(), np.full(mesh.n_nodes_l, -s0[1])])
+    fio.export_mooney_rivlin(a, mr)
+    fio.export_mooney_rivlin(b, Exact(mesh, exact))
+    assert fio.ndiff(a + ".part0.exnode", b + ".part0.exnode", "1e-9") == []
+    assert fio.ndiff(a + ".part0.exelem", b + ".part0.exelem", "0") == []
+    mr.destroy()
