@@ -291,11 +291,11 @@ def kernel_roofline(name, kind, per_launch, avg_launch_s, traffic_key, workload_
     return out
 
 
-def gather_bytes(eq, E, value_bytes):
-    """Algorithmic bytes of one CSR row gather: the element matrices read
-    (E*89*89 values), the neighbour-rank map (E*35*35 bytes) and the CSR values
-    written once."""
-    return E * 89 * 89 * value_bytes + E * 35 * 35 + eq.nnz * value_bytes
+def gather_bytes(eq, value_bytes):
+    """Algorithmic bytes of the scatter into CSR: the Jacobian's CSR values,
+    written once (the element matrices the row gather reads back are this
+    implementation's intermediate, not part of the count)."""
+    return eq.nnz * value_bytes
 
 
 def main():
@@ -377,7 +377,7 @@ def main():
     wl = f"mr{args.n}"
     # untimed pass with every kernel timed: which kernel dominates a step
     progress(f"MR {args.n}^3: {E} elements, {eq.n_rows} DOFs, {eq.nnz} nonzeros")
-    time_steps(step, 2, args.warmup, cm)
+    time_steps(step, args.steps, args.warmup, cm)
     pre = {k: cm.kernel_timing(k)[0] for k in ("mr_element_matrix_fp64", "csr_row_gather_fp64")}
     cm.kernel_timing_enable(False)
     dominant = max(pre, key=pre.get)
@@ -396,7 +396,7 @@ def main():
     rl_elem = kernel_roofline("mr_element_matrix_f64_kernel", "mfma64", E,
                               (dom_ms / dom_launches if dominant == "mr_element_matrix_fp64" else em / el) / 1e3,
                               "mr_element_matrix_fp64", wl)
-    rl_gather = kernel_roofline("csr_row_gather_kernel<double>", "hbm", gather_bytes(eq, E, 8),
+    rl_gather = kernel_roofline("csr_row_gather_kernel<double>", "hbm", gather_bytes(eq, 8),
                                 (dom_ms / dom_launches if dominant == "csr_row_gather_fp64" else gm / gl) / 1e3,
                                 "csr_row_gather_fp64", wl)
     roofline = rl_elem if dominant == "mr_element_matrix_fp64" else rl_gather
@@ -420,7 +420,7 @@ def main():
             "elements_per_s": E * args.steps / (ms32 / 1e3),
             "element_matrix": kernel_roofline("mr_element_matrix_kernel<float>", "mfma32", E, j32 / l32 / 1e3,
                                               "mr_element_matrix_fp32", wl),
-            "row_gather": kernel_roofline("csr_row_gather_kernel<float>", "hbm", gather_bytes(eq, E, 4),
+            "row_gather": kernel_roofline("csr_row_gather_kernel<float>", "hbm", gather_bytes(eq, 4),
                                           g32 / k32 / 1e3, "csr_row_gather_fp32", wl),
             "kernel_ms_per_step": {"mr_element_matrix_fp32": j32 / args.steps, "csr_row_gather_fp32": g32 / args.steps},
         }

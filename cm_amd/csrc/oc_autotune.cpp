@@ -88,31 +88,62 @@ extern "C" int oc_autotune(oc_equations_set h, int precision, char *report, int 
     }
     js += "}";
 
+    // SpMV: lanes per row (per node for node rows) x neighbours in flight per lane
+    OC_CHECK(prepare_node_rows(s));
     js += ", \"spmv_ms\": {";
     DevBuf<double> y;
     OC_CHECK(y.alloc(s.n_rows));
-    int best_lanes = 32;
+    int best_lanes = 32, best_unroll = 1;
     best_ms = 1e30f;
     const int lanes[] = {8, 16, 32, 64};
-    for (int i = 0; i < 4; ++i) {
-        float ms = 0.0f;
-        OC_CHECK(best_of([&] { return launch_spmv_lanes(s, precision == OC_FP32, lanes[i], s.dofs.ptr, y.ptr); },
-                         reps, &ms));
-        std::snprintf(buf, sizeof buf, "%s\"%d\": %.4f", i ? ", " : "", lanes[i], ms);
-        js += buf;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best_lanes = lanes[i];
+    bool first = true;
+    for (int i = 0; i < 4; ++i)
+        for (int u = 1; u <= 2; ++u) {
+            if (u == 2 && (s.node_rows != 1 || lanes[i] < 16)) continue;  // unrolled form: node rows only
+            float ms = 0.0f;
+            OC_CHECK(best_of(
+                [&] { return launch_spmv_lanes(s, precision == OC_FP32, lanes[i], s.dofs.ptr, y.ptr, u); }, reps,
+                &ms));
+            std::snprintf(buf, sizeof buf, "%s\"%dx%d\": %.4f", first ? "" : ", ", lanes[i], u, ms);
+            first = false;
+            js += buf;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_lanes = lanes[i];
+                best_unroll = u;
+            }
         }
-    }
     js += "}";
+    // level-0 multigrid node kernel (residual form), when the rows have the node layout
+    int best_smoother = -1;
+    if (s.node_rows == 1) {
+        js += ", \"smoother_residual_ms\": {";
+        DevBuf<double> r;
+        OC_CHECK(r.alloc(static_cast<size_t>(3) * s.n_nodes));
+        best_ms = 1e30f;
+        const char *names[] = {"32x1", "32x2", "16x2", "16x4", "64x1"};
+        for (int v = 0; v < 5; ++v) {
+            float ms = 0.0f;
+            OC_CHECK(best_of([&] { return launch_smoother_residual(s, precision, v, s.dofs.ptr, s.dofs.ptr, r.ptr); },
+                             reps, &ms));
+            std::snprintf(buf, sizeof buf, "%s\"%s\": %.4f", v ? ", " : "", names[v], ms);
+            js += buf;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_smoother = v;
+            }
+        }
+        js += "}";
+    }
     OC_HIP_CHECK(hipStreamSynchronize(runtime().stream));
     s.tuned_element_blocks_per_cu = best_bpc;
     s.tuned_gather_chunk = best_chunk;
     s.tuned_spmv_lanes = best_lanes;
+    s.tuned_spmv_unroll = best_unroll;
+    if (best_smoother >= 0) s.tuned_smoother = best_smoother;
     std::snprintf(buf, sizeof buf, ", \"chosen\": {\"element_blocks_per_cu\": %d, \"gather_chunk\": %d, "
-                                   "\"spmv_lanes\": %d}}",
-                  best_bpc, best_chunk, best_lanes);
+                                   "\"spmv_lanes\": %d, \"spmv_unroll\": %d, \"smoother_variant\": %d}}",
+                  best_bpc, best_chunk, best_lanes, best_unroll, best_smoother);
     js += buf;
     if (report && report_size > 0) std::snprintf(report, static_cast<size_t>(report_size), "%s", js.c_str());
     return OC_NO_ERROR;

@@ -306,4 +306,45 @@ __device__ __forceinline__ double csr_row_dot(int64_t k0, int64_t k1, int lane, 
     return a0 + a1;
 }
 
+// Node i's three displacement rows times x over their displacement columns
+// (rows with the node layout: row c of node i holds three blocks of `deg`
+// entries, block c' over the same sorted neighbour nodes j, from s[c]):
+// a[c] += sum_p sum_c' K[(c, i), (c', j_p)] x[c' nq + j_p].  kLanes lanes per
+// node; lane `lane` takes neighbours p = lane + kLanes (u + kU k), and the
+// loads of its kU neighbours are all issued before their products, so each
+// lane keeps kU independent index -> vector load chains in flight.
+template <typename VT, int kLanes, int kU>
+__device__ __forceinline__ void node_rows_uu(const int64_t (&s)[3], int deg, int lane, int64_t nq64,
+                                             const int *__restrict__ col_idx, const VT *__restrict__ vals,
+                                             const double *__restrict__ x, double (&a)[3])
+{
+    for (int p0 = lane; p0 < deg; p0 += kLanes * kU) {
+        int j[kU];
+        bool ok[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            ok[u] = p0 + u * kLanes < deg;
+            j[u] = ok[u] ? col_idx[s[0] + p0 + u * kLanes] : 0;
+        }
+        double v[kU][9];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    v[u][3 * r + c] = ok[u] ? static_cast<double>(vals[s[r] + c * deg + p0 + u * kLanes]) : 0.0;
+        double xv[kU][3];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xv[u][c] = ok[u] ? x[c * nq64 + j[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                a[r] += v[u][3 * r] * xv[u][0] + v[u][3 * r + 1] * xv[u][1] + v[u][3 * r + 2] * xv[u][2];
+    }
+}
+
 }  // namespace oc

@@ -127,8 +127,8 @@ struct Multigrid {
     DevBuf<double> t0, r0, x0;            // level-0 scratch (3*n_nodes each)
     DevBuf<double> coarse_inv;            // per mesh dense inverse of the coarsest level
     int coarse_stride = 0;                // (max DOFs per mesh at the coarsest level)^2
-    int pre = 1, post = 1;
-    double omega = 0.8;
+    int pre = 2, post = 2;
+    double omega = 0.5;
     int coarse_sweeps = 0;                // 0: dense solve of the coarsest level
 };
 
@@ -245,6 +245,8 @@ struct EquationsSet {
     int tuned_element_blocks_per_cu = 0;
     int tuned_spmv_lanes = 0;
     int tuned_gather_chunk = 0;
+    int tuned_smoother = -1;     // level-0 node-kernel shape (oc_multigrid.hip); -1 = default
+    int tuned_spmv_unroll = 0;   // neighbours per lane in flight in the node-wise SpMV
 
     // Newton-step graph: Jacobian assembly + preconditioner setup, recorded for
     // {precision, preconditioner, element blocks per CU, gather chunk,
@@ -295,10 +297,15 @@ int launch_mr_gauss(EquationsSet &s);
 // `precision`; apply z_u = V-cycle(t) for the displacement block
 int mg_setup(EquationsSet &s, int precision);
 int mg_apply(EquationsSet &s, int precision, const double *t, double *z);
+// pressure-column starts and the node-row layout check (oc_solver.hip)
+int prepare_node_rows(EquationsSet &s);
+// r = b - K_uu x with the level-0 node kernel of shape `variant` (autotuner)
+int launch_smoother_residual(EquationsSet &s, int precision, int variant, const double *b, const double *x,
+                             double *r);
 // solver matrix and vector from the last assembled matrix (oc_solver_matrices.hip)
 int solver_matrices_assemble(EquationsSet &s, int precision);
 // y = A x with `lanes` lanes per CSR row (A in FP64 or FP32 values)
-int launch_spmv_lanes(EquationsSet &s, bool fp32, int lanes, const double *x, double *y);
+int launch_spmv_lanes(EquationsSet &s, bool fp32, int lanes, const double *x, double *y, int unroll = 1);
 
 // Kernel timers are suspended while a stream capture records a graph.
 void set_capturing(bool on);

@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(256) values_bf16_kernel(int64_t n, const float
 // its three rows; r = b - K_uu x on free rows, 0 on fixed ones.  kSweep 0
 // writes r to out; kSweep 1 writes the block Jacobi update
 // x + omega Kblock^-1 r to out (not x: neighbours still read the old x).
-template <typename VT, int kLanes, int kSweep>
+template <typename VT, int kLanes, int kSweep, int kU>
 __global__ void __launch_bounds__(256)
 uu_node_residual_kernel(int nq, const int64_t *__restrict__ row_ptr, const int64_t *__restrict__ pstart,
                         const int *__restrict__ col_idx, const VT *__restrict__ vals,
@@ -388,19 +388,11 @@ uu_node_residual_kernel(int nq, const int64_t *__restrict__ row_ptr, const int64
     const int i = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes);
     if (i >= nq) return;  // whole groups
     const int64_t nq64 = nq;
-    const int64_t s0 = row_ptr[i], s1 = row_ptr[nq64 + i], s2 = row_ptr[2 * nq64 + i];
-    const int deg = static_cast<int>((pstart[i] - s0) / 3);
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int p = lane; p < deg; p += kLanes) {
-        const int j = col_idx[s0 + p];
-        const double x0 = x[j], x1 = x[nq64 + j], x2 = x[2 * nq64 + j];
-        a0 += static_cast<double>(vals[s0 + p]) * x0 + static_cast<double>(vals[s0 + deg + p]) * x1 +
-              static_cast<double>(vals[s0 + 2 * deg + p]) * x2;
-        a1 += static_cast<double>(vals[s1 + p]) * x0 + static_cast<double>(vals[s1 + deg + p]) * x1 +
-              static_cast<double>(vals[s1 + 2 * deg + p]) * x2;
-        a2 += static_cast<double>(vals[s2 + p]) * x0 + static_cast<double>(vals[s2 + deg + p]) * x1 +
-              static_cast<double>(vals[s2 + 2 * deg + p]) * x2;
-    }
+    const int64_t sr[3] = {row_ptr[i], row_ptr[nq64 + i], row_ptr[2 * nq64 + i]};
+    const int deg = static_cast<int>((pstart[i] - sr[0]) / 3);
+    double a[3] = {0.0, 0.0, 0.0};
+    node_rows_uu<VT, kLanes, kU>(sr, deg, lane, nq64, col_idx, vals, x, a);
+    double a0 = a[0], a1 = a[1], a2 = a[2];
 #pragma unroll
     for (int o = kLanes / 2; o > 0; o >>= 1) {
         a0 += __shfl_xor(a0, o, kLanes);
@@ -817,6 +809,47 @@ static int stencil_sweep(int M, MgLevel &F, double omega)
     return OC_NO_ERROR;
 }
 
+// Launch shapes of the level-0 node kernels: lanes per node (below) x
+// neighbours per lane in flight (1, 2, 2, 4, 1; the cases of launch_uu_node);
+// the autotuner picks one (EquationsSet::tuned_smoother).
+constexpr int kNodeVariants = 5;
+constexpr int kNodeVariantLanes[kNodeVariants] = {32, 32, 16, 16, 64};
+
+template <typename VT, int kSweep>
+static int launch_uu_node(int variant, int nq, const EquationsSet &s, const VT *vals, const double *b,
+                          const double *x, const double *inv_u, double omega, double *out)
+{
+    Runtime &rt = runtime();
+    if (variant < 0 || variant >= kNodeVariants) variant = 0;
+    const unsigned grid =
+        static_cast<unsigned>((static_cast<int64_t>(nq) * kNodeVariantLanes[variant] + 255) / 256);
+#define OC_UU_NODE(L, U)                                                                                       \
+    uu_node_residual_kernel<VT, L, kSweep, U><<<grid, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,   \
+                                                                         s.col_idx.ptr, vals, s.free_mask.ptr, \
+                                                                         b, x, inv_u, omega, out)
+    switch (variant) {
+    case 1: OC_UU_NODE(32, 2); break;
+    case 2: OC_UU_NODE(16, 2); break;
+    case 3: OC_UU_NODE(16, 4); break;
+    case 4: OC_UU_NODE(64, 1); break;
+    default: OC_UU_NODE(32, 1); break;
+    }
+#undef OC_UU_NODE
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
+int launch_smoother_residual(EquationsSet &s, int precision, int variant, const double *b, const double *x,
+                             double *r)
+{
+    if (s.node_rows != 1) return set_error(OC_ERR_UNSUPPORTED, "the node-wise smoother needs node rows");
+    if (precision == OC_FP32 && s.values_bf.ptr)
+        return launch_uu_node<Bf16, 0>(variant, s.n_nodes, s, reinterpret_cast<const Bf16 *>(s.values_bf.ptr), b, x,
+                                       nullptr, 0.0, r);
+    if (precision == OC_FP32) return launch_uu_node<float, 0>(variant, s.n_nodes, s, s.values_sp.ptr, b, x, nullptr, 0.0, r);
+    return launch_uu_node<double, 0>(variant, s.n_nodes, s, s.values.ptr, b, x, nullptr, 0.0, r);
+}
+
 // z (displacement part, component-major) = V-cycle applied to t
 int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
 {
@@ -825,22 +858,16 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     const int M = s.n_meshes, nq = s.n_nodes, nu = 3 * nq, nl = static_cast<int>(g.levels.size());
     const double *inv_u = s.precond.ptr;  // nodal block inverses of the set's block preconditioner
     const unsigned gu = static_cast<unsigned>((static_cast<int64_t>(nu) * 32 + 255) / 256);
-    const unsigned gn = group_grid(nq);
     const bool bf = precision == OC_FP32 && s.node_rows == 1 && s.values_bf.ptr;
     const Bf16 *vbf = reinterpret_cast<const Bf16 *>(s.values_bf.ptr);
+    const int variant = s.tuned_smoother;
     auto residual0 = [&](const double *b, const double *x, double *r) -> int {
         if (bf)
-            uu_node_residual_kernel<Bf16, 32, 0><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
-                                                                           s.col_idx.ptr, vbf, s.free_mask.ptr, b, x,
-                                                                           nullptr, 0.0, r);
+            OC_CHECK((launch_uu_node<Bf16, 0>(variant, nq, s, vbf, b, x, nullptr, 0.0, r)));
         else if (s.node_rows == 1 && precision == OC_FP32)
-            uu_node_residual_kernel<float, 32, 0><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
-                                                                            s.col_idx.ptr, s.values_sp.ptr,
-                                                                            s.free_mask.ptr, b, x, nullptr, 0.0, r);
+            OC_CHECK((launch_uu_node<float, 0>(variant, nq, s, s.values_sp.ptr, b, x, nullptr, 0.0, r)));
         else if (s.node_rows == 1)
-            uu_node_residual_kernel<double, 32, 0><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
-                                                                             s.col_idx.ptr, s.values.ptr,
-                                                                             s.free_mask.ptr, b, x, nullptr, 0.0, r);
+            OC_CHECK((launch_uu_node<double, 0>(variant, nq, s, s.values.ptr, b, x, nullptr, 0.0, r)));
         else if (precision == OC_FP32)
             uu_residual_kernel<float, 32><<<gu, 256, 0, rt.stream>>>(nu, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr,
                                                                     s.values_sp.ptr, s.free_mask.ptr, b, x, r);
@@ -858,20 +885,11 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     if (fused && ((std::max(g.pre - 1, 0) + g.post) & 1)) std::swap(cur, other);
     auto sweep0 = [&]() -> int {
         if (bf)
-            uu_node_residual_kernel<Bf16, 32, 1><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
-                                                                           s.col_idx.ptr, vbf, s.free_mask.ptr, t,
-                                                                           cur, inv_u, g.omega, other);
+            OC_CHECK((launch_uu_node<Bf16, 1>(variant, nq, s, vbf, t, cur, inv_u, g.omega, other)));
         else if (precision == OC_FP32)
-            uu_node_residual_kernel<float, 32, 1><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
-                                                                            s.col_idx.ptr, s.values_sp.ptr,
-                                                                            s.free_mask.ptr, t, cur, inv_u, g.omega,
-                                                                            other);
+            OC_CHECK((launch_uu_node<float, 1>(variant, nq, s, s.values_sp.ptr, t, cur, inv_u, g.omega, other)));
         else
-            uu_node_residual_kernel<double, 32, 1><<<gn, 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
-                                                                             s.col_idx.ptr, s.values.ptr,
-                                                                             s.free_mask.ptr, t, cur, inv_u, g.omega,
-                                                                             other);
-        OC_HIP_CHECK(hipGetLastError());
+            OC_CHECK((launch_uu_node<double, 1>(variant, nq, s, s.values.ptr, t, cur, inv_u, g.omega, other)));
         std::swap(cur, other);
         return OC_NO_ERROR;
     };

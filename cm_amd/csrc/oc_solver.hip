@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(256) node_pattern_check_kernel(int nq, const i
 // y = W (A x) (or A x when w is NULL) of a Mooney-Rivlin set with node rows:
 // groups g < nq take node g's three displacement rows (one read of the
 // neighbour and pressure column lists), groups nq + l take pressure row l
-template <typename VT, int kLanes>
+template <typename VT, int kLanes, int kU>
 __global__ void __launch_bounds__(256) spmv_node_kernel(int nq, int nl, const int64_t *__restrict__ row_ptr,
                                                        const int64_t *__restrict__ pstart,
                                                        const int *__restrict__ col_idx, const VT *__restrict__ vals,
@@ -333,23 +333,18 @@ __global__ void __launch_bounds__(256) spmv_node_kernel(int nq, int nl, const in
         return;
     }
     const int i = g;
-    const int64_t s0 = row_ptr[i], s1 = row_ptr[nq64 + i], s2 = row_ptr[2 * nq64 + i];
+    const int64_t sr[3] = {row_ptr[i], row_ptr[nq64 + i], row_ptr[2 * nq64 + i]};
     const int64_t p0 = pstart[i], p1 = pstart[nq64 + i], p2 = pstart[2 * nq64 + i];
-    const int deg = static_cast<int>((p0 - s0) / 3), dp = static_cast<int>(row_ptr[i + 1] - p0);
+    const int deg = static_cast<int>((p0 - sr[0]) / 3), dp = static_cast<int>(row_ptr[i + 1] - p0);
     const double w0 = w == nullptr ? 1.0 : w[i], w1 = w == nullptr ? 1.0 : w[nq64 + i],
                  w2 = w == nullptr ? 1.0 : w[2 * nq64 + i];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     if (w0 != 0.0 || w1 != 0.0 || w2 != 0.0) {
-        for (int p = lane; p < deg; p += kLanes) {
-            const int j = col_idx[s0 + p];
-            const double x0 = x[j], x1 = x[nq64 + j], x2 = x[2 * nq64 + j];
-            a0 += static_cast<double>(vals[s0 + p]) * x0 + static_cast<double>(vals[s0 + deg + p]) * x1 +
-                  static_cast<double>(vals[s0 + 2 * deg + p]) * x2;
-            a1 += static_cast<double>(vals[s1 + p]) * x0 + static_cast<double>(vals[s1 + deg + p]) * x1 +
-                  static_cast<double>(vals[s1 + 2 * deg + p]) * x2;
-            a2 += static_cast<double>(vals[s2 + p]) * x0 + static_cast<double>(vals[s2 + deg + p]) * x1 +
-                  static_cast<double>(vals[s2 + 2 * deg + p]) * x2;
-        }
+        double a[3] = {0.0, 0.0, 0.0};
+        node_rows_uu<VT, kLanes, kU>(sr, deg, lane, nq64, col_idx, vals, x, a);
+        a0 = a[0];
+        a1 = a[1];
+        a2 = a[2];
         for (int p = lane; p < dp; p += kLanes) {
             const double xp = x[col_idx[p0 + p]];
             a0 += static_cast<double>(vals[p0 + p]) * xp;
@@ -661,21 +656,27 @@ struct Krylov {
         return launch_spmv<double>(s.values.ptr, lanes, s.wgt.ptr, x, y);
     }
     template <typename VT>
-    int launch_spmv(const VT *vals, int lanes, const double *wv, const double *x, double *y)
+    int launch_spmv(const VT *vals, int lanes, const double *wv, const double *x, double *y, int unroll = -1)
     {
+        if (unroll < 0) unroll = s.tuned_spmv_unroll;
         Runtime &rt = runtime();
         if (s.cls == OC_EQUATIONS_SET_MOONEY_RIVLIN && s.node_rows == 1 && lanes >= 16) {
             const int nq = s.n_nodes, nl = s.n_nodes_l;
             const unsigned gn = static_cast<unsigned>((static_cast<int64_t>(nq + nl) * lanes + 255) / 256);
-            if (lanes == 16)
-                spmv_node_kernel<VT, 16><<<gn, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.pstart.ptr,
-                                                                    s.col_idx.ptr, vals, wv, x, y);
-            else if (lanes == 64)
-                spmv_node_kernel<VT, 64><<<gn, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.pstart.ptr,
-                                                                    s.col_idx.ptr, vals, wv, x, y);
-            else
-                spmv_node_kernel<VT, 32><<<gn, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.pstart.ptr,
-                                                                    s.col_idx.ptr, vals, wv, x, y);
+            const bool u2 = unroll == 2;
+#define OC_SPMV_NODE(L, U) \
+    spmv_node_kernel<VT, L, U><<<gn, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr, vals, wv, x, y)
+            if (lanes == 16) {
+                if (u2) OC_SPMV_NODE(16, 2);
+                else OC_SPMV_NODE(16, 1);
+            } else if (lanes == 64) {
+                if (u2) OC_SPMV_NODE(64, 2);
+                else OC_SPMV_NODE(64, 1);
+            } else {
+                if (u2) OC_SPMV_NODE(32, 2);
+                else OC_SPMV_NODE(32, 1);
+            }
+#undef OC_SPMV_NODE
             OC_HIP_CHECK(hipGetLastError());
             return OC_NO_ERROR;
         }
@@ -978,38 +979,47 @@ static int solver_prepare(EquationsSet &s, int restart, Krylov &K)
         K.diag_u = K.inv_p + n;
         K.inv_pt = K.diag_u + n;
         if (K.precond == OC_PRECONDITIONER_JACOBI) K.inv_u = K.inv_pt;  // n-sized point Jacobi
-        if (!s.pstart.ptr) {
-            const int nu = 3 * s.n_nodes;
-            OC_CHECK(s.pstart.alloc(nu));
-            pressure_start_kernel<<<static_cast<unsigned>((nu + 255) / 256), 256, 0, runtime().stream>>>(
-                nu, nu, s.row_ptr.ptr, s.col_idx.ptr, s.pstart.ptr);
-            OC_HIP_CHECK(hipGetLastError());
-        }
-        if (s.node_rows < 0) {
-            DevBuf<int32_t> bad_d;
-            OC_CHECK(bad_d.alloc(1));
-            Runtime &rt = runtime();
-            OC_HIP_CHECK(hipMemsetAsync(bad_d.ptr, 0, sizeof(int32_t), rt.stream));
-            node_pattern_check_kernel<<<static_cast<unsigned>((static_cast<int64_t>(s.n_nodes) * 32 + 255) / 256), 256,
-                                        0, rt.stream>>>(s.n_nodes, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr,
-                                                        bad_d.ptr);
-            OC_HIP_CHECK(hipGetLastError());
-            int32_t bad = 1;
-            OC_HIP_CHECK(hipMemcpyAsync(&bad, bad_d.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, rt.stream));
-            OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
-            s.node_rows = bad ? 0 : 1;
-        }
+        OC_CHECK(prepare_node_rows(s));
         K.pstart = s.pstart.ptr;
     }
     return OC_NO_ERROR;
 }
 
-int launch_spmv_lanes(EquationsSet &s, bool fp32, int lanes, const double *x, double *y)
+// First pressure column of every displacement row, and whether the set's rows
+// have the node layout the node-wise kernels read (once per set)
+int prepare_node_rows(EquationsSet &s)
+{
+    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN) return OC_NO_ERROR;
+    Runtime &rt = runtime();
+    if (!s.pstart.ptr) {
+        const int nu = 3 * s.n_nodes;
+        OC_CHECK(s.pstart.alloc(nu));
+        pressure_start_kernel<<<static_cast<unsigned>((nu + 255) / 256), 256, 0, rt.stream>>>(
+            nu, nu, s.row_ptr.ptr, s.col_idx.ptr, s.pstart.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+    }
+    if (s.node_rows < 0) {
+        DevBuf<int32_t> bad_d;
+        OC_CHECK(bad_d.alloc(1));
+        OC_HIP_CHECK(hipMemsetAsync(bad_d.ptr, 0, sizeof(int32_t), rt.stream));
+        node_pattern_check_kernel<<<static_cast<unsigned>((static_cast<int64_t>(s.n_nodes) * 32 + 255) / 256), 256,
+                                    0, rt.stream>>>(s.n_nodes, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr,
+                                                    bad_d.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+        int32_t bad = 1;
+        OC_HIP_CHECK(hipMemcpyAsync(&bad, bad_d.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, rt.stream));
+        OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+        s.node_rows = bad ? 0 : 1;
+    }
+    return OC_NO_ERROR;
+}
+
+int launch_spmv_lanes(EquationsSet &s, bool fp32, int lanes, const double *x, double *y, int unroll)
 {
     Krylov K{s};
     K.n = s.n_rows;
-    if (fp32) return K.launch_spmv<float>(s.values_sp.ptr, lanes, nullptr, x, y);
-    return K.launch_spmv<double>(s.values.ptr, lanes, nullptr, x, y);
+    if (fp32) return K.launch_spmv<float>(s.values_sp.ptr, lanes, nullptr, x, y, unroll);
+    return K.launch_spmv<double>(s.values.ptr, lanes, nullptr, x, y, unroll);
 }
 
 // Per-mesh norms of x over free rows (Mooney-Rivlin sets), one device->host copy.

