@@ -177,6 +177,7 @@ SIGNATURES = {
     "oc_mooney_rivlin_batch_create": [_c_int, _i32, _i32, _i32, _dp, _i32, _i32, _dp, _dp, _c_int,
                                       ctypes.POINTER(_H)],
     "oc_mesh_count": [_H, ctypes.POINTER(_c_int)],
+    "oc_residual_evaluate_noalloc": [_H, ctypes.c_void_p, ctypes.c_void_p],
     "oc_mesh_matrix_get": [_H, _c_int, _c_int, _c_int, ctypes.POINTER(_c_int), ctypes.POINTER(ctypes.c_longlong),
                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "oc_equations_set_mesh_get": [_H, _c_int] + [ctypes.POINTER(_c_int)] * 6,

@@ -174,6 +174,11 @@ int oc_initialise(int device)
     OC_HIP_CHECK(hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking));
     for (auto &e : rt.events) OC_HIP_CHECK(hipEventCreate(&e));
     rt.device = device;
+    // device-side set-up that later calls must not allocate: the basis tables
+    // and the address of this device's copy of the allocation-free scratch
+    const BasisTables *T = nullptr;
+    OC_CHECK(ensure_tables(&T));
+    OC_CHECK(noalloc_scratch_init(device));
     return OC_NO_ERROR;
 }
 

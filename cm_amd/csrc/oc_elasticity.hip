@@ -1156,6 +1156,31 @@ int launch_mr_jacobian(EquationsSet &s, int precision)
     return launch_mr_row_gather(s, precision, s.tuned_gather_chunk);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free residual evaluation (oc_residual_evaluate_noalloc): the
+// element residuals go through `scratch`, a fixed array in device memory (one
+// copy per GPU, held until the process ends) instead of the set's buffer.
+// Host code never passes the symbol itself to a kernel (on the host it only
+// stands for the device variable): it uses the address hipGetSymbolAddress
+// returned for the device, kept in g_scratch_ptr.  Every element residual
+// entry is written before the DOF gather reads it, so nothing is cleared
+// between calls; calls that overlap on one GPU share the array and race.
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[static_cast<size_t>(OC_NOALLOC_MAX_ELEMENTS) * 89];
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return set_error(OC_ERR_UNSUPPORTED, "device %d beyond the %d devices of the scratch table", device, kMaxDevices);
+    if (g_scratch_ptr[device]) return OC_NO_ERROR;
+    void *p = nullptr;
+    OC_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_error(OC_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return OC_NO_ERROR;
+}
+
 // (c10, c01) of every quadratic node from its elements (all elements around
 // a node belong to one mesh, so every writer of an entry writes the same pair)
 __global__ void __launch_bounds__(256) node_materials_kernel(int n_elements, const int *__restrict__ conn_q,
@@ -1180,3 +1205,35 @@ int mr_node_materials(EquationsSet &s)
 }
 
 }  // namespace oc
+
+using namespace oc;
+
+extern "C" int oc_residual_evaluate_noalloc(oc_equations_set h, const double *dofs_device, double *residual_device)
+{
+    OC_CHECK(require_runtime());
+    if (!h) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
+    if (!dofs_device || !residual_device)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "oc_residual_evaluate_noalloc: NULL device array");
+    EquationsSet &s = h->s;
+    if (s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "oc_residual_evaluate_noalloc is defined for Mooney-Rivlin sets");
+    if (s.n_elements > OC_NOALLOC_MAX_ELEMENTS)
+        return set_error(OC_ERR_UNSUPPORTED,
+                         "%d elements exceed the %d elements of the allocation-free residual's scratch; use "
+                         "oc_equations_set_residual_evaluate",
+                         s.n_elements, OC_NOALLOC_MAX_ELEMENTS);
+    Runtime &rt = runtime();
+    const BasisTables *T = device_basis_tables();
+    double *scr = (rt.device >= 0 && rt.device < kMaxDevices) ? g_scratch_ptr[rt.device] : nullptr;
+    if (!T || !scr || !s.gather_ptr.ptr)
+        return set_error(OC_ERR_NOT_INITIALISED, "device tables, scratch or the set's gather lists are not set up");
+    const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
+    mr_residual_element_kernel<false><<<blocks, kResThreads, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
+                                                                             s.coords.ptr, s.n_nodes, dofs_device,
+                                                                             s.mat.ptr, T, scr, 0.0, nullptr);
+    OC_HIP_CHECK(hipGetLastError());
+    dof_gather_kernel<<<static_cast<unsigned>((s.n_rows + 255) / 256), 256, 0, rt.stream>>>(
+        s.n_rows, s.gather_ptr.ptr, s.gather_idx.ptr, scr, residual_device);
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}

@@ -275,6 +275,10 @@ struct EquationsSet {
 // ---- shared device resources (oc_assembly.hip) --------------------------------
 struct BasisTables;
 int ensure_tables(const BasisTables **out);
+const BasisTables *device_basis_tables();
+// address of this device's allocation-free scratch (oc_elasticity.hip), looked
+// up once per device with the device current
+int noalloc_scratch_init(int device);
 template <typename T>
 int fill(T *p, int64_t n, T v);
 

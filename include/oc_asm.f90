@@ -28,6 +28,7 @@ MODULE OC_ASM
       & OC_CONVERGED_SNORM_RELATIVE = 4, OC_DIVERGED_FUNCTION_COUNT = -2, OC_DIVERGED_LINEAR_SOLVE = -3, &
       & OC_DIVERGED_FNORM_NAN = -4, OC_DIVERGED_MAX_IT = -5, OC_DIVERGED_LINE_SEARCH = -6
   INTEGER(C_INT), PARAMETER :: OC_SOLVER_NO_OUTPUT = 0, OC_SOLVER_PROGRESS_OUTPUT = 1
+  INTEGER(C_INT), PARAMETER :: OC_NOALLOC_MAX_ELEMENTS = 47000
 
   !> oc_newton_options (field order and types as in oc_asm.h)
   TYPE, BIND(C) :: OC_NEWTON_OPTIONS
@@ -75,6 +76,12 @@ MODULE OC_ASM
       INTEGER(C_INT), INTENT(OUT) :: n_rows
       INTEGER(C_LONG_LONG), INTENT(OUT) :: nnz
     END FUNCTION oc_mesh_matrix_get
+
+    INTEGER(C_INT) FUNCTION oc_residual_evaluate_noalloc(handle, dofs_device, residual_device) &
+        & BIND(C, NAME="oc_residual_evaluate_noalloc")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: handle, dofs_device, residual_device
+    END FUNCTION oc_residual_evaluate_noalloc
 
     INTEGER(C_INT) FUNCTION oc_laplace_create(n_nodes, coords, n_elements, elem_nodes, index_base, handle) &
         & BIND(C, NAME="oc_laplace_create")

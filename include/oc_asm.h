@@ -182,6 +182,20 @@ int oc_equations_set_jacobian_evaluate_fd(oc_equations_set equations_set);
  * change).  Results are identical to oc_equations_set_jacobian_evaluate. */
 int oc_jacobian_evaluate_graph(oc_equations_set equations_set, int precision);
 
+/* Residual evaluation that allocates nothing, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  R(x)
+ * of a Mooney-Rivlin set, as oc_equations_set_residual_evaluate computes it
+ * (same kernels, same values), with x = dofs_device and the result in
+ * residual_device: caller-owned device arrays of n_rows doubles in the set's
+ * DOF order.  The element residuals pass through a static device array of the
+ * library (OC_NOALLOC_MAX_ELEMENTS * 89 doubles, 31.9 MiB, one per GPU, held
+ * until the process ends), so sets of more than OC_NOALLOC_MAX_ELEMENTS
+ * elements fail with OC_ERR_UNSUPPORTED.  Calls that overlap on one GPU share
+ * that array and race; calls on the library stream are ordered. */
+#define OC_NOALLOC_MAX_ELEMENTS 47000
+int oc_residual_evaluate_noalloc(oc_equations_set equations_set, const double *dofs_device, double *residual_device);
+
 /* ---- copy-out / borrow ----------------------------------------------------- */
 int oc_matrix_values_get(oc_equations_set equations_set, double *values);
 int oc_matrix_values_get_sp(oc_equations_set equations_set, float *values);
