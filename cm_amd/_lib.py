@@ -25,6 +25,8 @@ OC_ERR_UNSUPPORTED = -8
 
 OC_EQUATIONS_SET_LAPLACE = 1
 OC_EQUATIONS_SET_MOONEY_RIVLIN = 2
+OC_MOONEY_RIVLIN_SUBTYPE = 1
+OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE = 21
 OC_FP64 = 0
 OC_FP32 = 1
 OC_BF16 = 2  # MLP inference only
@@ -177,6 +179,7 @@ SIGNATURES = {
     "oc_mooney_rivlin_batch_create": [_c_int, _i32, _i32, _i32, _dp, _i32, _i32, _dp, _dp, _c_int,
                                       ctypes.POINTER(_H)],
     "oc_mesh_count": [_H, ctypes.POINTER(_c_int)],
+    "oc_equations_set_subtype_set": [_H, _c_int],
     "oc_residual_evaluate_noalloc": [_H, ctypes.c_void_p, ctypes.c_void_p],
     "oc_mesh_matrix_get": [_H, _c_int, _c_int, _c_int, ctypes.POINTER(_c_int), ctypes.POINTER(ctypes.c_longlong),
                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],

@@ -400,6 +400,23 @@ int oc_mesh_matrix_get(oc_equations_set h, int mesh, int precision, int index_ba
     return OC_NO_ERROR;
 }
 
+int oc_equations_set_subtype_set(oc_equations_set h, int subtype)
+{
+    OC_CHECK(check_handle(h));
+    if (h->s.cls != OC_EQUATIONS_SET_MOONEY_RIVLIN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "subtypes are defined for Mooney-Rivlin sets");
+    if (subtype != OC_MOONEY_RIVLIN_SUBTYPE && subtype != OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "unknown Mooney-Rivlin subtype %d", subtype);
+    EquationsSet &s = h->s;
+    if (s.subtype != subtype && s.step_graph) {  // the Newton-step graph holds the other subtype's kernels
+        OC_HIP_CHECK(hipStreamSynchronize(g_runtime.stream));
+        (void)hipGraphExecDestroy(s.step_graph);
+        s.step_graph = nullptr;
+    }
+    s.subtype = subtype;
+    return OC_NO_ERROR;
+}
+
 int oc_mesh_count(oc_equations_set h, int *n_meshes)
 {
     OC_CHECK(check_handle(h));

@@ -262,7 +262,7 @@ int launch_mr_gauss(EquationsSet &s)
 // residual work items has a thread of its own.
 constexpr int kResThreads = 320;
 
-template <bool kFD>
+template <bool kFD, int kSub>
 __global__ void __launch_bounds__(kResThreads)
 mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const int *__restrict__ conn_l,
                            const double *__restrict__ coords, int nq, const double *__restrict__ dofs,
@@ -340,13 +340,21 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
             gauss_kinematics(s_tab, g, s_z[le], s_X[le], k);
             double P = 0.0;
             for (int n = 0; n < 8; ++n) P += T->lin27[(g * 8 + n) * 4] * s_p[le][n];
-            double sigma[6];
-            mr_cauchy_stress(k.F, k.Jznu, P, mat[2 * e], mat[2 * e + 1], sigma);
-            const double jgw = k.jz * T->w27[g];
+            double sigma[6], jgw, J;
+            if (kSub == OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE) {
+                // the 4-component residual loop of the subtype (:1654-1761):
+                // weight w Jxxi Jznu, Jznu from the stress routine
+                J = imr_cauchy_stress(k.F, P, mat[2 * e], mat[2 * e + 1], sigma);
+                jgw = T->w27[g] * k.jx * J;
+            } else {
+                mr_cauchy_stress(k.F, k.Jznu, P, mat[2 * e], mat[2 * e + 1], sigma);
+                jgw = k.jz * T->w27[g];
+                J = k.Jznu;
+            }
             for (int i = 0; i < 9; ++i) rdx[i] = k.dxidz[i];
             for (int a = 0; a < 3; ++a)
                 for (int i = 0; i < 3; ++i) rS[i + 3 * a] = jgw * sigma[t2v(i, a)];
-            rrp = k.jx * T->w27[g] * (k.Jznu - 1.0);
+            rrp = k.jx * T->w27[g] * (J - 1.0);
         }
         __syncthreads();  // X and z consumed: their LDS takes the results
         if (active) {
@@ -411,6 +419,24 @@ dof_gather_kernel(int n_rows, const int *__restrict__ ptr, const int *__restrict
     out[d] = acc;
 }
 
+// element residuals of the set's subtype into relem (kFD: the forward-
+// difference columns, `blocks` covering every (element, column) pair)
+template <bool kFD>
+static int launch_residual_elements(const EquationsSet &s, unsigned blocks, const double *dofs,
+                                    const BasisTables *T, double *relem, double delta, double *kfd)
+{
+    Runtime &rt = runtime();
+    const int nv = kFD ? s.n_elements * 89 : s.n_elements;
+    if (s.subtype == OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE)
+        mr_residual_element_kernel<kFD, OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE><<<blocks, kResThreads, 0, rt.stream>>>(
+            nv, s.conn.ptr, s.conn_l.ptr, s.coords.ptr, s.n_nodes, dofs, s.mat.ptr, T, relem, delta, kfd);
+    else
+        mr_residual_element_kernel<kFD, OC_MOONEY_RIVLIN_SUBTYPE><<<blocks, kResThreads, 0, rt.stream>>>(
+            nv, s.conn.ptr, s.conn_l.ptr, s.coords.ptr, s.n_nodes, dofs, s.mat.ptr, T, relem, delta, kfd);
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
 int launch_mr_residual(EquationsSet &s)
 {
     OC_CHECK(mr_prepare(s));
@@ -420,10 +446,7 @@ int launch_mr_residual(EquationsSet &s)
     const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
     {
         KernelTimer timer("mr_residual");
-        mr_residual_element_kernel<false><<<blocks, kResThreads, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
-                                                                         s.coords.ptr, s.n_nodes, s.dofs.ptr,
-                                                                         s.mat.ptr, T, s.relem.ptr, 0.0, nullptr);
-        OC_HIP_CHECK(hipGetLastError());
+        OC_CHECK(launch_residual_elements<false>(s, blocks, s.dofs.ptr, T, s.relem.ptr, 0.0, nullptr));
     }
     {
         KernelTimer timer("dof_gather");
@@ -1131,25 +1154,26 @@ int launch_mr_jacobian_fd(EquationsSet &s)
     for (double v : x) ss += v * v;
     const double delta = (1.0 + std::sqrt(ss)) * 1e-7;
     const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
-    mr_residual_element_kernel<false><<<blocks, kResThreads, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
-                                                                     s.coords.ptr, s.n_nodes, s.dofs.ptr, s.mat.ptr,
-                                                                     T, s.relem.ptr, 0.0, nullptr);
-    OC_HIP_CHECK(hipGetLastError());
+    OC_CHECK(launch_residual_elements<false>(s, blocks, s.dofs.ptr, T, s.relem.ptr, 0.0, nullptr));
     const int64_t nv = static_cast<int64_t>(s.n_elements) * 89;
     if (nv >= INT32_MAX) return set_error(OC_ERR_UNSUPPORTED, "too many elements for the finite-difference launch");
     {
         KernelTimer timer("mr_fd_columns");
         const unsigned fb = static_cast<unsigned>((nv + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
-        mr_residual_element_kernel<true><<<fb, kResThreads, 0, rt.stream>>>(static_cast<int>(nv), s.conn.ptr, s.conn_l.ptr,
-                                                                    s.coords.ptr, s.n_nodes, s.dofs.ptr, s.mat.ptr, T,
-                                                                    s.relem.ptr, delta, s.kbuf64.ptr);
-        OC_HIP_CHECK(hipGetLastError());
+        OC_CHECK(launch_residual_elements<true>(s, fb, s.dofs.ptr, T, s.relem.ptr, delta, s.kbuf64.ptr));
     }
     return launch_mr_row_gather(s, OC_FP64, s.tuned_gather_chunk);
 }
 
 int launch_mr_jacobian(EquationsSet &s, int precision)
 {
+    if (s.subtype == OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE) {
+        // the reference evaluates this subtype's Jacobian by forward differences
+        if (precision != OC_FP64)
+            return set_error(OC_ERR_UNSUPPORTED, "the incompressible Mooney-Rivlin subtype has FP64 "
+                                                 "forward-difference Jacobians only");
+        return launch_mr_jacobian_fd(s);
+    }
     OC_CHECK(mr_jacobian_buffers(s, precision));
     OC_CHECK(launch_mr_gauss(s));
     OC_CHECK(launch_mr_element_matrices(s, precision, s.tuned_element_blocks_per_cu));
@@ -1228,10 +1252,7 @@ extern "C" int oc_residual_evaluate_noalloc(oc_equations_set h, const double *do
     if (!T || !scr || !s.gather_ptr.ptr)
         return set_error(OC_ERR_NOT_INITIALISED, "device tables, scratch or the set's gather lists are not set up");
     const unsigned blocks = static_cast<unsigned>((s.n_elements + kGaussElemsPerBlock - 1) / kGaussElemsPerBlock);
-    mr_residual_element_kernel<false><<<blocks, kResThreads, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.conn_l.ptr,
-                                                                             s.coords.ptr, s.n_nodes, dofs_device,
-                                                                             s.mat.ptr, T, scr, 0.0, nullptr);
-    OC_HIP_CHECK(hipGetLastError());
+    OC_CHECK(launch_residual_elements<false>(s, blocks, dofs_device, T, scr, 0.0, nullptr));
     dof_gather_kernel<<<static_cast<unsigned>((s.n_rows + 255) / 256), 256, 0, rt.stream>>>(
         s.n_rows, s.gather_ptr.ptr, s.gather_idx.ptr, scr, residual_device);
     OC_HIP_CHECK(hipGetLastError());

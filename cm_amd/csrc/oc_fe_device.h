@@ -183,6 +183,40 @@ __device__ __forceinline__ void mr_cauchy_stress(const double *F, double Jznu, d
     for (int i = 0; i < 3; ++i) sigma[i] = sigma[i] - ot + p;
 }
 
+// FINITE_ELASTICITY_GAUSS_CAUCHY_TENSOR, INCOMPRESSIBLE_MOONEY_RIVLIN branch
+// (finite_elasticity_routines.f90:3476-3481, 3522-3549, 4316-4322): AZL =
+// F^T F, AZU = inv(AZL) with I3 = det AZL and Jznu = sqrt(I3); I1 = tr AZL,
+// I2 = (I1^2 - tr AZL^2)/2; S = 2 Jznu^(-2/3) ((c1 + c2 I1) I - c2 AZL
+// - (c1 I1 + 2 c2 I2 - 3/2 p Jznu^(5/3))/3 AZU); sigma = F S F^T / Jznu
+// (Voigt).  Returns Jznu.
+__device__ __forceinline__ double imr_cauchy_stress(const double *F, double p, double c10, double c01,
+                                                    double *sigma)
+{
+    double azl[9], azu[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            azl[i + 3 * j] = F[0 + 3 * i] * F[0 + 3 * j] + F[1 + 3 * i] * F[1 + 3 * j] + F[2 + 3 * i] * F[2 + 3 * j];
+    const double J = sqrt(invert3(azl, azu));
+    const double I1 = azl[0] + azl[4] + azl[8];
+    double sq[3];
+    for (int i = 0; i < 3; ++i) sq[i] = azl[i] * azl[3 * i] + azl[i + 3] * azl[1 + 3 * i] + azl[i + 6] * azl[2 + 3 * i];
+    const double I2 = 0.5 * (I1 * I1 - sq[0] - sq[1] - sq[2]);
+    const double coef = 2.0 * pow(J, -2.0 / 3.0);
+    const double t = (c10 * I1 + 2.0 * c01 * I2 - 1.5 * p * pow(J, 5.0 / 3.0)) / 3.0;
+    double S[9], FS[9];
+    for (int k = 0; k < 9; ++k) S[k] = coef * (((k % 4) == 0 ? c10 + c01 * I1 : 0.0) - c01 * azl[k] - t * azu[k]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            FS[i + 3 * j] = F[i] * S[3 * j] + F[i + 3] * S[1 + 3 * j] + F[i + 6] * S[2 + 3 * j];
+    const int va[6] = {0, 1, 2, 0, 0, 1}, vb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        const int i = va[a], j = vb[a];
+        sigma[a] = (FS[i] * F[j] + FS[i + 3] * F[j + 3] + FS[i + 6] * F[j + 6]) / J;
+    }
+    return J;
+}
+
 // FINITE_ELASTICITY_GAUSS_ELASTICITY_TENSOR, MR branch
 // (finite_elasticity_routines.f90:609-740): spatial elasticity tensor c
 // (6x6 column-major) and Cauchy stress sigma (Voigt, pressure included).

@@ -169,6 +169,7 @@ struct EquationsSet {
     int n_nodes = 0;         // nodes of the main component
     int n_nodes_l = 0;       // pressure component nodes (MR)
     double c10 = 0.0, c01 = 0.0;  // materials of a single-mesh set
+    int subtype = OC_MOONEY_RIVLIN_SUBTYPE;  // Mooney-Rivlin sets: constitutive subtype
     int n_meshes = 1;
     std::vector<int32_t> h_q_off, h_l_off, h_e_off;  // n_meshes + 1 each
     DevBuf<int32_t> q_off, l_off;

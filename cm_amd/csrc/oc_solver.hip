@@ -1056,6 +1056,12 @@ static int residual_mesh_norms(EquationsSet &s, std::vector<double> &out)
 static int step_assembly(EquationsSet &s, Krylov *K, int precision)
 {
     Runtime &rt = runtime();
+    if (s.subtype == OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE) {
+        // forward-difference Jacobians take their step size from the host:
+        // launched eagerly, not recorded
+        OC_CHECK(launch_mr_jacobian(s, precision));
+        return K ? K->setup_preconditioner() : OC_NO_ERROR;
+    }
     // the preconditioner kernels read the free-DOF mask, so a graph belongs to
     // one set of boundary conditions
     const long long key[5] = {precision, K ? K->precond : -1, s.tuned_element_blocks_per_cu, s.tuned_gather_chunk,

@@ -243,6 +243,13 @@ class MooneyRivlinEquationsSet(EquationsSet):
         call("oc_materials_set", self._h, float(c10), float(c01))
         self.c10, self.c01 = float(c10), float(c01)
 
+    def set_subtype(self, subtype):
+        """OC_MOONEY_RIVLIN_SUBTYPE (analytic Jacobian) or
+        OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE (2nd Piola form, forward-
+        difference FP64 Jacobian)."""
+        call("oc_equations_set_subtype_set", self._h, int(subtype))
+        self.subtype = int(subtype)
+
     def undeformed_dofs(self):
         """z = X and p = 0 (the initial dependent field)."""
         nq = self.mesh.n_nodes

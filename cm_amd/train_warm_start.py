@@ -213,6 +213,8 @@ def main():
     ap.add_argument("--big", type=int, default=0, help="also evaluate the n^3 case with 20%% stretch in 10 steps")
     ap.add_argument("--big-steps", type=int, default=10)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--krylov-rtol", type=float, default=PLAIN_KRYLOV_RTOL,
+                    help="GMRES forcing of both methods in the evaluation (<= 0: Eisenstat-Walker)")
     ap.add_argument("--eval-only", action="store_true", help="evaluate the weights at --out without training")
     ap.add_argument("--batched", action="store_true",
                     help="generate the data with batched solves (256 meshes per equations set)")
@@ -243,8 +245,8 @@ def main():
     rng = np.random.default_rng(args.seed + 7919)
     for k in range(args.heldout):
         case = draw_case(rng, 4, 16)
-        plain = summarise(solve_steps(case, args.steps))
-        learned = summarise(solve_steps(case, args.steps, net=net, precision=_lib.OC_FP32))
+        plain = summarise(solve_steps(case, args.steps, krylov_rtol=args.krylov_rtol))
+        learned = summarise(solve_steps(case, args.steps, net=net, precision=_lib.OC_FP32, krylov_rtol=args.krylov_rtol))
         report["heldout"].append(dict(case=case, plain_fp64=plain, learned_fp32=learned))
         log(f"heldout {case}: plain {plain} | learned {learned}")
         if args.report:
@@ -252,8 +254,9 @@ def main():
                 json.dump(report, f, indent=1)
     if args.big:
         case = dict(n=args.big, c10=2.0, c01=6.0, stretch=0.20, seed=2024)
-        learned = solve_steps(case, args.big_steps, net=net, precision=_lib.OC_FP32, log=log)
-        plain = solve_steps(case, args.big_steps, log=log)
+        learned = solve_steps(case, args.big_steps, net=net, precision=_lib.OC_FP32, log=log,
+                              krylov_rtol=args.krylov_rtol)
+        plain = solve_steps(case, args.big_steps, log=log, krylov_rtol=args.krylov_rtol)
         report["big"] = dict(case=case, plain_fp64=summarise(plain), learned_fp32=summarise(learned),
                              plain_rows=plain, learned_rows=learned)
         log(f"big: plain {summarise(plain)} | learned {summarise(learned)}")

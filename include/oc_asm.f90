@@ -17,6 +17,7 @@ MODULE OC_ASM
   INTEGER(C_INT), PARAMETER :: OC_ERR_NOT_INITIALISED = -7
   INTEGER(C_INT), PARAMETER :: OC_ERR_UNSUPPORTED = -8
   INTEGER(C_INT), PARAMETER :: OC_FP64 = 0, OC_FP32 = 1, OC_BF16 = 2
+  INTEGER(C_INT), PARAMETER :: OC_MOONEY_RIVLIN_SUBTYPE = 1, OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE = 21
   INTEGER(C_INT), PARAMETER :: OC_EQUATIONS_SET_LAPLACE = 1, OC_EQUATIONS_SET_MOONEY_RIVLIN = 2
   INTEGER(C_INT), PARAMETER :: OC_BUFFER_MATRIX_VALUES = 0, OC_BUFFER_MATRIX_VALUES_SP = 1, &
       & OC_BUFFER_ROW_PTR = 2, OC_BUFFER_COL_IDX = 3, OC_BUFFER_DOFS = 4, OC_BUFFER_RESIDUAL = 5, OC_BUFFER_RHS = 6, &
@@ -82,6 +83,12 @@ MODULE OC_ASM
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: handle, dofs_device, residual_device
     END FUNCTION oc_residual_evaluate_noalloc
+
+    INTEGER(C_INT) FUNCTION oc_equations_set_subtype_set(handle, subtype) BIND(C, NAME="oc_equations_set_subtype_set")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: handle
+      INTEGER(C_INT), VALUE :: subtype
+    END FUNCTION oc_equations_set_subtype_set
 
     INTEGER(C_INT) FUNCTION oc_laplace_create(n_nodes, coords, n_elements, elem_nodes, index_base, handle) &
         & BIND(C, NAME="oc_laplace_create")

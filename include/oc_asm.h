@@ -49,6 +49,10 @@ extern "C" {
 #define OC_EQUATIONS_SET_LAPLACE 1         /* classical field, Laplace, standard subtype */
 #define OC_EQUATIONS_SET_MOONEY_RIVLIN 2   /* elasticity, finite elasticity, MR subtype */
 
+/* ---- Mooney-Rivlin subtypes (equations_set_constants.f90:121, 141) -------- */
+#define OC_MOONEY_RIVLIN_SUBTYPE 1                  /* analytic Jacobian (the default) */
+#define OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE 21  /* PK2 form, forward-difference Jacobian */
+
 /* ---- precision flags ----------------------------------------------------- */
 #define OC_FP64 0
 #define OC_FP32 1
@@ -143,6 +147,15 @@ int oc_equations_set_info(oc_equations_set equations_set, int *set_class, int *n
                           int *n_elements, int *n_colours);
 /* Materials (c10, c01) of a Mooney-Rivlin set (every mesh of a batch). */
 int oc_materials_set(oc_equations_set equations_set, double c10, double c01);
+/* Constitutive subtype of a Mooney-Rivlin set (every mesh of a batch):
+ * OC_MOONEY_RIVLIN_SUBTYPE (EQUATIONS_SET_MOONEY_RIVLIN_SUBTYPE: deviatoric
+ * Cauchy stress plus p, analytic Jacobian) or
+ * OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE (EQUATIONS_SET_INCOMPRESSIBLE_
+ * MOONEY_RIVLIN_SUBTYPE: Cauchy stress pushed forward from the 2nd Piola tensor
+ * with the pressure inside it, finite_elasticity_routines.f90:3522-3549, in the
+ * generic residual loop :1654-1761; its Jacobian is the forward-difference one,
+ * FP64 only, as the reference evaluates it). */
+int oc_equations_set_subtype_set(oc_equations_set equations_set, int subtype);
 /* Per-mesh materials of a batch: c10[n_meshes], c01[n_meshes]. */
 int oc_mesh_materials_set(oc_equations_set equations_set, const double *c10, const double *c01);
 

@@ -714,21 +714,115 @@ int oco_mr_element_jacobian(const double *Xe, const double *ze, const double *pe
 /* EquationsSet_FiniteElementJacobianEvaluateFD, equations_set_routines.f90:
  * 3032-3150: column k = (R(x + delta e_k) - R(x)) / delta, columns in
  * component -> node order. */
-int oco_mr_element_jacobian_fd(const double *Xe, const double *ze, const double *pe,
-                               double c10, double c01, double delta, double *Je)
+typedef int (*element_residual_fn)(const double *, const double *, const double *, double, double, double *);
+
+static void element_jacobian_fd(element_residual_fn res, const double *Xe, const double *ze, const double *pe,
+                                double c10, double c01, double delta, double *Je)
 {
     double R0[89], R1[89], z[81], p[8];
-    oco_mr_element_residual(Xe, ze, pe, c10, c01, R0);
+    res(Xe, ze, pe, c10, c01, R0);
     memcpy(z, ze, sizeof z);
     memcpy(p, pe, sizeof p);
     for (int col = 0; col < 89; ++col) {
         double *v = col < 81 ? &z[col] : &p[col - 81];
         const double orig = *v;
         *v = orig + delta;
-        oco_mr_element_residual(Xe, z, p, c10, c01, R1);
+        res(Xe, z, p, c10, c01, R1);
         *v = orig;
         for (int r = 0; r < 89; ++r) Je[r * 89 + col] = (R1[r] - R0[r]) / delta;
     }
+}
+
+int oco_mr_element_jacobian_fd(const double *Xe, const double *ze, const double *pe,
+                               double c10, double c01, double delta, double *Je)
+{
+    element_jacobian_fd(oco_mr_element_residual, Xe, ze, pe, c10, c01, delta, Je);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Incompressible Mooney-Rivlin subtype                                      */
+/* (EQUATIONS_SET_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE): Cauchy stress from  */
+/* the 2nd Piola tensor with the pressure inside it, the generic 4-component */
+/* residual, and the forward-difference Jacobian the reference uses for it.  */
+
+/* FINITE_ELASTICITY_GAUSS_CAUCHY_TENSOR, INCOMPRESSIBLE_MOONEY_RIVLIN branch,
+ * finite_elasticity_routines.f90:3476-3481 (AZL = F^T F, AZU = AZL^-1,
+ * I3 = det AZL, Jznu = I3^(1/2)), :3522-3549 (I1 = tr AZL,
+ * I2 = (I1^2 - tr AZL^2)/2, S = 2 Jznu^(-2/3) ((c1 + c2 I1) I - c2 AZL
+ * - (c1 I1 + 2 c2 I2 - 3/2 p Jznu^(5/3))/3 AZU)) and :4316-4322
+ * (sigma = F S F^T / Jznu).  sigma in Voigt order; returns Jznu in *jznu. */
+int oco_imr_stress(const double *F, double p, double c10, double c01, double *sigma, double *jznu)
+{
+    double azl[9], azu[9], i3, S[9], tmp[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < 3; ++k) v += F[k + 3 * i] * F[k + 3 * j];
+            azl[i + 3 * j] = v;
+        }
+    oco_invert3(azl, azu, &i3);
+    const double J = pow(i3, 0.5);
+    const double I1 = azl[0] + azl[4] + azl[8];
+    double sq[3];
+    for (int i = 0; i < 3; ++i) {
+        double v = 0.0;
+        for (int k = 0; k < 3; ++k) v += azl[i + 3 * k] * azl[k + 3 * i];
+        sq[i] = v;
+    }
+    const double I2 = 0.5 * (I1 * I1 - sq[0] - sq[1] - sq[2]);
+    const double coef = 2.0 * pow(J, -2.0 / 3.0);
+    const double t = (c10 * I1 + 2.0 * c01 * I2 - 1.5 * p * pow(J, 5.0 / 3.0)) / 3.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            S[i + 3 * j] = coef * ((i == j ? c10 + c01 * I1 : 0.0) - c01 * azl[i + 3 * j] - t * azu[i + 3 * j]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < 3; ++k) v += F[i + 3 * k] * S[k + 3 * j];
+            tmp[i + 3 * j] = v;
+        }
+    for (int a = 0; a < 6; ++a) {
+        const int i = V2T[a][0], j = V2T[a][1];
+        double v = 0.0;
+        for (int k = 0; k < 3; ++k) v += tmp[i + 3 * k] * F[j + 3 * k];
+        sigma[a] = v / J;
+    }
+    *jznu = J;
+    return 0;
+}
+
+/* FiniteElasticity_FiniteElementResidualEvaluate, the 4-component branch that
+ * the incompressible MR subtype takes (finite_elasticity_routines.f90:
+ * 1654-1761): per Gauss point R(c, m) += w Jxxi Jznu sigma(c, c2)
+ * DFDZ(m, c2) for c2 = 1..3 (DFDZ = dphi/dxi . inv(dz/dxi), FINITE_ELASTICITY_
+ * GAUSS_DFDZ :4572-4643) and R_p(m) += w Jxxi psi_m (Jznu - 1), with Jxxi the
+ * undeformed volume Jacobian and Jznu from the stress routine. */
+int oco_imr_element_residual(const double *Xe, const double *ze, const double *pe,
+                             double c10, double c01, double *Re)
+{
+    tables_init();
+    for (int i = 0; i < 89; ++i) Re[i] = 0.0;
+    for (int g = 0; g < 27; ++g) {
+        gauss_state_t s;
+        gauss_state(g, Xe, ze, pe, &s);
+        double sig[6], J;
+        oco_imr_stress(s.F, s.P, c10, c01, sig, &J);
+        const double wjj = T.w[g] * s.jx * J;
+        int idx = 0;
+        for (int c = 0; c < 3; ++c)
+            for (int m = 0; m < 27; ++m, ++idx)
+                for (int c2 = 0; c2 < 3; ++c2) Re[idx] = Re[idx] + wjj * sig[T2V[c][c2]] * s.D[c2 * 27 + m];
+        for (int m = 0; m < 8; ++m, ++idx)
+            Re[idx] = Re[idx] + T.w[g] * s.jx * T.fl[((size_t)g * 8 + m) * 4 + 0] * (J - 1.0);
+    }
+    return 0;
+}
+
+int oco_imr_element_jacobian_fd(const double *Xe, const double *ze, const double *pe,
+                                double c10, double c01, double delta, double *Je)
+{
+    element_jacobian_fd(oco_imr_element_residual, Xe, ze, pe, c10, c01, delta, Je);
     return 0;
 }
 
@@ -750,8 +844,9 @@ static void mr_gather(int64_t e, const int32_t *elem_q, const int32_t *elem_l, i
 
 /* EQUATIONS_SET_RESIDUAL_EVALUATE_STATIC_FEM, equations_set_routines.f90:
  * 5483-5629: zero, elements in order, element residual, vector add. */
-int oco_mr_residual(int nelem, const int32_t *elem_q, const int32_t *elem_l, int nq, int nl,
-                    const double *Xq, const double *dofs, double c10, double c01, double *res)
+static int residual_assemble(element_residual_fn fn, int nelem, const int32_t *elem_q, const int32_t *elem_l,
+                             int nq, int nl, const double *Xq, const double *dofs, double c10, double c01,
+                             double *res)
 {
     const int64_t ndof = (int64_t)3 * nq + nl;
     memset(res, 0, sizeof(double) * (size_t)ndof);
@@ -759,10 +854,23 @@ int oco_mr_residual(int nelem, const int32_t *elem_q, const int32_t *elem_l, int
     int32_t ed[89];
     for (int64_t e = 0; e < nelem; ++e) {
         mr_gather(e, elem_q, elem_l, nq, Xq, dofs, Xe, ze, pe, ed);
-        oco_mr_element_residual(Xe, ze, pe, c10, c01, Re);
+        fn(Xe, ze, pe, c10, c01, Re);
         for (int i = 0; i < 89; ++i) res[ed[i]] += Re[i];
     }
     return 0;
+}
+
+int oco_mr_residual(int nelem, const int32_t *elem_q, const int32_t *elem_l, int nq, int nl,
+                    const double *Xq, const double *dofs, double c10, double c01, double *res)
+{
+    return residual_assemble(oco_mr_element_residual, nelem, elem_q, elem_l, nq, nl, Xq, dofs, c10, c01, res);
+}
+
+/* The residual driver for the incompressible MR subtype (same element loop). */
+int oco_imr_residual(int nelem, const int32_t *elem_q, const int32_t *elem_l, int nq, int nl,
+                     const double *Xq, const double *dofs, double c10, double c01, double *res)
+{
+    return residual_assemble(oco_imr_element_residual, nelem, elem_q, elem_l, nq, nl, Xq, dofs, c10, c01, res);
 }
 
 /* EQUATIONS_SET_JACOBIAN_EVALUATE_STATIC_FEM, equations_set_routines.f90:
@@ -774,7 +882,7 @@ int oco_mr_jacobian(int nelem, const int32_t *elem_q, const int32_t *elem_l, int
     const int64_t ndof = (int64_t)3 * nq + nl;
     memset(values, 0, sizeof(double) * (size_t)row_ptr[ndof]);
     double delta = 0.0;
-    if (mode == 1) {
+    if (mode == 1 || mode == 2) {
         double s = 0.0;
         for (int64_t i = 0; i < ndof; ++i) s += dofs[i] * dofs[i];
         delta = (1.0 + sqrt(s)) * 1e-7;
@@ -785,7 +893,8 @@ int oco_mr_jacobian(int nelem, const int32_t *elem_q, const int32_t *elem_l, int
     int rc = 0;
     for (int64_t e = 0; e < nelem && !rc; ++e) {
         mr_gather(e, elem_q, elem_l, nq, Xq, dofs, Xe, ze, pe, ed);
-        if (mode == 1) oco_mr_element_jacobian_fd(Xe, ze, pe, c10, c01, delta, Je);
+        if (mode == 2) oco_imr_element_jacobian_fd(Xe, ze, pe, c10, c01, delta, Je);
+        else if (mode == 1) oco_mr_element_jacobian_fd(Xe, ze, pe, c10, c01, delta, Je);
         else oco_mr_element_jacobian(Xe, ze, pe, c10, c01, Je);
         rc = oco_csr_add(row_ptr, col_idx, values, 89, ed, 89, ed, Je);
     }

@@ -84,7 +84,17 @@ int oco_mr_element_jacobian_fd(const double *Xe, const double *ze, const double 
 /* Global assembly.  dofs = [z1(nq) z2(nq) z3(nq) p(nl)]. */
 int oco_mr_residual(int nelem, const int32_t *elem_q, const int32_t *elem_l, int nq, int nl,
                     const double *Xq, const double *dofs, double c10, double c01, double *res);
-/* mode 0 analytic, 1 forward difference (delta=(1+||dofs||_2)*1e-7). */
+/* Incompressible MR subtype: Cauchy stress (returns Jznu), element residual,
+ * forward-difference element Jacobian, residual assembly. */
+int oco_imr_stress(const double *F, double p, double c10, double c01, double *sigma, double *jznu);
+int oco_imr_element_residual(const double *Xe, const double *ze, const double *pe,
+                             double c10, double c01, double *Re);
+int oco_imr_element_jacobian_fd(const double *Xe, const double *ze, const double *pe,
+                                double c10, double c01, double delta, double *Je);
+int oco_imr_residual(int nelem, const int32_t *elem_q, const int32_t *elem_l, int nq, int nl,
+                     const double *Xq, const double *dofs, double c10, double c01, double *res);
+/* mode 0 analytic, 1 forward difference (delta=(1+||dofs||_2)*1e-7), 2 forward
+ * difference of the incompressible MR subtype's residual (same delta). */
 int oco_mr_jacobian(int nelem, const int32_t *elem_q, const int32_t *elem_l, int nq, int nl,
                     const double *Xq, const double *dofs, double c10, double c01, int mode,
                     const int64_t *row_ptr, const int32_t *col_idx, double *values);

@@ -39,6 +39,10 @@ def load():
         "oco_mr_element_jacobian": (c_int, [_dp, _dp, _dp, c_double, c_double, _dp]),
         "oco_mr_element_jacobian_fd": (c_int, [_dp, _dp, _dp, c_double, c_double, c_double, _dp]),
         "oco_mr_residual": (c_int, [c_int, _i32, _i32, c_int, c_int, _dp, _dp, c_double, c_double, _dp]),
+        "oco_imr_stress": (c_int, [_dp, c_double, c_double, c_double, _dp, ctypes.POINTER(c_double)]),
+        "oco_imr_element_residual": (c_int, [_dp, _dp, _dp, c_double, c_double, _dp]),
+        "oco_imr_element_jacobian_fd": (c_int, [_dp, _dp, _dp, c_double, c_double, c_double, _dp]),
+        "oco_imr_residual": (c_int, [c_int, _i32, _i32, c_int, c_int, _dp, _dp, c_double, c_double, _dp]),
         "oco_mr_jacobian": (c_int, [c_int, _i32, _i32, c_int, c_int, _dp, _dp, c_double, c_double, c_int,
                                     _i64, _i32, _dp]),
         "oco_mr_assemble_parallel": (c_int, [c_int, _i32, _i32, c_int, c_int, _dp, _dp, c_double, c_double,
@@ -148,6 +152,33 @@ def mr_stress(F, p, c10, c01):
     return s
 
 
+def imr_stress(F, p, c10, c01):
+    """Cauchy stress (Voigt) and Jznu of the incompressible MR subtype."""
+    Fc = np.ascontiguousarray(np.asarray(F, dtype=np.float64).T).ravel()
+    s, J = np.zeros(6), ctypes.c_double(0.0)
+    _check(lib().oco_imr_stress(Fc, p, c10, c01, s, ctypes.byref(J)), "imr_stress")
+    return s, J.value
+
+
+def imr_element_residual(Xe, ze, pe, c10, c01):
+    R = np.zeros(89)
+    _check(lib().oco_imr_element_residual(np.ascontiguousarray(Xe).ravel(), np.ascontiguousarray(ze).ravel(),
+                                          np.ascontiguousarray(pe, dtype=np.float64), c10, c01, R), "imr_res")
+    return R
+
+
+def imr_residual(elem_q, elem_l, Xq, dofs, c10, c01):
+    E = elem_q.shape[0]
+    nq = Xq.shape[1]
+    nl = dofs.shape[0] - 3 * nq
+    res = np.zeros(dofs.shape[0])
+    _check(lib().oco_imr_residual(E, np.ascontiguousarray(elem_q, np.int32).ravel(),
+                                  np.ascontiguousarray(elem_l, np.int32).ravel(), nq, nl,
+                                  np.ascontiguousarray(Xq).ravel(), np.ascontiguousarray(dofs), c10, c01, res),
+           "imr_residual")
+    return res
+
+
 def mr_elasticity(F, p, c10, c01):
     Fc = np.ascontiguousarray(np.asarray(F, dtype=np.float64).T).ravel()
     J = abs(np.linalg.det(F))
@@ -191,6 +222,8 @@ def mr_residual(elem_q, elem_l, Xq, dofs, c10, c01):
 
 
 def mr_jacobian(elem_q, elem_l, Xq, dofs, c10, c01, row_ptr, col_idx, mode=0):
+    """mode 0: analytic MR Jacobian; 1: its forward differences; 2: forward
+    differences of the incompressible MR subtype's residual."""
     E = elem_q.shape[0]
     nq = Xq.shape[1]
     nl = dofs.shape[0] - 3 * nq

@@ -335,6 +335,99 @@ def test_mr_small_strain_stiffness_is_incompressible_elasticity():
     assert abs(u.ravel() @ Kuu @ u.ravel() - mu * gamma ** 2 * 1.0) < 1e-11
 
 
+# --- incompressible Mooney-Rivlin subtype (PK2 form, forward-difference Jacobian) ---
+def _voigt(s):
+    return np.array([[s[0], s[3], s[4]], [s[3], s[1], s[5]], [s[4], s[5], s[2]]])
+
+
+def test_imr_stress_neo_hookean_equals_mr_subtype():
+    """With c01 = 0 both subtypes are the same incompressible neo-Hookean law
+    for every F: sigma = (2 c10 / J) J^(-2/3) (b - tr(b)/3 I) + p I."""
+    rng = np.random.default_rng(11)
+    for _ in range(5):
+        F = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+        if np.linalg.det(F) < 0.2:
+            continue
+        p = rng.standard_normal()
+        s_imr, J = orc.imr_stress(F, p, 2.5, 0.0)
+        assert abs(J - np.linalg.det(F)) < 1e-13
+        b = F @ F.T
+        expect = 2 * 2.5 / J * J ** (-2 / 3) * (b - np.trace(b) / 3 * np.eye(3)) + p * np.eye(3)
+        assert np.abs(_voigt(s_imr) - expect).max() < 1e-12
+        assert np.abs(s_imr - orc.mr_stress(F, p, 2.5, 0.0)).max() < 1e-12
+
+
+def test_imr_stress_isochoric_states_equal_mr_subtype():
+    """At det F = 1 the two subtypes' stresses agree for any c10, c01 (the PK2
+    form's J^(-2/3) factors are 1 there); at det F != 1 with c01 != 0 they do
+    not (the subtype's c01 terms scale with J^(-2/3), not J^(-4/3))."""
+    rng = np.random.default_rng(12)
+    for _ in range(4):
+        A = np.eye(3) + 0.25 * rng.standard_normal((3, 3))
+        F = A / np.cbrt(np.linalg.det(A))
+        s_imr, J = orc.imr_stress(F, 0.4, 2.0, 6.0)
+        assert abs(J - 1.0) < 1e-13
+        assert np.abs(s_imr - orc.mr_stress(F, 0.4, 2.0, 6.0)).max() < 1e-11
+    F = np.diag([1.2, 1.0, 1.0])
+    assert np.abs(orc.imr_stress(F, 0.0, 2.0, 6.0)[0] - orc.mr_stress(F, 0.0, 2.0, 6.0)).max() > 1e-3
+
+
+def test_imr_stress_frame_indifference_and_rest_state():
+    rng = np.random.default_rng(13)
+    F = np.eye(3) + 0.2 * rng.standard_normal((3, 3))
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    Q *= np.sign(np.linalg.det(Q))
+    s1, _ = orc.imr_stress(F, 0.3, 2.0, 6.0)
+    s2, _ = orc.imr_stress(Q @ F, 0.3, 2.0, 6.0)
+    assert np.abs(Q @ _voigt(s1) @ Q.T - _voigt(s2)).max() < 1e-12
+    s0, _ = orc.imr_stress(np.eye(3), 0.0, 2.0, 6.0)
+    assert np.abs(s0).max() < 1e-14
+
+
+def test_imr_residual_equals_mr_residual_for_neo_hookean():
+    """Same law at c01 = 0, same residual loop up to the order of the
+    Jacobians' product: the element residuals agree to rounding on a jittered,
+    deformed element."""
+    rng = np.random.default_rng(14)
+    Xe = _element()
+    ze = Xe + 0.05 * rng.uniform(-1, 1, Xe.shape)
+    pe = rng.standard_normal(8)
+    R_imr = orc.imr_element_residual(Xe, ze, pe, 3.0, 0.0)
+    R_mr = orc.mr_element_residual(Xe, ze, pe, 3.0, 0.0)
+    assert np.abs(R_imr - R_mr).max() / np.abs(R_mr).max() < 1e-12
+    assert np.abs(orc.imr_element_residual(Xe, Xe.copy(), np.zeros(8), 2.0, 6.0)).max() < 1e-14
+
+
+def test_imr_homogeneous_uniaxial_state_is_exact():
+    """The homogeneous incompressible stretch with the traction-free pressure
+    is an exact discrete solution of the incompressible subtype too."""
+    c10, c01, lam = 2.0, 6.0, 1.1
+    m = RegularMesh.generate(2, 2)
+    z = m.coords * np.array([[lam], [lam ** -0.5], [lam ** -0.5]])
+    s0, _ = orc.imr_stress(np.diag([lam, lam ** -0.5, lam ** -0.5]), 0.0, c10, c01)
+    dofs = np.concatenate([z.ravel(), np.full(m.n_nodes_l, -s0[1])])
+    r = orc.imr_residual(m.elem_nodes, m.elem_nodes_l, m.coords, dofs, c10, c01)
+    nq = m.n_nodes
+    ru = r[:3 * nq].reshape(3, nq)
+    interior_x = np.setdiff1d(np.arange(nq), np.union1d(m.face_nodes(0, 0), m.face_nodes(0, 1)))
+    assert np.abs(ru[:, interior_x]).max() < 1e-12
+    assert np.abs(r[3 * nq:]).max() < 1e-13
+
+
+def test_imr_fd_jacobian_matches_analytic_mr_for_neo_hookean():
+    """The incompressible subtype's forward-difference Jacobian (the reference's
+    method for it) against the MR subtype's analytic Jacobian where the laws
+    coincide (c01 = 0): O(delta) apart."""
+    m = RegularMesh.generate(2, 2, jitter=0.1, seed=3)
+    rng = np.random.default_rng(15)
+    d = np.concatenate([(m.coords + 0.02 * rng.uniform(-1, 1, m.coords.shape)).ravel(),
+                        0.3 * rng.standard_normal(m.n_nodes_l)])
+    rp, ci = orc.pattern([(m.n_nodes, m.elem_nodes)] * 3 + [(m.n_nodes_l, m.elem_nodes_l)])
+    Jfd = orc.mr_jacobian(m.elem_nodes, m.elem_nodes_l, m.coords, d, 2.0, 0.0, rp, ci, mode=2)
+    Jan = orc.mr_jacobian(m.elem_nodes, m.elem_nodes_l, m.coords, d, 2.0, 0.0, rp, ci, mode=0)
+    assert np.linalg.norm(Jfd - Jan) / np.linalg.norm(Jan) < 1e-5
+
+
 # --- boundary-condition increments (equations_set_routines.f90:6560-6585) ---
 def test_bc_increment_schedule():
     cur = np.array([1.0, 5.0])
