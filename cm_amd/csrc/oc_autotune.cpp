@@ -116,6 +116,10 @@ extern "C" int oc_autotune(oc_equations_set h, int precision, char *report, int 
     js += "}";
     // level-0 multigrid node kernel (residual form), when the rows have the node layout
     int best_smoother = -1;
+    if (s.node_rows == 1 && !s.free_mask.ptr) {  // no boundary conditions yet: every DOF free
+        OC_CHECK(s.free_mask.alloc(s.n_rows));
+        OC_HIP_CHECK(hipMemset(s.free_mask.ptr, 1, s.n_rows));
+    }
     if (s.node_rows == 1) {
         js += ", \"smoother_residual_ms\": {";
         DevBuf<double> r;

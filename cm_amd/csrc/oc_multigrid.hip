@@ -843,6 +843,13 @@ int launch_smoother_residual(EquationsSet &s, int precision, int variant, const 
                              double *r)
 {
     if (s.node_rows != 1) return set_error(OC_ERR_UNSUPPORTED, "the node-wise smoother needs node rows");
+    // every array the kernel reads must exist: the free-DOF mask only does once
+    // boundary conditions are set or a solve has prepared the set
+    const void *vals = precision == OC_FP32 ? (s.values_bf.ptr ? static_cast<const void *>(s.values_bf.ptr)
+                                                               : static_cast<const void *>(s.values_sp.ptr))
+                                            : static_cast<const void *>(s.values.ptr);
+    if (!s.free_mask.ptr || !s.pstart.ptr || !vals || !b || !x || !r)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "smoother residual: free-DOF mask, pressure starts or values missing");
     if (precision == OC_FP32 && s.values_bf.ptr)
         return launch_uu_node<Bf16, 0>(variant, s.n_nodes, s, reinterpret_cast<const Bf16 *>(s.values_bf.ptr), b, x,
                                        nullptr, 0.0, r);
