@@ -94,22 +94,25 @@ def time_steps(fn, steps, warmup, cm, only=None):
 
 
 def laplace_line(cm, n, steps):
+    """Laplace assembly of an n^3 structured grid: the fused one-pass kernel
+    (the default for such grids) and the element buffer + row gather."""
     mesh = cm.RegularMesh.generate(1, n)
     eq = cm.LaplaceEquationsSet(mesh)
-    ms, _ = time_steps(eq.assemble, steps, 2, cm)
-    kms = cm.kernel_timing("laplace_element")[0] + cm.kernel_timing("laplace_row_gather")[0]
-    fms, _ = cm.kernel_timing("fill")
-    cm.kernel_timing_enable(False)
     nbytes = laplace_bytes(mesh.n_elements, mesh.n_nodes, eq.nnz)
-    out = {
-        "n": n, "elements": mesh.n_elements, "nnz": eq.nnz,
-        "elements_per_s": mesh.n_elements * steps / (ms / 1e3),
-        "ms_per_assembly": ms / steps,
-        "kernel_ms_per_assembly": (kms + fms) / steps,
-        "algorithmic_bytes": nbytes,
-        "achieved_GBs": nbytes / ((kms + fms) / steps / 1e3) / 1e9,
-    }
-    out["hbm_frac"] = out["achieved_GBs"] / HBM_PEAK_GBS
+    out = {"n": n, "elements": mesh.n_elements, "nnz": eq.nnz, "algorithmic_bytes": nbytes}
+    for form, fused, names in (("fused", True, ("laplace_fused",)),
+                               ("two_pass", False, ("laplace_element", "laplace_row_gather"))):
+        assert eq.set_fused(fused)
+        ms, _ = time_steps(eq.assemble, steps, 2, cm)
+        kms = sum(cm.kernel_timing(k)[0] for k in names)
+        fms, _ = cm.kernel_timing("fill")
+        cm.kernel_timing_enable(False)
+        r = {"elements_per_s": mesh.n_elements * steps / (ms / 1e3), "ms_per_assembly": ms / steps,
+             "kernel_ms_per_assembly": (kms + fms) / steps,
+             "achieved_GBs": nbytes / ((kms + fms) / steps / 1e3) / 1e9}
+        r["hbm_frac"] = r["achieved_GBs"] / HBM_PEAK_GBS
+        out[form] = r
+    out.update(out["fused"])
     eq.destroy()
     return out
 
@@ -393,10 +396,11 @@ def main():
     value = E * args.steps / (ms / 1e3)
     em, el = kernels["mr_element_matrix_fp64"]
     gm, gl = kernels["csr_row_gather_fp64"]
-    rl_elem = kernel_roofline("mr_element_matrix_f64_kernel", "mfma64", E,
+    # algorithmic count per launch: the step's count over the launches per step
+    rl_elem = kernel_roofline("mr_element_matrix_f64_kernel", "mfma64", E * args.steps / el,
                               (dom_ms / dom_launches if dominant == "mr_element_matrix_fp64" else em / el) / 1e3,
                               "mr_element_matrix_fp64", wl)
-    rl_gather = kernel_roofline("csr_row_gather_kernel<double>", "hbm", gather_bytes(eq, 8),
+    rl_gather = kernel_roofline("csr_row_gather_kernel<double>", "hbm", gather_bytes(eq, 8) * args.steps / gl,
                                 (dom_ms / dom_launches if dominant == "csr_row_gather_fp64" else gm / gl) / 1e3,
                                 "csr_row_gather_fp64", wl)
     roofline = rl_elem if dominant == "mr_element_matrix_fp64" else rl_gather
@@ -418,9 +422,9 @@ def main():
         secondary["mr_jacobian_fp32"] = {
             "error_vs_fp64_rel_frobenius": float(np.linalg.norm(v32 - v64) / np.linalg.norm(v64)),
             "elements_per_s": E * args.steps / (ms32 / 1e3),
-            "element_matrix": kernel_roofline("mr_element_matrix_kernel<float>", "mfma32", E, j32 / l32 / 1e3,
-                                              "mr_element_matrix_fp32", wl),
-            "row_gather": kernel_roofline("csr_row_gather_kernel<float>", "hbm", gather_bytes(eq, 4),
+            "element_matrix": kernel_roofline("mr_element_matrix_kernel<float>", "mfma32", E * args.steps / l32,
+                                              j32 / l32 / 1e3, "mr_element_matrix_fp32", wl),
+            "row_gather": kernel_roofline("csr_row_gather_kernel<float>", "hbm", gather_bytes(eq, 4) * args.steps / k32,
                                           g32 / k32 / 1e3, "csr_row_gather_fp32", wl),
             "kernel_ms_per_step": {"mr_element_matrix_fp32": j32 / args.steps, "csr_row_gather_fp32": g32 / args.steps},
         }

@@ -140,6 +140,7 @@ SIGNATURES = {
     "oc_dofs_get": [_H, _dp],
     "oc_dofs_set_device": [_H, ctypes.c_void_p],
     "oc_equations_set_assemble": [_H],
+    "oc_laplace_fused_set": [_H, _c_int, ctypes.POINTER(_c_int)],
     "oc_equations_set_residual_evaluate": [_H],
     "oc_equations_set_jacobian_evaluate": [_H, _c_int],
     "oc_equations_set_jacobian_evaluate_fd": [_H],

@@ -238,6 +238,7 @@ int oc_laplace_create(int n_nodes, const double *coords, int n_elements, const i
         delete h;
         return rc;
     }
+    laplace_detect_grid(s, conn);
     *equations_set = h;
     return OC_NO_ERROR;
 }
@@ -416,6 +417,17 @@ int oc_equations_set_subtype_set(oc_equations_set h, int subtype)
     s.subtype = subtype;
     return OC_NO_ERROR;
 }
+
+int oc_laplace_fused_set(oc_equations_set h, int fused, int *structured)
+{
+    OC_CHECK(check_handle(h));
+    if (h->s.cls != OC_EQUATIONS_SET_LAPLACE)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "the fused assembly form is defined for Laplace sets");
+    h->s.lap_fused = fused != 0;
+    if (structured) *structured = h->s.lap_grid[0] > 0 ? 1 : 0;
+    return OC_NO_ERROR;
+}
+
 
 int oc_mesh_count(oc_equations_set h, int *n_meshes)
 {

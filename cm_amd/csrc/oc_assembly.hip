@@ -155,10 +155,61 @@ laplace_slot_offsets_kernel(int n_elements, const int *__restrict__ conn, const 
     }
 }
 
-// One thread per element: the 36 upper-triangle entries of K_e stay in
-// registers over the 8 Gauss points (the basis table is read with uniform
-// addresses); each row goes to its place in the row-gather order, so the
-// gather reads every CSR row's contributions contiguously.
+// The 36 upper-triangle entries K[ms*(15-ms)/2 + ns] (ms <= ns) of one
+// element matrix from its node coordinates x[c][n], accumulated in registers
+// over the 8 Gauss points (the basis table is read with uniform addresses).
+__device__ __forceinline__ int lap_upper(int ms, int ns) { return ms * (15 - ms) / 2 + ns; }
+
+__device__ __forceinline__ void laplace_element_upper(const double (&x)[3][8], const BasisTables *__restrict__ T,
+                                                      double (&K)[36])
+{
+#pragma unroll
+    for (int i = 0; i < 36; ++i) K[i] = 0.0;
+#pragma unroll 1
+    for (int g = 0; g < 8; ++g) {
+        const double *f = T->lin8 + g * 8 * 4;
+        double dx[9];  // dx[c + 3 mi] = dX_c/dxi_mi (FIELD_INTERPOLATE_GAUSS)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int mi = 0; mi < 3; ++mi) {
+                double sum = 0.0;
+#pragma unroll
+                for (int n = 0; n < 8; ++n) sum += f[n * 4 + 1 + mi] * x[c][n];
+                dx[c + 3 * mi] = sum;
+            }
+        // GL = dX^T dX; GU JACOBIAN w_g = adj(GL) w_g sqrt|det| / det
+        // (COORDINATE_METRICS_CALCULATE and INVERT_FULL_DP with one division)
+        double gl[6];  // 00 11 22 01 02 12
+        gl[0] = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+        gl[1] = dx[3] * dx[3] + dx[4] * dx[4] + dx[5] * dx[5];
+        gl[2] = dx[6] * dx[6] + dx[7] * dx[7] + dx[8] * dx[8];
+        gl[3] = dx[0] * dx[3] + dx[1] * dx[4] + dx[2] * dx[5];
+        gl[4] = dx[0] * dx[6] + dx[1] * dx[7] + dx[2] * dx[8];
+        gl[5] = dx[3] * dx[6] + dx[4] * dx[7] + dx[5] * dx[8];
+        const double c00 = gl[1] * gl[2] - gl[5] * gl[5], c11 = gl[0] * gl[2] - gl[4] * gl[4];
+        const double c22 = gl[0] * gl[1] - gl[3] * gl[3], c01 = gl[4] * gl[5] - gl[3] * gl[2];
+        const double c02 = gl[3] * gl[5] - gl[4] * gl[1], c12 = gl[3] * gl[4] - gl[0] * gl[5];
+        const double det = gl[0] * c00 + gl[3] * c01 + gl[4] * c02;
+        const double sc = fabs(det) > 5.0 * 2.220446049250313e-16 ? T->w8[g] * sqrt(fabs(det)) / det : 0.0;
+        const double A[9] = {c00 * sc, c01 * sc, c02 * sc, c01 * sc, c11 * sc, c12 * sc, c02 * sc, c12 * sc, c22 * sc};
+        int idx = 0;
+#pragma unroll
+        for (int ms = 0; ms < 8; ++ms) {
+            double v[3];
+#pragma unroll
+            for (int ni = 0; ni < 3; ++ni)
+                v[ni] = f[ms * 4 + 1] * A[0 + 3 * ni] + f[ms * 4 + 2] * A[1 + 3 * ni] + f[ms * 4 + 3] * A[2 + 3 * ni];
+#pragma unroll
+            for (int ns = ms; ns < 8; ++ns, ++idx)
+                K[idx] = fma(v[2], f[ns * 4 + 3], fma(v[1], f[ns * 4 + 2], fma(v[0], f[ns * 4 + 1], K[idx])));
+        }
+    }
+}
+
+// One thread per element; each row of K_e goes to its place in the
+// row-gather order, so the gather reads every CSR row's contributions
+// contiguously.
 constexpr int kLapThreads = 64;
 
 __global__ void __launch_bounds__(kLapThreads)
@@ -166,47 +217,16 @@ laplace_element_kernel(int n_elements, const int *__restrict__ conn, const doubl
                        const BasisTables *__restrict__ T, const int *__restrict__ gpos, double *__restrict__ kbuf)
 {
     const int e = blockIdx.x * kLapThreads + threadIdx.x;
-    double K[36];
-#pragma unroll
-    for (int i = 0; i < 36; ++i) K[i] = 0.0;
-    if (e < n_elements) {
-        double x[3][8];
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            const int node = conn[static_cast<int64_t>(e) * 8 + n];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) x[c][n] = coords[static_cast<int64_t>(c) * n_nodes + node];
-        }
-#pragma unroll 1
-        for (int g = 0; g < 8; ++g) {
-            const double *f = T->lin8 + g * 8 * 4;
-            double dx[9], gu[9];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int ni = 0; ni < 3; ++ni) {
-                    double sum = 0.0;
-#pragma unroll
-                    for (int n = 0; n < 8; ++n) sum += f[n * 4 + 1 + ni] * x[c][n];
-                    dx[c + 3 * ni] = sum;
-                }
-            const double rwg = metric_jacobian(dx, gu) * T->w8[g];
-            int idx = 0;
-#pragma unroll
-            for (int ms = 0; ms < 8; ++ms) {
-                double v[3];
-#pragma unroll
-                for (int ni = 0; ni < 3; ++ni)
-                    v[ni] = f[ms * 4 + 1] * gu[0 + 3 * ni] + f[ms * 4 + 2] * gu[1 + 3 * ni] + f[ms * 4 + 3] * gu[2 + 3 * ni];
-#pragma unroll
-                for (int ns = ms; ns < 8; ++ns, ++idx) {
-                    const double sum = v[0] * f[ns * 4 + 1] + v[1] * f[ns * 4 + 2] + v[2] * f[ns * 4 + 3];
-                    K[idx] = K[idx] + sum * rwg;
-                }
-            }
-        }
-    }
     if (e >= n_elements) return;
+    double x[3][8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const int node = conn[static_cast<int64_t>(e) * 8 + n];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[c][n] = coords[static_cast<int64_t>(c) * n_nodes + node];
+    }
+    double K[36];
+    laplace_element_upper(x, T, K);
     // row ms of this element goes to its position in the row-gather order
     double row[64];
     int idx = 0;
@@ -278,6 +298,200 @@ laplace_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const
     for (int k = t; k < seg; k += 8 * kLapGatherRows) values[base + k] = s_seg[k];
 }
 
+// ---------------------------------------------------------------------------
+// Fused Laplace assembly of a structured hexahedral grid (nx x ny x nz
+// elements, nodes and elements numbered x fastest, local nodes xi1 fastest):
+// no element-matrix buffer.  A block owns the CSR rows of a kLapTX x kLapTY
+// column of nodes over kLapSeg element layers and walks up the layers.  Per
+// layer, each thread computes one element matrix of the column's element
+// footprint (one element wider on the low x and y sides) into LDS; each node
+// thread then
+//   * finishes the row of its node in the layer's bottom plane: the partial
+//     sums from the layer below (kept in its registers) plus the four
+//     elements above the plane, and writes it to its CSR segment once;
+//   * starts the row of its node in the top plane with the layer's four
+//     elements below that plane.
+// A row's entries thus add the element matrices that hold them in ascending
+// element order from 0, as the two-pass row gather does, so values are
+// bit-identical to it.  Blocks share only recomputed footprint elements
+// (x, y halo and the layer below a segment), never output.
+constexpr int kLapTX = 15, kLapTY = 7, kLapSeg = 32;
+constexpr int kLapFX = kLapTX + 1, kLapFusedThreads = (kLapTX + 1) * (kLapTY + 1);  // 128
+
+__global__ void __launch_bounds__(kLapFusedThreads)
+laplace_fused_kernel(int nx, int ny, int nz, const double *__restrict__ coords, int n_nodes,
+                     const BasisTables *__restrict__ T, const int64_t *__restrict__ row_ptr, double *__restrict__ values)
+{
+    __shared__ double sK[36][kLapFusedThreads];  // element matrices of the layer (upper triangles)
+    const int t = threadIdx.x;
+    const int tiles_x = (nx + kLapTX) / kLapTX, tiles_y = (ny + kLapTY) / kLapTY;  // nx + 1 nodes
+    int b = static_cast<int>(blockIdx.x);
+    const int i0 = (b % tiles_x) * kLapTX;
+    b /= tiles_x;
+    const int j0 = (b % tiles_y) * kLapTY;
+    const int k0 = (b / tiles_y) * kLapSeg, k1 = min(nz, k0 + kLapSeg);
+    const int NX = nx + 1;
+    const int64_t NXY = static_cast<int64_t>(nx + 1) * (ny + 1);
+    // this thread's footprint element and node column
+    const int ie = i0 - 1 + t % kLapFX, je = j0 - 1 + t / kLapFX;
+    const bool has_elem = ie >= 0 && ie < nx && je >= 0 && je < ny;
+    const int ni = i0 + t % kLapTX, nj = j0 + t / kLapTX;
+    const bool has_node = t < kLapTX * kLapTY && ni <= nx && nj <= ny;
+    const int dxlo = ni > 0 ? -1 : 0, cx = (ni < nx ? 2 : 1) - dxlo;
+    const int dylo = nj > 0 ? -1 : 0, cy = (nj < ny ? 2 : 1) - dylo;
+    // partial row of the column's node in the next plane (slots dz = -1, 0):
+    // the thread keeps its column over the layers, so it stays in registers
+    double P[18];
+#pragma unroll
+    for (int q = 0; q < 18; ++q) P[q] = 0.0;
+    // CSR row of node (ni, nj, k): entry (dx, dy, dz) of its stencil is at
+    // ((dz - dzlo) cy + dy - dylo) cx + dx - dxlo (columns ascend); the loops
+    // unroll, so R stays in registers
+    auto write_row = [&](int k, const double(&R)[27]) {
+        const int64_t base = row_ptr[ni + NX * nj + NXY * k];
+        const int dzlo = k > 0 ? -1 : 0, dzhi = k < nz ? 1 : 0;
+#pragma unroll
+        for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (dz < dzlo || dz > dzhi || dy < dylo || dy >= dylo + cy || dx < dxlo || dx >= dxlo + cx)
+                        continue;
+                    values[base + ((dz - dzlo) * cy + dy - dylo) * cx + dx - dxlo] =
+                        R[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1];
+                }
+    };
+    for (int k = max(k0 - 1, 0); k < k1; ++k) {
+        double K[36];
+        if (has_elem) {
+            double x[3][8];
+            const int64_t base = ie + static_cast<int64_t>(NX) * je + NXY * k;
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                const int64_t node = base + (n & 1) + NX * ((n >> 1) & 1) + NXY * (n >> 2);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) x[c][n] = coords[static_cast<int64_t>(c) * n_nodes + node];
+            }
+            laplace_element_upper(x, T, K);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 36; ++q) K[q] = 0.0;
+        }
+        __syncthreads();  // the previous layer's rows are read
+#pragma unroll
+        for (int q = 0; q < 36; ++q) sK[q][t] = K[q];
+        __syncthreads();
+        if (!has_node) continue;
+        // the elements around the column in this layer, ascending:
+        // (ni-1, nj-1), (ni, nj-1), (ni-1, nj), (ni, nj); local node a of the
+        // column's node is (a1, a2) = (ni - ex, nj - ey) in the plane
+        if (k >= k0) {  // plane k is this block's: finish and write its rows
+            double R[27];
+#pragma unroll
+            for (int q = 0; q < 18; ++q) R[q] = P[q];
+#pragma unroll
+            for (int q = 18; q < 27; ++q) R[q] = 0.0;
+#pragma unroll
+            for (int a2 = 1; a2 >= 0; --a2)
+#pragma unroll
+                for (int a1 = 1; a1 >= 0; --a1) {
+                    const int ex = ni - a1, ey = nj - a2;
+                    if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
+                    const int te = ex - i0 + 1 + kLapFX * (ey - j0 + 1);
+                    const int ms = a1 + 2 * a2;  // bottom face of the element
+#pragma unroll
+                    for (int ns = 0; ns < 8; ++ns) {
+                        const int b1 = ns & 1, b2 = (ns >> 1) & 1, b3 = ns >> 2;
+                        R[(b3 + 1) * 9 + (b2 - a2 + 1) * 3 + b1 - a1 + 1] +=
+                            sK[ms <= ns ? lap_upper(ms, ns) : lap_upper(ns, ms)][te];
+                    }
+                }
+            write_row(k, R);
+        }
+#pragma unroll
+        for (int q = 0; q < 18; ++q) P[q] = 0.0;
+#pragma unroll
+        for (int a2 = 1; a2 >= 0; --a2)
+#pragma unroll
+            for (int a1 = 1; a1 >= 0; --a1) {
+                const int ex = ni - a1, ey = nj - a2;
+                if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
+                const int te = ex - i0 + 1 + kLapFX * (ey - j0 + 1);
+                const int ms = a1 + 2 * a2 + 4;  // top face of the element
+#pragma unroll
+                for (int ns = 0; ns < 8; ++ns) {
+                    const int b1 = ns & 1, b2 = (ns >> 1) & 1, b3 = ns >> 2;
+                    P[b3 * 9 + (b2 - a2 + 1) * 3 + b1 - a1 + 1] +=
+                        sK[ms <= ns ? lap_upper(ms, ns) : lap_upper(ns, ms)][te];
+                }
+            }
+    }
+    if (k1 == nz && has_node) {  // the top plane has no elements above it
+        double R[27];
+#pragma unroll
+        for (int q = 0; q < 18; ++q) R[q] = P[q];
+#pragma unroll
+        for (int q = 18; q < 27; ++q) R[q] = 0.0;
+        write_row(nz, R);
+    }
+}
+
+// Structured-grid check of a Laplace set: nodes and elements numbered x
+// fastest on an nx x ny x nz grid with xi1-fastest local nodes, and the CSR
+// rows the 27-point stencils with ascending columns.  Sets s.lap_grid, or
+// leaves it zero (the two-pass assembly then serves the mesh).
+void laplace_detect_grid(EquationsSet &s, const std::vector<int32_t> &conn)
+{
+    s.lap_grid[0] = s.lap_grid[1] = s.lap_grid[2] = 0;
+    if (s.n_elements < 1 || conn.size() < 8 || conn[0] != 0 || conn[1] != 1) return;
+    const int64_t NX = conn[2];
+    if (NX < 2 || conn[4] % NX != 0) return;
+    const int64_t NY = conn[4] / NX;
+    if (NY < 2 || s.n_nodes % (NX * NY) != 0) return;
+    const int64_t NZ = s.n_nodes / (NX * NY);
+    const int64_t nx = NX - 1, ny = NY - 1, nz = NZ - 1;
+    if (nz < 1 || nx * ny * nz != s.n_elements) return;
+    for (int64_t e = 0; e < s.n_elements; ++e) {
+        const int64_t ie = e % nx, je = (e / nx) % ny, ke = e / (nx * ny);
+        const int64_t base = ie + NX * (je + NY * ke);
+        for (int n = 0; n < 8; ++n)
+            if (conn[e * 8 + n] != base + (n & 1) + NX * ((n >> 1) & 1) + NX * NY * (n >> 2)) return;
+    }
+    for (int64_t node = 0; node < s.n_nodes; ++node) {
+        const int64_t i = node % NX, j = (node / NX) % NY, k = node / (NX * NY);
+        int64_t p = s.h_row_ptr[node];
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (i + dx < 0 || i + dx > nx || j + dy < 0 || j + dy > ny || k + dz < 0 || k + dz > nz) continue;
+                    if (p >= s.h_row_ptr[node + 1] || s.h_col_idx[p] != node + dx + NX * dy + NX * NY * dz) return;
+                    ++p;
+                }
+        if (p != s.h_row_ptr[node + 1]) return;
+    }
+    s.lap_grid[0] = static_cast<int>(nx);
+    s.lap_grid[1] = static_cast<int>(ny);
+    s.lap_grid[2] = static_cast<int>(nz);
+}
+
+int launch_laplace_fused(EquationsSet &s)
+{
+    const BasisTables *T = nullptr;
+    OC_CHECK(ensure_tables(&T));
+    Runtime &rt = runtime();
+    const int nx = s.lap_grid[0], ny = s.lap_grid[1], nz = s.lap_grid[2];
+    if (nx < 1 || ny < 1 || nz < 1) return set_error(OC_ERR_UNSUPPORTED, "the Laplace set is not a structured grid");
+    OC_CHECK(fill<double>(s.rhs.ptr, s.n_rows, 0.0));  // RHS element vector is 0 (:542)
+    const int64_t blocks = static_cast<int64_t>((nx + kLapTX) / kLapTX) * ((ny + kLapTY) / kLapTY) *
+                           ((nz + kLapSeg - 1) / kLapSeg);
+    KernelTimer timer("laplace_fused");
+    laplace_fused_kernel<<<static_cast<unsigned>(blocks), kLapFusedThreads, 0, rt.stream>>>(
+        nx, ny, nz, s.coords.ptr, s.n_nodes, T, s.row_ptr.ptr, s.values.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
 static int laplace_prepare(EquationsSet &s)
 {
     if (s.slot_offsets8.ptr) return OC_NO_ERROR;
@@ -336,6 +550,7 @@ static int laplace_prepare(EquationsSet &s)
 
 int launch_laplace_assemble(EquationsSet &s)
 {
+    if (s.lap_grid[0] > 0 && s.lap_fused) return launch_laplace_fused(s);
     const BasisTables *T = nullptr;
     OC_CHECK(ensure_tables(&T));
     OC_CHECK(laplace_prepare(s));

@@ -53,7 +53,7 @@ extern "C" int oc_autotune(oc_equations_set h, int precision, char *report, int 
     std::string js = "{\"precision\": \"";
     js += precision == OC_FP64 ? "fp64" : "fp32";
     js += "\"";
-    char buf[128];
+    char buf[256];
     const int reps = 3;
 
     js += ", \"element_matrix_ms\": {";

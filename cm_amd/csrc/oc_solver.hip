@@ -650,7 +650,9 @@ struct Krylov {
     {
         KernelTimer timer("spmv");
         const bool lap = s.cls == OC_EQUATIONS_SET_LAPLACE;
-        const int lanes = s.tuned_spmv_lanes > 0 ? s.tuned_spmv_lanes : (lap ? 8 : 32);
+        // defaults: 8 lanes per Laplace row; per Mooney-Rivlin node 32 lanes
+        // (FP32) or 16 lanes with two neighbours in flight each (FP64)
+        const int lanes = s.tuned_spmv_lanes > 0 ? s.tuned_spmv_lanes : (lap ? 8 : (fp32 ? 32 : 16));
         if (lap) return launch_spmv<double>(s.values.ptr, lanes, s.wgt.ptr, x, y);
         if (fp32) return launch_spmv<float>(s.values_sp.ptr, lanes, s.wgt.ptr, x, y);
         return launch_spmv<double>(s.values.ptr, lanes, s.wgt.ptr, x, y);
@@ -658,7 +660,7 @@ struct Krylov {
     template <typename VT>
     int launch_spmv(const VT *vals, int lanes, const double *wv, const double *x, double *y, int unroll = -1)
     {
-        if (unroll < 0) unroll = s.tuned_spmv_unroll;
+        if (unroll < 0) unroll = s.tuned_spmv_unroll > 0 ? s.tuned_spmv_unroll : (fp32 ? 1 : 2);
         Runtime &rt = runtime();
         if (s.cls == OC_EQUATIONS_SET_MOONEY_RIVLIN && s.node_rows == 1 && lanes >= 16) {
             const int nq = s.n_nodes, nl = s.n_nodes_l;

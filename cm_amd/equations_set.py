@@ -212,6 +212,14 @@ class LaplaceEquationsSet(EquationsSet):
         """Zero and assemble the linear matrix and RHS vector on the GPU."""
         call("oc_equations_set_assemble", self._h)
 
+    def set_fused(self, fused=True):
+        """Fused one-pass assembly of a structured grid (True, the default) or
+        the element-matrix buffer + row gather; returns whether the mesh is a
+        structured grid (the fused pass serves only those)."""
+        structured = ctypes.c_int(0)
+        call("oc_laplace_fused_set", self._h, int(bool(fused)), ctypes.byref(structured))
+        return bool(structured.value)
+
     def solve(self, options=None, raise_on_failure=True):
         """Static linear solve with the current Dirichlet conditions."""
         opts = options if options is not None else newton_options()

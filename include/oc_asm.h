@@ -177,6 +177,13 @@ int oc_dofs_set_device(oc_equations_set equations_set, const double *dofs_device
  * equations_set_routines.f90:981, 1295-1492): zero and assemble the linear
  * matrix values and the RHS vector. */
 int oc_equations_set_assemble(oc_equations_set equations_set);
+/* Laplace assembly form (no counterpart in the reference): 1 (the default)
+ * assembles a structured grid (nodes and elements numbered x fastest, as the
+ * regular generated mesh) in one fused pass with no element-matrix buffer;
+ * 0 always uses the element-matrix buffer and the CSR row gather.  Other
+ * meshes use the latter either way; the values are the same bit for bit.
+ * structured (may be NULL) returns whether the set is a structured grid. */
+int oc_laplace_fused_set(oc_equations_set equations_set, int fused, int *structured);
 /* Nonlinear residual (EQUATIONS_SET_RESIDUAL_EVALUATE ->
  * ..._STATIC_FEM, equations_set_routines.f90:5184, 5483-5629). */
 int oc_equations_set_residual_evaluate(oc_equations_set equations_set);
@@ -359,9 +366,10 @@ int oc_multigrid_structure_set(oc_equations_set equations_set, const int *n_per_
  * layer, then stalls GMRES). */
 int oc_multigrid_options_set(oc_equations_set equations_set, int pre_sweeps, int post_sweeps, double omega);
 /* Launch-shape autotuner (no counterpart in the reference): times the
- * element-matrix kernel's grid, the CSR row gather's prefetch depth and the
- * SpMV's lanes per row on this set at its current DOFs, keeps the fastest of
- * each for later launches, and writes the timings as JSON into report. */
+ * element-matrix kernel's grid, the CSR row gather's prefetch depth, the
+ * SpMV's lanes per row and the level-0 multigrid kernel's shape on this set at
+ * its current DOFs, keeps the fastest of each for later launches, and writes
+ * the timings as JSON into report. */
 int oc_autotune(oc_equations_set equations_set, int precision, char *report, int report_size);
 /* Euclidean norm of the residual over the free DOFs at the current DOFs. */
 int oc_residual_norm(oc_equations_set equations_set, double *norm);

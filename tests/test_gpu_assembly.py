@@ -59,6 +59,29 @@ def test_laplace_jittered_parity(gpu):
     assert rel(eq.matrix_values(), ovals) < FP64_TOL
 
 
+@pytest.mark.parametrize("n,jitter", [((1, 1, 1), 0.0), ((3, 5, 4), 0.2), ((17, 9, 40), 0.1), ((33, 2, 65), 0.15)])
+def test_laplace_fused_equals_two_pass(gpu, n, jitter):
+    """The fused structured-grid assembly (node columns of 15 x 7 over
+    32-layer segments: these sizes give partial tiles in x and y and several
+    z segments) writes the same CSR values bit for bit as the element buffer +
+    row gather, and both match the oracle."""
+    cm = gpu
+    mesh = cm.RegularMesh.generate(1, n, jitter=jitter, seed=31)
+    eq = cm.LaplaceEquationsSet(mesh)
+    assert eq.set_fused(False)  # a structured grid
+    eq.assemble()
+    two_pass = eq.matrix_values()
+    eq.set_fused(True)
+    eq.assemble()
+    fused = eq.matrix_values()
+    assert np.array_equal(fused, two_pass)
+    assert np.array_equal(eq.rhs(), np.zeros(eq.n_rows))
+    rp, ci = eq.matrix_structure(0)
+    ovals, _ = orc.laplace_assemble(mesh.elem_nodes, mesh.coords, rp.astype(np.int64), ci)
+    assert rel(fused, ovals) < FP64_TOL
+    eq.destroy()
+
+
 def _deformed_state(mesh, scale, seed):
     rng = np.random.default_rng(seed)
     nq, nl = mesh.n_nodes, mesh.n_nodes_l
@@ -108,6 +131,7 @@ def test_laplace_parity_shuffled_numbering(gpu):
     cm = gpu
     mesh = _permuted(cm, cm.RegularMesh.generate(1, (5, 4, 3), jitter=0.1, seed=4), 17)
     eq = cm.LaplaceEquationsSet(mesh)
+    assert not eq.set_fused(True)  # not numbered as a structured grid: the row gather serves it
     eq.assemble()
     rp, ci = eq.matrix_structure(0)
     orp, oci = _pattern_oracle_laplace(mesh)
@@ -149,9 +173,11 @@ def test_mooney_rivlin_16_config(gpu):
     eq.residual_evaluate()
     eq.jacobian_evaluate(cm.OC_FP64)
     rp, ci = eq.matrix_structure(0)
+    orp, oci = _pattern_oracle_mr(mesh)
+    assert np.array_equal(rp.astype(np.int64), orp) and np.array_equal(ci, oci)
     oref = orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0)
     assert rel(eq.residual(), oref) < FP64_TOL
-    oJ = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, rp.astype(np.int64), ci)
+    oJ = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, orp, oci)
     J = eq.matrix_values()
     assert rel(J, oJ) < FP64_TOL
     # repeated assembly is deterministic (ordered row gather)
@@ -172,7 +198,10 @@ def test_mooney_rivlin_32_config(gpu):
     eq.residual_evaluate()
     assert rel(eq.residual(), orc.mr_residual(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0)) < FP64_TOL
     rp, ci = eq.matrix_structure(0)
-    oJ = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, rp.astype(np.int64), ci)
+    orp, oci = _pattern_oracle_mr(mesh)
+    assert np.array_equal(rp.astype(np.int64), orp) and np.array_equal(ci, oci)
+    del rp, ci
+    oJ = orc.mr_jacobian(mesh.elem_nodes, mesh.elem_nodes_l, mesh.coords, d, 2.0, 6.0, orp, oci)
     eq.jacobian_evaluate(cm.OC_FP64)
     assert rel(eq.matrix_values(), oJ) < FP64_TOL
     eq.jacobian_evaluate(cm.OC_FP32)
