@@ -100,9 +100,9 @@ def laplace_line(cm, n, steps):
     eq = cm.LaplaceEquationsSet(mesh)
     nbytes = laplace_bytes(mesh.n_elements, mesh.n_nodes, eq.nnz)
     out = {"n": n, "elements": mesh.n_elements, "nnz": eq.nnz, "algorithmic_bytes": nbytes}
-    for form, fused, names in (("fused", True, ("laplace_fused",)),
-                               ("two_pass", False, ("laplace_element", "laplace_row_gather"))):
-        assert eq.set_fused(fused)
+    for form, fused, names in (("fused", 2, ("laplace_fused",)),
+                               ("two_pass", 0, ("laplace_element", "laplace_row_gather"))):
+        eq.set_fused(fused)
         ms, _ = time_steps(eq.assemble, steps, 2, cm)
         kms = sum(cm.kernel_timing(k)[0] for k in names)
         fms, _ = cm.kernel_timing("fill")
@@ -112,7 +112,9 @@ def laplace_line(cm, n, steps):
              "achieved_GBs": nbytes / ((kms + fms) / steps / 1e3) / 1e9}
         r["hbm_frac"] = r["achieved_GBs"] / HBM_PEAK_GBS
         out[form] = r
-    out.update(out["fused"])
+    # the default form: fused where the grid fills the GPU
+    out["default_form"] = "fused" if eq.set_fused(1) else "two_pass"
+    out.update(out[out["default_form"]])
     eq.destroy()
     return out
 

@@ -418,13 +418,14 @@ int oc_equations_set_subtype_set(oc_equations_set h, int subtype)
     return OC_NO_ERROR;
 }
 
-int oc_laplace_fused_set(oc_equations_set h, int fused, int *structured)
+int oc_laplace_fused_set(oc_equations_set h, int fused, int *active)
 {
     OC_CHECK(check_handle(h));
     if (h->s.cls != OC_EQUATIONS_SET_LAPLACE)
         return set_error(OC_ERR_INVALID_ARGUMENT, "the fused assembly form is defined for Laplace sets");
-    h->s.lap_fused = fused != 0;
-    if (structured) *structured = h->s.lap_grid[0] > 0 ? 1 : 0;
+    if (fused < 0 || fused > 2) return set_error(OC_ERR_INVALID_ARGUMENT, "fused form %d not in [0, 2]", fused);
+    h->s.lap_fused = fused;
+    if (active) *active = laplace_fused_serves(h->s) ? 1 : 0;
     return OC_NO_ERROR;
 }
 

@@ -81,6 +81,7 @@ void build_basis_tables(BasisTables &t)
     tabulate(2, 3, t.w27, t.quad27);
     tabulate(1, 3, wtmp, t.lin27);
     tabulate(1, 2, t.w8, t.lin8);
+    gauss_1d(2, t.xi2, wtmp);
 }
 
 static BasisTables *g_device_tables = nullptr;
@@ -156,31 +157,57 @@ laplace_slot_offsets_kernel(int n_elements, const int *__restrict__ conn, const 
 }
 
 // The 36 upper-triangle entries K[ms*(15-ms)/2 + ns] (ms <= ns) of one
-// element matrix from its node coordinates x[c][n], accumulated in registers
-// over the 8 Gauss points (the basis table is read with uniform addresses).
+// trilinear element matrix from its node coordinates x[c][n]
+// (Laplace_equations_routines.f90:474-571).  Restated for the tensor-product
+// structure of the trilinear basis at the 2x2x2 Gauss points:
+//   * dX/dxi by sum factorisation: along xi_d a trilinear map's derivative is
+//     the bilinear interpolant, in the other two coordinates, of its four edge
+//     differences along d, so 36 values serve all eight Gauss points;
+//   * the basis derivatives are products of the 1-D factors 1 - xi, xi;
+//   * GU J w = adj(GL) w sqrt|det| / det, one division per Gauss point
+//     (COORDINATE_METRICS_CALCULATE and INVERT_FULL_DP, whose zero-tolerance
+//     rule it keeps).
+// The Gauss loop unrolls, so every table index is a constant.
 __device__ __forceinline__ int lap_upper(int ms, int ns) { return ms * (15 - ms) / 2 + ns; }
+
+__device__ __forceinline__ double lerp(double a, double b, double q) { return fma(q, b - a, a); }
 
 __device__ __forceinline__ void laplace_element_upper(const double (&x)[3][8], const BasisTables *__restrict__ T,
                                                       double (&K)[36])
 {
+    const double qx[2] = {T->xi2[0], T->xi2[1]};
+    // D[c][d][2 gb + ga]: dX_c/dxi_d at Gauss indices (ga, gb) of the other
+    // two directions (ascending)
+    double D[3][3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int sd = 1 << d, sa = d == 0 ? 2 : 1, sb = d == 2 ? 2 : 4;
+            double e[2][2];
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+                for (int na = 0; na < 2; ++na) e[nb][na] = x[c][sd + sa * na + sb * nb] - x[c][sa * na + sb * nb];
+#pragma unroll
+            for (int gb = 0; gb < 2; ++gb)
+#pragma unroll
+                for (int ga = 0; ga < 2; ++ga)
+                    D[c][d][2 * gb + ga] = lerp(lerp(e[0][0], e[0][1], qx[ga]), lerp(e[1][0], e[1][1], qx[ga]), qx[gb]);
+        }
 #pragma unroll
     for (int i = 0; i < 36; ++i) K[i] = 0.0;
-#pragma unroll 1
+#pragma unroll
     for (int g = 0; g < 8; ++g) {
-        const double *f = T->lin8 + g * 8 * 4;
-        double dx[9];  // dx[c + 3 mi] = dX_c/dxi_mi (FIELD_INTERPOLATE_GAUSS)
+        const int g1 = g & 1, g2 = (g >> 1) & 1, g3 = g >> 2;
+        double dx[9];  // dx[c + 3 mi] = dX_c/dxi_mi
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int mi = 0; mi < 3; ++mi) {
-                double sum = 0.0;
-#pragma unroll
-                for (int n = 0; n < 8; ++n) sum += f[n * 4 + 1 + mi] * x[c][n];
-                dx[c + 3 * mi] = sum;
-            }
-        // GL = dX^T dX; GU JACOBIAN w_g = adj(GL) w_g sqrt|det| / det
-        // (COORDINATE_METRICS_CALCULATE and INVERT_FULL_DP with one division)
-        double gl[6];  // 00 11 22 01 02 12
+        for (int c = 0; c < 3; ++c) {
+            dx[c] = D[c][0][2 * g3 + g2];
+            dx[c + 3] = D[c][1][2 * g3 + g1];
+            dx[c + 6] = D[c][2][2 * g2 + g1];
+        }
+        double gl[6];  // GL = dX^T dX: 00 11 22 01 02 12
         gl[0] = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
         gl[1] = dx[3] * dx[3] + dx[4] * dx[4] + dx[5] * dx[5];
         gl[2] = dx[6] * dx[6] + dx[7] * dx[7] + dx[8] * dx[8];
@@ -193,16 +220,26 @@ __device__ __forceinline__ void laplace_element_upper(const double (&x)[3][8], c
         const double det = gl[0] * c00 + gl[3] * c01 + gl[4] * c02;
         const double sc = fabs(det) > 5.0 * 2.220446049250313e-16 ? T->w8[g] * sqrt(fabs(det)) / det : 0.0;
         const double A[9] = {c00 * sc, c01 * sc, c02 * sc, c01 * sc, c11 * sc, c12 * sc, c02 * sc, c12 * sc, c22 * sc};
+        // dphi_n/dxi at g: (+-1) times the other two directions' factors
+        const double L1[2] = {1.0 - qx[g1], qx[g1]}, L2[2] = {1.0 - qx[g2], qx[g2]}, L3[2] = {1.0 - qx[g3], qx[g3]};
+        double f[8][3];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const int n1 = n & 1, n2 = (n >> 1) & 1, n3 = n >> 2;
+            f[n][0] = (n1 ? 1.0 : -1.0) * (L2[n2] * L3[n3]);
+            f[n][1] = (n2 ? 1.0 : -1.0) * (L1[n1] * L3[n3]);
+            f[n][2] = (n3 ? 1.0 : -1.0) * (L1[n1] * L2[n2]);
+        }
         int idx = 0;
 #pragma unroll
         for (int ms = 0; ms < 8; ++ms) {
             double v[3];
 #pragma unroll
             for (int ni = 0; ni < 3; ++ni)
-                v[ni] = f[ms * 4 + 1] * A[0 + 3 * ni] + f[ms * 4 + 2] * A[1 + 3 * ni] + f[ms * 4 + 3] * A[2 + 3 * ni];
+                v[ni] = f[ms][0] * A[0 + 3 * ni] + f[ms][1] * A[1 + 3 * ni] + f[ms][2] * A[2 + 3 * ni];
 #pragma unroll
             for (int ns = ms; ns < 8; ++ns, ++idx)
-                K[idx] = fma(v[2], f[ns * 4 + 3], fma(v[1], f[ns * 4 + 2], fma(v[0], f[ns * 4 + 1], K[idx])));
+                K[idx] = fma(v[2], f[ns][2], fma(v[1], f[ns][1], fma(v[0], f[ns][0], K[idx])));
         }
     }
 }
@@ -302,7 +339,7 @@ laplace_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const
 // Fused Laplace assembly of a structured hexahedral grid (nx x ny x nz
 // elements, nodes and elements numbered x fastest, local nodes xi1 fastest):
 // no element-matrix buffer.  A block owns the CSR rows of a kLapTX x kLapTY
-// column of nodes over kLapSeg element layers and walks up the layers.  Per
+// column of nodes over `seg` element layers and walks up the layers.  Per
 // layer, each thread computes one element matrix of the column's element
 // footprint (one element wider on the low x and y sides) into LDS; each node
 // thread then
@@ -315,11 +352,11 @@ laplace_row_gather_kernel(int n_rows, const int64_t *__restrict__ row_ptr, const
 // element order from 0, as the two-pass row gather does, so values are
 // bit-identical to it.  Blocks share only recomputed footprint elements
 // (x, y halo and the layer below a segment), never output.
-constexpr int kLapTX = 15, kLapTY = 7, kLapSeg = 32;
+constexpr int kLapTX = 15, kLapTY = 7, kLapSeg = 8;
 constexpr int kLapFX = kLapTX + 1, kLapFusedThreads = (kLapTX + 1) * (kLapTY + 1);  // 128
 
 __global__ void __launch_bounds__(kLapFusedThreads)
-laplace_fused_kernel(int nx, int ny, int nz, const double *__restrict__ coords, int n_nodes,
+laplace_fused_kernel(int nx, int ny, int nz, int seg, const double *__restrict__ coords, int n_nodes,
                      const BasisTables *__restrict__ T, const int64_t *__restrict__ row_ptr, double *__restrict__ values)
 {
     __shared__ double sK[36][kLapFusedThreads];  // element matrices of the layer (upper triangles)
@@ -329,7 +366,7 @@ laplace_fused_kernel(int nx, int ny, int nz, const double *__restrict__ coords, 
     const int i0 = (b % tiles_x) * kLapTX;
     b /= tiles_x;
     const int j0 = (b % tiles_y) * kLapTY;
-    const int k0 = (b / tiles_y) * kLapSeg, k1 = min(nz, k0 + kLapSeg);
+    const int k0 = (b / tiles_y) * seg, k1 = min(nz, k0 + seg);
     const int NX = nx + 1;
     const int64_t NXY = static_cast<int64_t>(nx + 1) * (ny + 1);
     // this thread's footprint element and node column
@@ -475,6 +512,21 @@ void laplace_detect_grid(EquationsSet &s, const std::vector<int32_t> &conn)
     s.lap_grid[2] = static_cast<int>(nz);
 }
 
+// Blocks of the fused launch with kLapSeg-layer segments; the fused form
+// serves a grid only when they fill the GPU several times over (a block walks
+// its layers in sequence, so a small grid runs faster as the two passes).
+static int64_t laplace_fused_blocks(const EquationsSet &s)
+{
+    const int nx = s.lap_grid[0], ny = s.lap_grid[1], nz = s.lap_grid[2];
+    return static_cast<int64_t>((nx + kLapTX) / kLapTX) * ((ny + kLapTY) / kLapTY) * ((nz + kLapSeg - 1) / kLapSeg);
+}
+
+bool laplace_fused_serves(const EquationsSet &s)
+{
+    if (s.lap_grid[0] <= 0 || s.lap_fused == 0) return false;
+    return s.lap_fused == 2 || laplace_fused_blocks(s) >= 4 * static_cast<int64_t>(runtime().num_cus);
+}
+
 int launch_laplace_fused(EquationsSet &s)
 {
     const BasisTables *T = nullptr;
@@ -483,11 +535,9 @@ int launch_laplace_fused(EquationsSet &s)
     const int nx = s.lap_grid[0], ny = s.lap_grid[1], nz = s.lap_grid[2];
     if (nx < 1 || ny < 1 || nz < 1) return set_error(OC_ERR_UNSUPPORTED, "the Laplace set is not a structured grid");
     OC_CHECK(fill<double>(s.rhs.ptr, s.n_rows, 0.0));  // RHS element vector is 0 (:542)
-    const int64_t blocks = static_cast<int64_t>((nx + kLapTX) / kLapTX) * ((ny + kLapTY) / kLapTY) *
-                           ((nz + kLapSeg - 1) / kLapSeg);
     KernelTimer timer("laplace_fused");
-    laplace_fused_kernel<<<static_cast<unsigned>(blocks), kLapFusedThreads, 0, rt.stream>>>(
-        nx, ny, nz, s.coords.ptr, s.n_nodes, T, s.row_ptr.ptr, s.values.ptr);
+    laplace_fused_kernel<<<static_cast<unsigned>(laplace_fused_blocks(s)), kLapFusedThreads, 0, rt.stream>>>(
+        nx, ny, nz, kLapSeg, s.coords.ptr, s.n_nodes, T, s.row_ptr.ptr, s.values.ptr);
     OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
 }
@@ -550,7 +600,7 @@ static int laplace_prepare(EquationsSet &s)
 
 int launch_laplace_assemble(EquationsSet &s)
 {
-    if (s.lap_grid[0] > 0 && s.lap_fused) return launch_laplace_fused(s);
+    if (laplace_fused_serves(s)) return launch_laplace_fused(s);
     const BasisTables *T = nullptr;
     OC_CHECK(ensure_tables(&T));
     OC_CHECK(laplace_prepare(s));

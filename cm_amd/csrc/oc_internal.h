@@ -211,9 +211,10 @@ struct EquationsSet {
     // in gather order)
     DevBuf<uint8_t> slot_offsets8;
     // Laplace sets on a structured grid: elements per direction (zero when
-    // the mesh is not one), and whether the fused assembly serves it
+    // the mesh is not one), and when the fused assembly serves it: 0 never,
+    // 1 when the grid fills the GPU, 2 always (oc_laplace_fused_set)
     int lap_grid[3] = {0, 0, 0};
-    bool lap_fused = true;
+    int lap_fused = 1;
 
     // Dirichlet conditions (0-based DOFs, full values) and the free-DOF mask;
     // bc_version counts oc_boundary_conditions_set calls
@@ -294,6 +295,7 @@ int build_pattern(int ncomp, int n_elements, const int32_t *nn, const int32_t *n
 // ---- kernel launchers --------------------------------------------------------
 int launch_laplace_assemble(EquationsSet &s);
 void laplace_detect_grid(EquationsSet &s, const std::vector<int32_t> &conn);
+bool laplace_fused_serves(const EquationsSet &s);  // the fused form assembles this set
 int mr_prepare(EquationsSet &s);  // builds the scatter map once
 int launch_mr_residual(EquationsSet &s);
 int launch_mr_jacobian(EquationsSet &s, int precision);

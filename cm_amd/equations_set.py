@@ -212,13 +212,14 @@ class LaplaceEquationsSet(EquationsSet):
         """Zero and assemble the linear matrix and RHS vector on the GPU."""
         call("oc_equations_set_assemble", self._h)
 
-    def set_fused(self, fused=True):
-        """Fused one-pass assembly of a structured grid (True, the default) or
-        the element-matrix buffer + row gather; returns whether the mesh is a
-        structured grid (the fused pass serves only those)."""
-        structured = ctypes.c_int(0)
-        call("oc_laplace_fused_set", self._h, int(bool(fused)), ctypes.byref(structured))
-        return bool(structured.value)
+    def set_fused(self, fused=1):
+        """Assembly form: 1 (the default) fused one-pass assembly of a
+        structured grid large enough to fill the GPU, 2 fused for any
+        structured grid, 0 the element-matrix buffer + row gather; returns
+        whether the fused pass now assembles this set."""
+        active = ctypes.c_int(0)
+        call("oc_laplace_fused_set", self._h, int(fused), ctypes.byref(active))
+        return bool(active.value)
 
     def solve(self, options=None, raise_on_failure=True):
         """Static linear solve with the current Dirichlet conditions."""

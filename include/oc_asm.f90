@@ -147,11 +147,11 @@ MODULE OC_ASM
       TYPE(C_PTR), VALUE :: handle
     END FUNCTION oc_equations_set_assemble
 
-    INTEGER(C_INT) FUNCTION oc_laplace_fused_set(handle, fused, structured) BIND(C, NAME="oc_laplace_fused_set")
+    INTEGER(C_INT) FUNCTION oc_laplace_fused_set(handle, fused, active) BIND(C, NAME="oc_laplace_fused_set")
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: handle
       INTEGER(C_INT), VALUE :: fused
-      INTEGER(C_INT), INTENT(OUT) :: structured
+      INTEGER(C_INT), INTENT(OUT) :: active
     END FUNCTION oc_laplace_fused_set
 
     INTEGER(C_INT) FUNCTION oc_equations_set_residual_evaluate(handle) &

@@ -177,13 +177,14 @@ int oc_dofs_set_device(oc_equations_set equations_set, const double *dofs_device
  * equations_set_routines.f90:981, 1295-1492): zero and assemble the linear
  * matrix values and the RHS vector. */
 int oc_equations_set_assemble(oc_equations_set equations_set);
-/* Laplace assembly form (no counterpart in the reference): 1 (the default)
- * assembles a structured grid (nodes and elements numbered x fastest, as the
- * regular generated mesh) in one fused pass with no element-matrix buffer;
- * 0 always uses the element-matrix buffer and the CSR row gather.  Other
- * meshes use the latter either way; the values are the same bit for bit.
- * structured (may be NULL) returns whether the set is a structured grid. */
-int oc_laplace_fused_set(oc_equations_set equations_set, int fused, int *structured);
+/* Laplace assembly form (no counterpart in the reference).  A structured grid
+ * (nodes and elements numbered x fastest, as the regular generated mesh) can
+ * assemble in one fused pass with no element-matrix buffer: fused = 1 (the
+ * default) does so when the grid is large enough to fill the GPU, 2 whenever
+ * the mesh is such a grid, 0 never (element-matrix buffer and CSR row gather).
+ * Other meshes always use the latter; the values are the same bit for bit.
+ * active (may be NULL) returns whether the fused pass now assembles the set. */
+int oc_laplace_fused_set(oc_equations_set equations_set, int fused, int *active);
 /* Nonlinear residual (EQUATIONS_SET_RESIDUAL_EVALUATE ->
  * ..._STATIC_FEM, equations_set_routines.f90:5184, 5483-5629). */
 int oc_equations_set_residual_evaluate(oc_equations_set equations_set);

@@ -62,16 +62,16 @@ def test_laplace_jittered_parity(gpu):
 @pytest.mark.parametrize("n,jitter", [((1, 1, 1), 0.0), ((3, 5, 4), 0.2), ((17, 9, 40), 0.1), ((33, 2, 65), 0.15)])
 def test_laplace_fused_equals_two_pass(gpu, n, jitter):
     """The fused structured-grid assembly (node columns of 15 x 7 over
-    32-layer segments: these sizes give partial tiles in x and y and several
+    8-layer segments: these sizes give partial tiles in x and y and several
     z segments) writes the same CSR values bit for bit as the element buffer +
     row gather, and both match the oracle."""
     cm = gpu
     mesh = cm.RegularMesh.generate(1, n, jitter=jitter, seed=31)
     eq = cm.LaplaceEquationsSet(mesh)
-    assert eq.set_fused(False)  # a structured grid
+    assert not eq.set_fused(0)
     eq.assemble()
     two_pass = eq.matrix_values()
-    eq.set_fused(True)
+    assert eq.set_fused(2)  # a structured grid: fused at any size
     eq.assemble()
     fused = eq.matrix_values()
     assert np.array_equal(fused, two_pass)
@@ -131,7 +131,7 @@ def test_laplace_parity_shuffled_numbering(gpu):
     cm = gpu
     mesh = _permuted(cm, cm.RegularMesh.generate(1, (5, 4, 3), jitter=0.1, seed=4), 17)
     eq = cm.LaplaceEquationsSet(mesh)
-    assert not eq.set_fused(True)  # not numbered as a structured grid: the row gather serves it
+    assert not eq.set_fused(2)  # not numbered as a structured grid: the row gather serves it
     eq.assemble()
     rp, ci = eq.matrix_structure(0)
     orp, oci = _pattern_oracle_laplace(mesh)
