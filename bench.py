@@ -317,9 +317,11 @@ def main():
     ap.add_argument("--newton-rtol", type=float, default=1e-5,
                     help="GMRES forcing of every Newton step in the load-step comparison (<= 0: Eisenstat-Walker)")
     ap.add_argument("--batched-meshes", type=int, default=256, help="0 skips the batched-meshes line")
-    ap.add_argument("--profile", choices=["jacobian", "jacobian32", "laplace", "copy", "newton"], default=None,
+    ap.add_argument("--profile", choices=["jacobian", "jacobian32", "laplace", "laplace2", "copy", "newton"],
+                    default=None,
                     help="run only one kernel sequence for rocprofv3: the FP64 (or FP32) Jacobian assembly of the "
-                         "MR workload (--steps times), the 128^3 Laplace assembly, a device copy of 512 MiB (the "
+                         "MR workload (--steps times), the 128^3 Laplace assembly (fused; laplace2: element buffer "
+                         "+ row gather), a device copy of 512 MiB (the "
                          "FETCH_SIZE/WRITE_SIZE calibration), or --steps load steps of the --n case (20%% in 10 "
                          "steps) with --profile-start learned or plain")
     ap.add_argument("--profile-start", choices=["learned", "plain"], default="learned")
@@ -342,13 +344,14 @@ def main():
 
     import cm_amd as cm
     cm.initialise(0)
-    if args.profile == "laplace":
+    if args.profile in ("laplace", "laplace2"):
         mesh = cm.RegularMesh.generate(1, 128)
         eq = cm.LaplaceEquationsSet(mesh)
+        eq.set_fused(1 if args.profile == "laplace" else 0)
         for _ in range(args.steps):
             eq.assemble()
         cm.synchronise()
-        print(json.dumps({"profile": "laplace", "steps": args.steps, "elements": mesh.n_elements, "nnz": eq.nnz,
+        print(json.dumps({"profile": args.profile, "steps": args.steps, "elements": mesh.n_elements, "nnz": eq.nnz,
                           "algorithmic_bytes": laplace_bytes(mesh.n_elements, mesh.n_nodes, eq.nnz)}))
         return
     if args.profile == "newton":
