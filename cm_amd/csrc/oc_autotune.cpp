@@ -2,9 +2,10 @@
 // Times, on the set's own mesh(es) and current DOFs, every candidate of
 //   * the element-matrix kernel's persistent grid (blocks per CU),
 //   * the CSR row gather's prefetch depth (element rows per chunk),
-//   * the SpMV's lanes per CSR row,
-// keeps the fastest of each in the set (later launches use them) and
-// reports the timings as JSON.
+//   * the SpMV's lanes per CSR row and neighbours in flight per lane,
+//   * the level-0 multigrid node kernel's shape,
+// keeps in the set (for later launches) the fastest of each when it beats the
+// default by more than the timing noise, and reports the timings as JSON.
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -34,6 +35,42 @@ int best_of(F launch, int reps, float *ms_out)
     return OC_NO_ERROR;
 }
 
+// A candidate replaces the default shape only when it is faster by more than
+// kMargin: the timings repeat to about 1-2%, and a choice inside that band is
+// noise.
+constexpr float kMargin = 0.03f;
+
+struct Choice {
+    int index;       // chosen candidate
+    float ms;        // its best time
+    float default_ms;
+};
+
+// Times candidates 0..n-1 (best of `reps` each), appends "name": {label: ms,
+// ...} to js and picks the fastest unless it beats candidate `def` by less
+// than kMargin.
+template <typename Launch, typename Label>
+int tune(std::string &js, const char *name, int n, int def, int reps, Launch launch, Label label, Choice *out)
+{
+    char buf[128];
+    std::vector<float> ms(static_cast<size_t>(n));
+    js += ", \"";
+    js += name;
+    js += "\": {";
+    for (int i = 0; i < n; ++i) {
+        OC_CHECK(best_of([&] { return launch(i); }, reps, &ms[static_cast<size_t>(i)]));
+        std::snprintf(buf, sizeof buf, "%s\"%s\": %.4f", i ? ", " : "", label(i).c_str(), ms[static_cast<size_t>(i)]);
+        js += buf;
+    }
+    js += "}";
+    int best = def;
+    for (int i = 0; i < n; ++i)
+        if (ms[static_cast<size_t>(i)] < ms[static_cast<size_t>(best)]) best = i;
+    if (ms[static_cast<size_t>(best)] > (1.0f - kMargin) * ms[static_cast<size_t>(def)]) best = def;
+    *out = {best, ms[static_cast<size_t>(best)], ms[static_cast<size_t>(def)]};
+    return OC_NO_ERROR;
+}
+
 }  // namespace
 }  // namespace oc
 
@@ -50,104 +87,79 @@ extern "C" int oc_autotune(oc_equations_set h, int precision, char *report, int 
         return set_error(OC_ERR_INVALID_ARGUMENT, "precision must be OC_FP64 or OC_FP32");
     OC_CHECK(mr_jacobian_buffers(s, precision));
     OC_CHECK(launch_mr_gauss(s));
+    const bool fp32 = precision == OC_FP32;
     std::string js = "{\"precision\": \"";
-    js += precision == OC_FP64 ? "fp64" : "fp32";
+    js += fp32 ? "fp32" : "fp64";
     js += "\"";
-    char buf[256];
     const int reps = 3;
+    auto num = [](int v) { return std::to_string(v); };
 
-    js += ", \"element_matrix_ms\": {";
-    int best_bpc = 4;
-    float best_ms = 1e30f;
+    // element-matrix grid (blocks per CU; default 1)
     const int bpcs[] = {1, 2, 3, 4, 6, 8};
-    for (int i = 0; i < 6; ++i) {
-        float ms = 0.0f;
-        OC_CHECK(best_of([&] { return launch_mr_element_matrices(s, precision, bpcs[i]); }, reps, &ms));
-        std::snprintf(buf, sizeof buf, "%s\"%d\": %.4f", i ? ", " : "", bpcs[i], ms);
-        js += buf;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best_bpc = bpcs[i];
-        }
-    }
-    js += "}";
-
-    js += ", \"row_gather_ms\": {";
-    int best_chunk = 8;
-    best_ms = 1e30f;
+    Choice c_elem{};
+    OC_CHECK(tune(js, "element_matrix_ms", 6, 0, reps,
+                  [&](int i) { return launch_mr_element_matrices(s, precision, bpcs[i]); },
+                  [&](int i) { return num(bpcs[i]); }, &c_elem));
+    // CSR row gather prefetch depth (element rows per chunk; default 8)
     const int chunks[] = {4, 8, 16};
-    for (int i = 0; i < 3; ++i) {
-        float ms = 0.0f;
-        OC_CHECK(best_of([&] { return launch_mr_row_gather(s, precision, chunks[i]); }, reps, &ms));
-        std::snprintf(buf, sizeof buf, "%s\"%d\": %.4f", i ? ", " : "", chunks[i], ms);
-        js += buf;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best_chunk = chunks[i];
-        }
-    }
-    js += "}";
+    Choice c_gather{};
+    OC_CHECK(tune(js, "row_gather_ms", 3, 1, reps, [&](int i) { return launch_mr_row_gather(s, precision, chunks[i]); },
+                  [&](int i) { return num(chunks[i]); }, &c_gather));
 
-    // SpMV: lanes per row (per node for node rows) x neighbours in flight per lane
+    // SpMV: lanes per row (per node for node rows) x neighbours in flight per
+    // lane; default 32 x 1 (FP32 values) or 16 x 2 (FP64) on node rows
     OC_CHECK(prepare_node_rows(s));
-    js += ", \"spmv_ms\": {";
     DevBuf<double> y;
     OC_CHECK(y.alloc(s.n_rows));
-    int best_lanes = 32, best_unroll = 1;
-    best_ms = 1e30f;
-    const int lanes[] = {8, 16, 32, 64};
-    bool first = true;
-    for (int i = 0; i < 4; ++i)
-        for (int u = 1; u <= 2; ++u) {
-            if (u == 2 && (s.node_rows != 1 || lanes[i] < 16)) continue;  // unrolled form: node rows only
-            float ms = 0.0f;
-            OC_CHECK(best_of(
-                [&] { return launch_spmv_lanes(s, precision == OC_FP32, lanes[i], s.dofs.ptr, y.ptr, u); }, reps,
-                &ms));
-            std::snprintf(buf, sizeof buf, "%s\"%dx%d\": %.4f", first ? "" : ", ", lanes[i], u, ms);
-            first = false;
-            js += buf;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best_lanes = lanes[i];
-                best_unroll = u;
-            }
-        }
-    js += "}";
-    // level-0 multigrid node kernel (residual form), when the rows have the node layout
+    std::vector<std::pair<int, int>> spmv;
+    for (int l : {8, 16, 32, 64})
+        for (int u = 1; u <= 2; ++u)
+            if (u == 1 || (s.node_rows == 1 && l >= 16)) spmv.push_back({l, u});  // unrolled form: node rows only
+    const std::pair<int, int> spmv_def = s.node_rows == 1 && !fp32 ? std::make_pair(16, 2) : std::make_pair(32, 1);
+    int spmv_def_i = 0;
+    for (size_t i = 0; i < spmv.size(); ++i)
+        if (spmv[i] == spmv_def) spmv_def_i = static_cast<int>(i);
+    Choice c_spmv{};
+    OC_CHECK(tune(js, "spmv_ms", static_cast<int>(spmv.size()), spmv_def_i, reps,
+                  [&](int i) {
+                      return launch_spmv_lanes(s, fp32, spmv[static_cast<size_t>(i)].first, s.dofs.ptr, y.ptr,
+                                               spmv[static_cast<size_t>(i)].second);
+                  },
+                  [&](int i) {
+                      return num(spmv[static_cast<size_t>(i)].first) + "x" + num(spmv[static_cast<size_t>(i)].second);
+                  },
+                  &c_spmv));
+
+    // level-0 multigrid node kernel (residual form), when the rows have the
+    // node layout; default 16 x 2 for FP64 values, 32 x 1 otherwise
     int best_smoother = -1;
     if (s.node_rows == 1 && !s.free_mask.ptr) {  // no boundary conditions yet: every DOF free
         OC_CHECK(s.free_mask.alloc(s.n_rows));
         OC_HIP_CHECK(hipMemset(s.free_mask.ptr, 1, s.n_rows));
     }
     if (s.node_rows == 1) {
-        js += ", \"smoother_residual_ms\": {";
         DevBuf<double> r;
         OC_CHECK(r.alloc(static_cast<size_t>(3) * s.n_nodes));
-        best_ms = 1e30f;
         const char *names[] = {"32x1", "32x2", "16x2", "16x4", "64x1"};
-        for (int v = 0; v < 5; ++v) {
-            float ms = 0.0f;
-            OC_CHECK(best_of([&] { return launch_smoother_residual(s, precision, v, s.dofs.ptr, s.dofs.ptr, r.ptr); },
-                             reps, &ms));
-            std::snprintf(buf, sizeof buf, "%s\"%s\": %.4f", v ? ", " : "", names[v], ms);
-            js += buf;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best_smoother = v;
-            }
-        }
-        js += "}";
+        Choice c_sm{};
+        OC_CHECK(tune(js, "smoother_residual_ms", 5, fp32 ? 0 : 2, reps,
+                      [&](int v) { return launch_smoother_residual(s, precision, v, s.dofs.ptr, s.dofs.ptr, r.ptr); },
+                      [&](int v) { return std::string(names[v]); }, &c_sm));
+        best_smoother = c_sm.index;
     }
     OC_HIP_CHECK(hipStreamSynchronize(runtime().stream));
+    const int best_bpc = bpcs[c_elem.index], best_chunk = chunks[c_gather.index];
+    const int best_lanes = spmv[static_cast<size_t>(c_spmv.index)].first;
+    const int best_unroll = spmv[static_cast<size_t>(c_spmv.index)].second;
     s.tuned_element_blocks_per_cu = best_bpc;
     s.tuned_gather_chunk = best_chunk;
     s.tuned_spmv_lanes = best_lanes;
     s.tuned_spmv_unroll = best_unroll;
     if (best_smoother >= 0) s.tuned_smoother = best_smoother;
-    std::snprintf(buf, sizeof buf, ", \"chosen\": {\"element_blocks_per_cu\": %d, \"gather_chunk\": %d, "
+    char buf[256];
+    std::snprintf(buf, sizeof buf, ", \"margin\": %.2f, \"chosen\": {\"element_blocks_per_cu\": %d, \"gather_chunk\": %d, "
                                    "\"spmv_lanes\": %d, \"spmv_unroll\": %d, \"smoother_variant\": %d}}",
-                  best_bpc, best_chunk, best_lanes, best_unroll, best_smoother);
+                  kMargin, best_bpc, best_chunk, best_lanes, best_unroll, best_smoother);
     js += buf;
     if (report && report_size > 0) std::snprintf(report, static_cast<size_t>(report_size), "%s", js.c_str());
     return OC_NO_ERROR;

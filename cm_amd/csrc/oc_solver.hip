@@ -78,10 +78,11 @@ __global__ void __launch_bounds__(256) mesh_weights_kernel(const int *__restrict
     }
 }
 
-// partial[m * gridDim.x + blockIdx.x] = sum of x_r^2 over the free rows of mesh m
-__global__ void __launch_bounds__(256) mesh_norms_kernel(const int *__restrict__ q_off, const int *__restrict__ l_off,
-                                                        int nq_total, const uint8_t *__restrict__ free_mask,
-                                                        const double *__restrict__ x, double *__restrict__ partial)
+// partial[m * gridDim.x + blockIdx.x] = sum of x_r y_r over the free rows of mesh m
+__global__ void __launch_bounds__(256) mesh_dots_kernel(const int *__restrict__ q_off, const int *__restrict__ l_off,
+                                                       int nq_total, const uint8_t *__restrict__ free_mask,
+                                                       const double *__restrict__ x, const double *__restrict__ y,
+                                                       double *__restrict__ partial)
 {
     __shared__ double sh[256];
     const int m = blockIdx.y;
@@ -90,7 +91,7 @@ __global__ void __launch_bounds__(256) mesh_norms_kernel(const int *__restrict__
         bool valid;
         const int64_t r = mesh_row(m, i, q_off, l_off, nq_total, &valid);
         if (!valid) break;
-        if (free_mask[r]) acc += x[r] * x[r];
+        if (free_mask[r]) acc += x[r] * y[r];
     }
     sh[threadIdx.x] = acc;
     __syncthreads();
@@ -99,6 +100,22 @@ __global__ void __launch_bounds__(256) mesh_norms_kernel(const int *__restrict__
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[m * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// x = x0 + lambda[m] dx on every row of mesh m (the per-mesh line search)
+__global__ void __launch_bounds__(256) mesh_axpy_kernel(const int *__restrict__ q_off, const int *__restrict__ l_off,
+                                                       int nq_total, const double *__restrict__ lambda,
+                                                       const double *__restrict__ dx, const double *__restrict__ x0,
+                                                       double *__restrict__ x)
+{
+    const int m = blockIdx.y;
+    const double a = lambda[m];
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;; i += gridDim.x * blockDim.x) {
+        bool valid;
+        const int64_t r = mesh_row(m, i, q_off, l_off, nq_total, &valid);
+        if (!valid) break;
+        x[r] = a * dx[r] + x0[r];
+    }
 }
 
 // partial[j * gridDim.x + blockIdx.x] = sum over this block's slice of V_j . w
@@ -1024,15 +1041,16 @@ int launch_spmv_lanes(EquationsSet &s, bool fp32, int lanes, const double *x, do
     return K.launch_spmv<double>(s.values.ptr, lanes, nullptr, x, y, unroll);
 }
 
-// Per-mesh norms of x over free rows (Mooney-Rivlin sets), one device->host copy.
-static int mesh_norms(EquationsSet &s, const double *x, std::vector<double> &out)
+// Per-mesh dot products x . y over free rows (Mooney-Rivlin sets), one
+// device->host copy.
+static int mesh_dots(EquationsSet &s, const double *x, const double *y, std::vector<double> &out)
 {
     Runtime &rt = runtime();
     const int M = s.n_meshes;
     const int chunks = M == 1 ? 256 : 32;
     dim3 grid(chunks, M);
-    mesh_norms_kernel<<<grid, 256, 0, rt.stream>>>(s.q_off.ptr, s.l_off.ptr, s.n_nodes, s.free_mask.ptr, x,
-                                                   s.partials.ptr);
+    mesh_dots_kernel<<<grid, 256, 0, rt.stream>>>(s.q_off.ptr, s.l_off.ptr, s.n_nodes, s.free_mask.ptr, x, y,
+                                                  s.partials.ptr);
     OC_HIP_CHECK(hipGetLastError());
     std::vector<double> h(static_cast<size_t>(M) * chunks);
     OC_HIP_CHECK(hipMemcpyAsync(h.data(), s.partials.ptr, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
@@ -1042,8 +1060,16 @@ static int mesh_norms(EquationsSet &s, const double *x, std::vector<double> &out
     for (int m = 0; m < M; ++m) {
         double acc = 0.0;
         for (int c = 0; c < chunks; ++c) acc += h[static_cast<size_t>(m) * chunks + c];
-        out[m] = std::sqrt(acc);
+        out[m] = acc;
     }
+    return OC_NO_ERROR;
+}
+
+// Per-mesh norms of x over free rows.
+static int mesh_norms(EquationsSet &s, const double *x, std::vector<double> &out)
+{
+    OC_CHECK(mesh_dots(s, x, x, out));
+    for (double &v : out) v = std::sqrt(v);
     return OC_NO_ERROR;
 }
 
@@ -1107,6 +1133,99 @@ static double plain_norm(const std::vector<double> &nrm)
     double acc = 0.0;
     for (double v : nrm) acc += v * v;
     return std::sqrt(acc);
+}
+
+// The backtracking line search of a batch, one per mesh: mesh m steps to
+// x_prev + lam_m dx and backtracks (quadratic, then cubic interpolation,
+// safeguarded to [0.1, 0.5] lam) until f_m = 1/2 (w_m ||F_m||)^2 meets
+// f_m(0) + alpha lam_m slope_m, with slope_m = (W F)_m . (W J dx)_m; a mesh
+// that has met it keeps its step while the others go on.  A mesh already
+// within its tolerance stays where it is (lam_m = 0), as its own solve would
+// have stopped.  Every trial is one residual launch over all meshes.  On return lam holds the steps taken, trial
+// the per-mesh residual norms there, *evals the trials run and *decreased
+// whether every mesh met its condition within 20 trials (the last trial is
+// kept otherwise, as for one mesh).
+static int batch_line_search(EquationsSet &s, const oc_newton_options *opt, const std::vector<double> &w,
+                             const std::vector<double> &target, const std::vector<double> &nrm, const double *rhs,
+                             const double *wjdx, const double *dx,
+                             const double *x_prev, std::vector<double> &lam, std::vector<double> &trial, int *evals,
+                             bool *decreased)
+{
+    Runtime &rt = runtime();
+    const int M = s.n_meshes;
+    std::vector<double> fjdx;
+    OC_CHECK(mesh_dots(s, rhs, wjdx, fjdx));  // rhs = -W F
+    std::vector<double> slope(M), f0(M), lam_prev(M, 0.0), f_prev(M, 0.0);
+    std::vector<int> tries(M, 0);
+    std::vector<char> done(M, 0);
+    for (int m = 0; m < M; ++m) {
+        slope[m] = -fjdx[m];
+        if (slope[m] > 0.0) slope[m] = -slope[m];
+        f0[m] = 0.5 * (w[m] * nrm[m]) * (w[m] * nrm[m]);
+        if (slope[m] == 0.0) slope[m] = -1.0;
+    }
+    lam.assign(M, 1.0);
+    for (int m = 0; m < M; ++m)
+        if (nrm[m] <= target[m]) {
+            lam[m] = 0.0;
+            done[m] = 1;
+        }
+    DevBuf<double> lam_dev;
+    OC_CHECK(lam_dev.alloc(M));
+    *decreased = false;
+    int ls = 0;
+    for (; ls < 20; ++ls) {
+        OC_HIP_CHECK(hipMemcpyAsync(lam_dev.ptr, lam.data(), sizeof(double) * M, hipMemcpyHostToDevice, rt.stream));
+        mesh_axpy_kernel<<<dim3(32, M), 256, 0, rt.stream>>>(s.q_off.ptr, s.l_off.ptr, s.n_nodes, lam_dev.ptr, dx,
+                                                              x_prev, s.dofs.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+        OC_CHECK(residual_mesh_norms(s, trial));
+        if (!opt->line_search) {
+            *decreased = true;
+            ++ls;
+            break;
+        }
+        bool all = true;
+        for (int m = 0; m < M; ++m) {
+            if (done[m]) continue;
+            const double f1 = 0.5 * (w[m] * trial[m]) * (w[m] * trial[m]);
+            if (std::isfinite(f1) && f1 <= f0[m] + opt->line_search_alpha * lam[m] * slope[m]) {
+                done[m] = 1;
+                continue;
+            }
+            all = false;
+            const double l = lam[m], sl = slope[m];
+            double lnew;
+            if (tries[m] == 0 || !std::isfinite(f1)) {
+                lnew = std::isfinite(f1) ? -sl / (2.0 * (f1 - f0[m] - sl)) : 0.1;
+            } else {
+                const double lp = lam_prev[m];
+                const double r1 = f1 - f0[m] - l * sl, r2 = f_prev[m] - f0[m] - lp * sl;
+                const double a = (r1 / (l * l) - r2 / (lp * lp)) / (l - lp);
+                const double b = (-lp * r1 / (l * l) + l * r2 / (lp * lp)) / (l - lp);
+                if (a == 0.0) lnew = -sl / (2.0 * b);
+                else {
+                    const double disc = b * b - 3.0 * a * sl;
+                    lnew = disc < 0.0 ? 0.5 * l : (-b + std::sqrt(disc)) / (3.0 * a);
+                }
+            }
+            lam_prev[m] = l;
+            f_prev[m] = f1;
+            ++tries[m];
+            lam[m] = std::min(0.5 * l, std::max(0.1 * l, lnew));
+        }
+        if (all) {
+            *decreased = true;
+            ++ls;
+            break;
+        }
+    }
+    if (!*decreased) {  // the meshes still backtracking keep their last trial
+        for (int m = 0; m < M; ++m)
+            if (!done[m]) lam[m] = lam_prev[m];
+    }
+    *evals = ls;
+    return OC_NO_ERROR;
 }
 
 // Inexact Newton with backtracking line search on every mesh of the set at
@@ -1241,18 +1360,31 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
         // line search on f = 1/2 ||W F||^2; initial slope (W F) . (W J dx), as
         // PETSc's bt line search takes it (made negative if it is not)
         OC_CHECK(K.spmv(dx, K.u));
-        double fjdx = 0.0;
-        OC_CHECK(K.multi_dot(rhs, 1, K.u, &fjdx));  // rhs = -W F
-        double slope = -fjdx;
-        if (slope > 0.0) slope = -slope;
-        if (slope == 0.0) slope = -1.0;
         OC_HIP_CHECK(hipMemcpyAsync(x_prev, s.dofs.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, rt.stream));
         double lambda = 1.0, lambda_taken = 1.0, fnew = 0.0;
         const double f0 = 0.5 * fnorm * fnorm;
-        double lambda_prev = 0.0, f_prev_trial = 0.0;
-        std::vector<double> trial;
+        std::vector<double> trial, lam_mesh;  // lam_mesh: the step length of each mesh of a batch
         bool decreased = !opt->line_search;
         int ls = 0;
+        double slope = 0.0;
+        if (M > 1) {
+            // a batch backtracks every mesh on its own f_m = 1/2 ||W F_m||^2,
+            // as independent solves would, in one residual launch per trial
+            OC_CHECK(batch_line_search(s, opt, w, target, nrm, rhs, K.u, dx, x_prev, lam_mesh, trial, &ls, &decreased));
+            stats->residual_evaluations += ls;
+            fnew = weighted_norm(trial, w);
+            lambda_taken = 0.0;  // the shortest step of the meshes that moved (monitor output)
+            for (double l : lam_mesh)
+                if (l > 0.0) lambda_taken = lambda_taken > 0.0 ? std::min(lambda_taken, l) : l;
+            if (!decreased) ls = 20;
+            else --ls;
+        } else {
+        double fjdx = 0.0;
+        OC_CHECK(K.multi_dot(rhs, 1, K.u, &fjdx));  // rhs = -W F
+        slope = -fjdx;
+        if (slope > 0.0) slope = -slope;
+        if (slope == 0.0) slope = -1.0;
+        double lambda_prev = 0.0, f_prev_trial = 0.0;
         for (; ls < 20; ++ls) {
             OC_HIP_CHECK(hipMemcpyAsync(s.dofs.ptr, x_prev, sizeof(double) * n, hipMemcpyDeviceToDevice,
                                         rt.stream));
@@ -1286,6 +1418,7 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
             f_prev_trial = f1;
             lambda = std::min(0.5 * lambda, std::max(0.1 * lambda, lnew));
         }
+        }
         nrm = trial;
         if (opt->output_type >= OC_SOLVER_PROGRESS_OUTPUT)
             std::fprintf(stderr, "  newton %3d  ||WF|| %.6e -> %.6e  eta %.2e  gmres %4d its %.3e/%.3e  slope/f0 %.3f  lambda %.3g (%d trials)\n",
@@ -1304,7 +1437,12 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
         if (opt->solution_tolerance > 0.0) {  // the step taken and the new iterate, free DOFs
             std::vector<double> part;
             OC_CHECK(mesh_norms(s, dx, part));
-            snorm = lambda_taken * plain_norm(part);
+            if (M > 1) {
+                for (int m = 0; m < M; ++m) part[m] *= lam_mesh[m];
+                snorm = plain_norm(part);
+            } else {
+                snorm = lambda_taken * plain_norm(part);
+            }
             OC_CHECK(mesh_norms(s, s.dofs.ptr, part));
             xnorm = plain_norm(part);
         }
