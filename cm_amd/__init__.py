@@ -10,7 +10,7 @@ from ._lib import (OC_CONVERGED_FNORM_ABS, OC_CONVERGED_FNORM_RELATIVE, OC_CONVE
                    OC_DIVERGED_FNORM_NAN, OC_DIVERGED_FUNCTION_COUNT, OC_DIVERGED_LINE_SEARCH,
                    OC_DIVERGED_LINEAR_SOLVE, OC_DIVERGED_MAX_IT, OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE,
                    OC_MOONEY_RIVLIN_SUBTYPE, OC_SOLVER_NO_OUTPUT, OC_SOLVER_PROGRESS_OUTPUT)
-from ._lib import (CMISSError, OC_BF16, OC_FP32, OC_FP64, OC_BUFFER_COL_IDX, OC_BUFFER_DOFS,  # noqa: F401
+from ._lib import (CMISSError, OC_BF16, OC_BF16_HIDDEN, OC_FP32, OC_FP64, OC_BUFFER_COL_IDX, OC_BUFFER_DOFS,  # noqa: F401
                    OC_BUFFER_MATRIX_VALUES, OC_BUFFER_MATRIX_VALUES_SP, OC_BUFFER_RESIDUAL, OC_BUFFER_RHS,
                    OC_BUFFER_ROW_PTR, OC_BUFFER_SOLVER_COL_IDX, OC_BUFFER_SOLVER_ROW_PTR, OC_BUFFER_SOLVER_VALUES,
                    OC_BUFFER_SOLVER_VALUES_SP, OC_BUFFER_SOLVER_VECTOR, LIB_PATH)

@@ -30,6 +30,7 @@ OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE = 21
 OC_FP64 = 0
 OC_FP32 = 1
 OC_BF16 = 2  # MLP inference only
+OC_BF16_HIDDEN = 3  # MLP inference only: bf16 hidden layers, FP32 first and last layers
 
 OC_BUFFER_MATRIX_VALUES = 0
 OC_BUFFER_MATRIX_VALUES_SP = 1

@@ -57,18 +57,31 @@ __device__ __forceinline__ double invert3(const double *A, double *B)
 #undef b
 }
 
-// COORDINATE_METRICS_CALCULATE, rectangular Cartesian, volume Jacobian
-// (coordinate_routines.f90:647-840): GL = DX_DXI^T DX_DXI, GU = inv(GL),
-// JACOBIAN = sqrt|det GL|.
-__device__ __forceinline__ double metric_jacobian(const double *dx, double *gu)
+// INVERT_FULL_DP as invert3, with one division: the cofactors times 1/det
+// (the same zero-tolerance rule; entries differ from invert3's by an ulp).
+__device__ __forceinline__ double invert3_rcp(const double *A, double *B)
 {
-    double gl[9];
-    for (int mi = 0; mi < 3; ++mi)
-        for (int ni = 0; ni < 3; ++ni)
-            gl[mi + 3 * ni] = dx[0 + 3 * mi] * dx[0 + 3 * ni] + dx[1 + 3 * mi] * dx[1 + 3 * ni] +
-                              dx[2 + 3 * mi] * dx[2 + 3 * ni];
-    const double det = invert3(gl, gu);
-    return sqrt(fabs(det));
+#define a(i, j) A[(i - 1) + 3 * (j - 1)]
+#define b(i, j) B[(i - 1) + 3 * (j - 1)]
+    const double d = a(1, 1) * a(2, 2) * a(3, 3) + a(1, 2) * a(2, 3) * a(3, 1) + a(1, 3) * a(2, 1) * a(3, 2) -
+                     a(1, 1) * a(3, 2) * a(2, 3) - a(2, 1) * a(1, 2) * a(3, 3) - a(3, 1) * a(2, 2) * a(1, 3);
+    if (fabs(d) > 5.0 * 2.220446049250313e-16) {
+        const double r = 1.0 / d;
+        b(1, 1) = (a(2, 2) * a(3, 3) - a(3, 2) * a(2, 3)) * r;
+        b(2, 1) = (a(2, 3) * a(3, 1) - a(3, 3) * a(2, 1)) * r;
+        b(3, 1) = (a(2, 1) * a(3, 2) - a(3, 1) * a(2, 2)) * r;
+        b(1, 2) = (a(3, 2) * a(1, 3) - a(1, 2) * a(3, 3)) * r;
+        b(2, 2) = (a(3, 3) * a(1, 1) - a(1, 3) * a(3, 1)) * r;
+        b(3, 2) = (a(3, 1) * a(1, 2) - a(1, 1) * a(3, 2)) * r;
+        b(1, 3) = (a(1, 2) * a(2, 3) - a(2, 2) * a(1, 3)) * r;
+        b(2, 3) = (a(1, 3) * a(2, 1) - a(2, 3) * a(1, 1)) * r;
+        b(3, 3) = (a(1, 1) * a(2, 2) - a(2, 1) * a(1, 2)) * r;
+    } else {
+        for (int i = 0; i < 9; ++i) B[i] = 0.0;
+    }
+    return d;
+#undef a
+#undef b
 }
 
 // Per-Gauss-point kinematics of a triquadratic element
@@ -95,7 +108,7 @@ constexpr int kTabSize = 3 * 27 * kTabStride;
 __device__ __forceinline__ void gauss_kinematics(const double *tab, int g, const double *ze, const double *Xe,
                                                  GaussKinematics &k)
 {
-    double dz[9], dX[9], gu[9], dxidnu[9];
+    double dz[9], dX[9], dxidnu[9];
     // one derivative direction at a time, the three components together (each
     // sum still runs over n in ascending order); the loops are kept rolled so
     // that the unrolled loads do not take hundreds of registers
@@ -117,10 +130,10 @@ __device__ __forceinline__ void gauss_kinematics(const double *tab, int g, const
             dX[x + 3 * ni] = sX[x];
         }
     }
-    k.jz = metric_jacobian(dz, gu);
-    k.jx = metric_jacobian(dX, gu);
-    invert3(dz, k.dxidz);
-    invert3(dX, dxidnu);
+    // volume Jacobians of COORDINATE_METRICS_CALCULATE (coordinate_routines.f90:
+    // 647-840), sqrt|det(dx^T dx)|, taken as |det dx| from the inversions
+    k.jz = fabs(invert3_rcp(dz, k.dxidz));
+    k.jx = fabs(invert3_rcp(dX, dxidnu));
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             k.F[i + 3 * j] = dz[i + 3 * 0] * dxidnu[0 + 3 * j] + dz[i + 3 * 1] * dxidnu[1 + 3 * j] +
@@ -148,7 +161,7 @@ __device__ __forceinline__ void voigt_push_matrix(const double *F, double *t)
 __device__ __forceinline__ void mr_fictitious_piola(const double *F, double Jznu, double c10, double c01,
                                                     double *mod, double *S)
 {
-    const double s = pow(Jznu, -1.0 / 3.0);
+    const double s = rcbrt(Jznu);  // Jznu^(-1/3)
     for (int i = 0; i < 9; ++i) mod[i] = F[i] * s;
     double azl[9];
     for (int i = 0; i < 3; ++i)
@@ -173,12 +186,13 @@ __device__ __forceinline__ void mr_cauchy_stress(const double *F, double Jznu, d
     double mod[9], S[6], t[36];
     mr_fictitious_piola(F, Jznu, c10, c01, mod, S);
     voigt_push_matrix(mod, t);
+    const double rJ = 1.0 / Jznu;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
         double v = 0.0;
 #pragma unroll
         for (int j = 0; j < 6; ++j) v += t[i + 6 * j] * S[j];
-        sigma[i] = v / Jznu;
+        sigma[i] = v * rJ;
     }
     const double ot = (sigma[0] + sigma[1] + sigma[2]) / 3.0;
     for (int i = 0; i < 3; ++i) sigma[i] = sigma[i] - ot + p;
@@ -228,12 +242,13 @@ __device__ __forceinline__ void mr_elasticity_tensor(const double *F, double Jzn
     double mod[9], S[6], t[36];
     mr_fictitious_piola(F, Jznu, c10, c01, mod, S);
     voigt_push_matrix(mod, t);
+    const double rJ = 1.0 / Jznu;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
         double v = 0.0;
 #pragma unroll
         for (int j = 0; j < 6; ++j) v += t[i + 6 * j] * S[j];
-        sigma[i] = v / Jznu;
+        sigma[i] = v * rJ;
     }
     // fictitious material tangent: 4 c01 off-diagonal in the normal block,
     // -2 c01 on the shear diagonal (:686-697); push-forward t E t^T / J.
@@ -255,7 +270,7 @@ __device__ __forceinline__ void mr_elasticity_tensor(const double *F, double Jzn
             double v = 0.0;
 #pragma unroll
             for (int k = 0; k < 6; ++k) v += tE[i + 6 * k] * t[j + 6 * k];
-            c[i + 6 * j] = v / Jznu;
+            c[i + 6 * j] = v * rJ;
         }
     const double trace = sigma[0] + sigma[1] + sigma[2];
     for (int i = 0; i < 3; ++i) sigma[i] = sigma[i] - ONETHIRD * trace;

@@ -122,14 +122,15 @@ __global__ void __launch_bounds__(256) rows_bf16_kernel(int64_t n, int cols, int
 // A: n x K bf16, W: N x K bf16 (K, N multiples of 16, at most 128), staged in
 // LDS.  A block is four waves of 16 rows; lane l holds A[l%16][k0 + 4(l/16) ..
 // +3] and W[16j + l%16][same k], and D[4(l/16) + r][16j + l%16].  Hidden
-// layers (out_bf) store bf16(GELU(y)) for all N columns; the last layer stores
-// y in FP32 for its n_valid columns.
+// layers (out_bf) store bf16(GELU(y)) for all N columns; otherwise the layer
+// stores y (gelu_f: GELU(y)) in FP32 for its n_valid columns.
 constexpr int kBfMax = 128;
 __global__ void __launch_bounds__(256) mlp_layer_bf16_kernel(int n, int K, int N, int n_valid,
                                                             const uint16_t *__restrict__ A,
                                                             const uint16_t *__restrict__ Wb,
                                                             const float *__restrict__ bias,
-                                                            uint16_t *__restrict__ out_bf, float *__restrict__ out_f)
+                                                            uint16_t *__restrict__ out_bf, float *__restrict__ out_f,
+                                                            int gelu_f)
 {
     __shared__ __align__(16) uint16_t Ws[kBfMax][kBfMax + 8];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -163,7 +164,7 @@ __global__ void __launch_bounds__(256) mlp_layer_bf16_kernel(int n, int K, int N
             if (row >= n) continue;
             const float v = acc[j][r] + bc;
             if (out_bf) out_bf[row * N + col] = f32_to_bf16(gelu(v));
-            else if (col < n_valid) out_f[row * n_valid + col] = v;
+            else if (col < n_valid) out_f[row * n_valid + col] = gelu_f ? gelu(v) : v;
         }
     }
 }
@@ -438,11 +439,10 @@ struct Mlp {
         return OC_NO_ERROR;
     }
 
-    // act[0] must hold the normalised input; result in out (FP32).  BF16
-    // operands and activations, FP32 accumulation, bias and GELU.
-    int forward_bf16(int n)
+    // bf16 copies of the weights (converted when the parameters changed) and
+    // bf16 activation buffers for n rows
+    int prepare_bf16(int n)
     {
-        KernelTimer timer("mlp_forward_bf16");
         const int L = layers();
         Runtime &rt = runtime();
         if (bf16_version != params_version) {
@@ -464,6 +464,17 @@ struct Mlp {
             OC_CHECK(hb1.alloc(static_cast<size_t>(n) * pad16(n_hidden)));
             bf_capacity = n;
         }
+        return OC_NO_ERROR;
+    }
+
+    // act[0] must hold the normalised input; result in out (FP32).  BF16
+    // operands and activations, FP32 accumulation, bias and GELU.
+    int forward_bf16(int n)
+    {
+        KernelTimer timer("mlp_forward_bf16");
+        const int L = layers();
+        Runtime &rt = runtime();
+        OC_CHECK(prepare_bf16(n));
         rows_bf16_kernel<<<blocks_for(static_cast<int64_t>(n) * pad16(n_in)), 256, 0, rt.stream>>>(
             n, n_in, pad16(n_in), act[0].ptr, xb.ptr);
         OC_HIP_CHECK(hipGetLastError());
@@ -474,7 +485,7 @@ struct Mlp {
             const bool last = l + 1 == L;
             mlp_layer_bf16_kernel<<<blocks, 256, 0, rt.stream>>>(n, pad16(in_dim[l]), pad16(out_dim[l]), out_dim[l], a,
                                                                  wbf.ptr + wbf_off[l], params.ptr + b_off[l],
-                                                                 last ? nullptr : h, last ? out.ptr : nullptr);
+                                                                 last ? nullptr : h, last ? out.ptr : nullptr, 0);
             OC_HIP_CHECK(hipGetLastError());
             a = h;
             std::swap(h, spare);
@@ -482,7 +493,52 @@ struct Mlp {
         return OC_NO_ERROR;
     }
 
-    int infer(int n) { return infer_precision == OC_BF16 ? forward_bf16(n) : forward(n); }
+    // BF16 hidden layers between an FP32 first and last layer: the input
+    // features and the output increment keep FP32; the hidden activations
+    // after the first layer are bf16, and the last hidden layer writes its
+    // GELU in FP32 for the FP32 output layer.
+    int forward_bf16_hidden(int n)
+    {
+        KernelTimer timer("mlp_forward_bf16_hidden");
+        const int L = layers();
+        if (L < 3) return forward(n);
+        Runtime &rt = runtime();
+        OC_CHECK(prepare_bf16(n));
+        auto fp32_layer = [&](int l, const float *in, float *h, int act) -> int {
+            OC_CHECK(gemm(n, out_dim[l], in_dim[l], 0, 1, in, in_dim[l], params.ptr + w_off[l], in_dim[l], 0.f,
+                          pre[l].ptr, out_dim[l]));
+            const int64_t total = static_cast<int64_t>(n) * out_dim[l];
+            bias_act_kernel<<<blocks_for(total), 256, 0, rt.stream>>>(total, out_dim[l], pre[l].ptr,
+                                                                      params.ptr + b_off[l], h, act);
+            OC_HIP_CHECK(hipGetLastError());
+            return OC_NO_ERROR;
+        };
+        OC_CHECK(fp32_layer(0, act[0].ptr, act[1].ptr, 1));
+        rows_bf16_kernel<<<blocks_for(static_cast<int64_t>(n) * pad16(n_hidden)), 256, 0, rt.stream>>>(
+            n, n_hidden, pad16(n_hidden), act[1].ptr, hb0.ptr);
+        OC_HIP_CHECK(hipGetLastError());
+        const uint16_t *a = hb0.ptr;
+        uint16_t *h = hb1.ptr, *spare = hb0.ptr;
+        const unsigned blocks = static_cast<unsigned>((static_cast<int64_t>(n) + 63) / 64);
+        for (int l = 1; l + 1 < L; ++l) {
+            const bool to_f32 = l + 2 == L;  // feeds the FP32 output layer
+            mlp_layer_bf16_kernel<<<blocks, 256, 0, rt.stream>>>(n, pad16(in_dim[l]), pad16(out_dim[l]), out_dim[l], a,
+                                                                 wbf.ptr + wbf_off[l], params.ptr + b_off[l],
+                                                                 to_f32 ? nullptr : h, to_f32 ? act[L - 1].ptr : nullptr,
+                                                                 1);
+            OC_HIP_CHECK(hipGetLastError());
+            a = h;
+            std::swap(h, spare);
+        }
+        return fp32_layer(L - 1, act[L - 1].ptr, out.ptr, 0);
+    }
+
+    int infer(int n)
+    {
+        if (infer_precision == OC_BF16) return forward_bf16(n);
+        if (infer_precision == OC_BF16_HIDDEN) return forward_bf16_hidden(n);
+        return forward(n);
+    }
 
     // after forward(n) with targets T on device: gradients into grads, loss partials into partial
     int backward(int n, const float *T, int *n_partials)
@@ -636,10 +692,10 @@ int oc_mlp_parameters_set(oc_mlp mlp, const float *parameters)
 int oc_mlp_inference_precision_set(oc_mlp mlp, int precision)
 {
     if (!mlp) return set_error(OC_ERR_INVALID_HANDLE, "MLP handle is NULL");
-    if (precision != OC_FP32 && precision != OC_BF16)
-        return set_error(OC_ERR_INVALID_ARGUMENT, "inference precision must be OC_FP32 or OC_BF16");
+    if (precision != OC_FP32 && precision != OC_BF16 && precision != OC_BF16_HIDDEN)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "inference precision must be OC_FP32, OC_BF16 or OC_BF16_HIDDEN");
     const Mlp &m = mlp->m;
-    if (precision == OC_BF16 && (m.n_in > kBfMax || m.n_hidden > kBfMax || m.n_out > kBfMax))
+    if (precision != OC_FP32 && (m.n_in > kBfMax || m.n_hidden > kBfMax || m.n_out > kBfMax))
         return set_error(OC_ERR_UNSUPPORTED, "BF16 inference needs layer widths of at most %d", kBfMax);
     mlp->m.infer_precision = precision;
     return OC_NO_ERROR;

@@ -71,7 +71,8 @@ class WarmStartMLP:
         call("oc_mlp_parameters_set", self._h, p)
 
     def set_inference_precision(self, precision):
-        """OC_FP32 (default) or OC_BF16 for forward() and the warm starts."""
+        """OC_FP32 (default), OC_BF16 or OC_BF16_HIDDEN (bf16 hidden layers
+        only) for forward() and the warm starts."""
         call("oc_mlp_inference_precision_set", self._h, int(precision))
 
     def forward(self, x):

@@ -16,7 +16,7 @@ MODULE OC_ASM
   INTEGER(C_INT), PARAMETER :: OC_ERR_PATTERN = -6
   INTEGER(C_INT), PARAMETER :: OC_ERR_NOT_INITIALISED = -7
   INTEGER(C_INT), PARAMETER :: OC_ERR_UNSUPPORTED = -8
-  INTEGER(C_INT), PARAMETER :: OC_FP64 = 0, OC_FP32 = 1, OC_BF16 = 2
+  INTEGER(C_INT), PARAMETER :: OC_FP64 = 0, OC_FP32 = 1, OC_BF16 = 2, OC_BF16_HIDDEN = 3
   INTEGER(C_INT), PARAMETER :: OC_MOONEY_RIVLIN_SUBTYPE = 1, OC_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE = 21
   INTEGER(C_INT), PARAMETER :: OC_EQUATIONS_SET_LAPLACE = 1, OC_EQUATIONS_SET_MOONEY_RIVLIN = 2
   INTEGER(C_INT), PARAMETER :: OC_BUFFER_MATRIX_VALUES = 0, OC_BUFFER_MATRIX_VALUES_SP = 1, &

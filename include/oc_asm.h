@@ -57,6 +57,7 @@ extern "C" {
 #define OC_FP64 0
 #define OC_FP32 1
 #define OC_BF16 2   /* MLP inference only */
+#define OC_BF16_HIDDEN 3  /* MLP inference only: bf16 hidden layers, FP32 first and last layers */
 
 /* ---- device buffers that can be borrowed --------------------------------- */
 #define OC_BUFFER_MATRIX_VALUES 0     /* double[nnz]: linear matrix or FP64 Jacobian */
@@ -395,8 +396,10 @@ int oc_mlp_normalisation_set(oc_mlp mlp, const float *shift, const float *scale)
 /* y = MLP(x) for n samples (host arrays, x: n*n_inputs, y: n*n_outputs). */
 int oc_mlp_forward(oc_mlp mlp, int n, const float *x, float *y);
 /* Inference precision of oc_mlp_forward and the warm starts: OC_FP32 (the
- * default) or OC_BF16 (bf16 weights and activations on MFMA with FP32
- * accumulation, bias and GELU; layer widths up to 128).  Training stays FP32.
+ * default), OC_BF16 (bf16 weights and activations on MFMA with FP32
+ * accumulation, bias and GELU; layer widths up to 128) or OC_BF16_HIDDEN (the
+ * same for the hidden layers only: the first and last layers, and so the input
+ * features and the output increment, stay FP32).  Training stays FP32.
  * No counterpart in the reference: the warm start is this library's own. */
 int oc_mlp_inference_precision_set(oc_mlp mlp, int precision);
 /* Loss 1/(n*n_out) sum (y - t)^2 and its gradient w.r.t. the parameters

@@ -95,6 +95,33 @@ def numpy_mlp_bf16(params, dims, x, shift, scale):
     return h
 
 
+def numpy_mlp_bf16_hidden(params, dims, x, shift, scale):
+    """The BF16_HIDDEN inference rule: the first and last layers in FP32;
+    between them, hidden activations and weights rounded to bf16, except the
+    last hidden activation (the FP32 output layer's input)."""
+    from math import sqrt
+    from scipy.special import erf
+    gelu = lambda z: 0.5 * z * (1 + erf(z / sqrt(2)))  # noqa: E731
+    h = ((x.astype(np.float32) - shift.astype(np.float32)) * scale.astype(np.float32)).astype(np.float64)
+    off = 0
+    L = len(dims)
+    for l, (i, o) in enumerate(dims):
+        W = params[off:off + i * o].reshape(o, i).astype(np.float64)
+        if 0 < l < L - 1:
+            W = bf16(W).astype(np.float64)
+        off += i * o
+        b = params[off:off + o].astype(np.float64)
+        off += o
+        z = h @ W.T + b
+        if l + 1 < L:
+            h = gelu(z)
+            if l + 2 < L:  # feeds a bf16 layer
+                h = bf16(h).astype(np.float64)
+        else:
+            h = z
+    return h
+
+
 def test_bf16_bit_rounding_rule():
     """The emulation's rounding: nearest, ties to even, on the upper 16 bits."""
     v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5, 0.0], np.float32)
@@ -134,6 +161,13 @@ def test_mlp_bf16_inference_matches_emulation(gpu, hidden, layers, n):
     emu2 = numpy_mlp_bf16(p2, dims, x, FEATURE_SHIFT, FEATURE_SCALE)
     assert np.abs(net.forward(x) - emu2).max() <= 2e-2 * np.abs(emu2).max()
     net.set_parameters(p)
+    # bf16 hidden layers between FP32 first and last layers
+    net.set_inference_precision(_lib.OC_BF16_HIDDEN)
+    yh = net.forward(x)
+    emu_h = numpy_mlp_bf16_hidden(p, dims, x, FEATURE_SHIFT, FEATURE_SCALE)
+    assert np.abs(yh - emu_h).max() <= 2e-2 * scale_y
+    assert np.median(np.abs(yh - emu_h)) <= 1e-3 * scale_y
+    assert np.linalg.norm(yh - exact) / np.linalg.norm(exact) < 2e-2
     net.set_inference_precision(_lib.OC_FP32)
     assert np.array_equal(net.forward(x), y32)
     with pytest.raises(_lib.CMISSError):
