@@ -81,7 +81,6 @@ void build_basis_tables(BasisTables &t)
     tabulate(2, 3, t.w27, t.quad27);
     tabulate(1, 3, wtmp, t.lin27);
     tabulate(1, 2, t.w8, t.lin8);
-    gauss_1d(2, t.xi2, wtmp);
 }
 
 static BasisTables *g_device_tables = nullptr;
@@ -163,19 +162,31 @@ laplace_slot_offsets_kernel(int n_elements, const int *__restrict__ conn, const 
 //   * dX/dxi by sum factorisation: along xi_d a trilinear map's derivative is
 //     the bilinear interpolant, in the other two coordinates, of its four edge
 //     differences along d, so 36 values serve all eight Gauss points;
-//   * the basis derivatives are products of the 1-D factors 1 - xi, xi;
-//   * GU J w = adj(GL) w sqrt|det| / det, one division per Gauss point
-//     (COORDINATE_METRICS_CALCULATE and INVERT_FULL_DP, whose zero-tolerance
-//     rule it keeps).
-// The Gauss loop unrolls, so every table index is a constant.
+//   * the basis derivatives are products of the 1-D factors 1 - xi, xi at
+//     the Gauss abscissae, compile-time constants;
+//   * GU J w = adj(GL) w / sqrt(det GL), since J = sqrt(det GL) and
+//     GU = adj(GL) / det GL (COORDINATE_METRICS_CALCULATE and
+//     INVERT_FULL_DP, whose zero-tolerance rule it keeps);
+//   * the basis derivatives sum to zero at every point, so each row of K_e
+//     sums to zero: the loop accumulates the 28 entries among local nodes
+//     0-6 and the entries of node 7 follow as negated row sums.
+// The Gauss loop unrolls, so every index is a constant.
 __device__ __forceinline__ int lap_upper(int ms, int ns) { return ms * (15 - ms) / 2 + ns; }
 
 __device__ __forceinline__ double lerp(double a, double b, double q) { return fma(q, b - a, a); }
 
-__device__ __forceinline__ void laplace_element_upper(const double (&x)[3][8], const BasisTables *__restrict__ T,
-                                                      double (&K)[36])
+// the 2-point Gauss abscissae on [0, 1], (1 -+ 1/sqrt 3) / 2 rounded as the
+// host basis tables compute them, and the linear factors L[g][n] of local
+// coordinate n (0 or 1) at abscissa g: 1 - xi, xi
+constexpr double kLapXi[2] = {0x1.b0cb174df99c6p-3, 0x1.93cd3a2c8198ep-1};
+__device__ constexpr double lap_factor(int g, int n) { return n ? kLapXi[g] : 1.0 - kLapXi[g]; }
+
+__device__ __forceinline__ void laplace_element_upper(const double (&x)[3][8], double (&K)[36])
 {
-    const double qx[2] = {T->xi2[0], T->xi2[1]};
+    // every multiply-add is an explicit fma (contraction is off here), so the
+    // rounding is the same wherever the function is compiled: the fused and
+    // the two-pass assembly write bit-identical values
+#pragma clang fp contract(off)
     // D[c][d][2 gb + ga]: dX_c/dxi_d at Gauss indices (ga, gb) of the other
     // two directions (ascending)
     double D[3][3][4];
@@ -193,10 +204,12 @@ __device__ __forceinline__ void laplace_element_upper(const double (&x)[3][8], c
             for (int gb = 0; gb < 2; ++gb)
 #pragma unroll
                 for (int ga = 0; ga < 2; ++ga)
-                    D[c][d][2 * gb + ga] = lerp(lerp(e[0][0], e[0][1], qx[ga]), lerp(e[1][0], e[1][1], qx[ga]), qx[gb]);
+                    D[c][d][2 * gb + ga] = lerp(lerp(e[0][0], e[0][1], kLapXi[ga]), lerp(e[1][0], e[1][1], kLapXi[ga]),
+                                                kLapXi[gb]);
         }
+    double Kr[28];  // entries (ms, ns), ms <= ns <= 6, at lap_upper(ms, ns) - ms
 #pragma unroll
-    for (int i = 0; i < 36; ++i) K[i] = 0.0;
+    for (int i = 0; i < 28; ++i) Kr[i] = 0.0;
 #pragma unroll
     for (int g = 0; g < 8; ++g) {
         const int g1 = g & 1, g2 = (g >> 1) & 1, g3 = g >> 2;
@@ -208,40 +221,59 @@ __device__ __forceinline__ void laplace_element_upper(const double (&x)[3][8], c
             dx[c + 6] = D[c][2][2 * g2 + g1];
         }
         double gl[6];  // GL = dX^T dX: 00 11 22 01 02 12
-        gl[0] = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
-        gl[1] = dx[3] * dx[3] + dx[4] * dx[4] + dx[5] * dx[5];
-        gl[2] = dx[6] * dx[6] + dx[7] * dx[7] + dx[8] * dx[8];
-        gl[3] = dx[0] * dx[3] + dx[1] * dx[4] + dx[2] * dx[5];
-        gl[4] = dx[0] * dx[6] + dx[1] * dx[7] + dx[2] * dx[8];
-        gl[5] = dx[3] * dx[6] + dx[4] * dx[7] + dx[5] * dx[8];
-        const double c00 = gl[1] * gl[2] - gl[5] * gl[5], c11 = gl[0] * gl[2] - gl[4] * gl[4];
-        const double c22 = gl[0] * gl[1] - gl[3] * gl[3], c01 = gl[4] * gl[5] - gl[3] * gl[2];
-        const double c02 = gl[3] * gl[5] - gl[4] * gl[1], c12 = gl[3] * gl[4] - gl[0] * gl[5];
-        const double det = gl[0] * c00 + gl[3] * c01 + gl[4] * c02;
-        const double sc = fabs(det) > 5.0 * 2.220446049250313e-16 ? T->w8[g] * sqrt(fabs(det)) / det : 0.0;
+        auto dot3 = [](double a0, double b0, double a1, double b1, double a2, double b2) {
+            return fma(a2, b2, fma(a1, b1, a0 * b0));
+        };
+        gl[0] = dot3(dx[0], dx[0], dx[1], dx[1], dx[2], dx[2]);
+        gl[1] = dot3(dx[3], dx[3], dx[4], dx[4], dx[5], dx[5]);
+        gl[2] = dot3(dx[6], dx[6], dx[7], dx[7], dx[8], dx[8]);
+        gl[3] = dot3(dx[0], dx[3], dx[1], dx[4], dx[2], dx[5]);
+        gl[4] = dot3(dx[0], dx[6], dx[1], dx[7], dx[2], dx[8]);
+        gl[5] = dot3(dx[3], dx[6], dx[4], dx[7], dx[5], dx[8]);
+        auto diff2 = [](double a, double b, double c, double d) { return fma(a, b, -(c * d)); };  // ab - cd
+        const double c00 = diff2(gl[1], gl[2], gl[5], gl[5]), c11 = diff2(gl[0], gl[2], gl[4], gl[4]);
+        const double c22 = diff2(gl[0], gl[1], gl[3], gl[3]), c01 = diff2(gl[4], gl[5], gl[3], gl[2]);
+        const double c02 = diff2(gl[3], gl[5], gl[4], gl[1]), c12 = diff2(gl[3], gl[4], gl[0], gl[5]);
+        const double det = dot3(gl[0], c00, gl[3], c01, gl[4], c02);  // det GL = det(dX)^2
+        // weight 1/8 of the 2x2x2 rule on [0, 1]^3
+        const double sc = fabs(det) > 5.0 * 2.220446049250313e-16 ? copysign(0.125 * rsqrt(fabs(det)), det) : 0.0;
         const double A[9] = {c00 * sc, c01 * sc, c02 * sc, c01 * sc, c11 * sc, c12 * sc, c02 * sc, c12 * sc, c22 * sc};
         // dphi_n/dxi at g: (+-1) times the other two directions' factors
-        const double L1[2] = {1.0 - qx[g1], qx[g1]}, L2[2] = {1.0 - qx[g2], qx[g2]}, L3[2] = {1.0 - qx[g3], qx[g3]};
         double f[8][3];
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
             const int n1 = n & 1, n2 = (n >> 1) & 1, n3 = n >> 2;
-            f[n][0] = (n1 ? 1.0 : -1.0) * (L2[n2] * L3[n3]);
-            f[n][1] = (n2 ? 1.0 : -1.0) * (L1[n1] * L3[n3]);
-            f[n][2] = (n3 ? 1.0 : -1.0) * (L1[n1] * L2[n2]);
+            f[n][0] = (n1 ? 1.0 : -1.0) * (lap_factor(g2, n2) * lap_factor(g3, n3));
+            f[n][1] = (n2 ? 1.0 : -1.0) * (lap_factor(g1, n1) * lap_factor(g3, n3));
+            f[n][2] = (n3 ? 1.0 : -1.0) * (lap_factor(g1, n1) * lap_factor(g2, n2));
         }
         int idx = 0;
 #pragma unroll
-        for (int ms = 0; ms < 8; ++ms) {
+        for (int ms = 0; ms < 7; ++ms) {
             double v[3];
 #pragma unroll
             for (int ni = 0; ni < 3; ++ni)
-                v[ni] = f[ms][0] * A[0 + 3 * ni] + f[ms][1] * A[1 + 3 * ni] + f[ms][2] * A[2 + 3 * ni];
+                v[ni] = dot3(f[ms][0], A[0 + 3 * ni], f[ms][1], A[1 + 3 * ni], f[ms][2], A[2 + 3 * ni]);
 #pragma unroll
-            for (int ns = ms; ns < 8; ++ns, ++idx)
-                K[idx] = fma(v[2], f[ns][2], fma(v[1], f[ns][1], fma(v[0], f[ns][0], K[idx])));
+            for (int ns = ms; ns < 7; ++ns, ++idx)
+                Kr[idx] = fma(v[2], f[ns][2], fma(v[1], f[ns][1], fma(v[0], f[ns][0], Kr[idx])));
         }
     }
+    int idx = 0;
+#pragma unroll
+    for (int ms = 0; ms < 7; ++ms)
+#pragma unroll
+        for (int ns = ms; ns < 7; ++ns, ++idx) K[lap_upper(ms, ns)] = Kr[idx];
+    double k77 = 0.0;
+#pragma unroll
+    for (int ms = 0; ms < 7; ++ms) {
+        double r = 0.0;
+#pragma unroll
+        for (int ns = 0; ns < 7; ++ns) r += K[ms <= ns ? lap_upper(ms, ns) : lap_upper(ns, ms)];
+        K[lap_upper(ms, 7)] = -r;
+        k77 += r;
+    }
+    K[lap_upper(7, 7)] = k77;
 }
 
 // One thread per element; each row of K_e goes to its place in the
@@ -251,7 +283,7 @@ constexpr int kLapThreads = 64;
 
 __global__ void __launch_bounds__(kLapThreads)
 laplace_element_kernel(int n_elements, const int *__restrict__ conn, const double *__restrict__ coords, int n_nodes,
-                       const BasisTables *__restrict__ T, const int *__restrict__ gpos, double *__restrict__ kbuf)
+                       const int *__restrict__ gpos, double *__restrict__ kbuf)
 {
     const int e = blockIdx.x * kLapThreads + threadIdx.x;
     if (e >= n_elements) return;
@@ -263,7 +295,7 @@ laplace_element_kernel(int n_elements, const int *__restrict__ conn, const doubl
         for (int c = 0; c < 3; ++c) x[c][n] = coords[static_cast<int64_t>(c) * n_nodes + node];
     }
     double K[36];
-    laplace_element_upper(x, T, K);
+    laplace_element_upper(x, K);
     // row ms of this element goes to its position in the row-gather order
     double row[64];
     int idx = 0;
@@ -357,7 +389,7 @@ constexpr int kLapFX = kLapTX + 1, kLapFusedThreads = (kLapTX + 1) * (kLapTY + 1
 
 __global__ void __launch_bounds__(kLapFusedThreads)
 laplace_fused_kernel(int nx, int ny, int nz, int seg, const double *__restrict__ coords, int n_nodes,
-                     const BasisTables *__restrict__ T, const int64_t *__restrict__ row_ptr, double *__restrict__ values)
+                     const int64_t *__restrict__ row_ptr, double *__restrict__ values)
 {
     __shared__ double sK[36][kLapFusedThreads];  // element matrices of the layer (upper triangles)
     const int t = threadIdx.x;
@@ -410,7 +442,7 @@ laplace_fused_kernel(int nx, int ny, int nz, int seg, const double *__restrict__
 #pragma unroll
                 for (int c = 0; c < 3; ++c) x[c][n] = coords[static_cast<int64_t>(c) * n_nodes + node];
             }
-            laplace_element_upper(x, T, K);
+            laplace_element_upper(x, K);
         } else {
 #pragma unroll
             for (int q = 0; q < 36; ++q) K[q] = 0.0;
@@ -529,15 +561,13 @@ bool laplace_fused_serves(const EquationsSet &s)
 
 int launch_laplace_fused(EquationsSet &s)
 {
-    const BasisTables *T = nullptr;
-    OC_CHECK(ensure_tables(&T));
     Runtime &rt = runtime();
     const int nx = s.lap_grid[0], ny = s.lap_grid[1], nz = s.lap_grid[2];
     if (nx < 1 || ny < 1 || nz < 1) return set_error(OC_ERR_UNSUPPORTED, "the Laplace set is not a structured grid");
     OC_CHECK(fill<double>(s.rhs.ptr, s.n_rows, 0.0));  // RHS element vector is 0 (:542)
     KernelTimer timer("laplace_fused");
     laplace_fused_kernel<<<static_cast<unsigned>(laplace_fused_blocks(s)), kLapFusedThreads, 0, rt.stream>>>(
-        nx, ny, nz, kLapSeg, s.coords.ptr, s.n_nodes, T, s.row_ptr.ptr, s.values.ptr);
+        nx, ny, nz, kLapSeg, s.coords.ptr, s.n_nodes, s.row_ptr.ptr, s.values.ptr);
     OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
 }
@@ -601,8 +631,6 @@ static int laplace_prepare(EquationsSet &s)
 int launch_laplace_assemble(EquationsSet &s)
 {
     if (laplace_fused_serves(s)) return launch_laplace_fused(s);
-    const BasisTables *T = nullptr;
-    OC_CHECK(ensure_tables(&T));
     OC_CHECK(laplace_prepare(s));
     Runtime &rt = runtime();
     OC_CHECK(fill<double>(s.rhs.ptr, s.n_rows, 0.0));  // RHS element vector is 0 (:542)
@@ -610,7 +638,7 @@ int launch_laplace_assemble(EquationsSet &s)
         KernelTimer timer("laplace_element");
         const unsigned blocks = static_cast<unsigned>((s.n_elements + kLapThreads - 1) / kLapThreads);
         laplace_element_kernel<<<blocks, kLapThreads, 0, rt.stream>>>(s.n_elements, s.conn.ptr, s.coords.ptr,
-                                                                     s.n_nodes, T, s.gather_idx.ptr, s.kbuf64.ptr);
+                                                                     s.n_nodes, s.gather_idx.ptr, s.kbuf64.ptr);
         OC_HIP_CHECK(hipGetLastError());
     }
     KernelTimer timer("laplace_row_gather");

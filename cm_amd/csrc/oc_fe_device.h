@@ -19,7 +19,6 @@ struct BasisTables {
     double quad27[27 * 27 * 4];  // triquadratic at 3x3x3
     double lin27[27 * 8 * 4];    // trilinear at 3x3x3
     double lin8[8 * 8 * 4];      // trilinear at 2x2x2
-    double xi2[2];               // the 2-point Gauss abscissae on [0, 1]
 };
 
 // Builds the tables on the host (restating GAUSS_LEGENDRE,
