@@ -109,6 +109,13 @@ extern "C" int oc_autotune(oc_equations_set h, int precision, char *report, int 
     // SpMV: lanes per row (per node for node rows) x neighbours in flight per
     // lane; default 32 x 1 (FP32 values) or 16 x 2 (FP64) on node rows
     OC_CHECK(prepare_node_rows(s));
+    if (fp32 && s.node_rows == 1 && !s.uu_x4.ptr) {  // the FP32 product's iterate quadruples, as a solve makes
+        OC_CHECK(s.uu_x4.alloc(static_cast<size_t>(s.n_nodes) * 4));
+        if (s.step_graph) {
+            (void)hipGraphExecDestroy(s.step_graph);
+            s.step_graph = nullptr;
+        }
+    }
     DevBuf<double> y;
     OC_CHECK(y.alloc(s.n_rows));
     std::vector<std::pair<int, int>> spmv;
@@ -131,7 +138,8 @@ extern "C" int oc_autotune(oc_equations_set h, int precision, char *report, int 
                   &c_spmv));
 
     // level-0 multigrid node kernel (residual form), when the rows have the
-    // node layout; default 16 x 2 for FP64 values, 32 x 1 otherwise
+    // node layout: FP64 values (default 16 x 2), or for an FP32 Jacobian the
+    // packed bf16 copy its smoother reads (default 16 lanes x 1 quad)
     int best_smoother = -1;
     if (s.node_rows == 1 && !s.free_mask.ptr) {  // no boundary conditions yet: every DOF free
         OC_CHECK(s.free_mask.alloc(s.n_rows));
@@ -140,7 +148,13 @@ extern "C" int oc_autotune(oc_equations_set h, int precision, char *report, int 
     if (s.node_rows == 1) {
         DevBuf<double> r;
         OC_CHECK(r.alloc(static_cast<size_t>(3) * s.n_nodes));
-        const char *names[] = {"32x1", "32x2", "16x2", "16x4", "64x1"};
+        const char *names64[] = {"32x1", "32x2", "16x2", "16x4", "64x1"};
+        const char *names_bf[] = {"16x1q", "8x1q", "32x1q", "8x2q", "16x2q"};
+        const char **names = fp32 ? names_bf : names64;
+        if (fp32) {
+            OC_CHECK(prepare_uu_bf16(s));
+            OC_CHECK(launch_uu_pack(s));
+        }
         Choice c_sm{};
         OC_CHECK(tune(js, "smoother_residual_ms", 5, fp32 ? 0 : 2, reps,
                       [&](int v) { return launch_smoother_residual(s, precision, v, s.dofs.ptr, s.dofs.ptr, r.ptr); },

@@ -188,7 +188,15 @@ struct EquationsSet {
     DevBuf<int32_t> col_idx;
     DevBuf<double> values;
     DevBuf<float> values_sp;
-    DevBuf<uint16_t> values_bf;  // bf16 copy of values_sp for the multigrid smoother (FP32 Jacobians)
+    // bf16 copy of the displacement block K_uu of an FP32 Jacobian, packed for
+    // the level-0 smoother (node rows only): node i's neighbours, padded to a
+    // multiple of four (padding: the node itself, value 0), start at
+    // uu_off[i]; columns uu_col[uu_off[i] + p], values
+    // uu_bf[9 uu_off[i] + (3 r + c) degp + p] for row r and component c
+    DevBuf<int64_t> uu_off;
+    DevBuf<int32_t> uu_col;
+    DevBuf<uint16_t> uu_bf;
+    DevBuf<float> uu_x4;  // the iterate those kernels gather, per node (x, y, z, 0) in FP32
     DevBuf<double> dofs, residual, rhs;
 
     // Mooney-Rivlin: per-Gauss-point records (field-major, see
@@ -311,6 +319,12 @@ int mg_setup(EquationsSet &s, int precision);
 int mg_apply(EquationsSet &s, int precision, const double *t, double *z);
 // pressure-column starts and the node-row layout check (oc_solver.hip)
 int prepare_node_rows(EquationsSet &s);
+// the packed bf16 displacement block's structure (uu_off, uu_col) and value
+// buffer, once per set (oc_multigrid.hip)
+int prepare_uu_bf16(EquationsSet &s);
+int launch_uu_pack(EquationsSet &s);  // the FP32 Jacobian's values into it
+// x4[j] = (x[j], x[nq + j], x[2 nq + j], 0) in FP32, on the runtime stream
+void pack_x4(int nq, const double *x, float4 *x4);
 // r = b - K_uu x with the level-0 node kernel of shape `variant` (autotuner)
 int launch_smoother_residual(EquationsSet &s, int precision, int variant, const double *b, const double *x,
                              double *r);

@@ -346,28 +346,145 @@ uu_residual_kernel(int nu, const int64_t *__restrict__ row_ptr, const int64_t *_
     if (lane == 0) r[row] = f ? b[row] - acc : 0.0;
 }
 
-// bf16 matrix values (the upper 16 bits of an FP32 value, rounded to nearest
-// even): the level-0 smoother's copy of an FP32 Jacobian
-struct Bf16 {
-    uint16_t bits;
-    __device__ explicit operator double() const
-    {
-        return static_cast<double>(__uint_as_float(static_cast<uint32_t>(bits) << 16));
-    }
-};
-
-__global__ void __launch_bounds__(256) values_bf16_kernel(int64_t n, const float *__restrict__ v,
-                                                         uint16_t *__restrict__ o)
+// bf16 values: the upper 16 bits of an FP32 value, rounded to nearest even
+// (NaN kept quiet)
+__device__ __forceinline__ uint16_t bf16_round(float f)
 {
-    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        uint32_t u = __float_as_uint(v[i]);
-        if ((u & 0x7fffffffu) > 0x7f800000u) {
-            o[i] = static_cast<uint16_t>((u >> 16) | 0x40u);
-            continue;
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+// The packed bf16 copy of K_uu (EquationsSet::uu_bf): padded neighbour
+// counts, then the padded column lists, once per set
+__global__ void __launch_bounds__(256) uu_degp_kernel(int nq, const int64_t *__restrict__ row_ptr,
+                                                     const int64_t *__restrict__ pstart, int64_t *__restrict__ degp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const int64_t deg = (pstart[i] - row_ptr[i]) / 3;
+    degp[i] = (deg + 3) & ~static_cast<int64_t>(3);
+}
+
+constexpr int kPackLanes = 32;  // lanes per node in the packing kernels
+
+__global__ void __launch_bounds__(256) uu_cols_kernel(int nq, const int64_t *__restrict__ row_ptr,
+                                                     const int64_t *__restrict__ pstart, const int *__restrict__ col_idx,
+                                                     const int64_t *__restrict__ uu_off, int *__restrict__ uu_col)
+{
+    const int lane = threadIdx.x % kPackLanes;
+    const int i = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kPackLanes);
+    if (i >= nq) return;
+    const int64_t s0 = row_ptr[i], off = uu_off[i];
+    const int deg = static_cast<int>((pstart[i] - s0) / 3), degp = static_cast<int>(uu_off[i + 1] - off);
+    for (int p = lane; p < degp; p += kPackLanes) uu_col[off + p] = p < deg ? col_idx[s0 + p] : i;
+}
+
+// values of one FP32 Jacobian into the packed bf16 copy (each Newton step)
+__global__ void __launch_bounds__(256) uu_pack_kernel(int nq, const int64_t *__restrict__ row_ptr,
+                                                     const int64_t *__restrict__ pstart, const float *__restrict__ v,
+                                                     const int64_t *__restrict__ uu_off, uint16_t *__restrict__ uu_bf)
+{
+    const int lane = threadIdx.x % kPackLanes;
+    const int i = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kPackLanes);
+    if (i >= nq) return;
+    const int64_t nq64 = nq, off = uu_off[i];
+    const int deg = static_cast<int>((pstart[i] - row_ptr[i]) / 3), degp = static_cast<int>(uu_off[i + 1] - off);
+    uint16_t *o = uu_bf + 9 * off;
+    for (int k = lane; k < 9 * degp; k += kPackLanes) {
+        const int rc = k / degp, p = k - rc * degp, r = rc / 3, c = rc - 3 * r;
+        o[k] = p < deg ? bf16_round(v[row_ptr[r * nq64 + i] + c * deg + p]) : static_cast<uint16_t>(0);
+    }
+}
+
+// the iterate as FP32 node quadruples (x, y, z, 0) for the bf16 kernels'
+// gathers: one 16-byte load per neighbour, and no accuracy lost against the
+// bf16 values' 8-bit significands
+__global__ void __launch_bounds__(256) pack_x4_kernel(int nq, const double *__restrict__ x, float4 *__restrict__ x4)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nq) return;
+    const int64_t nq64 = nq;
+    x4[j] = make_float4(static_cast<float>(x[j]), static_cast<float>(x[nq64 + j]), static_cast<float>(x[2 * nq64 + j]),
+                        0.0f);
+}
+
+// level 0 residual by nodes from the packed bf16 K_uu: one group of kLanes
+// lanes per node; a lane takes four neighbours at a time (kQ quads in
+// flight): one 16-byte column load, one 8-byte load per (row, component)
+// run and one 16-byte iterate load per neighbour (x4: pack_x4_kernel of x).
+// r = b - K_uu x on free rows, 0 on fixed ones; kSweep 1 writes the block
+// Jacobi update x + omega Kblock^-1 r instead (as uu_node_residual_kernel).
+template <int kLanes, int kSweep, int kQ>
+__global__ void __launch_bounds__(256)
+uu_node_bf16_kernel(int nq, const int64_t *__restrict__ uu_off, const int *__restrict__ uu_col,
+                    const uint16_t *__restrict__ uu_bf, const uint8_t *__restrict__ free_mask,
+                    const double *__restrict__ b, const float4 *__restrict__ x4, const double *__restrict__ x,
+                    const double *__restrict__ inv_u, double omega, double *__restrict__ out)
+{
+    const int lane = threadIdx.x % kLanes;
+    const int i = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes);
+    if (i >= nq) return;  // whole groups
+    const int64_t nq64 = nq, off = uu_off[i];
+    const int n4 = static_cast<int>((uu_off[i + 1] - off) >> 2);
+    const int4 *cols = reinterpret_cast<const int4 *>(uu_col + off);
+    const uint2 *vals = reinterpret_cast<const uint2 *>(uu_bf + 9 * off);  // run rc: vals[rc n4 + q]
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int q0 = lane; q0 < n4; q0 += kLanes * kQ) {
+        int4 cj[kQ];
+        uint2 v[kQ][9];
+#pragma unroll
+        for (int u = 0; u < kQ; ++u) {
+            const int q = q0 + u * kLanes;
+            const bool ok = q < n4;
+            cj[u] = ok ? cols[q] : make_int4(i, i, i, i);
+#pragma unroll
+            for (int rc = 0; rc < 9; ++rc) v[u][rc] = ok ? vals[rc * n4 + q] : make_uint2(0u, 0u);
         }
-        u += 0x7fffu + ((u >> 16) & 1u);
-        o[i] = static_cast<uint16_t>(u >> 16);
+#pragma unroll
+        for (int u = 0; u < kQ; ++u) {
+            const int j[4] = {cj[u].x, cj[u].y, cj[u].z, cj[u].w};
+            double xv[4][3];
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const float4 q4 = x4[j[n]];
+                xv[n][0] = q4.x;
+                xv[n][1] = q4.y;
+                xv[n][2] = q4.z;
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const uint2 w = v[u][3 * r + c];
+                    const float f[4] = {__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xffff0000u),
+                                        __uint_as_float(w.y << 16), __uint_as_float(w.y & 0xffff0000u)};
+#pragma unroll
+                    for (int n = 0; n < 4; ++n) a[r] = fma(static_cast<double>(f[n]), xv[n][c], a[r]);
+                }
+        }
+    }
+    double a0 = a[0], a1 = a[1], a2 = a[2];
+#pragma unroll
+    for (int o = kLanes / 2; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, kLanes);
+        a1 += __shfl_xor(a1, o, kLanes);
+        a2 += __shfl_xor(a2, o, kLanes);
+    }
+    if (kSweep == 0) {
+        if (lane < 3) {
+            const int64_t row = lane * nq64 + i;
+            const double a = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+            out[row] = free_mask[row] ? b[row] - a : 0.0;
+        }
+    } else if (lane < 3) {
+        const double r0 = free_mask[i] ? b[i] - a0 : 0.0;
+        const double r1 = free_mask[nq64 + i] ? b[nq64 + i] - a1 : 0.0;
+        const double r2 = free_mask[2 * nq64 + i] ? b[2 * nq64 + i] - a2 : 0.0;
+        const double *o = inv_u + static_cast<int64_t>(i) * 9 + 3 * lane;
+        const int64_t row = lane * nq64 + i;
+        out[row] = x[row] + omega * (o[0] * r0 + o[1] * r1 + o[2] * r2);
     }
 }
 
@@ -754,12 +871,7 @@ int mg_setup(EquationsSet &s, int precision)
         q1_galerkin_element_kernel<float><<<eb, 256, 0, rt.stream>>>(s.n_elements, s.kbuf32.ptr, s.conn.ptr,
                                                                     s.n_nodes, s.free_mask.ptr, g.ke1.ptr);
     OC_HIP_CHECK(hipGetLastError());
-    if (precision == OC_FP32 && s.values_bf.ptr) {
-        const int64_t nnz = static_cast<int64_t>(s.values_bf.n);
-        values_bf16_kernel<<<static_cast<unsigned>(std::min<int64_t>((nnz + 255) / 256, 16384)), 256, 0, rt.stream>>>(
-            nnz, s.values_sp.ptr, s.values_bf.ptr);
-        OC_HIP_CHECK(hipGetLastError());
-    }
+    if (precision == OC_FP32 && s.uu_bf.ptr) OC_CHECK(launch_uu_pack(s));
     MgLevel &L1 = g.levels[1];
     q1_stencil_kernel<<<grid1(static_cast<int64_t>(L1.nn) * 27), 256, 0, rt.stream>>>(
         M, g.n.ptr, g.e_off.ptr, L1.off.ptr, L1.nn, g.ke1.ptr, L1.A.ptr);
@@ -839,20 +951,88 @@ static int launch_uu_node(int variant, int nq, const EquationsSet &s, const VT *
     return OC_NO_ERROR;
 }
 
+// The packed bf16 level-0 kernels' shapes: lanes per node x quads of
+// neighbours per lane in flight (variants as kNodeVariants).
+constexpr int kBf16VariantLanes[kNodeVariants] = {16, 8, 32, 8, 16};
+
+template <int kSweep>
+static int launch_uu_bf16(int variant, const EquationsSet &s, const double *b, const double *x, const double *inv_u,
+                          double omega, double *out)
+{
+    Runtime &rt = runtime();
+    if (variant < 0 || variant >= kNodeVariants) variant = 0;
+    const int nq = s.n_nodes;
+    const unsigned grid = static_cast<unsigned>((static_cast<int64_t>(nq) * kBf16VariantLanes[variant] + 255) / 256);
+    float4 *x4 = reinterpret_cast<float4 *>(s.uu_x4.ptr);
+    pack_x4(nq, x, x4);
+#define OC_UU_BF16(L, Q)                                                                                          \
+    uu_node_bf16_kernel<L, kSweep, Q><<<grid, 256, 0, rt.stream>>>(nq, s.uu_off.ptr, s.uu_col.ptr, s.uu_bf.ptr, \
+                                                                 s.free_mask.ptr, b, x4, x, inv_u, omega, out)
+    switch (variant) {
+    case 1: OC_UU_BF16(8, 1); break;
+    case 2: OC_UU_BF16(32, 1); break;
+    case 3: OC_UU_BF16(8, 2); break;
+    case 4: OC_UU_BF16(16, 2); break;
+    default: OC_UU_BF16(16, 1); break;
+    }
+#undef OC_UU_BF16
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
+void pack_x4(int nq, const double *x, float4 *x4)
+{
+    pack_x4_kernel<<<static_cast<unsigned>((nq + 255) / 256), 256, 0, runtime().stream>>>(nq, x, x4);
+}
+
+int launch_uu_pack(EquationsSet &s)
+{
+    uu_pack_kernel<<<static_cast<unsigned>((static_cast<int64_t>(s.n_nodes) * kPackLanes + 255) / 256), 256, 0,
+                     runtime().stream>>>(s.n_nodes, s.row_ptr.ptr, s.pstart.ptr, s.values_sp.ptr, s.uu_off.ptr,
+                                         s.uu_bf.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
+int prepare_uu_bf16(EquationsSet &s)
+{
+    if (s.uu_bf.ptr) return OC_NO_ERROR;
+    if (s.node_rows != 1 || !s.pstart.ptr) return set_error(OC_ERR_UNSUPPORTED, "the packed bf16 block needs node rows");
+    Runtime &rt = runtime();
+    const int nq = s.n_nodes;
+    DevBuf<int64_t> degp;
+    OC_CHECK(degp.alloc(nq));
+    uu_degp_kernel<<<static_cast<unsigned>((nq + 255) / 256), 256, 0, rt.stream>>>(nq, s.row_ptr.ptr, s.pstart.ptr,
+                                                                                   degp.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    std::vector<int64_t> off(static_cast<size_t>(nq) + 1, 0);
+    OC_HIP_CHECK(hipMemcpyAsync(off.data() + 1, degp.ptr, sizeof(int64_t) * nq, hipMemcpyDeviceToHost, rt.stream));
+    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+    for (int i = 0; i < nq; ++i) off[i + 1] += off[i];
+    const int64_t total = off[nq];
+    OC_CHECK(s.uu_off.upload(off.data(), off.size()));
+    OC_CHECK(s.uu_col.alloc(static_cast<size_t>(total)));
+    OC_CHECK(s.uu_bf.alloc(static_cast<size_t>(total) * 9));
+    OC_CHECK(s.uu_x4.alloc(static_cast<size_t>(nq) * 4));
+    uu_cols_kernel<<<static_cast<unsigned>((static_cast<int64_t>(nq) * kPackLanes + 255) / 256), 256, 0, rt.stream>>>(
+        nq, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr, s.uu_off.ptr, s.uu_col.ptr);
+    OC_HIP_CHECK(hipGetLastError());
+    OC_HIP_CHECK(hipStreamSynchronize(rt.stream));
+    return OC_NO_ERROR;
+}
+
 int launch_smoother_residual(EquationsSet &s, int precision, int variant, const double *b, const double *x,
                              double *r)
 {
     if (s.node_rows != 1) return set_error(OC_ERR_UNSUPPORTED, "the node-wise smoother needs node rows");
     // every array the kernel reads must exist: the free-DOF mask only does once
     // boundary conditions are set or a solve has prepared the set
-    const void *vals = precision == OC_FP32 ? (s.values_bf.ptr ? static_cast<const void *>(s.values_bf.ptr)
-                                                               : static_cast<const void *>(s.values_sp.ptr))
+    const void *vals = precision == OC_FP32 ? (s.uu_bf.ptr ? static_cast<const void *>(s.uu_bf.ptr)
+                                                           : static_cast<const void *>(s.values_sp.ptr))
                                             : static_cast<const void *>(s.values.ptr);
     if (!s.free_mask.ptr || !s.pstart.ptr || !vals || !b || !x || !r)
         return set_error(OC_ERR_INVALID_ARGUMENT, "smoother residual: free-DOF mask, pressure starts or values missing");
-    if (precision == OC_FP32 && s.values_bf.ptr)
-        return launch_uu_node<Bf16, 0>(variant, s.n_nodes, s, reinterpret_cast<const Bf16 *>(s.values_bf.ptr), b, x,
-                                       nullptr, 0.0, r);
+    if (precision == OC_FP32 && s.uu_bf.ptr) return launch_uu_bf16<0>(variant, s, b, x, nullptr, 0.0, r);
     if (precision == OC_FP32) return launch_uu_node<float, 0>(variant, s.n_nodes, s, s.values_sp.ptr, b, x, nullptr, 0.0, r);
     return launch_uu_node<double, 0>(variant, s.n_nodes, s, s.values.ptr, b, x, nullptr, 0.0, r);
 }
@@ -865,14 +1045,13 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     const int M = s.n_meshes, nq = s.n_nodes, nu = 3 * nq, nl = static_cast<int>(g.levels.size());
     const double *inv_u = s.precond.ptr;  // nodal block inverses of the set's block preconditioner
     const unsigned gu = static_cast<unsigned>((static_cast<int64_t>(nu) * 32 + 255) / 256);
-    const bool bf = precision == OC_FP32 && s.node_rows == 1 && s.values_bf.ptr;
-    const Bf16 *vbf = reinterpret_cast<const Bf16 *>(s.values_bf.ptr);
+    const bool bf = precision == OC_FP32 && s.node_rows == 1 && s.uu_bf.ptr;
     // default shape: 16 lanes x 2 neighbours in flight for FP64 values (6%
     // faster than 32 x 1 at 32^3), 32 x 1 for bf16 and FP32
     const int variant = s.tuned_smoother >= 0 ? s.tuned_smoother : (precision == OC_FP64 ? 2 : 0);
     auto residual0 = [&](const double *b, const double *x, double *r) -> int {
         if (bf)
-            OC_CHECK((launch_uu_node<Bf16, 0>(variant, nq, s, vbf, b, x, nullptr, 0.0, r)));
+            OC_CHECK(launch_uu_bf16<0>(variant, s, b, x, nullptr, 0.0, r));
         else if (s.node_rows == 1 && precision == OC_FP32)
             OC_CHECK((launch_uu_node<float, 0>(variant, nq, s, s.values_sp.ptr, b, x, nullptr, 0.0, r)));
         else if (s.node_rows == 1)
@@ -894,7 +1073,7 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     if (fused && ((std::max(g.pre - 1, 0) + g.post) & 1)) std::swap(cur, other);
     auto sweep0 = [&]() -> int {
         if (bf)
-            OC_CHECK((launch_uu_node<Bf16, 1>(variant, nq, s, vbf, t, cur, inv_u, g.omega, other)));
+            OC_CHECK(launch_uu_bf16<1>(variant, s, t, cur, inv_u, g.omega, other));
         else if (precision == OC_FP32)
             OC_CHECK((launch_uu_node<float, 1>(variant, nq, s, s.values_sp.ptr, t, cur, inv_u, g.omega, other)));
         else
