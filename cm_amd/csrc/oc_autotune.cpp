@@ -109,13 +109,6 @@ extern "C" int oc_autotune(oc_equations_set h, int precision, char *report, int 
     // SpMV: lanes per row (per node for node rows) x neighbours in flight per
     // lane; default 32 x 1 (FP32 values) or 16 x 2 (FP64) on node rows
     OC_CHECK(prepare_node_rows(s));
-    if (fp32 && s.node_rows == 1 && !s.uu_x4.ptr) {  // the FP32 product's iterate quadruples, as a solve makes
-        OC_CHECK(s.uu_x4.alloc(static_cast<size_t>(s.n_nodes) * 4));
-        if (s.step_graph) {
-            (void)hipGraphExecDestroy(s.step_graph);
-            s.step_graph = nullptr;
-        }
-    }
     DevBuf<double> y;
     OC_CHECK(y.alloc(s.n_rows));
     std::vector<std::pair<int, int>> spmv;

@@ -396,39 +396,4 @@ __device__ __forceinline__ void node_rows_uu(const int64_t (&s)[3], int deg, int
     }
 }
 
-// node_rows_uu with the iterate gathered as FP32 node quadruples
-// (x, y, z, 0): one 16-byte load per neighbour, for FP32 matrix values
-template <typename VT, int kLanes, int kU>
-__device__ __forceinline__ void node_rows_uu_x4(const int64_t (&s)[3], int deg, int lane,
-                                                const int *__restrict__ col_idx, const VT *__restrict__ vals,
-                                                const float4 *__restrict__ x4, double (&a)[3])
-{
-    for (int p0 = lane; p0 < deg; p0 += kLanes * kU) {
-        int j[kU];
-        bool ok[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            ok[u] = p0 + u * kLanes < deg;
-            j[u] = ok[u] ? col_idx[s[0] + p0 + u * kLanes] : 0;
-        }
-        double v[kU][9];
-#pragma unroll
-        for (int u = 0; u < kU; ++u)
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    v[u][3 * r + c] = ok[u] ? static_cast<double>(vals[s[r] + c * deg + p0 + u * kLanes]) : 0.0;
-        float4 xq[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) xq[u] = ok[u] ? x4[j[u]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-        for (int u = 0; u < kU; ++u)
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-                a[r] += v[u][3 * r] * static_cast<double>(xq[u].x) + v[u][3 * r + 1] * static_cast<double>(xq[u].y) +
-                        v[u][3 * r + 2] * static_cast<double>(xq[u].z);
-    }
-}
-
 }  // namespace oc

@@ -12,7 +12,6 @@
 #include <chrono>
 #include <cstdio>
 #include <cmath>
-#include <type_traits>
 #include <vector>
 
 #include "oc_fe_device.h"
@@ -329,14 +328,12 @@ __global__ void __launch_bounds__(256) node_pattern_check_kernel(int nq, const i
 // y = W (A x) (or A x when w is NULL) of a Mooney-Rivlin set with node rows:
 // groups g < nq take node g's three displacement rows (one read of the
 // neighbour and pressure column lists), groups nq + l take pressure row l
-// (kX4: the displacement block gathers the iterate from x4, the FP32 node
-// quadruples of x, as the FP32 Jacobian's product does)
-template <typename VT, int kLanes, int kU, bool kX4>
+template <typename VT, int kLanes, int kU>
 __global__ void __launch_bounds__(256) spmv_node_kernel(int nq, int nl, const int64_t *__restrict__ row_ptr,
                                                        const int64_t *__restrict__ pstart,
                                                        const int *__restrict__ col_idx, const VT *__restrict__ vals,
                                                        const double *__restrict__ w, const double *__restrict__ x,
-                                                       const float4 *__restrict__ x4, double *__restrict__ y)
+                                                       double *__restrict__ y)
 {
     const int lane = threadIdx.x % kLanes;
     const int g = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kLanes);
@@ -361,10 +358,7 @@ __global__ void __launch_bounds__(256) spmv_node_kernel(int nq, int nl, const in
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     if (w0 != 0.0 || w1 != 0.0 || w2 != 0.0) {
         double a[3] = {0.0, 0.0, 0.0};
-        if (kX4)
-            node_rows_uu_x4<VT, kLanes, kU>(sr, deg, lane, col_idx, vals, x4, a);
-        else
-            node_rows_uu<VT, kLanes, kU>(sr, deg, lane, nq64, col_idx, vals, x, a);
+        node_rows_uu<VT, kLanes, kU>(sr, deg, lane, nq64, col_idx, vals, x, a);
         a0 = a[0];
         a1 = a[1];
         a2 = a[2];
@@ -689,22 +683,11 @@ struct Krylov {
             const int nq = s.n_nodes, nl = s.n_nodes_l;
             const unsigned gn = static_cast<unsigned>((static_cast<int64_t>(nq + nl) * lanes + 255) / 256);
             const bool u2 = unroll == 2;
-            // an FP32 Jacobian's product gathers the displacement iterate in
-            // FP32 (node quadruples made here), as its values are FP32
-            float4 *x4 = nullptr;
-            if (std::is_same<VT, float>::value && s.uu_x4.ptr) {
-                x4 = reinterpret_cast<float4 *>(s.uu_x4.ptr);
-                pack_x4(nq, x, x4);
-            }
-#define OC_SPMV_NODE(L, U)                                                                                             \
-    do {                                                                                                               \
-        if (x4)                                                                                                        \
-            spmv_node_kernel<VT, L, U, true><<<gn, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.pstart.ptr,          \
-                                                                      s.col_idx.ptr, vals, wv, x, x4, y);            \
-        else                                                                                                           \
-            spmv_node_kernel<VT, L, U, false><<<gn, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.pstart.ptr,         \
-                                                                       s.col_idx.ptr, vals, wv, x, nullptr, y);      \
-    } while (0)
+            // the Krylov operator gathers the iterate in FP64: FP32 node
+            // quadruples, as the bf16 smoother reads, made the product about
+            // 10% faster but perturbed it enough to cost Newton iterations
+#define OC_SPMV_NODE(L, U) \
+    spmv_node_kernel<VT, L, U><<<gn, 256, 0, rt.stream>>>(nq, nl, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr, vals, wv, x, y)
             if (lanes == 16) {
                 if (u2) OC_SPMV_NODE(16, 2);
                 else OC_SPMV_NODE(16, 1);
@@ -1265,22 +1248,13 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
     K.fp32 = opt->jacobian_precision == OC_FP32;
     OC_CHECK(solver_prepare(s, opt->krylov_restart, K));
     // an FP32 Jacobian's multigrid smoother reads a packed bf16 copy of K_uu
-    // (filled in the preconditioner setup), and its products gather the
-    // iterate as FP32 node quadruples (uu_x4): both are allocated here,
-    // outside the step graph, and a graph recorded before they existed is
-    // recorded again
-    if (K.fp32 && s.node_rows == 1) {
-        bool made = false;
-        if (K.precond == OC_PRECONDITIONER_MULTIGRID && !s.uu_bf.ptr) {
-            if (!s.values_sp.ptr) OC_CHECK(s.values_sp.alloc(s.nnz));
-            OC_CHECK(prepare_uu_bf16(s));
-            made = true;
-        }
-        if (!s.uu_x4.ptr) {
-            OC_CHECK(s.uu_x4.alloc(static_cast<size_t>(s.n_nodes) * 4));
-            made = true;
-        }
-        if (made && s.step_graph) {
+    // (filled in the preconditioner setup) and gathers the iterate as FP32
+    // node quadruples: allocated here, outside the step graph, and a graph
+    // recorded before the copy existed is recorded again
+    if (K.fp32 && K.precond == OC_PRECONDITIONER_MULTIGRID && s.node_rows == 1 && !s.uu_bf.ptr) {
+        if (!s.values_sp.ptr) OC_CHECK(s.values_sp.alloc(s.nnz));
+        OC_CHECK(prepare_uu_bf16(s));
+        if (s.step_graph) {
             (void)hipGraphExecDestroy(s.step_graph);
             s.step_graph = nullptr;
         }
