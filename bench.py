@@ -410,11 +410,23 @@ def main():
                                 "csr_row_gather_fp64", wl)
     roofline = rl_elem if dominant == "mr_element_matrix_fp64" else rl_gather
     other = rl_gather if dominant == "mr_element_matrix_fp64" else rl_elem
+    # the residual against both of its roofs (its algorithmic counts: about
+    # 3.2 kFLOP per Gauss point x 27, and per element the 27 x 6 + 8 nodal
+    # values read, 35 int32 gather indices; per DOF the residual written)
+    rm, rl = kernels["mr_residual"]
+    res_s = rm / rl / 1e3
+    res_flop, res_bytes = 3200.0 * 27 * E, (27 * 6 + 8) * 8.0 * E + 35 * 4.0 * E + 8.0 * eq.n_rows
+    rl_res = {"kernel": "mr_residual_element_kernel", "avg_launch_ms": rm / rl,
+              "fp64": {"achieved": res_flop / res_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": res_flop / res_s / 1e12 / FP64_PEAK_TFLOPS, "algorithmic_count": f"3200 FLOP x 27 GP x {E} elements"},
+              "hbm": {"achieved": res_bytes / res_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": res_bytes / res_s / 1e9 / HBM_PEAK_GBS, "algorithmic_count": f"{res_bytes:.0f} bytes"}}
     secondary = {
         "kernel_ms_per_step": {name: v[0] / args.steps for name, v in kernels.items()},
         "kernel_launches_per_step": {name: v[1] / args.steps for name, v in kernels.items()},
         "wall_ms_per_step": wall * 1e3 / args.steps,
         "roofline_other_assembly_kernel": other,
+        "roofline_residual": rl_res,
     }
     progress(f"assembly: {value:.4g} elements/s, {ms / args.steps:.3f} ms per step; dominant {dominant}")
     if not args.no_secondary:

@@ -247,9 +247,12 @@ int launch_mr_gauss(EquationsSet &s)
 // ---------------------------------------------------------------------------
 // Residual (:1491-1574), fused with the Gauss-point evaluation: one block of
 // 9 elements x 27 Gauss points evaluates kinematics and the stress routine,
-// keeps JGW*sigma and DXI_DX in LDS, then sums each element residual entry
+// then sums each element residual entry
 //   R_(a,m) += sum_i DPHIDZ(i,m) JGW sigma(i,a),  R_(p,m) += J_X w (J-1) psi_m
-// over the Gauss points in order into a per-element buffer.  A gather kernel
+// over the Gauss points in order into a per-element buffer.  With
+// DPHIDZ(i,m) = sum_j dphi_m/dxi_j DXI_DX(j,i), each Gauss point keeps in LDS
+// only T(j,a) = sum_i DXI_DX(j,i) JGW sigma(i,a), and the entry sums
+// dphi_m/dxi_j T(j,a) over j and the Gauss points.  A gather kernel
 // then adds the element contributions of every DOF in ascending element order
 // (the reference's element-loop order, equations_set_routines.f90:5535-5568).
 //
@@ -279,9 +282,8 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
     __shared__ double s_u[kNodal > kResults ? kNodal : kResults];
     double(*s_X)[81] = reinterpret_cast<double(*)[81]>(s_u);
     double(*s_z)[81] = reinterpret_cast<double(*)[81]>(s_u + kGaussElemsPerBlock * 81);
-    double(*s_dx)[9][27] = reinterpret_cast<double(*)[9][27]>(s_u);
-    double(*s_S)[9][27] = reinterpret_cast<double(*)[9][27]>(s_u + kGaussElemsPerBlock * 9 * 27);  // JGW sigma(i, a): [i + 3a][g]
-    double(*s_rp)[27] = reinterpret_cast<double(*)[27]>(s_u + 2 * kGaussElemsPerBlock * 9 * 27);
+    double(*s_T)[9][27] = reinterpret_cast<double(*)[9][27]>(s_u);  // T(j, a): [3a + j][g]
+    double(*s_rp)[27] = reinterpret_cast<double(*)[27]>(s_u + kGaussElemsPerBlock * 9 * 27);
     constexpr int kTabPer = (27 * 27 * 3 + kResThreads - 1) / kResThreads;
     constexpr int kNodePer = (kGaussElemsPerBlock * 81 + kResThreads - 1) / kResThreads;
     const int t = threadIdx.x;
@@ -334,7 +336,7 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
         const int le = t / 27, g = t % 27;
         const int v = e0 + le, e = kFD ? v / 89 : v;
         const bool active = t < kGaussElemsPerBlock * 27 && v < n_elements;
-        double rdx[9], rS[9], rrp = 0.0;
+        double rT[9], rrp = 0.0;
         if (active) {
             GaussKinematics k;
             gauss_kinematics(s_tab, g, s_z[le], s_X[le], k);
@@ -351,22 +353,24 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
                 jgw = k.jz * T->w27[g];
                 J = k.Jznu;
             }
-            for (int i = 0; i < 9; ++i) rdx[i] = k.dxidz[i];
+            // dxidz[j + 3 i] = DXI_DX(j, i)
             for (int a = 0; a < 3; ++a)
-                for (int i = 0; i < 3; ++i) rS[i + 3 * a] = jgw * sigma[t2v(i, a)];
+                for (int j = 0; j < 3; ++j) {
+                    double acc = 0.0;
+                    for (int i = 0; i < 3; ++i) acc = fma(k.dxidz[j + 3 * i], jgw * sigma[t2v(i, a)], acc);
+                    rT[3 * a + j] = acc;
+                }
             rrp = k.jx * T->w27[g] * (J - 1.0);
         }
         __syncthreads();  // X and z consumed: their LDS takes the results
         if (active) {
-            for (int i = 0; i < 9; ++i) s_dx[le][i][g] = rdx[i];
-            for (int i = 0; i < 9; ++i) s_S[le][i][g] = rS[i];
+            for (int i = 0; i < 9; ++i) s_T[le][i][g] = rT[i];
             s_rp[le][g] = rrp;
         }
     }
     __syncthreads();
-    // one work item per (element, displacement node m): the DPHIDZ of node m
-    // is formed once per Gauss point for its three component rows; then one
-    // item per pressure row
+    // one work item per (element, displacement node m), for its three
+    // component rows; then one item per pressure row
     for (int w = t; w < kGaussElemsPerBlock * 35; w += kResThreads) {
         const int le = w / 35, q = w % 35;
         const int v = e0 + le;
@@ -374,18 +378,13 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
         double acc[3] = {0.0, 0.0, 0.0};
         if (q < 27) {
             const int m = q;
-#pragma unroll 1
+#pragma unroll 3
             for (int g = 0; g < 27; ++g) {
-                const double f1 = s_tab[(0 * 27 + m) * kTabStride + g], f2 = s_tab[(1 * 27 + m) * kTabStride + g],
-                             f3 = s_tab[(2 * 27 + m) * kTabStride + g];
-                double d[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    d[i] = s_dx[le][0 + 3 * i][g] * f1 + s_dx[le][1 + 3 * i][g] * f2 + s_dx[le][2 + 3 * i][g] * f3;
+                const double f[3] = {s_tab[(0 * 27 + m) * kTabStride + g], s_tab[(1 * 27 + m) * kTabStride + g],
+                                     s_tab[(2 * 27 + m) * kTabStride + g]};
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    const double dot = d[0] * s_S[le][0 + 3 * a][g] + d[1] * s_S[le][1 + 3 * a][g] +
-                                       d[2] * s_S[le][2 + 3 * a][g];
+                    const double dot = fma(f[2], s_T[le][3 * a + 2][g], fma(f[1], s_T[le][3 * a + 1][g], f[0] * s_T[le][3 * a][g]));
                     acc[a] = acc[a] + dot;
                 }
             }
