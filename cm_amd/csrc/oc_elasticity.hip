@@ -406,6 +406,31 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
     }
 }
 
+// Schur-complement estimate of the pressure block, S = B K^-1 B^T ~ M_p / mu
+// (the pressure mass matrix over the shear modulus, as for Stokes flow):
+// lumped, in the current configuration, each element's part divided by its
+// linearised shear modulus mu = 2 (c10 + c01).  One thread per pressure node
+// sums its elements' integrals of psi_m J_z w (its gather list: entries
+// e*89 + 81 + m, ascending elements); inv_p = -1 / S.
+__global__ void __launch_bounds__(256)
+schur_mass_kernel(int nl, const int *__restrict__ gptr, const int *__restrict__ gidx, int n_elements,
+                  const double *__restrict__ rec, const BasisTables *__restrict__ T, const double *__restrict__ mat,
+                  double *__restrict__ inv_p)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nl) return;
+    const int64_t n_gp = static_cast<int64_t>(n_elements) * 27;
+    double sum = 0.0;
+    for (int k = gptr[p]; k < gptr[p + 1]; ++k) {
+        const int64_t e = gidx[k] / 89;
+        const int m = gidx[k] % 89 - 81;
+        double acc = 0.0;
+        for (int g = 0; g < 27; ++g) acc += T->lin27[(g * 8 + m) * 4] * rec[GP_JGW * n_gp + e * 27 + g];
+        sum += acc / (2.0 * (mat[2 * e] + mat[2 * e + 1]));
+    }
+    inv_p[p] = sum > 0.0 ? -1.0 / sum : -1.0;
+}
+
 // residual[d] = sum over the element entries of DOF d, ascending elements.
 __global__ void __launch_bounds__(256)
 dof_gather_kernel(int n_rows, const int *__restrict__ ptr, const int *__restrict__ idx,
@@ -1086,6 +1111,20 @@ static int launch_row_gather(EquationsSet &s, const Real *kbuf, Real *values, in
                                                                         values);
         break;
     }
+    OC_HIP_CHECK(hipGetLastError());
+    return OC_NO_ERROR;
+}
+
+int launch_schur_mass(EquationsSet &s, double *inv_p)
+{
+    if (!s.gp_records.ptr || s.subtype != OC_MOONEY_RIVLIN_SUBTYPE)
+        return set_error(OC_ERR_UNSUPPORTED, "the pressure-mass Schur estimate needs the analytic Jacobian's Gauss records");
+    const BasisTables *T = nullptr;
+    OC_CHECK(ensure_tables(&T));
+    const int nl = s.n_nodes_l;
+    schur_mass_kernel<<<static_cast<unsigned>((nl + 255) / 256), 256, 0, runtime().stream>>>(
+        nl, s.gather_ptr.ptr + 3 * static_cast<int64_t>(s.n_nodes), s.gather_idx.ptr, s.n_elements, s.gp_records.ptr, T,
+        s.mat.ptr, inv_p);
     OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
 }

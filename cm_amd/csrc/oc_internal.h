@@ -313,6 +313,9 @@ int mr_jacobian_buffers(EquationsSet &s, int precision);  // element-matrix and 
 int launch_mr_element_matrices(EquationsSet &s, int precision, int blocks_per_cu);
 int launch_mr_row_gather(EquationsSet &s, int precision, int chunk);
 int launch_mr_gauss(EquationsSet &s);
+// pressure-block Schur estimate from the pressure mass (analytic MR Jacobians:
+// it reads the Gauss records of the last Jacobian assembly)
+int launch_schur_mass(EquationsSet &s, double *inv_p);
 // multigrid (oc_multigrid.hip): setup after a Jacobian assembly in
 // `precision`; apply z_u = V-cycle(t) for the displacement block
 int mg_setup(EquationsSet &s, int precision);
