@@ -127,7 +127,7 @@ struct Multigrid {
     DevBuf<double> t0, r0, x0;            // level-0 scratch (3*n_nodes each)
     DevBuf<double> coarse_inv;            // per mesh dense inverse of the coarsest level
     int coarse_stride = 0;                // (max DOFs per mesh at the coarsest level)^2
-    int pre = 2, post = 2;
+    int pre = 1, post = 1;
     double omega = 0.5;
     int coarse_sweeps = 0;                // 0: dense solve of the coarsest level
 };

@@ -272,7 +272,7 @@ class MooneyRivlinEquationsSet(EquationsSet):
     def jacobian_evaluate(self, precision=_lib.OC_FP64):
         call("oc_equations_set_jacobian_evaluate", self._h, int(precision))
 
-    def set_multigrid(self, pre_sweeps=2, post_sweeps=2, omega=0.5):
+    def set_multigrid(self, pre_sweeps=1, post_sweeps=1, omega=0.5):
         """Declare the mesh(es) regular n^3 generated meshes and build the
         geometric multigrid hierarchy (OC_PRECONDITIONER_MULTIGRID)."""
         meshes = getattr(self, "meshes", None) or [self.mesh]

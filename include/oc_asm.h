@@ -362,7 +362,7 @@ int oc_linear_solve(oc_equations_set equations_set, const oc_newton_options *opt
  * current Jacobian (formed with the preconditioner setup of each Newton step). */
 int oc_multigrid_structure_set(oc_equations_set equations_set, const int *n_per_mesh);
 /* V-cycle smoothing: damped nodal block-Jacobi sweeps before and after the
- * coarse correction (defaults 2, 2, 0.5: damped block Jacobi on triquadratic
+ * coarse correction (defaults 1, 1, 0.5: damped block Jacobi on triquadratic
  * elements diverges on its highest modes for larger omega, and the first
  * Jacobian of a load step, with its whole boundary increment in one element
  * layer, then stalls GMRES). */
