@@ -50,7 +50,7 @@ mr_gauss_kernel(int n_elements, const int *__restrict__ conn_q, const int *__res
     constexpr int kTabPer = (27 * 27 * 3 + 255) / 256;
     constexpr int kNodePer = (kGaussElemsPerBlock * 81 + 255) / 256;
     const int t = threadIdx.x;
-    const int e0 = blockIdx.x * kGaussElemsPerBlock;
+    const int e0 = xcd_block(blockIdx.x, gridDim.x) * kGaussElemsPerBlock;
     // all global loads are issued before the LDS writes: the table and the
     // connectivity, then the nodal values they index
     double tab[kTabPer];
@@ -287,7 +287,7 @@ mr_residual_element_kernel(int n_elements, const int *__restrict__ conn_q, const
     constexpr int kTabPer = (27 * 27 * 3 + kResThreads - 1) / kResThreads;
     constexpr int kNodePer = (kGaussElemsPerBlock * 81 + kResThreads - 1) / kResThreads;
     const int t = threadIdx.x;
-    const int e0 = blockIdx.x * kGaussElemsPerBlock;
+    const int e0 = xcd_block(blockIdx.x, gridDim.x) * kGaussElemsPerBlock;
     // all global loads are issued before the LDS writes: the table and the
     // connectivity, then the nodal values they index
     double tab[kTabPer];

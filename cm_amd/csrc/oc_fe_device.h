@@ -27,6 +27,20 @@ void build_basis_tables(BasisTables &t);
 // Device copy owned by the runtime (built at oc_initialise).
 const BasisTables *device_basis_tables();
 
+// XCD-aware block order.  MI355X deals the workgroups of a launch out to its
+// eight XCDs round-robin (block b runs on XCD b % 8), and each XCD has its own
+// L2.  Row kernels whose neighbouring rows gather the same iterate entries
+// take their block index through this bijection of [0, nb): XCD x then works
+// through one contiguous share of the blocks, so the gathers of neighbouring
+// rows meet in one L2 instead of being fetched once per XCD.  The order of the
+// work within a row, and so every result, is unchanged.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    constexpr int kXcds = 8;
+    const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 // TENSOR_TO_VOIGT3, constants.f90:284 (0-based: 11,22,33,12,13,23).
 __device__ __forceinline__ int t2v(int i, int j) { return i == j ? i : 2 + i + j; }
 
