@@ -1,4 +1,4 @@
-// oc_solver.hip -- the GPU solver that judges the assembly path: Dirichlet
+// oc_solver.hip -- the GPU solver that checks the assembly path end to end: Dirichlet
 // elimination by masking, CSR SpMV, block preconditioners, right-
 // preconditioned GMRES and Newton with backtracking line search, plus the
 // load-increment rule for fixed-incremented boundary conditions.

@@ -1,7 +1,7 @@
 """Learned warm start: a per-node MLP, trained and evaluated by liboc_asm's
 own HIP kernels, that predicts each load step's nodal displacement increment
-as Newton's starting guess.  The plain Newton solve remains the judge: every
-accepted answer meets the same FP64 residual tolerance.
+as Newton's starting guess.  The Newton solve's own residual test still
+decides: every accepted answer meets the same FP64 residual tolerance.
 
 Features per quadratic node (9): reference coordinates X (3), current
 displacement z - X (3), load increment of the step, c10, c01.  Targets (4):

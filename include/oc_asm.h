@@ -245,7 +245,7 @@ int oc_solver_matrix_values_get(oc_equations_set equations_set, double *values);
 int oc_solver_matrix_values_get_sp(oc_equations_set equations_set, float *values);
 int oc_solver_vector_get(oc_equations_set equations_set, double *vector);
 
-/* ---- boundary conditions and solution (the solver "judge") ----------------- */
+/* ---- boundary conditions and solution --------------------------------------- */
 /* Fixed (or fixed-incremented) Dirichlet conditions on `n` DOFs with their
  * full values (BOUNDARY_CONDITION_FIXED[_INCREMENTED],
  * boundary_condition_routines.f90).  Replaces any previous set.  Does not
@@ -380,7 +380,7 @@ int oc_mesh_residual_norms(oc_equations_set equations_set, double *norms);
 
 /* ---- learned warm start (no counterpart in the reference) ------------------
  * A per-node MLP predicts each load step's nodal displacement increment as
- * Newton's starting guess; the plain Newton solve stays the judge.  Inputs
+ * Newton's starting guess; Newton's own residual test still decides.  Inputs
  * per quadratic node (9): X (3), current displacement z - X (3), load
  * increment, c10, c01; output (3): displacement increment per unit load
  * increment.  FP32 parameters, trained with Adam by the library's kernels. */

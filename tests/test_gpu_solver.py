@@ -1,4 +1,4 @@
-"""GPU solver (the judge of the assembly path): every accepted answer is
+"""GPU solver (the end-to-end check of the assembly path): every accepted answer is
 checked against the FP64 oracle residual, plus closed-form solutions."""
 import numpy as np
 import pytest
