@@ -1356,7 +1356,12 @@ static int newton_core(EquationsSet &s, const oc_newton_options *opt, const doub
         OC_HIP_CHECK(hipGetLastError());
         int kit = 0;
         double kres = 0.0;
-        const double ktol = eta * fnorm;
+        // the linear solve also stops once its residual is below half the
+        // Newton target, which the step then meets: an absolute floor as the
+        // KSP's ABSOLUTE_TOLERANCE (solver_routines.f90:10983-10984), tied to
+        // the Newton test so that the last step of a load step is not solved
+        // further than that test asks
+        const double ktol = std::max(eta * fnorm, 0.5 * min_wt);
         OC_CHECK(K.gmres(rhs, dx, ktol, opt->krylov_maximum_iterations, &kit, &kres));
         stats->krylov_iterations += kit;
         // a linear solve that ended above its tolerance (KSP_DIVERGED_ITS):
