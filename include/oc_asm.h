@@ -272,7 +272,9 @@ typedef struct {
     int jacobian_precision;      /* OC_FP64 or OC_FP32 */
     int krylov_restart;          /* GMRES restart length */
     int krylov_maximum_iterations;
-    double krylov_relative_tolerance; /* > 0: fixed forcing term; <= 0: Eisenstat-Walker capped by ||F||/||F_ref|| */
+    /* > 0: fixed forcing term; <= 0: Eisenstat-Walker capped by ||F||/||F_ref||.  Either way a linear solve also
+     * stops once its residual is below half the Newton target, which the step then meets */
+    double krylov_relative_tolerance;
     int line_search;             /* 0: full steps, 1: backtracking (cubic/quadratic, alpha below) */
     double line_search_alpha;    /* sufficient-decrease constant (reference default 1e-4) */
     int preconditioner;          /* OC_PRECONDITIONER_* */
