@@ -177,7 +177,8 @@ def newton_line(cm, n, stretch, load_steps, krylov_rtol):
     out = {"workload": f"{n}^3 Mooney-Rivlin, c10=2, c01=6, {round(100 * stretch)}% stretch in {load_steps} load "
                        "steps, each step to 1e-10 x the plain start's FP64 residual; GMRES(100), block-triangular "
                        "multigrid preconditioner, "
-                       + ("Eisenstat-Walker forcing" if krylov_rtol <= 0 else f"forcing rtol {krylov_rtol:g}")}
+                       + ("Eisenstat-Walker forcing" if krylov_rtol <= 0 else f"forcing rtol {krylov_rtol:g}")
+                       + ", linear solves floored at half the Newton target"}
     out.update({k: summary(v) for k, v in runs.items()})
     base = out["plain_fp64"]["seconds_per_step"]
     out["speedup_vs_plain_fp64"] = {k: base / out[k]["seconds_per_step"] for k in runs}
