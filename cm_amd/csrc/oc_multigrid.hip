@@ -1069,8 +1069,9 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     // kernel writing the other of z and x0; the first Jacobi step picks the
     // buffer that leaves the last post-smoothing sweep in z.
     const bool fused = s.node_rows == 1;
+    const int post = g.post >= 0 ? g.post : (precision == OC_FP32 ? 2 : 1);
     double *cur = z, *other = g.x0.ptr;
-    if (fused && ((std::max(g.pre - 1, 0) + g.post) & 1)) std::swap(cur, other);
+    if (fused && ((std::max(g.pre - 1, 0) + post) & 1)) std::swap(cur, other);
     auto sweep0 = [&]() -> int {
         if (bf)
             OC_CHECK(launch_uu_bf16<1>(variant, s, t, cur, inv_u, g.omega, other));
@@ -1098,12 +1099,12 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     // down the stencil levels
     for (int l = 1; l + 1 < nl; ++l) {
         MgLevel &F = g.levels[l], &C = g.levels[l + 1];
-        if (g.pre <= 1 && F.As.ptr) {
+        if (g.stencil_pre <= 1 && F.As.ptr) {
             stencil_first_residual_kernel<<<group_grid(F.nn), 256, 0, rt.stream>>>(
                 M, F.dim.ptr, F.off.ptr, F.nn, F.As.ptr, F.freed.ptr, F.b.ptr, F.Dinv.ptr, g.omega, F.x.ptr, F.r.ptr);
         } else {
             stencil_jacobi_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(F.nn, F.Dinv.ptr, F.b.ptr, F.x.ptr, g.omega, 1);
-            for (int k = 1; k < g.pre; ++k) OC_CHECK(stencil_sweep(M, F, g.omega));
+            for (int k = 1; k < g.stencil_pre; ++k) OC_CHECK(stencil_sweep(M, F, g.omega));
             stencil_apply_kernel<0><<<group_grid(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, F.A.ptr,
                                                                             F.freed.ptr, F.b.ptr, F.x.ptr, nullptr,
                                                                             0.0, F.r.ptr);
@@ -1121,13 +1122,13 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
         MgLevel &F = g.levels[l], &C = g.levels[l + 1];
         prolong_stencil_kernel<<<grid1(F.nn), 256, 0, rt.stream>>>(M, F.dim.ptr, F.off.ptr, F.nn, C.dim.ptr,
                                                                   C.off.ptr, F.freed.ptr, C.x.ptr, F.x.ptr);
-        for (int k = 0; k < g.post; ++k) OC_CHECK(stencil_sweep(M, F, g.omega));
+        for (int k = 0; k < g.stencil_post; ++k) OC_CHECK(stencil_sweep(M, F, g.omega));
         OC_HIP_CHECK(hipGetLastError());
     }
     prolong_q2_kernel<<<grid1(nq), 256, 0, rt.stream>>>(M, g.n.ptr, s.q_off.ptr, nq, L1.off.ptr, s.free_mask.ptr,
                                                         L1.x.ptr, cur);
     OC_HIP_CHECK(hipGetLastError());
-    for (int k = 0; k < g.post; ++k) {
+    for (int k = 0; k < post; ++k) {
         if (fused) {
             OC_CHECK(sweep0());
             continue;
@@ -1231,8 +1232,9 @@ extern "C" int oc_multigrid_options_set(oc_equations_set h, int pre_sweeps, int 
     OC_CHECK(require_runtime());
     if (!h) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
     if (!h->s.mg) return set_error(OC_ERR_INVALID_ARGUMENT, "no multigrid structure (oc_multigrid_structure_set)");
-    if (pre_sweeps < 1 || post_sweeps < 0 || !(omega > 0.0 && omega < 2.0))
-        return set_error(OC_ERR_INVALID_ARGUMENT, "multigrid options: pre >= 1, post >= 0, 0 < omega < 2");
+    if (pre_sweeps < 1 || !(omega > 0.0 && omega < 2.0))
+        return set_error(OC_ERR_INVALID_ARGUMENT, "multigrid options: pre >= 1, 0 < omega < 2");
+    if (post_sweeps < 0) post_sweeps = -1;  // by the Jacobian's precision
     h->s.mg->pre = pre_sweeps;
     h->s.mg->post = post_sweeps;
     h->s.mg->omega = omega;

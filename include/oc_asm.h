@@ -363,11 +363,14 @@ int oc_linear_solve(oc_equations_set equations_set, const oc_newton_options *opt
  * <= 3 nodes per direction.  Coarse operators are Galerkin products of the
  * current Jacobian (formed with the preconditioner setup of each Newton step). */
 int oc_multigrid_structure_set(oc_equations_set equations_set, const int *n_per_mesh);
-/* V-cycle smoothing: damped nodal block-Jacobi sweeps before and after the
- * coarse correction (defaults 1, 1, 0.5: damped block Jacobi on triquadratic
- * elements diverges on its highest modes for larger omega, and the first
- * Jacobian of a load step, with its whole boundary increment in one element
- * layer, then stalls GMRES). */
+/* V-cycle smoothing on the triquadratic level: damped nodal block-Jacobi
+ * sweeps before and after the coarse correction.  post_sweeps < 0 (the
+ * default) takes two after it with an FP32 Jacobian and one with an FP64
+ * Jacobian; pre 1 and omega 0.5 are the defaults (damped block Jacobi on
+ * triquadratic elements diverges on its highest modes for larger omega, and
+ * the first Jacobian of a load step, with its whole boundary increment in one
+ * element layer, then stalls GMRES).  The trilinear and coarser levels take
+ * one sweep each way. */
 int oc_multigrid_options_set(oc_equations_set equations_set, int pre_sweeps, int post_sweeps, double omega);
 /* Launch-shape autotuner (no counterpart in the reference): times the
  * element-matrix kernel's grid, the CSR row gather's prefetch depth, the
