@@ -127,14 +127,11 @@ struct Multigrid {
     DevBuf<double> t0, r0, x0;            // level-0 scratch (3*n_nodes each)
     DevBuf<double> coarse_inv;            // per mesh dense inverse of the coarsest level
     int coarse_stride = 0;                // (max DOFs per mesh at the coarsest level)^2
-    // level-0 sweeps (oc_multigrid_options_set); post < 0: two after the
-    // coarse correction with an FP32 Jacobian, whose passes read the packed
-    // bf16 copy, one with FP64 values, whose passes cost three times as much
-    int pre = 1, post = -1;
+    int pre = 1, post = 2;                // level-0 sweeps (oc_multigrid_options_set)
     // the trilinear and coarser stencil levels: one sweep each way (more
     // sweeps there cost Krylov iterations rather than saving them)
     int stencil_pre = 1, stencil_post = 1;
-    double omega = 0.5;
+    double omega = 0.8;
     int coarse_sweeps = 0;                // 0: dense solve of the coarsest level
 };
 

@@ -1069,7 +1069,7 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     // kernel writing the other of z and x0; the first Jacobi step picks the
     // buffer that leaves the last post-smoothing sweep in z.
     const bool fused = s.node_rows == 1;
-    const int post = g.post >= 0 ? g.post : (precision == OC_FP32 ? 2 : 1);
+    const int post = g.post;
     double *cur = z, *other = g.x0.ptr;
     if (fused && ((std::max(g.pre - 1, 0) + post) & 1)) std::swap(cur, other);
     auto sweep0 = [&]() -> int {
@@ -1232,9 +1232,8 @@ extern "C" int oc_multigrid_options_set(oc_equations_set h, int pre_sweeps, int 
     OC_CHECK(require_runtime());
     if (!h) return set_error(OC_ERR_INVALID_HANDLE, "equations set handle is NULL");
     if (!h->s.mg) return set_error(OC_ERR_INVALID_ARGUMENT, "no multigrid structure (oc_multigrid_structure_set)");
-    if (pre_sweeps < 1 || !(omega > 0.0 && omega < 2.0))
-        return set_error(OC_ERR_INVALID_ARGUMENT, "multigrid options: pre >= 1, 0 < omega < 2");
-    if (post_sweeps < 0) post_sweeps = -1;  // by the Jacobian's precision
+    if (pre_sweeps < 1 || post_sweeps < 0 || !(omega > 0.0 && omega < 2.0))
+        return set_error(OC_ERR_INVALID_ARGUMENT, "multigrid options: pre >= 1, post >= 0, 0 < omega < 2");
     h->s.mg->pre = pre_sweeps;
     h->s.mg->post = post_sweeps;
     h->s.mg->omega = omega;

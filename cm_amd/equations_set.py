@@ -272,11 +272,10 @@ class MooneyRivlinEquationsSet(EquationsSet):
     def jacobian_evaluate(self, precision=_lib.OC_FP64):
         call("oc_equations_set_jacobian_evaluate", self._h, int(precision))
 
-    def set_multigrid(self, pre_sweeps=1, post_sweeps=-1, omega=0.5):
+    def set_multigrid(self, pre_sweeps=1, post_sweeps=2, omega=0.8):
         """Declare the mesh(es) regular n^3 generated meshes and build the
-        geometric multigrid hierarchy (OC_PRECONDITIONER_MULTIGRID).  The sweep
-        counts are the triquadratic level's; post_sweeps < 0 takes two with an
-        FP32 Jacobian and one with an FP64 Jacobian."""
+        geometric multigrid hierarchy (OC_PRECONDITIONER_MULTIGRID); the sweep
+        counts are the triquadratic level's."""
         meshes = getattr(self, "meshes", None) or [self.mesh]
         ns = []
         for m in meshes:

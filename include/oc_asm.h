@@ -364,13 +364,12 @@ int oc_linear_solve(oc_equations_set equations_set, const oc_newton_options *opt
  * current Jacobian (formed with the preconditioner setup of each Newton step). */
 int oc_multigrid_structure_set(oc_equations_set equations_set, const int *n_per_mesh);
 /* V-cycle smoothing on the triquadratic level: damped nodal block-Jacobi
- * sweeps before and after the coarse correction.  post_sweeps < 0 (the
- * default) takes two after it with an FP32 Jacobian and one with an FP64
- * Jacobian; pre 1 and omega 0.5 are the defaults (damped block Jacobi on
- * triquadratic elements diverges on its highest modes for larger omega, and
- * the first Jacobian of a load step, with its whole boundary increment in one
- * element layer, then stalls GMRES).  The trilinear and coarser levels take
- * one sweep each way. */
+ * sweeps before and after the coarse correction (defaults 1, 2, 0.8).  One
+ * post-sweep needs omega <= 0.6: with larger omega it amplifies the
+ * highest modes of the element layer that takes a load step's boundary
+ * increment, and GMRES stalls on the first Jacobian of the step; two pre-sweeps
+ * at 0.8 stall it too.  The trilinear and coarser levels take one sweep each
+ * way. */
 int oc_multigrid_options_set(oc_equations_set equations_set, int pre_sweeps, int post_sweeps, double omega);
 /* Launch-shape autotuner (no counterpart in the reference): times the
  * element-matrix kernel's grid, the CSR row gather's prefetch depth, the
