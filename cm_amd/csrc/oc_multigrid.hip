@@ -53,18 +53,30 @@ q1_galerkin_element_kernel(int n_elements, const Real *__restrict__ kbuf, const 
 {
     __shared__ Real sK[81][82];  // K_e in its own precision: an FP32 Jacobian halves the block's LDS
     __shared__ double sT[81][25];
-    __shared__ double sW[27][8];
+    // W[a][c], the trilinear corner function c at quadratic node a, is nonzero
+    // at exactly the 2 x 2 x 2 nodes a on c's side of the element: both
+    // products below run over those eight (ascending a), which leaves out only
+    // exact zero terms and so gives the same sums as the full 27-term loops
+    __shared__ double sWn[8][8];
+    __shared__ int sA[8][8];
     __shared__ uint8_t sf[81];
     const int t = threadIdx.x;
-    for (int k = t; k < 27 * 8; k += 256) {
-        const int a = k / 8, c = k % 8;
-        double w = 1.0;
-        int ad[3] = {a % 3, (a / 3) % 3, a / 9};
-        for (int d = 0; d < 3; ++d) {
-            const double x = 0.5 * ad[d];
-            w *= ((c >> d) & 1) ? x : 1.0 - x;
+    if (t < 8) {
+        const int c = t;
+        int q = 0;
+        for (int a = 0; a < 27; ++a) {
+            double w = 1.0;
+            const int ad[3] = {a % 3, (a / 3) % 3, a / 9};
+            for (int d = 0; d < 3; ++d) {
+                const double x = 0.5 * ad[d];
+                w *= ((c >> d) & 1) ? x : 1.0 - x;
+            }
+            if (w != 0.0 && q < 8) {
+                sA[c][q] = a;
+                sWn[c][q] = w;
+                ++q;
+            }
         }
-        sW[a][c] = w;
     }
     for (int e = blockIdx.x; e < n_elements; e += gridDim.x) {
         __syncthreads();
@@ -79,14 +91,16 @@ q1_galerkin_element_kernel(int n_elements, const Real *__restrict__ kbuf, const 
         for (int k = t; k < 81 * 24; k += 256) {
             const int i = k / 24, col = k % 24, cp = col / 8, kk = col % 8;
             double acc = 0.0;
-            for (int a = 0; a < 27; ++a) acc += static_cast<double>(sK[i][cp * 27 + a]) * sW[a][kk];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc += static_cast<double>(sK[i][cp * 27 + sA[kk][q]]) * sWn[kk][q];
             sT[i][col] = acc;
         }
         __syncthreads();
         for (int k = t; k < 24 * 24; k += 256) {
             const int row = k / 24, col = k % 24, c = row / 8, kr = row % 8;
             double acc = 0.0;
-            for (int a = 0; a < 27; ++a) acc += sW[a][kr] * sT[c * 27 + a][col];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc += sWn[kr][q] * sT[c * 27 + sA[kr][q]][col];
             ke1[static_cast<int64_t>(e) * 576 + k] = acc;
         }
     }
@@ -159,20 +173,29 @@ galerkin_coarsen_kernel(int M, const int *__restrict__ dim_f, const int *__restr
             i[1] >= 0 && i[1] < Nf && i[2] >= 0 && i[2] < Nf) {
             const double wi = lin_weight(s % 3 - 1) * lin_weight((s / 3) % 3 - 1) * lin_weight(s / 9 - 1);
             const int64_t fi = off_f[m] + i[0] + static_cast<int64_t>(Nf) * (i[1] + static_cast<int64_t>(Nf) * i[2]);
-            for (int q = 0; q < 27; ++q) {
-                const int j[3] = {2 * J[0] + q % 3 - 1, 2 * J[1] + (q / 3) % 3 - 1, 2 * J[2] + q / 9 - 1};
-                if (j[0] < 0 || j[0] >= Nf || j[1] < 0 || j[1] >= Nf || j[2] < 0 || j[2] >= Nf) continue;
-                const int e0 = j[0] - i[0], e1 = j[1] - i[1], e2 = j[2] - i[2];
-                if (e0 < -1 || e0 > 1 || e1 < -1 || e1 > 1 || e2 < -1 || e2 > 1) continue;
-                const double w = wi * lin_weight(q % 3 - 1) * lin_weight((q / 3) % 3 - 1) * lin_weight(q / 9 - 1);
-                const int64_t fj = off_f[m] + j[0] + static_cast<int64_t>(Nf) * (j[1] + static_cast<int64_t>(Nf) * j[2]);
-                const double *a = Af + (fi * 27 + (e0 + 1) + 3 * (e1 + 1) + 9 * (e2 + 1)) * 9;
-                for (int c = 0; c < 3; ++c) {
-                    if (!freed_f[fi * 3 + c]) continue;
-                    for (int cp = 0; cp < 3; ++cp)
-                        if (freed_f[fj * 3 + cp]) blk[3 * c + cp] += w * a[3 * c + cp];
-                }
+            // fine neighbours j = 2J + q - 1 of J inside the grid and within
+            // one node of i, per direction (the other q add nothing), in
+            // ascending q = q0 + 3 q1 + 9 q2
+            int qlo[3], qhi[3];
+            for (int k = 0; k < 3; ++k) {
+                qlo[k] = max(max(0, 1 - 2 * J[k]), i[k] - 2 * J[k]);
+                qhi[k] = min(min(2, Nf - 2 * J[k]), i[k] - 2 * J[k] + 2);
             }
+            for (int q2 = qlo[2]; q2 <= qhi[2]; ++q2)
+                for (int q1 = qlo[1]; q1 <= qhi[1]; ++q1)
+                    for (int q0 = qlo[0]; q0 <= qhi[0]; ++q0) {
+                        const int j[3] = {2 * J[0] + q0 - 1, 2 * J[1] + q1 - 1, 2 * J[2] + q2 - 1};
+                        const int e0 = j[0] - i[0], e1 = j[1] - i[1], e2 = j[2] - i[2];
+                        const double w = wi * lin_weight(q0 - 1) * lin_weight(q1 - 1) * lin_weight(q2 - 1);
+                        const int64_t fj =
+                            off_f[m] + j[0] + static_cast<int64_t>(Nf) * (j[1] + static_cast<int64_t>(Nf) * j[2]);
+                        const double *a = Af + (fi * 27 + (e0 + 1) + 3 * (e1 + 1) + 9 * (e2 + 1)) * 9;
+                        for (int c = 0; c < 3; ++c) {
+                            if (!freed_f[fi * 3 + c]) continue;
+                            for (int cp = 0; cp < 3; ++cp)
+                                if (freed_f[fj * 3 + cp]) blk[3 * c + cp] += w * a[3 * c + cp];
+                        }
+                    }
         }
     }
 #pragma unroll
