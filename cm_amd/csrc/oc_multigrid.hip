@@ -414,11 +414,23 @@ __global__ void __launch_bounds__(256) uu_pack_kernel(int nq, const int64_t *__r
     if (i >= nq) return;
     const int64_t nq64 = nq, off = uu_off[i];
     const int deg = static_cast<int>((pstart[i] - row_ptr[i]) / 3), degp = static_cast<int>(uu_off[i + 1] - off);
-    uint16_t *o = uu_bf + 9 * off;
-    for (int k = lane; k < 9 * degp; k += kPackLanes) {
-        const int rc = k / degp, p = k - rc * degp, r = rc / 3, c = rc - 3 * r;
-        o[k] = p < deg ? bf16_round(v[row_ptr[r * nq64 + i] + c * deg + p]) : static_cast<uint16_t>(0);
-    }
+    const int64_t rows[3] = {row_ptr[i], row_ptr[nq64 + i], row_ptr[2 * nq64 + i]};
+    // run (r, c) starts at 9 off + (3 r + c) degp, a multiple of four (degp
+    // and every offset are), so a lane writes two neighbours as one 4-byte
+    // word: the lower address holds the even neighbour
+    uint32_t *o = reinterpret_cast<uint32_t *>(uu_bf + 9 * off);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float *src = v + rows[r] + static_cast<int64_t>(c) * deg;
+            uint32_t *dst = o + (3 * r + c) * (degp / 2);
+            for (int p = 2 * lane; p < degp; p += 2 * kPackLanes) {
+                const uint32_t lo = p < deg ? bf16_round(src[p]) : 0u;
+                const uint32_t hi = p + 1 < deg ? bf16_round(src[p + 1]) : 0u;
+                dst[p / 2] = lo | (hi << 16);
+            }
+        }
 }
 
 // the iterate as FP32 node quadruples (x, y, z, 0) for the bf16 kernels'
