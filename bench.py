@@ -232,6 +232,30 @@ def batched_line(cm, n_meshes, reps=3):
                                 "meshes": n_meshes, "newton_iterations": st["newton_iterations"],
                                 "krylov_iterations": st["krylov_iterations"],
                                 "max_final_over_reference": float(np.max(finals / refs))}
+    # load step 2 of every mesh from the step-1 solution, once from the plain
+    # start with FP64 Jacobians and once from the learned start with FP32
+    # Jacobians, with the forcing rule of newton_per_load_step; both after the
+    # set's one-time solver setup
+    from cm_amd.train_warm_start import PLAIN_KRYLOV_RTOL
+    d1 = batch.get_dofs()
+    batch.load_increment_apply(2, 5)
+    refs2 = batch.mesh_residual_norms()
+    runs = {}
+    for name, prec, warm in (("plain_fp64", cm.OC_FP64, False), ("learned_fp32", cm.OC_FP32, True)):
+        batch.set_dofs(d1)
+        cm.synchronise()
+        t0 = time.perf_counter()
+        if warm:
+            net.warm_start(batch, np.array([c["stretch"] / 5 for c in cases]))
+        batch.load_increment_apply(2, 5)
+        st, finals = batch.newton_solve(solver_options(prec, krylov_rtol=PLAIN_KRYLOV_RTOL), reference_norms=refs2,
+                                        raise_on_failure=False)
+        cm.synchronise()
+        runs[name] = {"seconds": time.perf_counter() - t0, "converged_meshes": int(np.sum(finals <= 1e-10 * refs2)),
+                      "newton_iterations": st["newton_iterations"], "krylov_iterations": st["krylov_iterations"],
+                      "max_final_over_reference": float(np.max(finals / refs2))}
+    runs["speedup"] = runs["plain_fp64"]["seconds"] / runs["learned_fp32"]["seconds"]
+    out["load_step_2"] = runs
     net.destroy()
     batch.destroy()
     return out
