@@ -459,7 +459,8 @@ uu_node_bf16_kernel(int nq, const int64_t *__restrict__ uu_off, const int *__res
                     const double *__restrict__ inv_u, double omega, double *__restrict__ out)
 {
     const int lane = threadIdx.x % kLanes;
-    const int i = static_cast<int>((static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x) / kLanes);
+    const int64_t xb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = static_cast<int>((xb * blockDim.x + threadIdx.x) / kLanes);
     if (i >= nq) return;  // whole groups
     const int64_t nq64 = nq, off = uu_off[i];
     const int n4 = static_cast<int>((uu_off[i + 1] - off) >> 2);
@@ -537,7 +538,8 @@ uu_node_residual_kernel(int nq, const int64_t *__restrict__ row_ptr, const int64
                         double *__restrict__ out)
 {
     const int lane = threadIdx.x % kLanes;
-    const int i = static_cast<int>((static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x) / kLanes);
+    const int64_t xb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = static_cast<int>((xb * blockDim.x + threadIdx.x) / kLanes);
     if (i >= nq) return;  // whole groups
     const int64_t nq64 = nq;
     const int64_t sr[3] = {row_ptr[i], row_ptr[nq64 + i], row_ptr[2 * nq64 + i]};

@@ -280,7 +280,8 @@ __global__ void __launch_bounds__(256) spmv_kernel(int n, const int64_t *__restr
                                                   double *__restrict__ y)
 {
     const int lane = threadIdx.x % kLanes;
-    const int row = static_cast<int>((static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x) / kLanes);
+    const int64_t xb = xcd_block(blockIdx.x, gridDim.x);
+    const int row = static_cast<int>((xb * blockDim.x + threadIdx.x) / kLanes);
     if (row >= n) return;
     double acc = 0.0;
     const double wr = w == nullptr ? 1.0 : w[row];
@@ -336,7 +337,8 @@ __global__ void __launch_bounds__(256) spmv_node_kernel(int nq, int nl, const in
                                                        double *__restrict__ y)
 {
     const int lane = threadIdx.x % kLanes;
-    const int g = static_cast<int>((static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x) / kLanes);
+    const int64_t xb = xcd_block(blockIdx.x, gridDim.x);
+    const int g = static_cast<int>((xb * blockDim.x + threadIdx.x) / kLanes);
     const int64_t nq64 = nq;
     if (g >= nq + nl) return;  // whole groups
     if (g >= nq) {
@@ -395,7 +397,8 @@ __global__ void __launch_bounds__(256) schur_rhs_node_kernel(int nq, int nl, con
                                                             double *__restrict__ t)
 {
     const int lane = threadIdx.x % kLanes;
-    const int i = static_cast<int>((static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x) / kLanes);
+    const int64_t xb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = static_cast<int>((xb * blockDim.x + threadIdx.x) / kLanes);
     if (i >= nq) return;  // whole groups
     const int64_t nq64 = nq;
     if (i < nl && lane == 0) z[3 * nq64 + i] = inv_p[i] * r[3 * nq64 + i];
