@@ -199,7 +199,7 @@ struct EquationsSet {
     DevBuf<int64_t> uu_off;
     DevBuf<int32_t> uu_col;
     DevBuf<uint16_t> uu_bf;
-    DevBuf<float> uu_x4;  // the iterate the smoother gathers, per node (x, y, z, 0) in FP32
+    DevBuf<float> uu_x4;  // the iterate the smoother gathers, per node (x, y, z, -) in FP32; two buffers
     DevBuf<double> dofs, residual, rhs;
 
     // Mooney-Rivlin: per-Gauss-point records (field-major, see

@@ -450,13 +450,15 @@ __global__ void __launch_bounds__(256) pack_x4_kernel(int nq, const double *__re
 // flight): one 16-byte column load, one 8-byte load per (row, component)
 // run and one 16-byte iterate load per neighbour (x4: pack_x4_kernel of x).
 // r = b - K_uu x on free rows, 0 on fixed ones; kSweep 1 writes the block
-// Jacobi update x + omega Kblock^-1 r instead (as uu_node_residual_kernel).
+// Jacobi update x + omega Kblock^-1 r instead (as uu_node_residual_kernel),
+// and with out4 also its FP32 node quadruples for the next pass.
 template <int kLanes, int kSweep, int kQ>
 __global__ void __launch_bounds__(256)
 uu_node_bf16_kernel(int nq, const int64_t *__restrict__ uu_off, const int *__restrict__ uu_col,
                     const uint16_t *__restrict__ uu_bf, const uint8_t *__restrict__ free_mask,
                     const double *__restrict__ b, const float4 *__restrict__ x4, const double *__restrict__ x,
-                    const double *__restrict__ inv_u, double omega, double *__restrict__ out)
+                    const double *__restrict__ inv_u, double omega, double *__restrict__ out,
+                    float *__restrict__ out4)
 {
     const int lane = threadIdx.x % kLanes;
     const int64_t xb = xcd_block(blockIdx.x, gridDim.x);
@@ -520,7 +522,9 @@ uu_node_bf16_kernel(int nq, const int64_t *__restrict__ uu_off, const int *__res
         const double r2 = free_mask[2 * nq64 + i] ? b[2 * nq64 + i] - a2 : 0.0;
         const double *o = inv_u + static_cast<int64_t>(i) * 9 + 3 * lane;
         const int64_t row = lane * nq64 + i;
-        out[row] = x[row] + omega * (o[0] * r0 + o[1] * r1 + o[2] * r2);
+        const double xn = x[row] + omega * (o[0] * r0 + o[1] * r1 + o[2] * r2);
+        out[row] = xn;
+        if (out4) out4[4 * static_cast<int64_t>(i) + lane] = static_cast<float>(xn);
     }
 }
 
@@ -569,10 +573,11 @@ uu_node_residual_kernel(int nq, const int64_t *__restrict__ row_ptr, const int64
     }
 }
 
-// level 0: x = (first ? 0 : x) + omega Kblock^-1 r, component-major
+// level 0: x = (first ? 0 : x) + omega Kblock^-1 r, component-major (and,
+// with x4, the result's FP32 node quadruples for the bf16 passes)
 __global__ void __launch_bounds__(256) q2_jacobi_kernel(int nq, const double *__restrict__ inv_u,
                                                        const double *__restrict__ r, double *__restrict__ x,
-                                                       double omega, int first)
+                                                       double omega, int first, float *__restrict__ x4)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
         const double *o = inv_u + static_cast<int64_t>(i) * 9;
@@ -581,7 +586,9 @@ __global__ void __launch_bounds__(256) q2_jacobi_kernel(int nq, const double *__
                              o[6] * r0 + o[7] * r1 + o[8] * r2};
         for (int c = 0; c < 3; ++c) {
             const int64_t k = static_cast<int64_t>(c) * nq + i;
-            x[k] = (first ? 0.0 : x[k]) + omega * d[c];
+            const double xn = (first ? 0.0 : x[k]) + omega * d[c];
+            x[k] = xn;
+            if (x4) x4[4 * static_cast<int64_t>(i) + c] = static_cast<float>(xn);
         }
     }
 }
@@ -623,12 +630,14 @@ __global__ void __launch_bounds__(256) restrict_q2_kernel(int M, const int *__re
     }
 }
 
-// level 1 -> 0 prolongation x0 += P0 x1 on free DOFs
+// level 1 -> 0 prolongation x0 += P0 x1 on free DOFs (and, with x4, the
+// result's FP32 node quadruples for the bf16 passes)
 __global__ void __launch_bounds__(256) prolong_q2_kernel(int M, const int *__restrict__ n_mesh,
                                                         const int *__restrict__ q_off, int nq_total,
                                                         const int *__restrict__ off1,
                                                         const uint8_t *__restrict__ free_mask,
-                                                        const double *__restrict__ x1, double *__restrict__ x0)
+                                                        const double *__restrict__ x1, double *__restrict__ x0,
+                                                        float *__restrict__ x4)
 {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq_total; q += gridDim.x * blockDim.x) {
         const int m = find_mesh(q_off, M, q);
@@ -654,6 +663,7 @@ __global__ void __launch_bounds__(256) prolong_q2_kernel(int M, const int *__res
         for (int c = 0; c < 3; ++c) {
             const int64_t k = static_cast<int64_t>(c) * nq_total + q;
             if (free_mask[k]) x0[k] += v[c];
+            if (x4) x4[4 * static_cast<int64_t>(q) + c] = static_cast<float>(x0[k]);
         }
     }
 }
@@ -993,18 +1003,24 @@ static int launch_uu_node(int variant, int nq, const EquationsSet &s, const VT *
 constexpr int kBf16VariantLanes[kNodeVariants] = {16, 8, 32, 8, 16};
 
 template <int kSweep>
+// x4: the FP32 node quadruples of x when the caller has them (nullptr: packed
+// here, into the first of the set's two quadruple buffers); out4: where a
+// sweep also writes its result's quadruples (nullptr: not written)
 static int launch_uu_bf16(int variant, const EquationsSet &s, const double *b, const double *x, const double *inv_u,
-                          double omega, double *out)
+                          double omega, double *out, const float *x4_in = nullptr, float *out4 = nullptr)
 {
     Runtime &rt = runtime();
     if (variant < 0 || variant >= kNodeVariants) variant = 0;
     const int nq = s.n_nodes;
     const unsigned grid = static_cast<unsigned>((static_cast<int64_t>(nq) * kBf16VariantLanes[variant] + 255) / 256);
-    float4 *x4 = reinterpret_cast<float4 *>(s.uu_x4.ptr);
-    pack_x4(nq, x, x4);
+    const float4 *x4 = reinterpret_cast<const float4 *>(x4_in);
+    if (!x4) {
+        pack_x4(nq, x, reinterpret_cast<float4 *>(s.uu_x4.ptr));
+        x4 = reinterpret_cast<const float4 *>(s.uu_x4.ptr);
+    }
 #define OC_UU_BF16(L, Q)                                                                                          \
     uu_node_bf16_kernel<L, kSweep, Q><<<grid, 256, 0, rt.stream>>>(nq, s.uu_off.ptr, s.uu_col.ptr, s.uu_bf.ptr, \
-                                                                 s.free_mask.ptr, b, x4, x, inv_u, omega, out)
+                                                                 s.free_mask.ptr, b, x4, x, inv_u, omega, out, out4)
     switch (variant) {
     case 1: OC_UU_BF16(8, 1); break;
     case 2: OC_UU_BF16(32, 1); break;
@@ -1050,7 +1066,7 @@ int prepare_uu_bf16(EquationsSet &s)
     OC_CHECK(s.uu_off.upload(off.data(), off.size()));
     OC_CHECK(s.uu_col.alloc(static_cast<size_t>(total)));
     OC_CHECK(s.uu_bf.alloc(static_cast<size_t>(total) * 9));
-    OC_CHECK(s.uu_x4.alloc(static_cast<size_t>(nq) * 4));
+    OC_CHECK(s.uu_x4.alloc(static_cast<size_t>(nq) * 8));  // two quadruple buffers
     uu_cols_kernel<<<static_cast<unsigned>((static_cast<int64_t>(nq) * kPackLanes + 255) / 256), 256, 0, rt.stream>>>(
         nq, s.row_ptr.ptr, s.pstart.ptr, s.col_idx.ptr, s.uu_off.ptr, s.uu_col.ptr);
     OC_HIP_CHECK(hipGetLastError());
@@ -1086,9 +1102,13 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     // default shape: 16 lanes x 2 neighbours in flight for FP64 values (6%
     // faster than 32 x 1 at 32^3), 32 x 1 for bf16 and FP32
     const int variant = s.tuned_smoother >= 0 ? s.tuned_smoother : (precision == OC_FP64 ? 2 : 0);
+    // with the packed bf16 values the kernels that produce the iterate also
+    // write its FP32 node quadruples, into the two halves of uu_x4 in turn
+    // (a sweep gathers one half and writes the other); q4 is the current one
+    float *q4 = bf ? s.uu_x4.ptr : nullptr, *q4_other = bf ? s.uu_x4.ptr + 4 * static_cast<int64_t>(nq) : nullptr;
     auto residual0 = [&](const double *b, const double *x, double *r) -> int {
         if (bf)
-            OC_CHECK(launch_uu_bf16<0>(variant, s, b, x, nullptr, 0.0, r));
+            OC_CHECK(launch_uu_bf16<0>(variant, s, b, x, nullptr, 0.0, r, q4));
         else if (s.node_rows == 1 && precision == OC_FP32)
             OC_CHECK((launch_uu_node<float, 0>(variant, nq, s, s.values_sp.ptr, b, x, nullptr, 0.0, r)));
         else if (s.node_rows == 1)
@@ -1111,22 +1131,23 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
     if (fused && ((std::max(g.pre - 1, 0) + post) & 1)) std::swap(cur, other);
     auto sweep0 = [&]() -> int {
         if (bf)
-            OC_CHECK(launch_uu_bf16<1>(variant, s, t, cur, inv_u, g.omega, other));
+            OC_CHECK(launch_uu_bf16<1>(variant, s, t, cur, inv_u, g.omega, other, q4, q4_other));
         else if (precision == OC_FP32)
             OC_CHECK((launch_uu_node<float, 1>(variant, nq, s, s.values_sp.ptr, t, cur, inv_u, g.omega, other)));
         else
             OC_CHECK((launch_uu_node<double, 1>(variant, nq, s, s.values.ptr, t, cur, inv_u, g.omega, other)));
         std::swap(cur, other);
+        std::swap(q4, q4_other);
         return OC_NO_ERROR;
     };
-    q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, t, cur, g.omega, 1);
+    q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, t, cur, g.omega, 1, q4);
     for (int k = 1; k < g.pre; ++k) {
         if (fused) {
             OC_CHECK(sweep0());
             continue;
         }
         OC_CHECK(residual0(t, cur, g.r0.ptr));
-        q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, g.r0.ptr, cur, g.omega, 0);
+        q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, g.r0.ptr, cur, g.omega, 0, q4);
     }
     OC_CHECK(residual0(t, cur, g.r0.ptr));
     MgLevel &L1 = g.levels[1];
@@ -1163,7 +1184,7 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
         OC_HIP_CHECK(hipGetLastError());
     }
     prolong_q2_kernel<<<grid1(nq), 256, 0, rt.stream>>>(M, g.n.ptr, s.q_off.ptr, nq, L1.off.ptr, s.free_mask.ptr,
-                                                        L1.x.ptr, cur);
+                                                        L1.x.ptr, cur, q4);
     OC_HIP_CHECK(hipGetLastError());
     for (int k = 0; k < post; ++k) {
         if (fused) {
@@ -1171,7 +1192,7 @@ int mg_apply(EquationsSet &s, int precision, const double *t, double *z)
             continue;
         }
         OC_CHECK(residual0(t, cur, g.r0.ptr));
-        q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, g.r0.ptr, cur, g.omega, 0);
+        q2_jacobi_kernel<<<grid1(nq), 256, 0, rt.stream>>>(nq, inv_u, g.r0.ptr, cur, g.omega, 0, q4);
     }
     OC_HIP_CHECK(hipGetLastError());
     return OC_NO_ERROR;
